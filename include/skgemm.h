@@ -1,0 +1,289 @@
+/*
+ * skgemm.h - C ABI of the MI355X-native FP64 Stream-K GEMM / SYRK library.
+ *
+ * This is the only surface host code links against.  Every entry point is
+ * extern "C", takes plain pointers and sizes, and returns an int status
+ * (SKGEMM_OK == 0, negative on error; skgemm_last_error() gives the text of
+ * the calling thread's last error).  No HIP or torch types appear here: a
+ * stream is passed as void* (a hipStream_t), a context as an opaque handle.
+ *
+ * Reference surface replaced (shriramch/stream-k-gemm, C++ class templates
+ * with static members, host pointers, void returns):
+ *   gemm<T>::compute_test / compute_streamk_packed / compute_streamk_packed_core
+ *       gemm_crsp.cpp:261,316,369         -> skgemm_dgemm(_host)
+ *   gemm<T>::compute_packed               gemm_opt.cpp:314  -> skgemm_dgemm(_host)
+ *   gemm<T>::pack_A / pack_B              gemm_crsp.cpp:342,355 -> skgemm_pack
+ *   sme_syrk<T>::compute                  include/sme_syrk.hpp:137,
+ *                                         syrk_crsp.cpp:306 -> skgemm_dsyrk(_host)
+ *   sme_gemm_trp_3d<T>::compute           include/sme_trp_3d.hpp:194
+ *                                                         -> skgemm_dgemm(_host)
+ *   linalg_pack_c16_f64 / unpack_c16      include/sme_pack.hpp:319-352
+ *                                                         -> skgemm_pack
+ *   hbm_alloc / hbm_calloc / hbm_free     include/hbm_alloc.hpp:21-50
+ *                                                         -> skgemm_dev_alloc/free
+ *   run_fmopa_loop peak probe             sme_peak_microbench.cpp:16-60
+ *                                                         -> skgemm_peak_probe
+ * The header-only C++ facade include/skgemm.hpp keeps the reference's class
+ * names and signatures on top of these functions.
+ *
+ * Operand layouts.  An operand is addressed by its free index o (m for A,
+ * n for B, n for D) and its other index r (k for A and B, m for D):
+ *   SKGEMM_ROW_MAJOR  A: m*ld + k      B: k*ld + n      D: m*ld + n
+ *   SKGEMM_COL_MAJOR  A: m + k*ld      B: k + n*ld      D: m + n*ld
+ *   SKGEMM_PACK16     (o/16)*ld*16 + r*16 + o%16   (ld >= extent of r)
+ *   SKGEMM_PACK32     (o/32)*ld*32 + r*32 + o%32
+ * PACK16 on A is the reference's PA [M/16][K][16] and the rotation's
+ * grid-packed input; PACK32 on B is PB [N/32][K][32]; PACK16 on the SYRK
+ * operand is P16(M,K) [ceil(M/16)][K][16]; PACK16 on D is the rotation's
+ * band-packed output [nb/16][Nd][16].  Unlike the reference, any M, N, K >= 0
+ * is accepted (packed panels past the extent are never read or written).
+ *
+ * Threading: a context owns one HIP stream, a device workspace and the tuning
+ * table.  Calls on one context must not overlap from several host threads;
+ * use one context per thread.  Device-pointer calls are asynchronous on the
+ * context's stream; *_host calls copy in, run and copy out synchronously.
+ */
+#ifndef SKGEMM_H_
+#define SKGEMM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SKGEMM_OK 0
+#define SKGEMM_ERR_ARG (-1)       /* invalid argument (shape, ld, layout, null) */
+#define SKGEMM_ERR_HIP (-2)       /* a HIP runtime call failed */
+#define SKGEMM_ERR_NOMEM (-3)     /* device allocation failed or budget exceeded */
+#define SKGEMM_ERR_TIMEOUT (-4)   /* a Stream-K fix-up wait hit its spin limit */
+#define SKGEMM_ERR_NOTSUP (-5)    /* unsupported combination */
+#define SKGEMM_ERR_STATE (-6)     /* call on a destroyed / wrong context */
+
+enum skgemm_layout {
+  SKGEMM_ROW_MAJOR = 0,
+  SKGEMM_COL_MAJOR = 1,
+  SKGEMM_PACK16 = 2,
+  SKGEMM_PACK32 = 3
+};
+
+/* Precision of a GEMM/SYRK call. */
+enum skgemm_precision {
+  SKGEMM_FP64 = 0,        /* plain FP64 MFMA (v_mfma_f64_16x16x4) */
+  SKGEMM_OZAKI_INT8 = 1   /* Ozaki split: INT8 slices on INT8 MFMA, FP64 sum */
+};
+
+typedef struct skgemm_ctx skgemm_ctx;
+
+/* ---- library / context ------------------------------------------------- */
+
+/* Text of the calling thread's last error ("" if none). */
+const char* skgemm_last_error(void);
+
+/* Library version string. */
+const char* skgemm_version(void);
+
+/* Create a context on HIP device `device`: a non-blocking stream at default
+ * priority, device workspace and flags for Stream-K fix-up.
+ * No counterpart in the reference (it has no device state). */
+int skgemm_create(int device, skgemm_ctx** out);
+int skgemm_destroy(skgemm_ctx* ctx);
+
+/* Use caller's stream (hipStream_t) instead of the context's own; NULL
+ * restores the context's stream. */
+int skgemm_set_stream(skgemm_ctx* ctx, void* stream);
+int skgemm_synchronize(skgemm_ctx* ctx);
+
+/* Bound on the s_sleep rounds a Stream-K owner waits for a peer's partial
+ * before it gives up and the call reports SKGEMM_ERR_TIMEOUT (default 2^26,
+ * many seconds).  Exists so a broken co-residency assumption surfaces as an
+ * error instead of a hang. */
+int skgemm_set_spin_limit(skgemm_ctx* ctx, int64_t limit);
+
+/* Device facts used by the scheduler: CU count, XCD count, LDS per CU. */
+int skgemm_device_info(skgemm_ctx* ctx, int* num_cus, int* num_xcds,
+                       int* lds_bytes_per_cu, int* clock_mhz);
+
+/* ---- device memory (include/hbm_alloc.hpp:21-50) ----------------------- */
+
+/* hbm_alloc / hbm_calloc / hbm_free on HBM of the context's device. */
+int skgemm_dev_alloc(skgemm_ctx* ctx, size_t bytes, int zero, void** out);
+int skgemm_dev_free(skgemm_ctx* ctx, void* ptr);
+int skgemm_memcpy_h2d(skgemm_ctx* ctx, void* dst, const void* src, size_t bytes);
+int skgemm_memcpy_d2h(skgemm_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* ---- plain FP64 GEMM --------------------------------------------------- */
+
+/* D = alpha * A * B + beta * D + gamma  (device pointers, async).
+ * A is M x K, B is K x N, D is M x N with the layouts above.  gamma is the
+ * reference's additive scalar (gemm_crsp.cpp:8, :297); beta is the BLAS
+ * term the reference lacks (beta == 0 never reads D).
+ * gemm_crsp.cpp:261-311 (compute_streamk_packed_core), gemm_opt.cpp:314-337
+ * (compute_packed), include/sme_trp_3d.hpp:194-215 (rotation, alpha=1,
+ * beta=gamma=0, A PACK16, B ROW_MAJOR, D PACK16). */
+int skgemm_dgemm(skgemm_ctx* ctx, int layout_a, int layout_b, int layout_d,
+                 int M, int N, int K, double alpha, const double* A,
+                 int64_t lda, const double* B, int64_t ldb, double beta,
+                 double* D, int64_t ldd, double gamma);
+
+/* Host-pointer form: copies A, B (and D if beta != 0) to the device, runs,
+ * copies D back, synchronously.  gemm_crsp.cpp:369-386 (compute_test). */
+int skgemm_dgemm_host(skgemm_ctx* ctx, int layout_a, int layout_b,
+                      int layout_d, int M, int N, int K, double alpha,
+                      const double* A, int64_t lda, const double* B,
+                      int64_t ldb, double beta, double* D, int64_t ldd,
+                      double gamma);
+
+/* ---- plain FP64 SYRK --------------------------------------------------- */
+
+/* C = alpha * A^T * A + beta * C on the lower triangle (i >= j) of the
+ * row-major N x N matrix C (== the reference's column-major upper
+ * triangle), A is K x N in layout layout_a (B-operand convention: ROW_MAJOR
+ * k*ld+n, COL_MAJOR k+n*ld, PACK16 = P16).  Entries above the diagonal are
+ * never written.  include/sme_syrk.hpp:137-149, syrk_crsp.cpp:306-348.
+ * Unlike the reference (quirk: alpha applied to the last KC block only,
+ * include/sme_syrk.hpp:123-133) alpha scales the whole product. */
+int skgemm_dsyrk(skgemm_ctx* ctx, int layout_a, int N, int K, double alpha,
+                 const double* A, int64_t lda, double beta, double* C,
+                 int64_t ldc);
+int skgemm_dsyrk_host(skgemm_ctx* ctx, int layout_a, int N, int K,
+                      double alpha, const double* A, int64_t lda, double beta,
+                      double* C, int64_t ldc);
+
+/* ---- Ozaki split form (INT8 slices, INT32 accumulation, FP64 sum) ------ */
+
+/* Same contracts as skgemm_dgemm / skgemm_dsyrk (beta/gamma supported) with
+ * the product computed by the Ozaki split: power-of-two scaling of rows of
+ * A and columns of B, `slices` INT8 slices per operand (2..8; 0 = choose
+ * from the requested accuracy), the triangular product set p+q <= slices+1
+ * on INT8 MFMA with exact INT32 accumulation, and an FP64 cross-slice sum.
+ * *err_bound (may be NULL) receives the a-priori bound on
+ * ||C - C_exact||_F / ||C_exact||_F for this call's inputs.
+ * No counterpart in the reference (its path is FP64 only). */
+int skgemm_dgemm_ozaki(skgemm_ctx* ctx, int layout_a, int layout_b,
+                       int layout_d, int M, int N, int K, double alpha,
+                       const double* A, int64_t lda, const double* B,
+                       int64_t ldb, double beta, double* D, int64_t ldd,
+                       double gamma, int slices, double* err_bound);
+int skgemm_dsyrk_ozaki(skgemm_ctx* ctx, int layout_a, int N, int K,
+                       double alpha, const double* A, int64_t lda, double beta,
+                       double* C, int64_t ldc, int slices, double* err_bound);
+
+/* ---- layout conversion (include/sme_pack.hpp, gemm_crsp.cpp:342-366) --- */
+
+/* Copy a rows x cols matrix (element (r, o) in the free-index convention
+ * above: r = k for A/B, o = m/n) from layout src_layout to dst_layout on
+ * the device.  pack_A = (COL_MAJOR -> PACK16) with o = m,
+ * pack_B = (ROW_MAJOR -> PACK32) with o = n, linalg_pack_c16_f64 =
+ * (COL_MAJOR [Nd,nb] -> PACK16 over grids).  `o_extent` x `r_extent`. */
+int skgemm_pack(skgemm_ctx* ctx, int src_layout, const double* src,
+                int64_t ld_src, int dst_layout, double* dst, int64_t ld_dst,
+                int o_extent, int r_extent);
+
+/* ---- batched / grouped launch ------------------------------------------ */
+
+/* One problem of a batch.  kind: 0 = GEMM D = alpha*A*B + beta*D + gamma,
+ * 1 = SYRK C = alpha*A^T*A + beta*C (lower).  Device pointers. */
+typedef struct skgemm_problem {
+  int kind;
+  int layout_a, layout_b, layout_d;
+  int M, N, K;
+  double alpha, beta, gamma;
+  const double* A;
+  int64_t lda;
+  const double* B;
+  int64_t ldb;
+  double* D;
+  int64_t ldd;
+} skgemm_problem;
+
+/* Run `count` independent problems in ONE persistent Stream-K launch over
+ * the flattened iteration space of all of them (plain FP64), or, for
+ * precision == SKGEMM_OZAKI_INT8, one split pass per stage over all of them.
+ * use_graph != 0 captures the launch sequence in a hipGraph on first use of
+ * a batch shape and replays it (descriptors are refreshed in device memory
+ * before each replay).  No counterpart in the reference. */
+int skgemm_batched(skgemm_ctx* ctx, const skgemm_problem* problems, int count,
+                   int precision, int slices, int use_graph);
+
+/* ---- tuning ------------------------------------------------------------- */
+
+/* Force a tile configuration (index into the kernel table) and a persistent
+ * grid size for subsequent calls on ctx; tile < 0 / grid <= 0 restore the
+ * autotuner's choice.  The reference's analogue is rebuilding with
+ * -DMC/-DNC/-DKC (sub_gemm.sh:31-52). */
+int skgemm_set_tuning(skgemm_ctx* ctx, int tile, int grid);
+/* Number of tile configurations and their shapes. */
+int skgemm_num_tiles(void);
+int skgemm_tile_info(int tile, int* bm, int* bn, int* bk, int* waves);
+/* Time candidate (tile, grid) pairs on the given shape with real launches
+ * and record the fastest in the context's table (and in `cache_path` if not
+ * NULL, one line per shape bucket).  kind 0 GEMM / 1 SYRK. */
+int skgemm_autotune(skgemm_ctx* ctx, int kind, int M, int N, int K,
+                    const char* cache_path, int* best_tile, int* best_grid,
+                    double* best_us);
+int skgemm_load_tuning(skgemm_ctx* ctx, const char* cache_path);
+/* The tile and grid the next call on this shape would use. */
+int skgemm_query_plan(skgemm_ctx* ctx, int kind, int M, int N, int K,
+                      int* tile, int* grid, int64_t* tiles, int64_t* iters);
+
+/* ---- measurement -------------------------------------------------------- */
+
+/* Elapsed milliseconds of the last compute launch on this context, timed
+ * with HIP events recorded on the stream the kernel ran on. */
+int skgemm_last_kernel_ms(skgemm_ctx* ctx, double* ms);
+
+/* MFMA peak probe: back-to-back independent MFMAs on every SIMD, no memory
+ * traffic.  which: 0 = v_mfma_f64_16x16x4_f64, 1 = v_mfma_i32_32x32x32_i8.
+ * Returns achieved TFLOP/s (TOP/s for INT8).  sme_peak_microbench.cpp:16-60. */
+int skgemm_peak_probe(skgemm_ctx* ctx, int which, double* tflops);
+/* HBM copy probe: device-to-device streaming copy of `bytes`; GB/s counts
+ * read + write bytes. */
+int skgemm_copy_probe(skgemm_ctx* ctx, size_t bytes, double* gbps);
+
+/* Fill device memory with seeded inputs: uniform(lo, hi) or normal(0, sigma)
+ * from a counter-based SplitMix64 stream (element i depends only on seed, i). */
+int skgemm_fill_uniform(skgemm_ctx* ctx, double* x, int64_t n, uint64_t seed,
+                        double lo, double hi);
+int skgemm_fill_normal(skgemm_ctx* ctx, double* x, int64_t n, uint64_t seed,
+                       double sigma);
+/* x[:, j] *= 10^u_j, u_j = uniform(0, umax) from stream `seed`, for a column-
+ * major rows x cols matrix with leading dimension ld (dynamic-range inputs). */
+int skgemm_scale_columns_pow10(skgemm_ctx* ctx, double* x, int rows, int cols,
+                               int64_t ld, uint64_t seed, double umax);
+
+/* ---- serving ------------------------------------------------------------ */
+
+/* Continuous-batching request loop over resident systems.  See
+ * skgemm_serve.h-style docs in skgemm_serve_* below. */
+typedef struct skgemm_server skgemm_server;
+
+/* Create a server with an HBM budget for per-request outputs and workspace
+ * (bytes), at most `max_batch` problems per launch and `max_inflight`
+ * launches in flight (separate streams, default priority). */
+int skgemm_server_create(int device, size_t hbm_budget, int max_batch,
+                         int max_inflight, int precision, int slices,
+                         skgemm_server** out);
+int skgemm_server_destroy(skgemm_server* s);
+/* Register a resident system: Psi (Nd x Ns, column-major, ld = Nd) already
+ * in device memory, owned by the caller.  Returns its id. */
+int skgemm_server_add_system(skgemm_server* s, const double* psi, int Nd,
+                             int Ns, int* id);
+/* Submit a request: kind 0 = overlap Ms = Psi^T Psi (lower, row-major Ns x
+ * Ns written to out), kind 1 = rotation out = Psi * Q (Q device Ns x Ns
+ * row-major; out Nd x Ns column-major).  `out` is a device buffer owned by
+ * the caller.  Thread-safe.  Returns a ticket. */
+int skgemm_server_submit(skgemm_server* s, int system, int kind,
+                         const double* q, double* out, int64_t* ticket);
+/* Block until ticket completes; *latency_us = completion - submit. */
+int skgemm_server_wait(skgemm_server* s, int64_t ticket, double* latency_us);
+/* Aggregate statistics since creation. */
+int skgemm_server_stats(skgemm_server* s, int64_t* completed, int64_t* launches,
+                        double* flops, double* busy_s, int* peak_inflight);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#endif /* SKGEMM_H_ */

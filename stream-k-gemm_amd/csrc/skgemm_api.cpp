@@ -1,0 +1,773 @@
+// C ABI of the FP64 Stream-K GEMM / SYRK library: contexts, argument
+// checking, operand descriptors, launch planning and the host-pointer
+// wrappers.  Kernels live in skgemm_kernels.hip (plain FP64) and
+// skgemm_ozaki.hip (split form).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/skgemm.h"
+#include "skgemm_ctx.h"
+#include "skgemm_internal.h"
+
+using namespace skg;
+
+namespace skg {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int hip_error(hipError_t e, const char* what) {
+  return set_error(SKGEMM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+// ---- operand descriptors --------------------------------------------------
+
+static bool valid_layout(int l) { return l >= SKGEMM_ROW_MAJOR && l <= SKGEMM_PACK32; }
+static int panel_shift(int l) { return l == SKGEMM_PACK16 ? 4 : 5; }
+
+// Input operand with free index o (extent `ext`) and contraction index k
+// (extent K).  `a_convention`: A (ROW_MAJOR = o*ld + k) vs B (ROW_MAJOR =
+// k*ld + o).
+int make_operand(int layout, bool a_convention, const double* p, int64_t ld,
+                 int ext, int K, Operand* op, const char* name) {
+  if (!valid_layout(layout))
+    return set_error(SKGEMM_ERR_ARG, "%s: unknown layout %d", name, layout);
+  Operand o{};
+  o.ptr = p;
+  o.extent = ext;
+  bool o_major;  // o varies fastest
+  if (layout == SKGEMM_ROW_MAJOR) o_major = !a_convention;
+  else if (layout == SKGEMM_COL_MAJOR) o_major = a_convention;
+  else o_major = true;
+  if (layout == SKGEMM_PACK16 || layout == SKGEMM_PACK32) {
+    const int s = panel_shift(layout);
+    if (ld < std::max(K, 1))
+      return set_error(SKGEMM_ERR_ARG, "%s: packed ld %lld < K %d", name,
+                       (long long)ld, K);
+    o.pshift = s;
+    o.pstride = ld << s;
+    o.os = 1;
+    o.ks = 1ll << s;
+    o.kcontig = 0;
+  } else if (o_major) {
+    if (ld < std::max(ext, 1))
+      return set_error(SKGEMM_ERR_ARG, "%s: ld %lld < %d", name, (long long)ld,
+                       ext);
+    o.pshift = kNoPanelShift;
+    o.pstride = 0;
+    o.os = 1;
+    o.ks = ld;
+    o.kcontig = 0;
+  } else {
+    if (ld < std::max(K, 1))
+      return set_error(SKGEMM_ERR_ARG, "%s: ld %lld < K %d", name,
+                       (long long)ld, K);
+    o.pshift = kNoPanelShift;
+    o.pstride = 0;
+    o.os = ld;
+    o.ks = 1;
+    o.kcontig = 1;
+  }
+  const bool aligned = (reinterpret_cast<uintptr_t>(p) & 15) == 0;
+  o.vec2 = aligned && (o.kcontig ? (o.os % 2 == 0)
+                                 : (o.ks % 2 == 0 && o.pstride % 2 == 0));
+  *op = o;
+  return SKGEMM_OK;
+}
+
+// Output D (m, n), M x N.
+int make_output(int layout, double* p, int64_t ld, int M, int N, Output* out,
+                const char* name) {
+  if (!valid_layout(layout))
+    return set_error(SKGEMM_ERR_ARG, "%s: unknown layout %d", name, layout);
+  Output d{};
+  d.ptr = p;
+  if (layout == SKGEMM_ROW_MAJOR) {
+    if (ld < std::max(N, 1))
+      return set_error(SKGEMM_ERR_ARG, "%s: ld %lld < N %d", name, (long long)ld, N);
+    d.pshift = kNoPanelShift; d.pstride = 0; d.ns = 1; d.ms = ld;
+  } else if (layout == SKGEMM_COL_MAJOR) {
+    if (ld < std::max(M, 1))
+      return set_error(SKGEMM_ERR_ARG, "%s: ld %lld < M %d", name, (long long)ld, M);
+    d.pshift = kNoPanelShift; d.pstride = 0; d.ns = ld; d.ms = 1;
+  } else {
+    if (ld < std::max(M, 1))
+      return set_error(SKGEMM_ERR_ARG, "%s: packed ld %lld < M %d", name,
+                       (long long)ld, M);
+    const int s = panel_shift(layout);
+    d.pshift = s; d.pstride = ld << s; d.ns = 1; d.ms = 1ll << s;
+  }
+  *out = d;
+  return SKGEMM_OK;
+}
+
+// Number of elements from the base pointer to one past the last element an
+// operand of this layout touches (0 if empty).
+int64_t span_operand(int layout, bool a_convention, int64_t ld, int ext, int K) {
+  if (ext <= 0 || K <= 0) return 0;
+  if (layout == SKGEMM_PACK16 || layout == SKGEMM_PACK32) {
+    const int s = panel_shift(layout);
+    const int64_t P = 1ll << s;
+    return ((int64_t)(ext - 1) >> s) * ld * P + (int64_t)(K - 1) * P +
+           ((ext - 1) & (P - 1)) + 1;
+  }
+  const bool o_major = (layout == SKGEMM_ROW_MAJOR) ? !a_convention : a_convention;
+  return o_major ? (int64_t)(K - 1) * ld + ext : (int64_t)(ext - 1) * ld + K;
+}
+int64_t span_output(int layout, int64_t ld, int M, int N) {
+  if (M <= 0 || N <= 0) return 0;
+  if (layout == SKGEMM_ROW_MAJOR) return (int64_t)(M - 1) * ld + N;
+  if (layout == SKGEMM_COL_MAJOR) return (int64_t)(N - 1) * ld + M;
+  const int s = panel_shift(layout);
+  const int64_t P = 1ll << s;
+  return ((int64_t)(N - 1) >> s) * ld * P + (int64_t)(M - 1) * P +
+         ((N - 1) & (P - 1)) + 1;
+}
+
+}  // namespace skg
+
+// ---- context ----------------------------------------------------------------
+
+static const int kCtxMagic = 0x5ca1ab1e;
+
+static int check_ctx(skgemm_ctx* c) {
+  if (c == nullptr || c->magic != kCtxMagic)
+    return set_error(SKGEMM_ERR_STATE, "invalid or destroyed context");
+  return SKGEMM_OK;
+}
+
+#define SKG_TRY(expr)              \
+  do {                             \
+    int _rc = (expr);              \
+    if (_rc != SKGEMM_OK) return _rc; \
+  } while (0)
+#define SKG_HIP(expr, what)                             \
+  do {                                                  \
+    hipError_t _e = (expr);                             \
+    if (_e != hipSuccess) return hip_error(_e, what);   \
+  } while (0)
+
+extern "C" const char* skgemm_last_error(void) { return g_last_error.c_str(); }
+extern "C" const char* skgemm_version(void) { return "skgemm 0.1 (gfx950)"; }
+
+extern "C" int skgemm_create(int device, skgemm_ctx** out) {
+  if (out == nullptr) return set_error(SKGEMM_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  SKG_HIP(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+  if (device < 0 || device >= ndev)
+    return set_error(SKGEMM_ERR_ARG, "device %d out of range (%d devices)",
+                     device, ndev);
+  SKG_HIP(hipSetDevice(device), "hipSetDevice");
+  hipDeviceProp_t prop;
+  SKG_HIP(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return set_error(SKGEMM_ERR_NOTSUP,
+                     "device %d is %s; this library is built for gfx950 only",
+                     device, prop.gcnArchName);
+  skgemm_ctx* c = new skgemm_ctx();
+  c->magic = kCtxMagic;
+  c->device = device;
+  c->num_cus = prop.multiProcessorCount;
+  c->num_xcds = 8;
+  c->lds_per_cu = (int)prop.maxSharedMemoryPerMultiProcessor;
+  c->clock_mhz = prop.clockRate / 1000;
+  hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return hip_error(e, "hipStreamCreate");
+  }
+  c->stream = c->own_stream;
+  if ((e = hipEventCreate(&c->ev0)) != hipSuccess ||
+      (e = hipEventCreate(&c->ev1)) != hipSuccess ||
+      (e = hipMalloc(&c->status, sizeof(int))) != hipSuccess ||
+      (e = hipMemset(c->status, 0, sizeof(int))) != hipSuccess) {
+    skgemm_destroy(c);
+    return hip_error(e, "context setup");
+  }
+  c->spin_limit = 1ll << 26;
+  *out = c;
+  return SKGEMM_OK;
+}
+
+extern "C" int skgemm_destroy(skgemm_ctx* c) {
+  SKG_TRY(check_ctx(c));
+  hipSetDevice(c->device);
+  if (c->own_stream) hipStreamSynchronize(c->own_stream);
+  if (c->stream && c->stream != c->own_stream) hipStreamSynchronize(c->stream);
+  c->free_all();
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->own_stream) hipStreamDestroy(c->own_stream);
+  c->magic = 0;
+  delete c;
+  return SKGEMM_OK;
+}
+
+extern "C" int skgemm_set_stream(skgemm_ctx* c, void* stream) {
+  SKG_TRY(check_ctx(c));
+  c->stream = stream ? reinterpret_cast<hipStream_t>(stream) : c->own_stream;
+  return SKGEMM_OK;
+}
+
+extern "C" int skgemm_synchronize(skgemm_ctx* c) {
+  SKG_TRY(check_ctx(c));
+  SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
+  SKG_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  return c->check_status();
+}
+
+extern "C" int skgemm_device_info(skgemm_ctx* c, int* num_cus, int* num_xcds,
+                                  int* lds, int* clock_mhz) {
+  SKG_TRY(check_ctx(c));
+  if (num_cus) *num_cus = c->num_cus;
+  if (num_xcds) *num_xcds = c->num_xcds;
+  if (lds) *lds = c->lds_per_cu;
+  if (clock_mhz) *clock_mhz = c->clock_mhz;
+  return SKGEMM_OK;
+}
+
+extern "C" int skgemm_set_spin_limit(skgemm_ctx* c, int64_t limit) {
+  SKG_TRY(check_ctx(c));
+  if (limit < 1) return set_error(SKGEMM_ERR_ARG, "spin limit must be >= 1");
+  c->spin_limit = limit;
+  return SKGEMM_OK;
+}
+
+// ---- memory -----------------------------------------------------------------
+
+extern "C" int skgemm_dev_alloc(skgemm_ctx* c, size_t bytes, int zero,
+                                void** out) {
+  SKG_TRY(check_ctx(c));
+  if (out == nullptr) return set_error(SKGEMM_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+  if (e != hipSuccess)
+    return set_error(SKGEMM_ERR_NOMEM, "hipMalloc(%zu): %s", bytes,
+                     hipGetErrorString(e));
+  if (zero) SKG_HIP(hipMemset(p, 0, bytes), "hipMemset");
+  *out = p;
+  return SKGEMM_OK;
+}
+
+extern "C" int skgemm_dev_free(skgemm_ctx* c, void* p) {
+  SKG_TRY(check_ctx(c));
+  SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
+  if (p) SKG_HIP(hipFree(p), "hipFree");
+  return SKGEMM_OK;
+}
+
+extern "C" int skgemm_memcpy_h2d(skgemm_ctx* c, void* dst, const void* src,
+                                 size_t bytes) {
+  SKG_TRY(check_ctx(c));
+  SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
+  SKG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream),
+          "hipMemcpyAsync H2D");
+  SKG_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  return SKGEMM_OK;
+}
+
+extern "C" int skgemm_memcpy_d2h(skgemm_ctx* c, void* dst, const void* src,
+                                 size_t bytes) {
+  SKG_TRY(check_ctx(c));
+  SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
+  SKG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream),
+          "hipMemcpyAsync D2H");
+  SKG_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  return SKGEMM_OK;
+}
+
+// ---- skgemm_ctx members -------------------------------------------------------
+
+int skgemm_ctx::check_status() {
+  int h = 0;
+  SKG_HIP(hipMemcpy(&h, status, sizeof(int), hipMemcpyDeviceToHost),
+          "status readback");
+  if (h != 0) {
+    hipMemset(status, 0, sizeof(int));
+    if (flags) hipMemset(flags, 0, sizeof(int) * flags_n);
+    hipDeviceSynchronize();
+    return set_error(SKGEMM_ERR_TIMEOUT,
+                     "a Stream-K fix-up wait exceeded its spin limit (%lld)",
+                     (long long)spin_limit);
+  }
+  return SKGEMM_OK;
+}
+
+int skgemm_ctx::ensure_ws(size_t ws_doubles, int nflags) {
+  if (ws_doubles > ws_n) {
+    if (ws) {
+      hipStreamSynchronize(stream);
+      hipFree(ws);
+      ws = nullptr;
+    }
+    hipError_t e = hipMalloc(&ws, ws_doubles * sizeof(double));
+    if (e != hipSuccess) {
+      ws_n = 0;
+      return set_error(SKGEMM_ERR_NOMEM, "workspace of %zu doubles: %s",
+                       ws_doubles, hipGetErrorString(e));
+    }
+    ws_n = ws_doubles;
+  }
+  if (nflags > flags_n) {
+    if (flags) {
+      hipStreamSynchronize(stream);
+      hipFree(flags);
+      flags = nullptr;
+    }
+    hipError_t e = hipMalloc(&flags, sizeof(int) * nflags);
+    if (e != hipSuccess) {
+      flags_n = 0;
+      return set_error(SKGEMM_ERR_NOMEM, "flags: %s", hipGetErrorString(e));
+    }
+    // Flags start at zero; every consumer resets the flag it consumed, so
+    // they are zero again at the end of every launch.
+    SKG_HIP(hipMemset(flags, 0, sizeof(int) * nflags), "hipMemset flags");
+    SKG_HIP(hipDeviceSynchronize(), "flags init");
+    flags_n = nflags;
+  }
+  return SKGEMM_OK;
+}
+
+double* skgemm_ctx::staging(int i, size_t bytes) {
+  if (bytes > stage_n[i]) {
+    if (stage[i]) {
+      hipStreamSynchronize(stream);
+      hipFree(stage[i]);
+      stage[i] = nullptr;
+      stage_n[i] = 0;
+    }
+    if (hipMalloc(&stage[i], bytes) != hipSuccess) return nullptr;
+    stage_n[i] = bytes;
+  }
+  return reinterpret_cast<double*>(stage[i]);
+}
+
+void skgemm_ctx::free_all() {
+  if (ws) hipFree(ws);
+  if (flags) hipFree(flags);
+  if (status) hipFree(status);
+  for (int i = 0; i < kStages; ++i)
+    if (stage[i]) hipFree(stage[i]);
+  ws = nullptr;
+  flags = nullptr;
+  status = nullptr;
+  free_extra();
+}
+
+int skgemm_ctx::capacity(int cfg) {
+  auto it = occ_cache.find(cfg);
+  int occ;
+  if (it == occ_cache.end()) {
+    occ = occupancy_dgemm_sk(cfg);
+    occ_cache[cfg] = occ;
+  } else {
+    occ = it->second;
+  }
+  return std::max(occ, 1) * num_cus;
+}
+
+// ---- planning ---------------------------------------------------------------
+
+namespace skg {
+
+int64_t tiles_for(int kind, int cfg, int M, int N) {
+  const TileCfg& t = tile_cfg(cfg);
+  if (kind == 1) {
+    const int64_t T = (N + t.bm - 1) / t.bm;
+    return T * (T + 1) / 2;
+  }
+  return (int64_t)((M + t.bm - 1) / t.bm) * ((N + t.bn - 1) / t.bn);
+}
+
+// Default plan when the autotuner has no entry for this shape.
+void default_plan(skgemm_ctx* c, int kind, int M, int N, int K, int* cfg,
+                  int* grid) {
+  int best = 1;
+  if (kind == 1) {
+    best = (N >= 256) ? 0 : 1;
+  } else {
+    if (M >= 128 && N >= 128) {
+      const int64_t t128 = tiles_for(0, 0, M, N);
+      best = (t128 >= c->num_cus / 2) ? 0 : 1;
+    } else if (M >= 128) {
+      best = 2;
+    } else if (N >= 128) {
+      best = 3;
+    } else {
+      best = 1;
+    }
+  }
+  const TileCfg& t = tile_cfg(best);
+  const int64_t ipt = std::max<int64_t>(1, (K + t.bk - 1) / t.bk);
+  const int64_t total = tiles_for(kind, best, M, N) * ipt;
+  const int cap = c->capacity(best);
+  *cfg = best;
+  *grid = (int)std::max<int64_t>(1, std::min<int64_t>(cap, total));
+}
+
+void plan(skgemm_ctx* c, int kind, int M, int N, int K, int* cfg, int* grid) {
+  if (c->forced_cfg >= 0) {
+    *cfg = c->forced_cfg;
+    if (kind == 1 && tile_cfg(*cfg).bm != tile_cfg(*cfg).bn) *cfg = 0;
+  } else {
+    auto it = c->tuned.find(std::make_tuple(kind, M, N, K));
+    if (it != c->tuned.end()) {
+      *cfg = it->second.first;
+      *grid = it->second.second;
+    } else {
+      default_plan(c, kind, M, N, K, cfg, grid);
+    }
+  }
+  const TileCfg& t = tile_cfg(*cfg);
+  const int64_t ipt = std::max<int64_t>(1, (K + t.bk - 1) / t.bk);
+  const int64_t total = tiles_for(kind, *cfg, M, N) * ipt;
+  const int cap = c->capacity(*cfg);
+  int g = (c->forced_grid > 0) ? c->forced_grid
+          : (c->forced_cfg >= 0 || c->tuned.find(std::make_tuple(kind, M, N, K)) == c->tuned.end())
+              ? cap
+              : *grid;
+  g = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)g, (int64_t)cap, total}));
+  *grid = g;
+}
+
+// Fill a Problem for the Stream-K kernel (tile counts depend on the cfg).
+void fill_problem(Problem* p, int kind, int cfg, int M, int N, int K) {
+  const TileCfg& t = tile_cfg(cfg);
+  p->M = M;
+  p->N = N;
+  p->K = K;
+  p->syrk = kind == 1;
+  p->tiles_m = (M + t.bm - 1) / t.bm;
+  p->tiles_n = (N + t.bn - 1) / t.bn;
+  p->iters_per_tile = (int)std::max<int64_t>(1, (K + t.bk - 1) / t.bk);
+  p->num_tiles = tiles_for(kind, cfg, M, N);
+}
+
+// Launch one problem (already described) with the given cfg / grid.
+int run_single(skgemm_ctx* c, Problem& p, int cfg, int grid) {
+  const TileCfg& t = tile_cfg(cfg);
+  p.tile_begin = 0;
+  p.iter_begin = 0;
+  Launch L{};
+  L.problems = nullptr;
+  L.single = p;
+  L.num_problems = 1;
+  L.total_iters = p.num_tiles * p.iters_per_tile;
+  L.grid = grid;
+  SKG_TRY(c->ensure_ws((size_t)grid * t.bm * t.bn, grid));
+  L.ws = c->ws;
+  L.flags = c->flags;
+  L.status = c->status;
+  L.spin_limit = c->spin_limit;
+  SKG_HIP(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
+  SKG_HIP(launch_dgemm_sk(cfg, L, c->stream), "Stream-K launch");
+  SKG_HIP(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
+  c->timed = true;
+  return SKGEMM_OK;
+}
+
+}  // namespace skg
+
+static int check_dims(int M, int N, int K) {
+  if (M < 0 || N < 0 || K < 0)
+    return set_error(SKGEMM_ERR_ARG, "negative dimension (M=%d N=%d K=%d)", M, N, K);
+  if (M >= (1 << 30) || N >= (1 << 30) || K >= (1 << 30))
+    return set_error(SKGEMM_ERR_ARG, "dimension >= 2^30");
+  return SKGEMM_OK;
+}
+
+namespace skg {
+
+int prepare_gemm(int la, int lb, int ld_, int M, int N, int K, const double* A,
+                 int64_t lda, const double* B, int64_t ldb, double* D,
+                 int64_t ldd, Problem* p) {
+  SKG_TRY(check_dims(M, N, K));
+  if (M > 0 && N > 0) {
+    if (D == nullptr) return set_error(SKGEMM_ERR_ARG, "D is NULL");
+    if (K > 0 && (A == nullptr || B == nullptr))
+      return set_error(SKGEMM_ERR_ARG, "A or B is NULL");
+  }
+  SKG_TRY(make_operand(la, true, A, lda, M, K, &p->a, "A"));
+  SKG_TRY(make_operand(lb, false, B, ldb, N, K, &p->b, "B"));
+  SKG_TRY(make_output(ld_, D, ldd, M, N, &p->d, "D"));
+  return SKGEMM_OK;
+}
+
+int prepare_syrk(int la, int N, int K, const double* A, int64_t lda, double* C,
+                 int64_t ldc, Problem* p) {
+  SKG_TRY(check_dims(N, N, K));
+  if (N > 0) {
+    if (C == nullptr) return set_error(SKGEMM_ERR_ARG, "C is NULL");
+    if (K > 0 && A == nullptr) return set_error(SKGEMM_ERR_ARG, "A is NULL");
+  }
+  SKG_TRY(make_operand(la, false, A, lda, N, K, &p->a, "A"));
+  p->b = p->a;
+  SKG_TRY(make_output(SKGEMM_ROW_MAJOR, C, ldc, N, N, &p->d, "C"));
+  return SKGEMM_OK;
+}
+
+}  // namespace skg
+
+extern "C" int skgemm_dgemm(skgemm_ctx* c, int la, int lb, int ld_, int M,
+                            int N, int K, double alpha, const double* A,
+                            int64_t lda, const double* B, int64_t ldb,
+                            double beta, double* D, int64_t ldd, double gamma) {
+  SKG_TRY(check_ctx(c));
+  Problem p{};
+  SKG_TRY(prepare_gemm(la, lb, ld_, M, N, K, A, lda, B, ldb, D, ldd, &p));
+  if (M == 0 || N == 0) return SKGEMM_OK;
+  SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
+  int cfg, grid;
+  plan(c, 0, M, N, K, &cfg, &grid);
+  fill_problem(&p, 0, cfg, M, N, K);
+  p.alpha = alpha;
+  p.beta = beta;
+  p.gamma = gamma;
+  return run_single(c, p, cfg, grid);
+}
+
+extern "C" int skgemm_dsyrk(skgemm_ctx* c, int la, int N, int K, double alpha,
+                            const double* A, int64_t lda, double beta,
+                            double* C, int64_t ldc) {
+  SKG_TRY(check_ctx(c));
+  Problem p{};
+  SKG_TRY(prepare_syrk(la, N, K, A, lda, C, ldc, &p));
+  if (N == 0) return SKGEMM_OK;
+  SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
+  int cfg, grid;
+  plan(c, 1, N, N, K, &cfg, &grid);
+  fill_problem(&p, 1, cfg, N, N, K);
+  p.alpha = alpha;
+  p.beta = beta;
+  p.gamma = 0.0;
+  return run_single(c, p, cfg, grid);
+}
+
+// ---- host-pointer wrappers ----------------------------------------------------
+
+extern "C" int skgemm_dgemm_host(skgemm_ctx* c, int la, int lb, int ld_,
+                                 int M, int N, int K, double alpha,
+                                 const double* A, int64_t lda, const double* B,
+                                 int64_t ldb, double beta, double* D,
+                                 int64_t ldd, double gamma) {
+  SKG_TRY(check_ctx(c));
+  Problem p{};
+  SKG_TRY(prepare_gemm(la, lb, ld_, M, N, K, A, lda, B, ldb, D, ldd, &p));
+  if (M == 0 || N == 0) return SKGEMM_OK;
+  SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
+  const int64_t na = span_operand(la, true, lda, M, K);
+  const int64_t nb = span_operand(lb, false, ldb, N, K);
+  const int64_t nd = span_output(ld_, ldd, M, N);
+  double* dA = c->staging(0, std::max<int64_t>(na, 1) * 8);
+  double* dB = c->staging(1, std::max<int64_t>(nb, 1) * 8);
+  double* dD = c->staging(2, std::max<int64_t>(nd, 1) * 8);
+  if (!dA || !dB || !dD) return set_error(SKGEMM_ERR_NOMEM, "staging buffers");
+  if (na) SKG_HIP(hipMemcpyAsync(dA, A, na * 8, hipMemcpyHostToDevice, c->stream), "H2D A");
+  if (nb) SKG_HIP(hipMemcpyAsync(dB, B, nb * 8, hipMemcpyHostToDevice, c->stream), "H2D B");
+  SKG_HIP(hipMemcpyAsync(dD, D, nd * 8, hipMemcpyHostToDevice, c->stream), "H2D D");
+  SKG_TRY(skgemm_dgemm(c, la, lb, ld_, M, N, K, alpha, dA, lda, dB, ldb, beta,
+                       dD, ldd, gamma));
+  SKG_HIP(hipMemcpyAsync(D, dD, nd * 8, hipMemcpyDeviceToHost, c->stream), "D2H D");
+  SKG_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  return c->check_status();
+}
+
+extern "C" int skgemm_dsyrk_host(skgemm_ctx* c, int la, int N, int K,
+                                 double alpha, const double* A, int64_t lda,
+                                 double beta, double* C, int64_t ldc) {
+  SKG_TRY(check_ctx(c));
+  Problem p{};
+  SKG_TRY(prepare_syrk(la, N, K, A, lda, C, ldc, &p));
+  if (N == 0) return SKGEMM_OK;
+  SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
+  const int64_t na = span_operand(la, false, lda, N, K);
+  const int64_t nc = span_output(SKGEMM_ROW_MAJOR, ldc, N, N);
+  double* dA = c->staging(0, std::max<int64_t>(na, 1) * 8);
+  double* dC = c->staging(2, std::max<int64_t>(nc, 1) * 8);
+  if (!dA || !dC) return set_error(SKGEMM_ERR_NOMEM, "staging buffers");
+  if (na) SKG_HIP(hipMemcpyAsync(dA, A, na * 8, hipMemcpyHostToDevice, c->stream), "H2D A");
+  SKG_HIP(hipMemcpyAsync(dC, C, nc * 8, hipMemcpyHostToDevice, c->stream), "H2D C");
+  SKG_TRY(skgemm_dsyrk(c, la, N, K, alpha, dA, lda, beta, dC, ldc));
+  SKG_HIP(hipMemcpyAsync(C, dC, nc * 8, hipMemcpyDeviceToHost, c->stream), "D2H C");
+  SKG_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  return c->check_status();
+}
+
+// ---- layout conversion --------------------------------------------------------
+
+extern "C" int skgemm_pack(skgemm_ctx* c, int src_layout, const double* src,
+                           int64_t ld_src, int dst_layout, double* dst,
+                           int64_t ld_dst, int o_extent, int r_extent) {
+  SKG_TRY(check_ctx(c));
+  SKG_TRY(check_dims(o_extent, r_extent, 1));
+  if (o_extent == 0 || r_extent == 0) return SKGEMM_OK;
+  if (src == nullptr || dst == nullptr)
+    return set_error(SKGEMM_ERR_ARG, "src or dst is NULL");
+  // Both sides in the B convention: element (r, o), ROW_MAJOR = r*ld + o.
+  Operand s{};
+  SKG_TRY(make_operand(src_layout, false, src, ld_src, o_extent, r_extent, &s, "src"));
+  Output d{};
+  SKG_TRY(make_output(dst_layout, dst, ld_dst, r_extent, o_extent, &d, "dst"));
+  SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
+  SKG_HIP(launch_pack(s, r_extent, d, c->stream), "pack launch");
+  return SKGEMM_OK;
+}
+
+// ---- tuning -------------------------------------------------------------------
+
+extern "C" int skgemm_num_tiles(void) { return num_tile_cfgs(); }
+
+extern "C" int skgemm_tile_info(int tile, int* bm, int* bn, int* bk, int* waves) {
+  if (tile < 0 || tile >= num_tile_cfgs())
+    return set_error(SKGEMM_ERR_ARG, "tile %d out of range", tile);
+  const TileCfg& t = tile_cfg(tile);
+  if (bm) *bm = t.bm;
+  if (bn) *bn = t.bn;
+  if (bk) *bk = t.bk;
+  if (waves) *waves = t.wm * t.wn;
+  return SKGEMM_OK;
+}
+
+extern "C" int skgemm_set_tuning(skgemm_ctx* c, int tile, int grid) {
+  SKG_TRY(check_ctx(c));
+  if (tile >= num_tile_cfgs())
+    return set_error(SKGEMM_ERR_ARG, "tile %d out of range", tile);
+  c->forced_cfg = tile < 0 ? -1 : tile;
+  c->forced_grid = grid <= 0 ? 0 : grid;
+  return SKGEMM_OK;
+}
+
+extern "C" int skgemm_query_plan(skgemm_ctx* c, int kind, int M, int N, int K,
+                                 int* tile, int* grid, int64_t* tiles,
+                                 int64_t* iters) {
+  SKG_TRY(check_ctx(c));
+  SKG_TRY(check_dims(M, N, K));
+  if (kind != 0 && kind != 1) return set_error(SKGEMM_ERR_ARG, "kind must be 0 or 1");
+  int cfg, g;
+  plan(c, kind, M, N, K, &cfg, &g);
+  const TileCfg& t = tile_cfg(cfg);
+  const int64_t nt = tiles_for(kind, cfg, M, N);
+  if (tile) *tile = cfg;
+  if (grid) *grid = g;
+  if (tiles) *tiles = nt;
+  if (iters) *iters = nt * std::max<int64_t>(1, (K + t.bk - 1) / t.bk);
+  return SKGEMM_OK;
+}
+
+extern "C" int skgemm_last_kernel_ms(skgemm_ctx* c, double* ms) {
+  SKG_TRY(check_ctx(c));
+  if (!c->timed) return set_error(SKGEMM_ERR_STATE, "no timed launch yet");
+  SKG_HIP(hipEventSynchronize(c->ev1), "hipEventSynchronize");
+  float f = 0.f;
+  SKG_HIP(hipEventElapsedTime(&f, c->ev0, c->ev1), "hipEventElapsedTime");
+  *ms = f;
+  return SKGEMM_OK;
+}
+
+// ---- probes and fills -----------------------------------------------------------
+
+extern "C" int skgemm_peak_probe(skgemm_ctx* c, int which, double* tflops) {
+  SKG_TRY(check_ctx(c));
+  if (which != 0 && which != 1) return set_error(SKGEMM_ERR_ARG, "which must be 0 or 1");
+  SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
+  double* sink = c->staging(3, 65536 * sizeof(double));
+  if (!sink) return set_error(SKGEMM_ERR_NOMEM, "probe sink");
+  const int blocks = c->num_cus * 8;
+  const int iters = which == 0 ? 2000 : 20000;
+  SKG_HIP(launch_peak_probe(which, blocks, iters / 10, sink, c->stream), "probe warmup");
+  SKG_HIP(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
+  SKG_HIP(launch_peak_probe(which, blocks, iters, sink, c->stream), "probe");
+  SKG_HIP(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
+  SKG_HIP(hipEventSynchronize(c->ev1), "hipEventSynchronize");
+  float ms = 0.f;
+  SKG_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1), "hipEventElapsedTime");
+  const double per_mfma = which == 0 ? 2.0 * 16 * 16 * 4 : 2.0 * 32 * 32 * 32;
+  const double n_mfma = (double)blocks * 4 /*waves*/ * iters * (which == 0 ? 8 : 4);
+  *tflops = n_mfma * per_mfma / (ms * 1e-3) / 1e12;
+  return SKGEMM_OK;
+}
+
+extern "C" int skgemm_copy_probe(skgemm_ctx* c, size_t bytes, double* gbps) {
+  SKG_TRY(check_ctx(c));
+  SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
+  bytes &= ~(size_t)15;
+  void *a = nullptr, *b = nullptr;
+  SKG_HIP(hipMalloc(&a, bytes), "hipMalloc probe");
+  hipError_t e = hipMalloc(&b, bytes);
+  if (e != hipSuccess) {
+    hipFree(a);
+    return hip_error(e, "hipMalloc probe");
+  }
+  hipMemsetAsync(a, 0, bytes, c->stream);
+  const int64_t n = bytes / 8;
+  launch_copy_probe((double*)a, (double*)b, n, c->stream);
+  hipEventRecord(c->ev0, c->stream);
+  const int reps = 10;
+  for (int i = 0; i < reps; ++i) launch_copy_probe((double*)a, (double*)b, n, c->stream);
+  hipEventRecord(c->ev1, c->stream);
+  e = hipEventSynchronize(c->ev1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  hipFree(a);
+  hipFree(b);
+  if (e != hipSuccess) return hip_error(e, "copy probe");
+  *gbps = 2.0 * bytes * reps / (ms * 1e-3) / 1e9;
+  return SKGEMM_OK;
+}
+
+extern "C" int skgemm_fill_uniform(skgemm_ctx* c, double* x, int64_t n,
+                                   uint64_t seed, double lo, double hi) {
+  SKG_TRY(check_ctx(c));
+  if (n < 0 || (n > 0 && x == nullptr)) return set_error(SKGEMM_ERR_ARG, "bad buffer");
+  if (n == 0) return SKGEMM_OK;
+  SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
+  SKG_HIP(launch_fill_uniform(x, n, seed, lo, hi, c->stream), "fill_uniform");
+  return SKGEMM_OK;
+}
+
+extern "C" int skgemm_fill_normal(skgemm_ctx* c, double* x, int64_t n,
+                                  uint64_t seed, double sigma) {
+  SKG_TRY(check_ctx(c));
+  if (n < 0 || (n > 0 && x == nullptr)) return set_error(SKGEMM_ERR_ARG, "bad buffer");
+  if (n == 0) return SKGEMM_OK;
+  SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
+  SKG_HIP(launch_fill_normal(x, n, seed, sigma, c->stream), "fill_normal");
+  return SKGEMM_OK;
+}
+
+extern "C" int skgemm_scale_columns_pow10(skgemm_ctx* c, double* x, int rows,
+                                          int cols, int64_t ld, uint64_t seed,
+                                          double umax) {
+  SKG_TRY(check_ctx(c));
+  if (rows < 0 || cols < 0 || ld < rows || (rows > 0 && cols > 0 && !x))
+    return set_error(SKGEMM_ERR_ARG, "bad matrix");
+  if (rows == 0 || cols == 0) return SKGEMM_OK;
+  SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
+  SKG_HIP(launch_scale_columns_pow10(x, rows, cols, ld, seed, umax, c->stream),
+          "scale_columns");
+  return SKGEMM_OK;
+}

@@ -1,0 +1,77 @@
+// The context behind the opaque skgemm_ctx handle, and helpers shared by
+// the host translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <tuple>
+#include <utility>
+
+#include "skgemm_internal.h"
+
+struct skgemm_ctx {
+  static constexpr int kStages = 4;
+  int magic = 0;
+  int device = 0;
+  int num_cus = 0, num_xcds = 8, lds_per_cu = 0, clock_mhz = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+
+  // Stream-K fix-up workspace: one partial tile per persistent workgroup.
+  double* ws = nullptr;
+  size_t ws_n = 0;  // doubles
+  int* flags = nullptr;
+  int flags_n = 0;
+  int* status = nullptr;
+  int64_t spin_limit = 1ll << 26;
+
+  // Device staging for host-pointer calls and probes.
+  void* stage[kStages] = {nullptr, nullptr, nullptr, nullptr};
+  size_t stage_n[kStages] = {0, 0, 0, 0};
+
+  // Tuning: forced configuration, autotuned plans per exact shape
+  // (kind, M, N, K) -> (cfg, grid), occupancy per cfg.
+  int forced_cfg = -1;
+  int forced_grid = 0;
+  std::map<std::tuple<int, int, int, int>, std::pair<int, int>> tuned;
+  std::map<int, int> occ_cache;
+
+  // Split-form and batched state (skgemm_ozaki.hip / skgemm_batch.cpp).
+  struct Extra;
+  Extra* extra = nullptr;
+
+  int check_status();
+  int ensure_ws(size_t ws_doubles, int nflags);
+  double* staging(int i, size_t bytes);
+  int capacity(int cfg);
+  void free_all();
+  void free_extra();
+};
+
+namespace skg {
+
+int set_error(int code, const char* fmt, ...);
+int hip_error(hipError_t e, const char* what);
+
+int make_operand(int layout, bool a_convention, const double* p, int64_t ld,
+                 int ext, int K, Operand* op, const char* name);
+int make_output(int layout, double* p, int64_t ld, int M, int N, Output* out,
+                const char* name);
+int64_t span_operand(int layout, bool a_convention, int64_t ld, int ext, int K);
+int64_t span_output(int layout, int64_t ld, int M, int N);
+
+int64_t tiles_for(int kind, int cfg, int M, int N);
+void plan(skgemm_ctx* c, int kind, int M, int N, int K, int* cfg, int* grid);
+void fill_problem(Problem* p, int kind, int cfg, int M, int N, int K);
+int run_single(skgemm_ctx* c, Problem& p, int cfg, int grid);
+int prepare_gemm(int la, int lb, int ld_, int M, int N, int K, const double* A,
+                 int64_t lda, const double* B, int64_t ldb, double* D,
+                 int64_t ldd, Problem* p);
+int prepare_syrk(int la, int N, int K, const double* A, int64_t lda, double* C,
+                 int64_t ldc, Problem* p);
+
+}  // namespace skg

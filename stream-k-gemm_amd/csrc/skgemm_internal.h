@@ -1,0 +1,94 @@
+// Internal declarations shared by the kernel and host translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace skg {
+
+// Free-index panel shift used for unpacked layouts: o >> 30 == 0 for every
+// extent the C ABI accepts (int, < 2^30 checked on entry).
+constexpr int kNoPanelShift = 30;
+
+// One input operand, addressed by its free index o (m of A, n of B) and the
+// contraction index k:
+//   addr(o, k) = (o >> pshift) * pstride + (o & pmask) * os + k * ks
+struct Operand {
+  const double* ptr;
+  int64_t pstride;
+  int64_t os;
+  int64_t ks;
+  int pshift;
+  int extent;   // M or N
+  int kcontig;  // ks == 1: pairs are loaded along k, else along o
+  int vec2;     // 16-byte loads of a pair are aligned
+};
+
+// Output D (m, n): addr = (n >> pshift) * pstride + (n & pmask) * ns + m * ms
+struct Output {
+  double* ptr;
+  int64_t pstride;
+  int64_t ns;
+  int64_t ms;
+  int pshift;
+};
+
+// One problem as the Stream-K kernel sees it.
+struct Problem {
+  Operand a, b;
+  Output d;
+  int M, N, K;
+  int syrk;            // 1: tiles of the lower triangle only, diagonal masked
+  double alpha, beta, gamma;
+  int tiles_m, tiles_n;
+  int iters_per_tile;  // ceil(K / BK), at least 1
+  int64_t tile_begin;  // first flattened tile index of this problem
+  int64_t iter_begin;  // first flattened MAC-iteration index
+  int64_t num_tiles;
+};
+
+// Launch-wide state of one Stream-K launch.
+struct Launch {
+  const Problem* problems;  // device table (batched) or nullptr
+  Problem single;           // used when problems == nullptr
+  int num_problems;
+  int64_t total_iters;
+  int grid;             // persistent workgroups
+  double* ws;           // grid partial tiles of BM*BN doubles
+  int* flags;           // grid flags: 0 empty, 1 partial ready
+  int* status;          // bit 0: a fix-up wait hit its spin limit
+  int64_t spin_limit;   // s_sleep rounds before a wait gives up
+};
+
+// Tile configurations (index = kernel table entry).
+struct TileCfg {
+  int bm, bn, bk, wm, wn;
+};
+int num_tile_cfgs();
+const TileCfg& tile_cfg(int i);
+
+// Launch the FP64 Stream-K kernel with tile configuration `cfg`.
+hipError_t launch_dgemm_sk(int cfg, const Launch& L, hipStream_t s);
+// Workgroups of configuration `cfg` that fit on one CU at once.
+int occupancy_dgemm_sk(int cfg);
+// LDS bytes one workgroup of `cfg` uses.
+size_t lds_bytes_dgemm_sk(int cfg);
+
+// Utility kernels.
+hipError_t launch_fill_uniform(double* x, int64_t n, uint64_t seed, double lo,
+                               double hi, hipStream_t s);
+hipError_t launch_fill_normal(double* x, int64_t n, uint64_t seed,
+                              double sigma, hipStream_t s);
+hipError_t launch_scale_columns_pow10(double* x, int rows, int cols,
+                                      int64_t ld, uint64_t seed, double umax,
+                                      hipStream_t s);
+hipError_t launch_pack(const Operand& src, int r_extent, const Output& dst,
+                       hipStream_t s);
+hipError_t launch_zero_ints(int* p, int n, hipStream_t s);
+hipError_t launch_peak_probe(int which, int blocks, int iters, double* sink,
+                             hipStream_t s);
+hipError_t launch_copy_probe(const double* src, double* dst, int64_t n,
+                             hipStream_t s);
+
+}  // namespace skg

@@ -1,0 +1,553 @@
+// FP64 Stream-K GEMM / SYRK kernels for CDNA4 (gfx950), plus the utility
+// kernels of the library (seeded fills, layout conversion, peak probes).
+//
+// Design (no counterpart in the reference's decomposition, which splits K
+// evenly over OpenMP threads with one full M x N partial per thread,
+// gemm_crsp.cpp:261-311):
+//   * One persistent grid of G workgroups, G <= the number that fit on the
+//     chip at once, so every fix-up wait has a resident producer.
+//   * The MAC-iteration space (tiles x ceil(K/BK)) of one or many problems
+//     is split into G contiguous, equal ranges.  A range may start and end
+//     inside a tile.  The workgroup that computes a tile's first iteration
+//     owns the tile: the others spill their partial accumulators to their
+//     own workspace slot and raise their flag; the owner adds the peers'
+//     partials in peer order (deterministic for a fixed G) and runs the
+//     epilogue D = alpha*acc + beta*D + gamma.
+//   * Workgroup ids are remapped so that consecutive ranges (neighbouring
+//     tiles, shared A/B panels) run on the same XCD and share its L2.
+//   * Inner loop: v_mfma_f64_16x16x4_f64 (lane l supplies A[l%16][l/16] and
+//     B[l/16][l%16]; accumulator element v of lane l is D[4v + l/16][l%16],
+//     measured on gfx950 hardware); operands staged through LDS,
+//     double-buffered, one barrier per BK step; global loads of step i+1
+//     are issued before the MFMAs of step i.
+//   * SYRK enumerates lower-triangle tiles only (row-major lower ==
+//     the reference's column-major upper) and masks the diagonal tile.
+
+#include <hip/hip_runtime.h>
+
+#include "skgemm_internal.h"
+
+namespace skg {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+static const TileCfg kCfgs[] = {
+    {128, 128, 16, 2, 2},  // 0: 4 waves, 64x64 per wave
+    {64, 64, 16, 2, 2},    // 1: 4 waves, 32x32 per wave
+    {128, 64, 16, 2, 2},   // 2: 4 waves, 64x32 per wave
+    {64, 128, 16, 2, 2},   // 3: 4 waves, 32x64 per wave
+    {256, 128, 16, 4, 2},  // 4: 8 waves, 64x64 per wave
+    {128, 128, 32, 2, 2},  // 5: as 0 with BK = 32
+};
+constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
+
+int num_tile_cfgs() { return kNumCfgs; }
+const TileCfg& tile_cfg(int i) { return kCfgs[i]; }
+
+constexpr int kPad = 2;  // LDS row padding in doubles (keeps 16 B alignment)
+
+__device__ __forceinline__ int64_t wg_start(int w, int64_t total, int G) {
+  const int64_t base = total / G, extra = total % G;
+  return (int64_t)w * base + ((int64_t)w < extra ? (int64_t)w : extra);
+}
+
+__device__ __forceinline__ int64_t op_addr(const Operand& op, int o, int k) {
+  const int pm = (1 << op.pshift) - 1;
+  return (int64_t)(o >> op.pshift) * op.pstride + (int64_t)(o & pm) * op.os +
+         (int64_t)k * op.ks;
+}
+
+__device__ __forceinline__ int64_t out_addr(const Output& d, int m, int n) {
+  const int pm = (1 << d.pshift) - 1;
+  return (int64_t)(n >> d.pshift) * d.pstride + (int64_t)(n & pm) * d.ns +
+         (int64_t)m * d.ms;
+}
+
+// Global -> register -> LDS staging of one BO x BK operand tile.  Elements
+// move in pairs along the operand's contiguous index (k or o) so that a pair
+// is one 16-byte load when aligned.  LDS holds the tile as [k][BO + kPad].
+template <int BO, int BK, int NT>
+struct Stage {
+  static constexpr int kPairs = BO * BK / 2 / NT;
+  static_assert(BO * BK / 2 % NT == 0, "tile not divisible over threads");
+  static constexpr int kPitch = BO + kPad;
+  double x[2 * kPairs];
+
+  __device__ __forceinline__ void load(const Operand& op, int o0, int k0,
+                                       int K, int tid) {
+#pragma unroll
+    for (int p = 0; p < kPairs; ++p) {
+      const int q = p * NT + tid;
+      double x0 = 0.0, x1 = 0.0;
+      if (op.kcontig) {
+        const int o = q / (BK / 2), k = (q % (BK / 2)) * 2;
+        const int og = o0 + o, kg = k0 + k;
+        const bool in0 = og < op.extent && kg < K;
+        const bool in1 = og < op.extent && kg + 1 < K;
+        const double* src = op.ptr + (in0 ? op_addr(op, og, kg) : 0);
+        if (in1 && op.vec2) {
+          const double2 t = *reinterpret_cast<const double2*>(src);
+          x0 = t.x;
+          x1 = t.y;
+        } else {
+          if (in0) x0 = src[0];
+          if (in1) x1 = src[1];
+        }
+      } else {
+        const int k = q / (BO / 2), o = (q % (BO / 2)) * 2;
+        const int og = o0 + o, kg = k0 + k;
+        const bool in0 = og < op.extent && kg < K;
+        const bool in1 = og + 1 < op.extent && kg < K;
+        const double* src = op.ptr + (in0 ? op_addr(op, og, kg) : 0);
+        if (in1 && op.vec2) {
+          const double2 t = *reinterpret_cast<const double2*>(src);
+          x0 = t.x;
+          x1 = t.y;
+        } else {
+          if (in0) x0 = src[0];
+          if (in1) x1 = src[op.os];
+        }
+      }
+      x[2 * p] = x0;
+      x[2 * p + 1] = x1;
+    }
+  }
+
+  __device__ __forceinline__ void store(double* lds, int kcontig, int tid) {
+#pragma unroll
+    for (int p = 0; p < kPairs; ++p) {
+      const int q = p * NT + tid;
+      if (kcontig) {
+        const int o = q / (BK / 2), k = (q % (BK / 2)) * 2;
+        lds[k * kPitch + o] = x[2 * p];
+        lds[(k + 1) * kPitch + o] = x[2 * p + 1];
+      } else {
+        const int k = q / (BO / 2), o = (q % (BO / 2)) * 2;
+        *reinterpret_cast<double2*>(lds + k * kPitch + o) =
+            make_double2(x[2 * p], x[2 * p + 1]);
+      }
+    }
+  }
+};
+
+// Row-major lower-triangle tile index t -> (ti, tj), ti >= tj.
+__device__ __forceinline__ void tri_tile(int64_t t, int& ti, int& tj) {
+  int64_t i = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while (i * (i + 1) / 2 > t) --i;
+  while ((i + 1) * (i + 2) / 2 <= t) ++i;
+  ti = (int)i;
+  tj = (int)(t - i * (i + 1) / 2);
+}
+
+template <int BM, int BN, int BK, int WM, int WN>
+__global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
+  constexpr int NT = WM * WN * 64;
+  constexpr int TM = BM / WM, TN = BN / WN;
+  constexpr int MB = TM / 16, NB = TN / 16;
+  static_assert(TM % 16 == 0 && TN % 16 == 0 && BK % 4 == 0, "bad tile");
+  using StA = Stage<BM, BK, NT>;
+  using StB = Stage<BN, BK, NT>;
+  constexpr int kBufDoubles = BK * (StA::kPitch + StB::kPitch);
+
+  extern __shared__ __align__(16) double lds[];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int lq = lane >> 4, lr = lane & 15;
+
+  // XCD-aware logical workgroup id: hardware places workgroup g on XCD g % 8;
+  // logical ids of one XCD are made contiguous.
+  const int G = L.grid;
+  const int g = blockIdx.x;
+  const int per = G >> 3, rem = G & 7, xcd = g & 7;
+  const int w = xcd * per + (xcd < rem ? xcd : rem) + (g >> 3);
+
+  int64_t it = wg_start(w, L.total_iters, G);
+  const int64_t end = wg_start(w + 1, L.total_iters, G);
+  int pi = 0;
+
+  StA sa;
+  StB sb;
+
+  while (it < end) {
+    // Locate the problem holding iteration `it`.
+    if (L.problems != nullptr) {
+      while (pi + 1 < L.num_problems && L.problems[pi + 1].iter_begin <= it)
+        ++pi;
+    }
+    const Problem& P = (L.problems != nullptr) ? L.problems[pi] : L.single;
+    const int ipt = P.iters_per_tile;
+    const int64_t local = it - P.iter_begin;
+    const int64_t tile = local / ipt;
+    const int64_t t0 = P.iter_begin + tile * ipt;
+    const int kb = (int)(it - t0);
+    const int ke = (int)((end - t0) < ipt ? (end - t0) : ipt);
+
+    int ti, tj;
+    if (P.syrk) {
+      tri_tile(tile, ti, tj);
+    } else {
+      ti = (int)(tile % P.tiles_m);
+      tj = (int)(tile / P.tiles_m);
+    }
+    const int m0 = ti * BM, n0 = tj * BN;
+
+    v4d acc[MB][NB];
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+    // ---- MAC loop over BK steps [kb, ke) --------------------------------
+    sa.load(P.a, m0, kb * BK, P.K, tid);
+    sb.load(P.b, n0, kb * BK, P.K, tid);
+    sa.store(lds, P.a.kcontig, tid);
+    sb.store(lds + BK * StA::kPitch, P.b.kcontig, tid);
+    __syncthreads();
+    for (int kk = kb; kk < ke; ++kk) {
+      const int cur = (kk - kb) & 1;
+      const bool more = kk + 1 < ke;
+      if (more) {
+        sa.load(P.a, m0, (kk + 1) * BK, P.K, tid);
+        sb.load(P.b, n0, (kk + 1) * BK, P.K, tid);
+      }
+      const double* la = lds + cur * kBufDoubles;
+      const double* lb = la + BK * StA::kPitch;
+#pragma unroll
+      for (int s = 0; s < BK / 4; ++s) {
+        double av[MB], bv[NB];
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+          av[i] = la[(s * 4 + lq) * StA::kPitch + wm * TM + i * 16 + lr];
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+          bv[j] = lb[(s * 4 + lq) * StB::kPitch + wn * TN + j * 16 + lr];
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j],
+                                                             acc[i][j], 0, 0, 0);
+      }
+      if (more) {
+        double* na = lds + (cur ^ 1) * kBufDoubles;
+        sa.store(na, P.a.kcontig, tid);
+        sb.store(na + BK * StA::kPitch, P.b.kcontig, tid);
+      }
+      __syncthreads();
+    }
+
+    // ---- Stream-K fix-up -------------------------------------------------
+    bool write_out = true;
+    if (kb != 0) {
+      // Not the owner: spill the partial to this workgroup's slot.
+      double* slot = L.ws + (int64_t)w * (BM * BN);
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            slot[((i * NB + j) * 4 + v) * NT + tid] = acc[i][j][v];
+      __threadfence();
+      __syncthreads();
+      if (tid == 0)
+        __hip_atomic_store(&L.flags[w], 1, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      write_out = false;
+    } else if (ke < ipt) {
+      // Owner of a split tile: add the peers' partials in peer order.
+      const int64_t tile_end = t0 + ipt;
+      for (int w2 = w + 1; w2 < G && wg_start(w2, L.total_iters, G) < tile_end;
+           ++w2) {
+        if (tid == 0) {
+          int64_t n = 0;
+          while (__hip_atomic_load(&L.flags[w2], __ATOMIC_ACQUIRE,
+                                   __HIP_MEMORY_SCOPE_AGENT) == 0) {
+            __builtin_amdgcn_s_sleep(8);
+            if (++n > L.spin_limit) {
+              atomicOr(L.status, 1);
+              break;
+            }
+          }
+          __hip_atomic_store(&L.flags[w2], 0, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        __threadfence();
+        const double* slot = L.ws + (int64_t)w2 * (BM * BN);
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+              acc[i][j][v] += __builtin_nontemporal_load(
+                  slot + ((i * NB + j) * 4 + v) * NT + tid);
+      }
+    }
+
+    // ---- Epilogue: D = alpha*acc + beta*D + gamma --------------------------
+    if (write_out) {
+      const double alpha = P.alpha, beta = P.beta, gamma = P.gamma;
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = m0 + wm * TM + i * 16 + v * 4 + lq;
+            const int col = n0 + wn * TN + j * 16 + lr;
+            if (row < P.M && col < P.N && (!P.syrk || row >= col)) {
+              double* dst = P.d.ptr + out_addr(P.d, row, col);
+              double val = alpha * acc[i][j][v];
+              if (beta != 0.0) val = fma(beta, *dst, val);
+              *dst = val + gamma;
+            }
+          }
+    }
+    it = t0 + ke;
+  }
+}
+
+template <int BM, int BN, int BK, int WM, int WN>
+static size_t lds_bytes_t() {
+  return 2ull * BK * ((BM + kPad) + (BN + kPad)) * sizeof(double);
+}
+
+template <int BM, int BN, int BK, int WM, int WN>
+static hipError_t launch_t(const Launch& L, hipStream_t s) {
+  auto kfn = sk_dgemm_kernel<BM, BN, BK, WM, WN>;
+  const size_t lds = lds_bytes_t<BM, BN, BK, WM, WN>();
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(kfn),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(kfn, dim3(L.grid), dim3(WM * WN * 64), lds, s, L);
+  return hipGetLastError();
+}
+
+template <int BM, int BN, int BK, int WM, int WN>
+static int occupancy_t() {
+  auto kfn = sk_dgemm_kernel<BM, BN, BK, WM, WN>;
+  const size_t lds = lds_bytes_t<BM, BN, BK, WM, WN>();
+  hipFuncSetAttribute(reinterpret_cast<const void*>(kfn),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &nb, reinterpret_cast<const void*>(kfn), WM * WN * 64, lds) !=
+      hipSuccess)
+    return 0;
+  return nb;
+}
+
+#define SKG_CFG_SWITCH(cfg, FN, ...)                  \
+  switch (cfg) {                                      \
+    case 0: return FN<128, 128, 16, 2, 2>(__VA_ARGS__); \
+    case 1: return FN<64, 64, 16, 2, 2>(__VA_ARGS__);   \
+    case 2: return FN<128, 64, 16, 2, 2>(__VA_ARGS__);  \
+    case 3: return FN<64, 128, 16, 2, 2>(__VA_ARGS__);  \
+    case 4: return FN<256, 128, 16, 4, 2>(__VA_ARGS__); \
+    case 5: return FN<128, 128, 32, 2, 2>(__VA_ARGS__); \
+    default: break;                                   \
+  }
+
+hipError_t launch_dgemm_sk(int cfg, const Launch& L, hipStream_t s) {
+  SKG_CFG_SWITCH(cfg, launch_t, L, s);
+  return hipErrorInvalidValue;
+}
+int occupancy_dgemm_sk(int cfg) {
+  SKG_CFG_SWITCH(cfg, occupancy_t);
+  return 0;
+}
+size_t lds_bytes_dgemm_sk(int cfg) {
+  SKG_CFG_SWITCH(cfg, lds_bytes_t);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Seeded fills: counter-based SplitMix64, element i from (seed, i) only.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t sm64_at(uint64_t seed, uint64_t i) {
+  uint64_t z = seed * 0x9E3779B97F4A7C15ULL + (i + 1) * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double u01(uint64_t z) {
+  return (double)(z >> 11) * 0x1.0p-53;
+}
+
+__global__ void fill_uniform_kernel(double* x, int64_t n, uint64_t seed,
+                                    double lo, double hi) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = lo + (hi - lo) * u01(sm64_at(seed, i));
+}
+
+// Box-Muller on consecutive pairs (2j, 2j+1).
+__global__ void fill_normal_kernel(double* x, int64_t n, uint64_t seed,
+                                   double sigma) {
+  const int64_t pairs = (n + 1) / 2;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < pairs;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const double u1 = u01(sm64_at(seed, 2 * j));
+    const double u2 = u01(sm64_at(seed, 2 * j + 1));
+    const double r = sqrt(-2.0 * log(1.0 - u1));
+    double sn, cs;
+    sincospi(2.0 * u2, &sn, &cs);
+    x[2 * j] = sigma * r * cs;
+    if (2 * j + 1 < n) x[2 * j + 1] = sigma * r * sn;
+  }
+}
+
+__global__ void scale_columns_pow10_kernel(double* x, int rows, int cols,
+                                           int64_t ld, uint64_t seed,
+                                           double umax) {
+  const int j = blockIdx.y;
+  if (j >= cols) return;
+  const double f = exp10(umax * u01(sm64_at(seed, (uint64_t)j)));
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < rows;
+       i += gridDim.x * blockDim.x)
+    x[(int64_t)j * ld + i] *= f;
+}
+
+static int fill_blocks(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  return (int)(b < 65536 ? (b < 1 ? 1 : b) : 65536);
+}
+
+hipError_t launch_fill_uniform(double* x, int64_t n, uint64_t seed, double lo,
+                               double hi, hipStream_t s) {
+  hipLaunchKernelGGL(fill_uniform_kernel, dim3(fill_blocks(n)), dim3(256), 0,
+                     s, x, n, seed, lo, hi);
+  return hipGetLastError();
+}
+hipError_t launch_fill_normal(double* x, int64_t n, uint64_t seed,
+                              double sigma, hipStream_t s) {
+  hipLaunchKernelGGL(fill_normal_kernel, dim3(fill_blocks((n + 1) / 2)),
+                     dim3(256), 0, s, x, n, seed, sigma);
+  return hipGetLastError();
+}
+hipError_t launch_scale_columns_pow10(double* x, int rows, int cols,
+                                      int64_t ld, uint64_t seed, double umax,
+                                      hipStream_t s) {
+  int bx = (rows + 255) / 256;
+  if (bx > 64) bx = 64;
+  if (bx < 1) bx = 1;
+  hipLaunchKernelGGL(scale_columns_pow10_kernel, dim3(bx, cols), dim3(256), 0,
+                     s, x, rows, cols, ld, seed, umax);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Layout conversion: dst(o, r) = src(o, r) for o < o_extent (src.extent),
+// r < r_extent.  One thread per element, r fastest across a block row so
+// that the side contiguous in r is coalesced.
+// ---------------------------------------------------------------------------
+__global__ void pack_kernel(Operand src, int r_extent, Output dst) {
+  const int o = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (o >= src.extent) return;
+  for (int r = blockIdx.x * 64 + (threadIdx.x & 63); r < r_extent;
+       r += gridDim.x * 64)
+    dst.ptr[out_addr(dst, r, o)] = src.ptr[op_addr(src, o, r)];
+}
+
+hipError_t launch_pack(const Operand& src, int r_extent, const Output& dst,
+                       hipStream_t s) {
+  if (src.extent <= 0 || r_extent <= 0) return hipSuccess;
+  int bx = (r_extent + 63) / 64;
+  if (bx > 1024) bx = 1024;
+  const int by = (src.extent + 3) / 4;
+  hipLaunchKernelGGL(pack_kernel, dim3(bx, by), dim3(256), 0, s, src,
+                     r_extent, dst);
+  return hipGetLastError();
+}
+
+__global__ void zero_ints_kernel(int* p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+hipError_t launch_zero_ints(int* p, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  int b = (n + 255) / 256;
+  if (b > 1024) b = 1024;
+  hipLaunchKernelGGL(zero_ints_kernel, dim3(b), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Peak probes (analogue of sme_peak_microbench.cpp:16-60): 8 independent
+// accumulators per wave, register operands only.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void peak_f64_kernel(int iters, double* sink) {
+  v4d acc[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) acc[j] = v4d{0.0, 0.0, 0.0, 0.0};
+  const double a = 1.0 + threadIdx.x * 1e-6, b = 1e-9;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
+  if (s == 12345.0) sink[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void peak_i8_kernel(int iters, double* sink) {
+  v16i acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[j][v] = 0;
+  const v4i a = {(int)threadIdx.x, 1, 2, 3}, b = {1, 1, 1, (int)blockIdx.x};
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      acc[j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, acc[j], 0, 0, 0);
+  }
+  int s = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) s += acc[j][v];
+  if (s == 12345) sink[blockIdx.x] = (double)s;
+}
+
+hipError_t launch_peak_probe(int which, int blocks, int iters, double* sink,
+                             hipStream_t s) {
+  if (which == 0)
+    hipLaunchKernelGGL(peak_f64_kernel, dim3(blocks), dim3(256), 0, s, iters,
+                       sink);
+  else
+    hipLaunchKernelGGL(peak_i8_kernel, dim3(blocks), dim3(256), 0, s, iters,
+                       sink);
+  return hipGetLastError();
+}
+
+__global__ void copy_kernel(const double2* __restrict__ src,
+                            double2* __restrict__ dst, int64_t n2) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n2;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+hipError_t launch_copy_probe(const double* src, double* dst, int64_t n,
+                             hipStream_t s) {
+  const int64_t n2 = n / 2;
+  hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(256), 0, s,
+                     reinterpret_cast<const double2*>(src),
+                     reinterpret_cast<double2*>(dst), n2);
+  return hipGetLastError();
+}
+
+}  // namespace skg
