@@ -234,6 +234,15 @@ int skgemm_query_plan(skgemm_ctx* ctx, int kind, int M, int N, int K,
  * with HIP events recorded on the stream the kernel ran on. */
 int skgemm_last_kernel_ms(skgemm_ctx* ctx, double* ms);
 
+/* Per-launch kernel timing without host synchronisation: after
+ * skgemm_timing_enable(ctx, capacity) every compute launch on ctx is
+ * bracketed by a pair of HIP events recorded on the launch stream (up to
+ * `capacity` launches; 0 disables).  skgemm_timing_read waits for the last
+ * recorded launch, writes up to `max` durations in launch order (ms) and
+ * starts a new recording. */
+int skgemm_timing_enable(skgemm_ctx* ctx, int capacity);
+int skgemm_timing_read(skgemm_ctx* ctx, double* ms, int max, int* count);
+
 /* MFMA peak probe: back-to-back independent MFMAs on every SIMD, no memory
  * traffic.  which: 0 = v_mfma_f64_16x16x4_f64, 1 = v_mfma_i32_32x32x32_i8.
  * Returns achieved TFLOP/s (TOP/s for INT8).  sme_peak_microbench.cpp:16-60. */

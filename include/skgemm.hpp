@@ -1,0 +1,276 @@
+// skgemm.hpp - header-only C++ facade with the reference's class names and
+// call signatures, implemented on the C ABI in skgemm.h (no HIP types, no
+// HIP headers: a host program links libskgemm.so and includes this file).
+//
+// Reference surface kept (shriramch/stream-k-gemm):
+//   gemm<T>                 gemm_crsp.cpp:57-431, gemm_opt.cpp:55-426
+//   sme_syrk<T>             include/sme_syrk.hpp:22-150, syrk_crsp.cpp:51-349
+//   sme_gemm_trp_3d<T>      include/sme_trp_3d.hpp:48-220
+//   sme_pack<T> C wrappers  include/sme_pack.hpp:316-352
+//   hbm_init/hbm_alloc/...  include/hbm_alloc.hpp:14-50
+// Only T = double is provided (the reference instantiates double only).
+//
+// Conventions, as in the reference: host pointers, void returns, same
+// layouts (A column-major / B row-major / D row-major for gemm::compute_test;
+// PA [M/16][K][16] and PB [N/32][K][32] for the packed entry points; P16 for
+// sme_syrk; grid-packed in / band-packed out for the rotation).  Each call
+// copies its operands to the GPU, runs the Stream-K kernel and copies the
+// result back.  A failing call cannot report through a void signature, so
+// it prints the library's error text and aborts.
+//
+// Differences (supersets): any M, N, K (the reference truncates to
+// 16*floor(M/16) rows and 32*floor(N/32) columns, gemm_crsp.cpp:197-198);
+// sme_syrk applies alpha to the whole product (the reference applies it to
+// the last KC block only, include/sme_syrk.hpp:123-133) and never writes
+// outside the M x M matrix (include/sme_syrk.hpp:66-108 does when M % 16).
+#pragma once
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+#include "skgemm.h"
+
+namespace skgemm_facade {
+
+// One context per host thread (contexts are not shared across threads).
+inline skgemm_ctx* ctx() {
+  static thread_local skgemm_ctx* c = nullptr;
+  if (c == nullptr) {
+    const char* dev = std::getenv("SKGEMM_DEVICE");
+    if (skgemm_create(dev ? std::atoi(dev) : 0, &c) != SKGEMM_OK) {
+      std::fprintf(stderr, "skgemm: %s\n", skgemm_last_error());
+      std::abort();
+    }
+  }
+  return c;
+}
+
+inline void check(int rc, const char* what) {
+  if (rc != SKGEMM_OK) {
+    std::fprintf(stderr, "skgemm: %s failed: %s\n", what, skgemm_last_error());
+    std::abort();
+  }
+}
+
+}  // namespace skgemm_facade
+
+// ---- include/hbm_alloc.hpp:14-50 ------------------------------------------
+// Host allocations, as in the reference without USE_HBM; operands are moved
+// to HBM by each call.
+inline void hbm_init() {}
+template <typename T>
+inline T* hbm_alloc(size_t count) {
+  return new T[count];
+}
+template <typename T>
+inline T* hbm_calloc(size_t count) {
+  return new T[count]();
+}
+template <typename T>
+inline void hbm_free(T* ptr) {
+  delete[] ptr;
+}
+
+// ---- gemm_crsp.cpp / gemm_opt.cpp -------------------------------------------
+template <typename T>
+class gemm;
+
+template <>
+class gemm<double> {
+ public:
+  using T = double;
+  static constexpr int ZA_TILE_M = 16;  // PA panel width (gemm_crsp.cpp:62)
+  static constexpr int ZA_TILE_N = 32;  // PB panel width (gemm_crsp.cpp:63)
+
+  // gemm_crsp.cpp:261-311.  C_local is the reference's per-thread workspace;
+  // the GPU path keeps its workspace inside the library, so it is unused.
+  static void compute_streamk_packed_core(const int M, const int N,
+                                          const int K, const T* A_packed,
+                                          const T* B_packed, T* D,
+                                          const T alpha, const T gamma,
+                                          T** /*C_local*/) {
+    compute_streamk_packed(M, N, K, A_packed, B_packed, D, alpha, gamma);
+  }
+
+  // gemm_crsp.cpp:316-338: D = alpha * A * B + gamma on PA / PB.
+  static void compute_streamk_packed(const int M, const int N, const int K,
+                                     const T* A_packed, const T* B_packed,
+                                     T* D, const T alpha, const T gamma) {
+    skgemm_facade::check(
+        skgemm_dgemm_host(skgemm_facade::ctx(), SKGEMM_PACK16, SKGEMM_PACK32,
+                          SKGEMM_ROW_MAJOR, M, N, K, alpha, A_packed, K,
+                          B_packed, K, 0.0, D, N, gamma),
+        "gemm::compute_streamk_packed");
+  }
+
+  // gemm_opt.cpp:314-337 (same contract, M-tile decomposition there).
+  static void compute_packed(const int M, const int N, const int K,
+                             const T* A_packed, const T* B_packed, T* D,
+                             const T alpha, const T gamma) {
+    compute_streamk_packed(M, N, K, A_packed, B_packed, D, alpha, gamma);
+  }
+
+  // gemm_crsp.cpp:342-353: column-major A -> [M/16][K][16] (truncating).
+  static void pack_A(const int M, const int K, const T* A_colmajor,
+                     T* A_packed) {
+    const int tiles_m = M / ZA_TILE_M;
+    for (int tm = 0; tm < tiles_m; ++tm)
+      for (int k = 0; k < K; ++k)
+        std::memcpy(A_packed + ((size_t)tm * K + k) * ZA_TILE_M,
+                    A_colmajor + (size_t)tm * ZA_TILE_M + (size_t)k * M,
+                    ZA_TILE_M * sizeof(T));
+  }
+
+  // gemm_crsp.cpp:355-366: row-major B -> [N/32][K][32] (truncating).
+  static void pack_B(const int N, const int K, const T* B_rowmajor,
+                     T* B_packed) {
+    const int tiles_n = N / ZA_TILE_N;
+    for (int tn = 0; tn < tiles_n; ++tn)
+      for (int k = 0; k < K; ++k)
+        std::memcpy(B_packed + ((size_t)tn * K + k) * ZA_TILE_N,
+                    B_rowmajor + (size_t)k * N + (size_t)tn * ZA_TILE_N,
+                    ZA_TILE_N * sizeof(T));
+  }
+
+  // gemm_crsp.cpp:369-386: A column-major M x K, B row-major K x N, D
+  // row-major M x N, D = alpha * A * B + gamma.  The GPU reads the unpacked
+  // layouts directly, so no packing pass is needed.
+  static void compute_test(const int M, const int N, const int K,
+                           const T* A_colmajor, const T* B_rowmajor, T* D,
+                           const T alpha, const T gamma) {
+    skgemm_facade::check(
+        skgemm_dgemm_host(skgemm_facade::ctx(), SKGEMM_COL_MAJOR,
+                          SKGEMM_ROW_MAJOR, SKGEMM_ROW_MAJOR, M, N, K, alpha,
+                          A_colmajor, M > 0 ? M : 1, B_rowmajor,
+                          N > 0 ? N : 1, 0.0, D, N > 0 ? N : 1, gamma),
+        "gemm::compute_test");
+  }
+
+  // gemm_crsp.cpp:388-430: ns per call, operands resident on the device,
+  // WITERS warm-up calls then ITERS timed calls (5 and 3 in the reference),
+  // kernel time from HIP events.
+  static double compute_benchmark(const int M, const int N, const int K,
+                                  const T* A_colmajor, const T* B_rowmajor,
+                                  T* D, const T alpha, const T gamma,
+                                  int witers = 5, int iters = 3) {
+    skgemm_ctx* c = skgemm_facade::ctx();
+    void *dA, *dB, *dD;
+    skgemm_facade::check(skgemm_dev_alloc(c, sizeof(T) * M * K, 0, &dA), "alloc");
+    skgemm_facade::check(skgemm_dev_alloc(c, sizeof(T) * K * N, 0, &dB), "alloc");
+    skgemm_facade::check(skgemm_dev_alloc(c, sizeof(T) * M * N, 1, &dD), "alloc");
+    skgemm_memcpy_h2d(c, dA, A_colmajor, sizeof(T) * M * K);
+    skgemm_memcpy_h2d(c, dB, B_rowmajor, sizeof(T) * K * N);
+    auto run = [&] {
+      skgemm_facade::check(
+          skgemm_dgemm(c, SKGEMM_COL_MAJOR, SKGEMM_ROW_MAJOR, SKGEMM_ROW_MAJOR,
+                       M, N, K, alpha, (const T*)dA, M, (const T*)dB, N, 0.0,
+                       (T*)dD, N, gamma),
+          "gemm::compute_benchmark");
+    };
+    for (int i = 0; i < witers; ++i) run();
+    double total_ms = 0.0;
+    for (int i = 0; i < iters; ++i) {
+      run();
+      double ms = 0.0;
+      skgemm_last_kernel_ms(c, &ms);
+      total_ms += ms;
+    }
+    skgemm_facade::check(skgemm_synchronize(c), "synchronize");
+    skgemm_memcpy_d2h(c, D, dD, sizeof(T) * M * N);
+    skgemm_dev_free(c, dA);
+    skgemm_dev_free(c, dB);
+    skgemm_dev_free(c, dD);
+    return total_ms * 1e6 / (iters > 0 ? iters : 1);
+  }
+};
+
+// ---- include/sme_syrk.hpp / syrk_crsp.cpp -----------------------------------
+template <typename T>
+class sme_syrk;
+
+template <>
+class sme_syrk<double> {
+ public:
+  using T = double;
+  static constexpr int TILE_M = 16;
+
+  // include/sme_syrk.hpp:137-149: C = alpha * A^T * A + beta * C on the
+  // upper triangle (m1 <= m2) of the M x M column-major C; A is P16
+  // [ceil(M/16)][K][16].  Column-major upper == row-major lower, which is
+  // what the C ABI writes.
+  static void compute(const int K, const int M, const T* A_packed, T* C,
+                      const T alpha, const T beta) {
+    skgemm_facade::check(
+        skgemm_dsyrk_host(skgemm_facade::ctx(), SKGEMM_PACK16, M, K, alpha,
+                          A_packed, K > 0 ? K : 1, beta, C, M > 0 ? M : 1),
+        "sme_syrk::compute");
+  }
+};
+
+// ---- include/sme_trp_3d.hpp ---------------------------------------------------
+template <typename T>
+class sme_gemm_trp_3d;
+
+template <>
+class sme_gemm_trp_3d<double> {
+ public:
+  using T = double;
+  // include/sme_trp_3d.hpp:194-215: D[g, bp] = sum_b A[g, b] * B[b, bp] with
+  // A grid-packed [Nd/16][nb][16], B row-major [nb][nb], D band-packed
+  // [nb/16][Nd][16]; no alpha / gamma.
+  static void compute(const int Nd, const int nb, const T* A, const T* B,
+                      T* D) {
+    skgemm_facade::check(
+        skgemm_dgemm_host(skgemm_facade::ctx(), SKGEMM_PACK16,
+                          SKGEMM_ROW_MAJOR, SKGEMM_PACK16, Nd, nb, nb, 1.0, A,
+                          nb > 0 ? nb : 1, B, nb > 0 ? nb : 1, 0.0, D,
+                          Nd > 0 ? Nd : 1, 0.0),
+        "sme_gemm_trp_3d::compute");
+  }
+};
+template <typename T>
+using sme_gemm_trp = sme_gemm_trp_3d<T>;
+
+// ---- include/sme_pack.hpp:316-352 (C wrappers) ------------------------------
+// Column-major [Nd, nb] <-> grid-packed [Nd/16][nb][16], on the device.
+inline void linalg_pack_c16_f64(const int Nd, const int nb, const double* src,
+                                double* dst) {
+  skgemm_ctx* c = skgemm_facade::ctx();
+  const size_t n = (size_t)Nd * nb, np = (size_t)(Nd / 16) * nb * 16;
+  void *ds, *dd;
+  skgemm_facade::check(skgemm_dev_alloc(c, n * 8, 0, &ds), "alloc");
+  skgemm_facade::check(skgemm_dev_alloc(c, np * 8 + 8, 0, &dd), "alloc");
+  skgemm_memcpy_h2d(c, ds, src, n * 8);
+  // skgemm_pack addresses element (r, o) = (band b, grid g); the source
+  // src[g + b*Nd] is r*ld + o with ld = Nd, i.e. ROW_MAJOR in that convention.
+  skgemm_facade::check(skgemm_pack(c, SKGEMM_ROW_MAJOR, (const double*)ds, Nd,
+                                   SKGEMM_PACK16, (double*)dd, nb,
+                                   (Nd / 16) * 16, nb),
+                       "linalg_pack_c16_f64");
+  skgemm_facade::check(skgemm_synchronize(c), "synchronize");
+  skgemm_memcpy_d2h(c, dst, dd, np * 8);
+  skgemm_dev_free(c, ds);
+  skgemm_dev_free(c, dd);
+}
+
+inline void linalg_unpack_c16_f64(const int Nd, const int nb,
+                                  const double* src, double* dst) {
+  skgemm_ctx* c = skgemm_facade::ctx();
+  const size_t n = (size_t)Nd * nb, np = (size_t)(Nd / 16) * nb * 16;
+  void *ds, *dd;
+  skgemm_facade::check(skgemm_dev_alloc(c, np * 8 + 8, 0, &ds), "alloc");
+  skgemm_facade::check(skgemm_dev_alloc(c, n * 8, 0, &dd), "alloc");
+  skgemm_memcpy_h2d(c, ds, src, np * 8);
+  skgemm_memcpy_h2d(c, dd, dst, n * 8);
+  skgemm_facade::check(skgemm_pack(c, SKGEMM_PACK16, (const double*)ds, nb,
+                                   SKGEMM_ROW_MAJOR, (double*)dd, Nd,
+                                   (Nd / 16) * 16, nb),
+                       "linalg_unpack_c16_f64");
+  skgemm_facade::check(skgemm_synchronize(c), "synchronize");
+  skgemm_memcpy_d2h(c, dst, dd, n * 8);
+  skgemm_dev_free(c, ds);
+  skgemm_dev_free(c, dd);
+}

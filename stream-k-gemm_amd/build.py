@@ -20,7 +20,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
           "-Wno-unused-command-line-argument", "-Wno-unknown-warning-option",
           "-Wno-unused-value", "-Wno-unused-result"]
 DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
-          "-packed-fp32-ops", "-munsafe-fp-atomics"]
+          "-packed-fp32-ops", "-munsafe-fp-atomics",
+          "-mllvm", "-amdgpu-mfma-vgpr-form"]
 
 
 def sources():

@@ -89,9 +89,22 @@ int make_operand(int layout, bool a_convention, const double* p, int64_t ld,
     o.ks = 1;
     o.kcontig = 1;
   }
+  // 16-byte pair loads need a 16-byte aligned pair start everywhere and an
+  // even extent along the contiguous index (a pair never straddles the edge).
   const bool aligned = (reinterpret_cast<uintptr_t>(p) & 15) == 0;
-  o.vec2 = aligned && (o.kcontig ? (o.os % 2 == 0)
-                                 : (o.ks % 2 == 0 && o.pstride % 2 == 0));
+  o.vec2 = aligned && (o.kcontig ? (o.os % 2 == 0 && K % 2 == 0)
+                                 : (o.ks % 2 == 0 && o.pstride % 2 == 0 &&
+                                    ext % 2 == 0));
+  // Per-thread tile offsets are 32-bit byte offsets from the tile origin
+  // (largest tile: 256 along o, 32 along k).
+  const int64_t pm = (1ll << o.pshift) - 1;
+  const int64_t max_rel =
+      ((255 >> o.pshift) * o.pstride + std::min<int64_t>(255, pm) * o.os +
+       31 * o.ks + 2) * 8;
+  if (max_rel >= 0x7FFFFF00ll)
+    return set_error(SKGEMM_ERR_NOTSUP,
+                     "%s: leading dimension %lld too large for 32-bit tile "
+                     "offsets", name, (long long)ld);
   *op = o;
   return SKGEMM_OK;
 }
@@ -367,6 +380,11 @@ double* skgemm_ctx::staging(int i, size_t bytes) {
 }
 
 void skgemm_ctx::free_all() {
+  for (auto& pr : tring) {
+    hipEventDestroy(pr.first);
+    hipEventDestroy(pr.second);
+  }
+  tring.clear();
   if (ws) hipFree(ws);
   if (flags) hipFree(flags);
   if (status) hipFree(status);
@@ -376,6 +394,21 @@ void skgemm_ctx::free_all() {
   flags = nullptr;
   status = nullptr;
   free_extra();
+}
+
+void skgemm_ctx::record_begin() {
+  hipEventRecord(ev0, stream);
+  if (tring_used < (int)tring.size())
+    hipEventRecord(tring[tring_used].first, stream);
+}
+
+void skgemm_ctx::record_end() {
+  hipEventRecord(ev1, stream);
+  timed = true;
+  if (tring_used < (int)tring.size()) {
+    hipEventRecord(tring[tring_used].second, stream);
+    ++tring_used;
+  }
 }
 
 int skgemm_ctx::capacity(int cfg) {
@@ -483,10 +516,9 @@ int run_single(skgemm_ctx* c, Problem& p, int cfg, int grid) {
   L.flags = c->flags;
   L.status = c->status;
   L.spin_limit = c->spin_limit;
-  SKG_HIP(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
+  c->record_begin();
   SKG_HIP(launch_dgemm_sk(cfg, L, c->stream), "Stream-K launch");
-  SKG_HIP(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
-  c->timed = true;
+  c->record_end();
   return SKGEMM_OK;
 }
 
@@ -676,6 +708,49 @@ extern "C" int skgemm_query_plan(skgemm_ctx* c, int kind, int M, int N, int K,
   if (tiles) *tiles = nt;
   if (iters) *iters = nt * std::max<int64_t>(1, (K + t.bk - 1) / t.bk);
   return SKGEMM_OK;
+}
+
+extern "C" int skgemm_timing_enable(skgemm_ctx* c, int capacity) {
+  SKG_TRY(check_ctx(c));
+  if (capacity < 0) return set_error(SKGEMM_ERR_ARG, "capacity < 0");
+  SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
+  SKG_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  for (auto& pr : c->tring) {
+    hipEventDestroy(pr.first);
+    hipEventDestroy(pr.second);
+  }
+  c->tring.clear();
+  c->tring_used = 0;
+  for (int i = 0; i < capacity; ++i) {
+    hipEvent_t a, b;
+    SKG_HIP(hipEventCreate(&a), "hipEventCreate");
+    hipError_t e = hipEventCreate(&b);
+    if (e != hipSuccess) {
+      hipEventDestroy(a);
+      return hip_error(e, "hipEventCreate");
+    }
+    c->tring.emplace_back(a, b);
+  }
+  return SKGEMM_OK;
+}
+
+extern "C" int skgemm_timing_read(skgemm_ctx* c, double* ms, int max,
+                                  int* count) {
+  SKG_TRY(check_ctx(c));
+  const int n = c->tring_used;
+  if (n > 0)
+    SKG_HIP(hipEventSynchronize(c->tring[n - 1].second), "hipEventSynchronize");
+  int w = 0;
+  for (int i = 0; i < n && i < max; ++i) {
+    float f = 0.f;
+    SKG_HIP(hipEventElapsedTime(&f, c->tring[i].first, c->tring[i].second),
+            "hipEventElapsedTime");
+    if (ms) ms[i] = f;
+    ++w;
+  }
+  if (count) *count = w;
+  c->tring_used = 0;
+  return c->check_status();
 }
 
 extern "C" int skgemm_last_kernel_ms(skgemm_ctx* c, double* ms) {

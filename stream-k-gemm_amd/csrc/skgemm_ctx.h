@@ -8,6 +8,7 @@
 #include <map>
 #include <tuple>
 #include <utility>
+#include <vector>
 
 #include "skgemm_internal.h"
 
@@ -20,6 +21,11 @@ struct skgemm_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;
+  // Optional ring of event pairs, one per launch (skgemm_timing_*).
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> tring;
+  int tring_used = 0;
+  void record_begin();
+  void record_end();
 
   // Stream-K fix-up workspace: one partial tile per persistent workgroup.
   double* ws = nullptr;

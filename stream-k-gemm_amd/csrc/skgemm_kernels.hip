@@ -38,8 +38,8 @@ static const TileCfg kCfgs[] = {
     {64, 64, 16, 2, 2},    // 1: 4 waves, 32x32 per wave
     {128, 64, 16, 2, 2},   // 2: 4 waves, 64x32 per wave
     {64, 128, 16, 2, 2},   // 3: 4 waves, 32x64 per wave
-    {256, 128, 16, 4, 2},  // 4: 8 waves, 64x64 per wave
-    {128, 128, 32, 2, 2},  // 5: as 0 with BK = 32
+    {128, 128, 16, 4, 2},  // 4: 8 waves, 32x64 per wave
+    {256, 128, 16, 4, 2},  // 5: 8 waves, 64x64 per wave
 };
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
@@ -65,66 +65,100 @@ __device__ __forceinline__ int64_t out_addr(const Output& d, int m, int n) {
          (int64_t)m * d.ms;
 }
 
+// Raw buffer loads: operand tiles are read through a buffer resource whose
+// base is the tile's first element, so per-thread byte offsets are loop
+// invariant.  An element outside the matrix is given an offset past
+// num_records, which the hardware returns as 0 without touching memory: no
+// branches on the load path.
+constexpr int kRsrcWord3 = 0x00020000;  // gfx9-family raw buffer (CDNA3/CDNA4)
+constexpr int kNumRecords = 0x7FFFFF00;
+constexpr int kOOB = (int)0x80000000;   // >= kNumRecords: reads as zero
+
+typedef int v2i __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) double gdouble;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const double* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), (short)0,
+                                           kNumRecords, kRsrcWord3);
+}
+
 // Global -> register -> LDS staging of one BO x BK operand tile.  Elements
-// move in pairs along the operand's contiguous index (k or o) so that a pair
-// is one 16-byte load when aligned.  LDS holds the tile as [k][BO + kPad].
-template <int BO, int BK, int NT>
+// move in pairs along the operand's contiguous index (k when KC, else o), one
+// 16-byte load per pair when VEC.  LDS holds the tile as [k][BO + kPad].
+template <int BO, int BK, int NT, int KC, int VEC>
 struct Stage {
   static constexpr int kPairs = BO * BK / 2 / NT;
   static_assert(BO * BK / 2 % NT == 0, "tile not divisible over threads");
   static constexpr int kPitch = BO + kPad;
+  int rel[kPairs];  // byte offset of pair p from the tile's (o0, k0) element
   double x[2 * kPairs];
 
-  __device__ __forceinline__ void load(const Operand& op, int o0, int k0,
-                                       int K, int tid) {
-#pragma unroll
-    for (int p = 0; p < kPairs; ++p) {
-      const int q = p * NT + tid;
-      double x0 = 0.0, x1 = 0.0;
-      if (op.kcontig) {
-        const int o = q / (BK / 2), k = (q % (BK / 2)) * 2;
-        const int og = o0 + o, kg = k0 + k;
-        const bool in0 = og < op.extent && kg < K;
-        const bool in1 = og < op.extent && kg + 1 < K;
-        const double* src = op.ptr + (in0 ? op_addr(op, og, kg) : 0);
-        if (in1 && op.vec2) {
-          const double2 t = *reinterpret_cast<const double2*>(src);
-          x0 = t.x;
-          x1 = t.y;
-        } else {
-          if (in0) x0 = src[0];
-          if (in1) x1 = src[1];
-        }
-      } else {
-        const int k = q / (BO / 2), o = (q % (BO / 2)) * 2;
-        const int og = o0 + o, kg = k0 + k;
-        const bool in0 = og < op.extent && kg < K;
-        const bool in1 = og + 1 < op.extent && kg < K;
-        const double* src = op.ptr + (in0 ? op_addr(op, og, kg) : 0);
-        if (in1 && op.vec2) {
-          const double2 t = *reinterpret_cast<const double2*>(src);
-          x0 = t.x;
-          x1 = t.y;
-        } else {
-          if (in0) x0 = src[0];
-          if (in1) x1 = src[op.os];
-        }
-      }
-      x[2 * p] = x0;
-      x[2 * p + 1] = x1;
+  __device__ __forceinline__ static void coords(int p, int tid, int& o,
+                                                int& k) {
+    const int q = p * NT + tid;
+    if (KC) {
+      o = q / (BK / 2);
+      k = (q % (BK / 2)) * 2;
+    } else {
+      k = q / (BO / 2);
+      o = (q % (BO / 2)) * 2;
     }
   }
 
-  __device__ __forceinline__ void store(double* lds, int kcontig, int tid) {
+  // Tile origins o0 are multiples of 64, hence of every panel width, so the
+  // offset of (o0 + o, k0 + k) from (o0, k0) depends on (o, k) only.
+  __device__ __forceinline__ void init(const Operand& op, int tid) {
+    const int pm = (1 << op.pshift) - 1;
 #pragma unroll
     for (int p = 0; p < kPairs; ++p) {
-      const int q = p * NT + tid;
-      if (kcontig) {
-        const int o = q / (BK / 2), k = (q % (BK / 2)) * 2;
+      int o, k;
+      coords(p, tid, o, k);
+      rel[p] = (int)(((int64_t)(o >> op.pshift) * op.pstride +
+                      (int64_t)(o & pm) * op.os + (int64_t)k * op.ks) *
+                     8);
+    }
+  }
+
+  // o_lim = extent - o0, k_lim = K - k0: elements at o >= o_lim or
+  // k >= k_lim read as zero.
+  __device__ __forceinline__ void load(__amdgpu_buffer_rsrc_t r, int o_lim,
+                                       int k_lim, int tid) {
+#pragma unroll
+    for (int p = 0; p < kPairs; ++p) {
+      int o, k;
+      coords(p, tid, o, k);
+      if (VEC) {
+        // VEC implies an even extent along the contiguous index, so the
+        // pair is entirely inside or entirely outside.
+        const bool ok = o < o_lim && k < k_lim;
+        const v4i v =
+            __builtin_amdgcn_raw_buffer_load_b128(r, ok ? rel[p] : kOOB, 0, 0);
+        const double2 d = __builtin_bit_cast(double2, v);
+        x[2 * p] = d.x;
+        x[2 * p + 1] = d.y;
+      } else {
+        const bool ok0 = o < o_lim && k < k_lim;
+        const bool ok1 = KC ? (o < o_lim && k + 1 < k_lim)
+                            : (o + 1 < o_lim && k < k_lim);
+        const v2i a =
+            __builtin_amdgcn_raw_buffer_load_b64(r, ok0 ? rel[p] : kOOB, 0, 0);
+        const v2i b = __builtin_amdgcn_raw_buffer_load_b64(
+            r, ok1 ? rel[p] + 8 : kOOB, 0, 0);
+        x[2 * p] = __builtin_bit_cast(double, a);
+        x[2 * p + 1] = __builtin_bit_cast(double, b);
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store(double* lds, int tid) {
+#pragma unroll
+    for (int p = 0; p < kPairs; ++p) {
+      int o, k;
+      coords(p, tid, o, k);
+      if (KC) {
         lds[k * kPitch + o] = x[2 * p];
         lds[(k + 1) * kPitch + o] = x[2 * p + 1];
       } else {
-        const int k = q / (BO / 2), o = (q % (BO / 2)) * 2;
         *reinterpret_cast<double2*>(lds + k * kPitch + o) =
             make_double2(x[2 * p], x[2 * p + 1]);
       }
@@ -142,14 +176,102 @@ __device__ __forceinline__ void tri_tile(int64_t t, int& ti, int& tj) {
 }
 
 template <int BM, int BN, int BK, int WM, int WN>
-__global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
-  constexpr int NT = WM * WN * 64;
-  constexpr int TM = BM / WM, TN = BN / WN;
-  constexpr int MB = TM / 16, NB = TN / 16;
+struct TileShape {
+  static constexpr int NT = WM * WN * 64;
+  static constexpr int TM = BM / WM, TN = BN / WN;
+  static constexpr int MB = TM / 16, NB = TN / 16;
+  static constexpr int kPitchA = BM + kPad, kPitchB = BN + kPad;
+  static constexpr int kBufDoubles = BK * (kPitchA + kPitchB);
   static_assert(TM % 16 == 0 && TN % 16 == 0 && BK % 4 == 0, "bad tile");
-  using StA = Stage<BM, BK, NT>;
-  using StB = Stage<BN, BK, NT>;
-  constexpr int kBufDoubles = BK * (StA::kPitch + StB::kPitch);
+};
+
+// MFMAs of one staged BK step from LDS buffer `la` (A) / `lb` (B).
+template <int BM, int BN, int BK, int WM, int WN>
+__device__ __forceinline__ void mma_step(
+    v4d (&acc)[BM / WM / 16][BN / WN / 16],
+    const double* la, const double* lb, int wm, int wn, int lq, int lr) {
+  using S = TileShape<BM, BN, BK, WM, WN>;
+#pragma unroll
+  for (int s = 0; s < BK / 4; ++s) {
+    double av[S::MB], bv[S::NB];
+#pragma unroll
+    for (int i = 0; i < S::MB; ++i)
+      av[i] = la[(s * 4 + lq) * S::kPitchA + wm * S::TM + i * 16 + lr];
+#pragma unroll
+    for (int j = 0; j < S::NB; ++j)
+      bv[j] = lb[(s * 4 + lq) * S::kPitchB + wn * S::TN + j * 16 + lr];
+#pragma unroll
+    for (int i = 0; i < S::MB; ++i)
+#pragma unroll
+      for (int j = 0; j < S::NB; ++j)
+        acc[i][j] =
+            __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+  }
+}
+
+// The MAC loop over BK steps [kb, ke) of one tile, for one layout variant:
+// AKC / BKC = operand contiguous along k, VEC = 16-byte pair loads.
+// Loads of step i+1 are issued before the MFMAs of step i; the last step is
+// peeled so the loop body has no branches.
+template <int BM, int BN, int BK, int WM, int WN, int AKC, int BKC, int VEC>
+__device__ __forceinline__ void mac_loop(
+    v4d (&acc)[BM / WM / 16][BN / WN / 16],
+    const Operand& A, const Operand& B, int K, int m0, int n0, int kb, int ke,
+    double* lds, int tid) {
+  using S = TileShape<BM, BN, BK, WM, WN>;
+  constexpr int NT = S::NT;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int lq = lane >> 4, lr = lane & 15;
+  Stage<BM, BK, NT, AKC, VEC> sa;
+  Stage<BN, BK, NT, BKC, VEC> sb;
+  sa.init(A, tid);
+  sb.init(B, tid);
+  const double* baseA = A.ptr + op_addr(A, m0, 0);
+  const double* baseB = B.ptr + op_addr(B, n0, 0);
+  const int64_t stepA = (int64_t)BK * A.ks, stepB = (int64_t)BK * B.ks;
+  const int olimA = A.extent - m0, olimB = B.extent - n0;
+
+  sa.load(make_rsrc(baseA + kb * stepA), olimA, K - kb * BK, tid);
+  sb.load(make_rsrc(baseB + kb * stepB), olimB, K - kb * BK, tid);
+  sa.store(lds, tid);
+  sb.store(lds + BK * S::kPitchA, tid);
+  __syncthreads();
+  int cur = 0;
+  for (int kk = kb; kk + 1 < ke; ++kk) {
+    sa.load(make_rsrc(baseA + (kk + 1) * stepA), olimA, K - (kk + 1) * BK, tid);
+    sb.load(make_rsrc(baseB + (kk + 1) * stepB), olimB, K - (kk + 1) * BK, tid);
+    const double* la = lds + cur * S::kBufDoubles;
+    mma_step<BM, BN, BK, WM, WN>(acc, la, la + BK * S::kPitchA, wm, wn, lq, lr);
+    double* na = lds + (cur ^ 1) * S::kBufDoubles;
+    sa.store(na, tid);
+    sb.store(na + BK * S::kPitchA, tid);
+    __syncthreads();
+    cur ^= 1;
+  }
+  const double* la = lds + cur * S::kBufDoubles;
+  mma_step<BM, BN, BK, WM, WN>(acc, la, la + BK * S::kPitchA, wm, wn, lq, lr);
+  __syncthreads();
+}
+
+// Uniform read of a descriptor held in global memory: through the constant
+// address space the loads become scalar (s_load) and land in SGPRs.
+template <typename T>
+__device__ __forceinline__ T load_uniform(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "dword-sized descriptor");
+  T out;
+  const __attribute__((address_space(4))) int* src =
+      (const __attribute__((address_space(4))) int*)p;
+  int* dst = reinterpret_cast<int*>(&out);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = src[i];
+  return out;
+}
+
+template <int BM, int BN, int BK, int WM, int WN, bool BATCHED>
+__global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
+  using S = TileShape<BM, BN, BK, WM, WN>;
+  constexpr int NT = S::NT, TM = S::TM, TN = S::TN, MB = S::MB, NB = S::NB;
 
   extern __shared__ __align__(16) double lds[];
 
@@ -168,30 +290,36 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
   int64_t it = wg_start(w, L.total_iters, G);
   const int64_t end = wg_start(w + 1, L.total_iters, G);
   int pi = 0;
-
-  StA sa;
-  StB sb;
+  const Problem* tab = L.problems;
 
   while (it < end) {
-    // Locate the problem holding iteration `it`.
-    if (L.problems != nullptr) {
-      while (pi + 1 < L.num_problems && L.problems[pi + 1].iter_begin <= it)
+    if (BATCHED) {
+      while (pi + 1 < L.num_problems &&
+             load_uniform(&tab[pi + 1].iter_begin) <= it)
         ++pi;
     }
-    const Problem& P = (L.problems != nullptr) ? L.problems[pi] : L.single;
-    const int ipt = P.iters_per_tile;
-    const int64_t local = it - P.iter_begin;
-    const int64_t tile = local / ipt;
-    const int64_t t0 = P.iter_begin + tile * ipt;
+    const Operand A = BATCHED ? load_uniform(&tab[pi].a) : L.single.a;
+    const Operand B = BATCHED ? load_uniform(&tab[pi].b) : L.single.b;
+    const int K = BATCHED ? load_uniform(&tab[pi].K) : L.single.K;
+    const int ipt = BATCHED ? load_uniform(&tab[pi].iters_per_tile)
+                            : L.single.iters_per_tile;
+    const int64_t ib =
+        BATCHED ? load_uniform(&tab[pi].iter_begin) : L.single.iter_begin;
+    const int syrk = BATCHED ? load_uniform(&tab[pi].syrk) : L.single.syrk;
+    const int tiles_m =
+        BATCHED ? load_uniform(&tab[pi].tiles_m) : L.single.tiles_m;
+
+    const int64_t tile = (it - ib) / ipt;
+    const int64_t t0 = ib + tile * ipt;
     const int kb = (int)(it - t0);
     const int ke = (int)((end - t0) < ipt ? (end - t0) : ipt);
 
     int ti, tj;
-    if (P.syrk) {
+    if (syrk) {
       tri_tile(tile, ti, tj);
     } else {
-      ti = (int)(tile % P.tiles_m);
-      tj = (int)(tile / P.tiles_m);
+      ti = (int)(tile % tiles_m);
+      tj = (int)(tile / tiles_m);
     }
     const int m0 = ti * BM, n0 = tj * BN;
 
@@ -201,50 +329,25 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
 #pragma unroll
       for (int j = 0; j < NB; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
-    // ---- MAC loop over BK steps [kb, ke) --------------------------------
-    sa.load(P.a, m0, kb * BK, P.K, tid);
-    sb.load(P.b, n0, kb * BK, P.K, tid);
-    sa.store(lds, P.a.kcontig, tid);
-    sb.store(lds + BK * StA::kPitch, P.b.kcontig, tid);
-    __syncthreads();
-    for (int kk = kb; kk < ke; ++kk) {
-      const int cur = (kk - kb) & 1;
-      const bool more = kk + 1 < ke;
-      if (more) {
-        sa.load(P.a, m0, (kk + 1) * BK, P.K, tid);
-        sb.load(P.b, n0, (kk + 1) * BK, P.K, tid);
-      }
-      const double* la = lds + cur * kBufDoubles;
-      const double* lb = la + BK * StA::kPitch;
-#pragma unroll
-      for (int s = 0; s < BK / 4; ++s) {
-        double av[MB], bv[NB];
-#pragma unroll
-        for (int i = 0; i < MB; ++i)
-          av[i] = la[(s * 4 + lq) * StA::kPitch + wm * TM + i * 16 + lr];
-#pragma unroll
-        for (int j = 0; j < NB; ++j)
-          bv[j] = lb[(s * 4 + lq) * StB::kPitch + wn * TN + j * 16 + lr];
-#pragma unroll
-        for (int i = 0; i < MB; ++i)
-#pragma unroll
-          for (int j = 0; j < NB; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j],
-                                                             acc[i][j], 0, 0, 0);
-      }
-      if (more) {
-        double* na = lds + (cur ^ 1) * kBufDoubles;
-        sa.store(na, P.a.kcontig, tid);
-        sb.store(na + BK * StA::kPitch, P.b.kcontig, tid);
-      }
-      __syncthreads();
+    // ---- MAC loop, one branch-free variant per operand layout -----------
+    const int variant = (A.kcontig ? 4 : 0) | (B.kcontig ? 2 : 0) |
+                        ((A.vec2 && B.vec2) ? 1 : 0);
+#define SKG_MAC(V)                                                          \
+  case V:                                                                   \
+    mac_loop<BM, BN, BK, WM, WN, (V >> 2) & 1, (V >> 1) & 1, V & 1>(        \
+        acc, A, B, K, m0, n0, kb, ke, lds, tid);                            \
+    break;
+    switch (variant) {
+      SKG_MAC(0) SKG_MAC(1) SKG_MAC(2) SKG_MAC(3)
+      SKG_MAC(4) SKG_MAC(5) SKG_MAC(6) SKG_MAC(7)
     }
+#undef SKG_MAC
 
     // ---- Stream-K fix-up -------------------------------------------------
     bool write_out = true;
     if (kb != 0) {
       // Not the owner: spill the partial to this workgroup's slot.
-      double* slot = L.ws + (int64_t)w * (BM * BN);
+      gdouble* slot = (gdouble*)(L.ws) + (int64_t)w * (BM * BN);
 #pragma unroll
       for (int i = 0; i < MB; ++i)
 #pragma unroll
@@ -278,7 +381,7 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
         }
         __syncthreads();
         __threadfence();
-        const double* slot = L.ws + (int64_t)w2 * (BM * BN);
+        const gdouble* slot = (const gdouble*)(L.ws) + (int64_t)w2 * (BM * BN);
 #pragma unroll
         for (int i = 0; i < MB; ++i)
 #pragma unroll
@@ -292,22 +395,34 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
 
     // ---- Epilogue: D = alpha*acc + beta*D + gamma --------------------------
     if (write_out) {
-      const double alpha = P.alpha, beta = P.beta, gamma = P.gamma;
+      const Output D = BATCHED ? load_uniform(&tab[pi].d) : L.single.d;
+      const double alpha =
+          BATCHED ? load_uniform(&tab[pi].alpha) : L.single.alpha;
+      const double beta = BATCHED ? load_uniform(&tab[pi].beta) : L.single.beta;
+      const double gamma =
+          BATCHED ? load_uniform(&tab[pi].gamma) : L.single.gamma;
+      const int M = BATCHED ? load_uniform(&tab[pi].M) : L.single.M;
+      const int N = BATCHED ? load_uniform(&tab[pi].N) : L.single.N;
+      gdouble* dbase = (gdouble*)(D.ptr);
+      const int64_t rstep = 4 * D.ms;  // element v -> v + 1 moves 4 rows
 #pragma unroll
-      for (int i = 0; i < MB; ++i)
+      for (int i = 0; i < MB; ++i) {
+        const int row0 = m0 + wm * TM + i * 16 + lq;
 #pragma unroll
-        for (int j = 0; j < NB; ++j)
+        for (int j = 0; j < NB; ++j) {
+          const int col = n0 + wn * TN + j * 16 + lr;
+          gdouble* dst = dbase + out_addr(D, row0, col);
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            const int row = m0 + wm * TM + i * 16 + v * 4 + lq;
-            const int col = n0 + wn * TN + j * 16 + lr;
-            if (row < P.M && col < P.N && (!P.syrk || row >= col)) {
-              double* dst = P.d.ptr + out_addr(P.d, row, col);
+            const int row = row0 + 4 * v;
+            if (row < M && col < N && (!syrk || row >= col)) {
               double val = alpha * acc[i][j][v];
-              if (beta != 0.0) val = fma(beta, *dst, val);
-              *dst = val + gamma;
+              if (beta != 0.0) val = fma(beta, dst[v * rstep], val);
+              dst[v * rstep] = val + gamma;
             }
           }
+        }
+      }
     }
     it = t0 + ke;
   }
@@ -320,7 +435,21 @@ static size_t lds_bytes_t() {
 
 template <int BM, int BN, int BK, int WM, int WN>
 static hipError_t launch_t(const Launch& L, hipStream_t s) {
-  auto kfn = sk_dgemm_kernel<BM, BN, BK, WM, WN>;
+  if (L.problems != nullptr) {
+    auto kb = sk_dgemm_kernel<BM, BN, BK, WM, WN, true>;
+    const size_t lds = lds_bytes_t<BM, BN, BK, WM, WN>();
+    static bool attr_b = false;
+    if (!attr_b) {
+      hipError_t e = hipFuncSetAttribute(
+          reinterpret_cast<const void*>(kb),
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+      attr_b = true;
+    }
+    hipLaunchKernelGGL(kb, dim3(L.grid), dim3(WM * WN * 64), lds, s, L);
+    return hipGetLastError();
+  }
+  auto kfn = sk_dgemm_kernel<BM, BN, BK, WM, WN, false>;
   const size_t lds = lds_bytes_t<BM, BN, BK, WM, WN>();
   static bool attr_set = false;
   if (!attr_set) {
@@ -336,16 +465,23 @@ static hipError_t launch_t(const Launch& L, hipStream_t s) {
 
 template <int BM, int BN, int BK, int WM, int WN>
 static int occupancy_t() {
-  auto kfn = sk_dgemm_kernel<BM, BN, BK, WM, WN>;
+  // Both variants (single problem / batched table) must fit: the persistent
+  // grid is sized from the smaller of the two.
+  const void* fns[2] = {
+      reinterpret_cast<const void*>(sk_dgemm_kernel<BM, BN, BK, WM, WN, true>),
+      reinterpret_cast<const void*>(sk_dgemm_kernel<BM, BN, BK, WM, WN, false>)};
   const size_t lds = lds_bytes_t<BM, BN, BK, WM, WN>();
-  hipFuncSetAttribute(reinterpret_cast<const void*>(kfn),
-                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &nb, reinterpret_cast<const void*>(kfn), WM * WN * 64, lds) !=
-      hipSuccess)
-    return 0;
-  return nb;
+  int best = 1 << 30;
+  for (const void* f : fns) {
+    hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)lds);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, WM * WN * 64,
+                                                     lds) != hipSuccess)
+      return 0;
+    if (nb < best) best = nb;
+  }
+  return best;
 }
 
 #define SKG_CFG_SWITCH(cfg, FN, ...)                  \
@@ -354,8 +490,8 @@ static int occupancy_t() {
     case 1: return FN<64, 64, 16, 2, 2>(__VA_ARGS__);   \
     case 2: return FN<128, 64, 16, 2, 2>(__VA_ARGS__);  \
     case 3: return FN<64, 128, 16, 2, 2>(__VA_ARGS__);  \
-    case 4: return FN<256, 128, 16, 4, 2>(__VA_ARGS__); \
-    case 5: return FN<128, 128, 32, 2, 2>(__VA_ARGS__); \
+    case 4: return FN<128, 128, 16, 4, 2>(__VA_ARGS__); \
+    case 5: return FN<256, 128, 16, 4, 2>(__VA_ARGS__); \
     default: break;                                   \
   }
 

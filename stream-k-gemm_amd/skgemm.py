@@ -30,7 +30,8 @@ EXPORTS = [
     "skgemm_dgemm_ozaki", "skgemm_dsyrk_ozaki", "skgemm_pack",
     "skgemm_batched", "skgemm_set_tuning", "skgemm_num_tiles",
     "skgemm_tile_info", "skgemm_autotune", "skgemm_load_tuning",
-    "skgemm_query_plan", "skgemm_last_kernel_ms", "skgemm_peak_probe",
+    "skgemm_query_plan", "skgemm_timing_enable", "skgemm_timing_read",
+    "skgemm_last_kernel_ms", "skgemm_peak_probe",
     "skgemm_copy_probe", "skgemm_fill_uniform", "skgemm_fill_normal",
     "skgemm_scale_columns_pow10", "skgemm_server_create",
     "skgemm_server_destroy", "skgemm_server_add_system",
@@ -111,6 +112,8 @@ def lib():
         "skgemm_load_tuning": (c_int, [P, ctypes.c_char_p]),
         "skgemm_query_plan": (c_int, [P, c_int, c_int, c_int, c_int, P, P, P,
                                       P]),
+        "skgemm_timing_enable": (c_int, [P, c_int]),
+        "skgemm_timing_read": (c_int, [P, P, c_int, P]),
         "skgemm_last_kernel_ms": (c_int, [P, P]),
         "skgemm_peak_probe": (c_int, [P, c_int, P]),
         "skgemm_copy_probe": (c_int, [P, c_size_t, P]),
@@ -210,6 +213,16 @@ class Context:
                                       ctypes.byref(ni)))
         return dict(tile=t.value, grid=g.value, tiles=nt.value,
                     iters=ni.value)
+
+    def timing_enable(self, capacity):
+        check(lib().skgemm_timing_enable(self.h, capacity))
+
+    def timing_read(self, max_n=100000):
+        buf = np.zeros(max_n, np.float64)
+        n = c_int()
+        check(lib().skgemm_timing_read(self.h, buf.ctypes.data, max_n,
+                                       ctypes.byref(n)))
+        return buf[:n.value].copy()
 
     def last_kernel_ms(self):
         d = _out_double()

@@ -1,0 +1,253 @@
+"""GPU parity of the FP64 Stream-K GEMM against the oracle, through the C ABI.
+
+Bar: relative Frobenius error <= 1e-13 against the double-double oracle
+(plain FP64 contract); exact where the math makes it exact.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_fro
+from test_oracle import decode_d, encode_a, encode_b, encode_d
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-13
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def dev(skg, ctx, a):
+    return skg.DeviceArray.from_host(ctx, a)
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_reference_kats(skg, ctx, orc, case):
+    """gemm_crsp.cpp:555-565 through the compute_test layouts (A column-major,
+    B row-major, D row-major, D = alpha*A*B + gamma) vs the exact closed form."""
+    g = np.load(os.path.join(GOLDEN, "gemm_kat.npz"))
+    M, N, K = (int(x) for x in g["case%d_shape" % case])
+    alpha, gamma = (float(x) for x in g["case%d_coef" % case])
+    A, B = orc.gemm_test_inputs(M, N, K)
+    D = np.zeros(M * N)
+    print("case", case, M, N, K, flush=True)
+    ctx.set_tuning(-1, 0)
+    ctx.dgemm_host(skg.COL_MAJOR, skg.ROW_MAJOR, skg.ROW_MAJOR, M, N, K, alpha,
+                   A, M, B, N, 0.0, D, N, gamma)
+    exact = g["case%d_D" % case].ravel()
+    assert rel_fro(D, exact) < TOL
+    assert np.max(np.abs(D - exact) / (np.abs(exact) + 1e-10)) < 1e-12
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_packed_entry_matches_crsp_restatement(skg, ctx, orc, case):
+    """compute_streamk_packed: PA [M/16][K][16], PB [N/32][K][32]."""
+    g = np.load(os.path.join(GOLDEN, "gemm_kat.npz"))
+    M, N, K = (int(x) for x in g["case%d_shape" % case])
+    alpha, gamma = (float(x) for x in g["case%d_coef" % case])
+    A, B = orc.gemm_test_inputs(M, N, K)
+    pa, pb = orc.pack_A(M, K, A), orc.pack_B(N, K, B)
+    ref = orc.gemm_crsp(M, N, K, pa, pb, alpha, gamma, nthreads=4)
+    D = np.zeros(M * N)
+    ctx.dgemm_host(skg.PACK16, skg.PACK32, skg.ROW_MAJOR, M, N, K, alpha, pa, K,
+                   pb, K, 0.0, D, N, gamma)
+    assert rel_fro(D, ref) < TOL
+
+
+def test_config1_gemm_1024(skg, ctx, orc):
+    """c1: M=N=K=1024, row-major A and B uniform(-1,1) seeds 42 / 43."""
+    M = N = K = 1024
+    A = orc.fill_uniform(M * K, 42)
+    B = orc.fill_uniform(K * N, 43)
+    ref = np.zeros(M * N)
+    orc.dgemm(0, 0, 0, M, N, K, 1.0, A, K, B, N, 0.0, ref, N, 0.0, dd=True)
+    for tile in [-1] + list(range(skg.num_tiles())):
+        ctx.set_tuning(tile, 0)
+        D = np.zeros(M * N)
+        ctx.dgemm_host(skg.ROW_MAJOR, skg.ROW_MAJOR, skg.ROW_MAJOR, M, N, K, 1.0,
+                       A, K, B, N, 0.0, D, N, 0.0)
+        err = rel_fro(D, ref)
+        print("tile", tile, "err", err, flush=True)
+        assert err < TOL
+    ctx.set_tuning(-1, 0)
+
+
+def test_all_layouts_ragged(skg, ctx, orc, rng):
+    """Every A/B/D layout (row, col, PACK16, PACK32) with padded lds on a
+    shape that is a multiple of no tile."""
+    M, N, K = 77, 45, 133
+    A = rng.uniform(-1, 1, (M, K))
+    B = rng.uniform(-1, 1, (K, N))
+    ref = A @ B
+    ctx.set_tuning(-1, 0)
+    for la in range(4):
+        Ab, lda = encode_a(A, la)
+        for lb in range(4):
+            Bb, ldb = encode_b(B, lb)
+            for ld_ in range(4):
+                Db, ldd = encode_d(np.zeros((M, N)), ld_)
+                ctx.dgemm_host(la, lb, ld_, M, N, K, 1.0, Ab, lda, Bb, ldb, 0.0,
+                               Db, ldd, 0.0)
+                got = decode_d(Db, ld_, M, N, ldd)
+                chk = np.zeros(M * N)
+                assert rel_fro(got, ref) < TOL, (la, lb, ld_)
+                del chk
+
+
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (1, 300, 7), (300, 1, 7),
+                                   (17, 33, 1), (65, 129, 2), (63, 64, 17),
+                                   (16, 16, 4096), (200, 3, 999), (513, 257, 65)])
+def test_edge_shapes(skg, ctx, orc, rng, M, N, K):
+    A = rng.uniform(-1, 1, (M, K))
+    B = rng.uniform(-1, 1, (K, N))
+    ref = np.zeros(M * N)
+    orc.dgemm(0, 0, 0, M, N, K, 1.0, np.ascontiguousarray(A.ravel()), K,
+              np.ascontiguousarray(B.ravel()), N, 0.0, ref, N, 0.0)
+    D = np.zeros(M * N)
+    ctx.dgemm_host(0, 0, 0, M, N, K, 1.0, np.ascontiguousarray(A.ravel()), K,
+                   np.ascontiguousarray(B.ravel()), N, 0.0, D, N, 0.0)
+    assert rel_fro(D, ref) < TOL
+
+
+def test_zero_sizes_and_k0(skg, ctx, rng):
+    D = np.full(12, 5.0)
+    ctx.dgemm_host(0, 0, 0, 0, 4, 3, 1.0, np.zeros(12), 3, np.zeros(12), 4, 0.0,
+                   D, 4, 0.0)
+    assert np.all(D == 5.0)  # M == 0: nothing written
+    C0 = rng.uniform(-1, 1, 12)
+    D = C0.copy()
+    ctx.dgemm_host(0, 0, 0, 3, 4, 0, 2.0, np.zeros(1), 1, np.zeros(1), 4, 0.5,
+                   D, 4, 0.25)
+    assert np.allclose(D, 0.5 * C0 + 0.25, rtol=0, atol=1e-15)  # K == 0
+
+
+def test_alpha_beta_gamma(skg, ctx, orc, rng):
+    M, N, K = 150, 90, 300
+    A = rng.uniform(-1, 1, (M, K))
+    B = rng.uniform(-1, 1, (K, N))
+    C0 = rng.uniform(-1, 1, (M, N))
+    for alpha, beta, gamma in [(2.0, 0.0, 0.5), (-1.0, 1.0, 0.0),
+                               (0.5, -2.0, 0.25), (0.0, 3.0, 1.0)]:
+        D = C0.copy().ravel()
+        ctx.dgemm_host(0, 0, 0, M, N, K, alpha, np.ascontiguousarray(A.ravel()),
+                       K, np.ascontiguousarray(B.ravel()), N, beta, D, N, gamma)
+        ref = alpha * (A @ B) + beta * C0 + gamma
+        assert rel_fro(D, ref) < TOL
+
+
+def test_beta_zero_never_reads_d(skg, ctx, rng):
+    M, N, K = 64, 64, 64
+    A = rng.uniform(-1, 1, (M, K))
+    B = rng.uniform(-1, 1, (K, N))
+    D = np.full(M * N, np.nan)
+    ctx.dgemm_host(0, 0, 0, M, N, K, 1.0, np.ascontiguousarray(A.ravel()), K,
+                   np.ascontiguousarray(B.ravel()), N, 0.0, D, N, 0.0)
+    assert np.all(np.isfinite(D)) and rel_fro(D, A @ B) < TOL
+
+
+def test_streamk_partition_invariance(skg, ctx, orc, rng):
+    """Every tile configuration and grid size (data-parallel, split tiles,
+    many peers per tile) agrees with the oracle."""
+    M, N, K = 300, 200, 1500
+    A = rng.uniform(-1, 1, (M, K)).ravel()
+    B = rng.uniform(-1, 1, (K, N)).ravel()
+    ref = np.zeros(M * N)
+    orc.dgemm(0, 0, 0, M, N, K, 1.0, A, K, B, N, 0.0, ref, N, 0.0, dd=True)
+    results = []
+    for tile in range(skg.num_tiles()):
+        for grid in (1, 2, 3, 7, 64, 0):
+            ctx.set_tuning(tile, grid)
+            D = np.zeros(M * N)
+            ctx.dgemm_host(0, 0, 0, M, N, K, 1.0, A, K, B, N, 0.0, D, N, 0.0)
+            err = rel_fro(D, ref)
+            assert err < TOL, (tile, grid, err)
+            results.append(D)
+    ctx.set_tuning(-1, 0)
+    for D in results[1:]:
+        assert rel_fro(D, results[0]) < 1e-14
+
+
+def test_repeated_calls_reuse_workspace(skg, ctx, rng):
+    """Fix-up flags are consumed and cleared by every launch: back-to-back
+    split-tile launches of different shapes stay correct."""
+    for it in range(6):
+        M, N, K = [(256, 256, 4096), (100, 60, 5000), (512, 128, 777)][it % 3]
+        A = rng.uniform(-1, 1, (M, K))
+        B = rng.uniform(-1, 1, (K, N))
+        ctx.set_tuning(1, 97)
+        D = np.zeros(M * N)
+        ctx.dgemm_host(0, 0, 0, M, N, K, 1.0, np.ascontiguousarray(A.ravel()),
+                       K, np.ascontiguousarray(B.ravel()), N, 0.0, D, N, 0.0)
+        assert rel_fro(D, A @ B) < TOL
+    ctx.set_tuning(-1, 0)
+
+
+@pytest.mark.parametrize("Nd,nb", [(64, 32), (256, 64), (512, 128),
+                                   (1024, 256), (2048, 128), (4096, 128)])
+def test_rotation_trp3d_cases(skg, ctx, orc, Nd, nb):
+    """trp_3d_test.cpp:237-238: grid-packed A, row-major Q, band-packed D."""
+    A, Q = orc.trp_test_inputs(Nd, nb)
+    ref = orc.trp_reference(Nd, nb, A, Q)
+    D = np.zeros(((nb + 15) // 16) * Nd * 16)
+    ctx.dgemm_host(skg.PACK16, skg.ROW_MAJOR, skg.PACK16, Nd, nb, nb, 1.0, A, nb,
+                   Q, nb, 0.0, D, Nd, 0.0)
+    assert rel_fro(D, ref) < TOL
+
+
+def test_config2_rotation_full_size(skg, ctx, orc):
+    """c2 rotation Psi (32768 x 520, normal(0, 1/sqrt(Nd))) * Q (520 x 520
+    uniform(-1,1)), column-major Psi.  Checked on 256 sampled rows against
+    the double-double oracle, and Q = I reproduces Psi bit for bit."""
+    Nd, Ns = 32768, 520
+    psi = skg.DeviceArray(ctx, Nd * Ns)
+    q = skg.DeviceArray(ctx, Ns * Ns)
+    out = skg.DeviceArray(ctx, Nd * Ns)
+    ctx.fill_normal(psi.ptr, Nd * Ns, 1, 1.0 / np.sqrt(Nd))
+    ctx.fill_uniform(q.ptr, Ns * Ns, 2, -1.0, 1.0)
+    ctx.set_tuning(-1, 0)
+    ctx.dgemm(skg.COL_MAJOR, skg.ROW_MAJOR, skg.COL_MAJOR, Nd, Ns, Ns, 1.0,
+              psi.ptr, Nd, q.ptr, Ns, 0.0, out.ptr, Nd, 0.0)
+    ctx.synchronize()
+    P = psi.download().reshape(Ns, Nd).T      # [g, b]
+    Q = q.download().reshape(Ns, Ns)
+    R = out.download().reshape(Ns, Nd).T
+    rows = np.random.default_rng(7).choice(Nd, 256, replace=False)
+    sub = np.ascontiguousarray(P[rows])
+    ref = np.zeros(256 * Ns)
+    orc.dgemm(0, 0, 0, 256, Ns, Ns, 1.0, sub.ravel(), Ns, Q.ravel().copy(), Ns,
+              0.0, ref, Ns, 0.0, dd=True)
+    assert rel_fro(R[rows], ref) < TOL
+    eye = skg.DeviceArray.from_host(ctx, np.eye(Ns).ravel())
+    ctx.dgemm(skg.COL_MAJOR, skg.ROW_MAJOR, skg.COL_MAJOR, Nd, Ns, Ns, 1.0,
+              psi.ptr, Nd, eye.ptr, Ns, 0.0, out.ptr, Nd, 0.0)
+    ctx.synchronize()
+    assert np.array_equal(out.download(), psi.download())
+
+
+def test_device_pack_matches_reference_layouts(skg, ctx, orc, rng):
+    """skgemm_pack: linalg_pack_c16_f64 and gemm::pack_A / pack_B layouts."""
+    Nd, nb = 96, 40
+    src = rng.uniform(-1, 1, Nd * nb)
+    d_src = dev(skg, ctx, src)
+    d_dst = skg.DeviceArray(ctx, (Nd // 16) * nb * 16, zero=True)
+    ctx.pack(skg.ROW_MAJOR, d_src.ptr, Nd, skg.PACK16, d_dst.ptr, nb, Nd, nb)
+    ctx.synchronize()
+    assert np.array_equal(d_dst.download(), orc.pack_c16(Nd, nb, src))
+    back = skg.DeviceArray(ctx, Nd * nb, zero=True)
+    ctx.pack(skg.PACK16, d_dst.ptr, nb, skg.ROW_MAJOR, back.ptr, Nd, Nd, nb)
+    ctx.synchronize()
+    assert np.array_equal(back.download(), src)
+    M, K = 48, 20
+    A = rng.uniform(-1, 1, M * K)                 # column-major M x K
+    dA = dev(skg, ctx, A)
+    dpa = skg.DeviceArray(ctx, M * K, zero=True)
+    # pack_A: element (r = k, o = m); source m + k*M is ROW_MAJOR (B conv.)
+    ctx.pack(skg.ROW_MAJOR, dA.ptr, M, skg.PACK16, dpa.ptr, K, M, K)
+    ctx.synchronize()
+    assert np.array_equal(dpa.download(), orc.pack_A(M, K, A))
+    N = 64
+    Bm = rng.uniform(-1, 1, K * N)                # row-major K x N
+    dB = dev(skg, ctx, Bm)
+    dpb = skg.DeviceArray(ctx, K * N, zero=True)
+    ctx.pack(skg.ROW_MAJOR, dB.ptr, N, skg.PACK32, dpb.ptr, K, N, K)
+    ctx.synchronize()
+    assert np.array_equal(dpb.download(), orc.pack_B(N, K, Bm))

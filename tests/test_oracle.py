@@ -1,0 +1,358 @@
+"""Pins the oracle (CPU restatement of the reference path) against the
+reference's own known-answer inputs, closed forms, exact arithmetic and
+invariants.  CPU only.
+
+Each check is chosen so that a plausible mistake in the oracle (a dropped
+term, a transposed operand, a wrong index, a wrong alpha/beta placement)
+fails it; test_oracle_mutations plants such mistakes and shows they are
+caught.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_fro
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+GEMM_CASES = [(32, 64, 128, 1.0, 0.0), (32, 64, 4096, 1.0, 0.0),
+              (32, 64, 4096, 2.0, 0.5), (64, 128, 2048, 1.0, 0.0)]
+SYRK_CASES = [(64, 32), (256, 64), (512, 96), (1024, 128), (2048, 256)]
+
+
+def unpack_p16(ap, K, M):
+    """P16 [ceil(M/16)][K][16] -> dense K x M."""
+    t = (M + 15) // 16
+    return ap.reshape(t, K, 16).transpose(1, 0, 2).reshape(K, t * 16)[:, :M]
+
+
+# ---- GEMM (gemm_crsp.cpp / gemm_opt.cpp) ----------------------------------
+
+@pytest.mark.parametrize("case", range(4))
+def test_gemm_kats_vs_golden_closed_form(orc, case):
+    """gemm_crsp.cpp:555-565 cases against the exact closed form."""
+    g = np.load(os.path.join(GOLDEN, "gemm_kat.npz"))
+    M, N, K = (int(x) for x in g["case%d_shape" % case])
+    alpha, gamma = (float(x) for x in g["case%d_coef" % case])
+    D_exact = g["case%d_D" % case].ravel()
+    A, B = orc.gemm_test_inputs(M, N, K)
+    pa, pb = orc.pack_A(M, K, A), orc.pack_B(N, K, B)
+    for nth in (1, 3, 8):
+        D = orc.gemm_crsp(M, N, K, pa, pb, alpha, gamma, nthreads=nth)
+        assert rel_fro(D, D_exact) < 1e-14
+    D = orc.gemm_opt(M, N, K, pa, pb, alpha, gamma, nthreads=4)
+    assert rel_fro(D, D_exact) < 1e-14
+    D = orc.gemm_naive(M, N, K, A, B, alpha, gamma)
+    assert rel_fro(D, D_exact) < 1e-14
+    # elementwise, as the reference checks (gemm_crsp.cpp:487-497)
+    assert np.max(np.abs(D - D_exact) / (np.abs(D_exact) + 1e-10)) < 1e-12
+
+
+def test_gemm_random_all_layouts_vs_numpy(orc, rng):
+    """General-layout checker vs numpy, all A/B/D layouts, ragged shape."""
+    M, N, K = 37, 45, 29
+    A = rng.uniform(-1, 1, (M, K))
+    B = rng.uniform(-1, 1, (K, N))
+    ref = A @ B
+    for la in range(4):
+        Ab, lda = encode_a(A, la)
+        for lb in range(4):
+            Bb, ldb = encode_b(B, lb)
+            for ld_ in range(4):
+                D, ldd = encode_d(np.zeros((M, N)), ld_)
+                orc.dgemm(la, lb, ld_, M, N, K, 1.0, Ab, lda, Bb, ldb, 0.0, D,
+                          ldd, 0.0)
+                assert rel_fro(decode_d(D, ld_, M, N, ldd), ref) < 1e-15
+
+
+def test_dd_exact_on_integers(orc, rng):
+    """Double-double accumulation is exact where the true result fits."""
+    M, N, K = 9, 11, 1000
+    A = rng.integers(-2**20, 2**20, (M, K)).astype(np.float64)
+    B = rng.integers(-2**20, 2**20, (K, N)).astype(np.float64)
+    exact = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float64)
+    D = np.zeros(M * N)
+    orc.dgemm(0, 0, 0, M, N, K, 1.0, np.ascontiguousarray(A.ravel()), K,
+              np.ascontiguousarray(B.ravel()), N, 0.0, D, N, 0.0, dd=True)
+    assert np.array_equal(D.reshape(M, N), exact)
+
+
+def test_dd_recovers_cancellation(orc):
+    """[1e16, 1, -1e16] . [1, 1, 1] = 1: lost by plain FP64, kept by dd."""
+    A = np.array([1e16, 1.0, -1e16])
+    B = np.ones(3)
+    D = np.zeros(1)
+    orc.dgemm(0, 0, 0, 1, 1, 3, 1.0, A, 3, B, 1, 0.0, D, 1, 0.0, dd=True)
+    assert D[0] == 1.0
+    D0 = np.zeros(1)
+    orc.dgemm(0, 0, 0, 1, 1, 3, 1.0, A, 3, B, 1, 0.0, D0, 1, 0.0, dd=False)
+    assert D0[0] == 0.0
+
+
+def test_gemm_beta_gamma_placement(orc, rng):
+    M, N, K = 8, 6, 5
+    A = rng.uniform(-1, 1, (M, K))
+    B = rng.uniform(-1, 1, (K, N))
+    C0 = rng.uniform(-1, 1, (M, N))
+    D = C0.copy().ravel()
+    orc.dgemm(0, 0, 0, M, N, K, 0.5, A.ravel().copy(), K, B.ravel().copy(), N,
+              -2.0, D, N, 0.25)
+    assert rel_fro(D, 0.5 * A @ B - 2.0 * C0 + 0.25) < 1e-15
+
+
+def test_crsp_partial_panels_untouched(orc):
+    """Quirk Q1: the crsp restatement covers 16*floor(M/16) x 32*floor(N/32)."""
+    M, N, K = 40, 70, 16
+    A, B = orc.gemm_test_inputs(M, N, K)
+    D = np.full(M * N, -7.0)
+    orc.gemm_crsp(M, N, K, orc.pack_A(M, K, A), orc.pack_B(N, K, B), 1.0, 0.0,
+                  nthreads=2, D=D)
+    D = D.reshape(M, N)
+    assert np.all(D[32:, :] == -7.0) and np.all(D[:, 64:] == -7.0)
+    assert np.all(D[:32, :64] != -7.0)
+
+
+# ---- SYRK (syrk_crsp.cpp / include/sme_syrk.hpp) ----------------------------
+
+def test_syrk_glibc_fixture(orc):
+    """The first reference SYRK case reproduces the committed glibc inputs
+    and the exact integer A^T A."""
+    g = np.load(os.path.join(GOLDEN, "syrk_glibc.npz"))
+    K, M, seed = (int(x) for x in g["shape"])
+    ap = orc.init_A_packed(K, M, seed)
+    assert np.array_equal(np.rint(ap * 100).astype(np.uint8), g["a_packed_int"])
+    exact = g["ata_int"] / 1e4
+    C = orc.syrk_reference(K, M, ap, 1.0, 0.0)
+    C = C.reshape(M, M).T  # column-major -> [m1, m2]
+    iu = np.triu_indices(M)
+    assert np.max(np.abs(C[iu] - exact[iu])) < 1e-12 * np.max(exact)
+
+
+@pytest.mark.parametrize("t", range(5))
+def test_syrk_reference_cases(orc, t):
+    """syrk_crsp.cpp:474-493: five cases, seeds 42..46, exact integer check
+    of syrk_reference and of the tile-pair restatement."""
+    K, M = SYRK_CASES[t]
+    ap = orc.init_A_packed(K, M, 42 + t)
+    ints = np.rint(unpack_p16(ap, K, M) * 100).astype(np.int64)
+    exact = (ints.T @ ints) / 1e4
+    iu = np.triu_indices(M)
+    for C in (orc.syrk_reference(K, M, ap, 1.0, 0.0),
+              orc.syrk_tiles(K, M, ap, 1.0, 0.0, nthreads=4)):
+        Cm = C.reshape(M, M).T
+        err = np.abs(Cm[iu] - exact[iu]) / np.maximum(1.0, np.abs(exact[iu]))
+        assert err.max() < 1e-13  # reference tolerance is 1e-9
+        assert np.all(np.tril(Cm, -1) == 0.0)  # lower never written
+
+
+def test_syrk_alpha_beta_and_ragged(orc, rng):
+    """alpha / beta on the whole product, M % 16 != 0 (the reference's store
+    overruns the matrix here, quirk Q2; the restatement clips)."""
+    K, M = 300, 37
+    A = rng.uniform(-1, 1, (K, M))
+    t = (M + 15) // 16
+    ap = np.zeros((t, K, 16))
+    for m in range(M):
+        ap[m // 16, :, m % 16] = A[:, m]
+    C0 = rng.uniform(-1, 1, (M, M))           # C0[m1, m2]
+    C = C0.ravel(order="F").copy()            # column-major storage
+    C = orc.syrk_tiles(K, M, ap.ravel(), 0.75, -1.5, C=C, nthreads=3)
+    Cm = C.reshape(M, M).T                    # back to [m1, m2]
+    iu = np.triu_indices(M)
+    ref = 0.75 * (A.T @ A) - 1.5 * C0
+    assert np.max(np.abs(Cm[iu] - ref[iu])) < 1e-13
+    il = np.tril_indices(M, -1)
+    assert np.array_equal(Cm[il], C0[il])     # strict lower untouched
+
+
+def test_syrk_orthonormal_columns_give_identity(orc, rng):
+    K, M = 4096, 48
+    Q, _ = np.linalg.qr(rng.standard_normal((K, M)))
+    ap = np.zeros((M // 16, K, 16))
+    for m in range(M):
+        ap[m // 16, :, m % 16] = Q[:, m]
+    C = orc.syrk_tiles(K, M, ap.ravel(), 1.0, 0.0).reshape(M, M).T
+    iu = np.triu_indices(M)
+    assert np.max(np.abs(C[iu] - np.eye(M)[iu])) < 1e-13
+
+
+def test_syrk_power_of_two_scaling_is_exact(orc, rng):
+    """Ms(A diag(2^e)) = diag(2^e) Ms(A) diag(2^e), bit for bit."""
+    K, M = 512, 32
+    A = rng.uniform(-1, 1, (K, M))
+    e = rng.integers(-20, 20, M)
+    As = A * np.ldexp(1.0, e)[None, :]
+
+    def packed(X):
+        ap = np.zeros((M // 16, K, 16))
+        for m in range(M):
+            ap[m // 16, :, m % 16] = X[:, m]
+        return ap.ravel()
+
+    C = orc.syrk_tiles(K, M, packed(A), 1.0, 0.0, nthreads=1).reshape(M, M).T
+    Cs = orc.syrk_tiles(K, M, packed(As), 1.0, 0.0, nthreads=1).reshape(M, M).T
+    scale = np.ldexp(1.0, e[:, None] + e[None, :])
+    iu = np.triu_indices(M)
+    assert np.array_equal(Cs[iu], (C * scale)[iu])
+
+
+def test_dsyrk_general_vs_numpy(orc, rng):
+    N, K = 41, 77
+    A = rng.uniform(-1, 1, (K, N))
+    ref = A.T @ A
+    il = np.tril_indices(N)
+    for la in range(4):
+        Ab, lda = encode_b(A, la)
+        C = np.full(N * N, 3.0)
+        orc.dsyrk(la, N, K, 1.0, Ab, lda, 0.0, C, N)
+        Cm = C.reshape(N, N)
+        assert rel_fro(Cm[il], ref[il]) < 1e-15
+        assert np.all(Cm[np.triu_indices(N, 1)] == 3.0)
+
+
+# ---- rotation (include/sme_trp_3d.hpp, trp_3d_test.cpp) -------------------
+
+@pytest.mark.parametrize("Nd,nb", [(64, 32), (256, 64), (512, 128)])
+def test_trp_closed_form(orc, Nd, nb):
+    """trp_3d_test.cpp:125-138 inputs; D[g,bp] = 1e-5 (G+P) S1 +
+    1e-6 G P nb + 1e-4 S2 with G = g+1, P = bp+1."""
+    A, B = orc.trp_test_inputs(Nd, nb)
+    S1 = nb * (nb + 1) / 2
+    S2 = nb * (nb + 1) * (2 * nb + 1) / 6
+    G = np.arange(1, Nd + 1)[:, None]
+    P = np.arange(1, nb + 1)[None, :]
+    Dx = 1e-5 * (G + P) * S1 + 1e-6 * G * P * nb + 1e-4 * S2
+    for D in (orc.trp_reference(Nd, nb, A, B), orc.trp_3d(Nd, nb, A, B, 3)):
+        Dd = D.reshape(nb // 16, Nd, 16).transpose(1, 0, 2).reshape(Nd, nb)
+        assert rel_fro(Dd, Dx) < 1e-14
+
+
+def test_trp_identity_and_permutation(orc, rng):
+    Nd, nb = 128, 48
+    X = rng.uniform(-1, 1, (Nd, nb))
+    Ag = X.reshape(Nd // 16, 16, nb).transpose(0, 2, 1).ravel().copy()
+    perm = rng.permutation(nb)
+    Q = np.eye(nb)[:, perm].ravel().copy()  # D = X[:, perm]
+    for Qm, expect in ((np.eye(nb).ravel().copy(), X), (Q, X[:, perm])):
+        D = orc.trp_3d(Nd, nb, Ag, Qm, 2)
+        Dd = D.reshape(nb // 16, Nd, 16).transpose(1, 0, 2).reshape(Nd, nb)
+        assert np.array_equal(Dd, expect)
+
+
+# ---- packing (pack.cpp / include/sme_pack.hpp) ------------------------------
+
+def test_pack_roundtrip_and_layout(orc, rng):
+    Nd, nb = 96, 40
+    src = rng.uniform(-1, 1, Nd * nb)  # column-major [Nd, nb]
+    p = orc.pack_c16(Nd, nb, src)
+    g, b = 37, 11
+    assert p[(g // 16) * nb * 16 + b * 16 + g % 16] == src[g + b * Nd]
+    assert np.array_equal(orc.unpack_c16(Nd, nb, p), src)
+    M, K = 48, 20
+    A = rng.uniform(-1, 1, M * K)
+    pa = orc.pack_A(M, K, A)
+    assert pa[(2 * K + 5) * 16 + 3] == A[(2 * 16 + 3) + 5 * M]
+    N = 64
+    B = rng.uniform(-1, 1, K * N)
+    pb = orc.pack_B(N, K, B)
+    assert pb[(1 * K + 7) * 32 + 9] == B[7 * N + 32 + 9]
+
+
+# ---- planted mistakes are caught -----------------------------------------------
+
+def test_oracle_mutations_are_detected(orc, rng):
+    """Transposing B, dropping the last k, or flipping gamma's sign in the
+    checker's output must fail the checks above."""
+    g = np.load(os.path.join(GOLDEN, "gemm_kat.npz"))
+    M, N, K = (int(x) for x in g["case2_shape"])
+    alpha, gamma = (float(x) for x in g["case2_coef"])
+    D_exact = g["case2_D"].ravel()
+    A, B = orc.gemm_test_inputs(M, N, K)
+    good = orc.gemm_naive(M, N, K, A, B, alpha, gamma)
+    assert rel_fro(good, D_exact) < 1e-14
+    dropped = orc.gemm_naive(M, N, K - 1, A, B[:(K - 1) * N].copy(), alpha, gamma)
+    assert rel_fro(dropped, D_exact) > 1e-6
+    flipped = orc.gemm_naive(M, N, K, A, B, alpha, -gamma)
+    assert rel_fro(flipped, D_exact) > 1e-6
+    # transposed operand on random data (the KAT closed form is symmetric)
+    X = rng.uniform(-1, 1, (16, 16))
+    Y = rng.uniform(-1, 1, (16, 16))
+    D = np.zeros(256)
+    orc.dgemm(0, 1, 0, 16, 16, 16, 1.0, X.ravel().copy(), 16, Y.ravel().copy(),
+              16, 0.0, D, 16, 0.0)  # B read column-major = Y^T
+    assert rel_fro(D, X @ Y) > 1e-3
+
+
+# ---- layout helpers (shared with the GPU tests) ----------------------------------
+
+def encode_a(A, layout, pad=3):
+    """Dense M x K -> flat buffer in `layout` (A convention) and its ld."""
+    M, K = A.shape
+    if layout == 0:
+        ld = K + pad
+        buf = np.zeros((M, ld)); buf[:, :K] = A
+        return buf.ravel(), ld
+    if layout == 1:
+        ld = M + pad
+        buf = np.zeros((K, ld)); buf[:, :M] = A.T
+        return buf.ravel(), ld
+    P = 16 if layout == 2 else 32
+    t = (M + P - 1) // P
+    ld = K + pad
+    buf = np.zeros((t, ld, P))
+    for m in range(M):
+        buf[m // P, :K, m % P] = A[m]
+    return buf.ravel(), ld
+
+
+def encode_b(B, layout, pad=3):
+    """Dense K x N -> flat buffer in `layout` (B convention) and its ld."""
+    K, N = B.shape
+    if layout == 0:
+        ld = N + pad
+        buf = np.zeros((K, ld)); buf[:, :N] = B
+        return buf.ravel(), ld
+    if layout == 1:
+        ld = K + pad
+        buf = np.zeros((N, ld)); buf[:, :K] = B.T
+        return buf.ravel(), ld
+    P = 16 if layout == 2 else 32
+    t = (N + P - 1) // P
+    ld = K + pad
+    buf = np.zeros((t, ld, P))
+    for n in range(N):
+        buf[n // P, :K, n % P] = B[:, n]
+    return buf.ravel(), ld
+
+
+def encode_d(D, layout, pad=2):
+    M, N = D.shape
+    if layout == 0:
+        ld = N + pad
+        buf = np.zeros((M, ld)); buf[:, :N] = D
+        return buf.ravel(), ld
+    if layout == 1:
+        ld = M + pad
+        buf = np.zeros((N, ld)); buf[:, :M] = D.T
+        return buf.ravel(), ld
+    P = 16 if layout == 2 else 32
+    t = (N + P - 1) // P
+    ld = M + pad
+    buf = np.zeros((t, ld, P))
+    for n in range(N):
+        buf[n // P, :M, n % P] = D[:, n]
+    return buf.ravel(), ld
+
+
+def decode_d(buf, layout, M, N, ld):
+    if layout == 0:
+        return buf.reshape(M, ld)[:, :N]
+    if layout == 1:
+        return buf.reshape(N, ld)[:, :M].T
+    P = 16 if layout == 2 else 32
+    t = (N + P - 1) // P
+    b = buf.reshape(t, ld, P)
+    out = np.empty((M, N))
+    for n in range(N):
+        out[:, n] = b[n // P, :M, n % P]
+    return out

@@ -1,0 +1,139 @@
+"""GPU parity of the FP64 Stream-K SYRK (lower-triangle tiles only) against
+the oracle through the C ABI.  Bar: 1e-13 relative Frobenius on the lower
+triangle; entries above the diagonal are never written."""
+import numpy as np
+import pytest
+
+from conftest import rel_fro
+from test_oracle import SYRK_CASES, encode_b, unpack_p16
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-13
+
+
+@pytest.mark.parametrize("t", range(5))
+def test_reference_cases_p16(skg, ctx, orc, t):
+    """syrk_crsp.cpp:474-493 through sme_syrk's layouts: P16 input, M x M
+    column-major upper triangle (== row-major lower) output, glibc inputs."""
+    K, M = SYRK_CASES[t]
+    ap = orc.init_A_packed(K, M, 42 + t)
+    ref = orc.syrk_reference(K, M, ap, 1.0, 0.0)      # column-major upper
+    C = np.full(M * M, 0.1)                            # syrk_crsp.cpp:499
+    ctx.set_tuning(-1, 0)
+    ctx.dsyrk_host(skg.PACK16, M, K, 1.0, ap, K, 0.0, C, M)
+    iu = np.triu_indices(M)
+    got = C.reshape(M, M).T[iu]                        # [m1, m2], m1 <= m2
+    want = ref.reshape(M, M).T[iu]
+    assert rel_fro(got, want) < TOL
+    err = np.abs(got - want) / np.maximum(1.0, np.abs(want))
+    assert err.max() < 1e-12                           # reference: 1e-9
+    assert np.all(C.reshape(M, M).T[np.tril_indices(M, -1)] == 0.1)
+
+
+@pytest.mark.parametrize("layout", [0, 1, 2, 3])
+def test_layouts_ragged_alpha_beta(skg, ctx, orc, rng, layout):
+    N, K = 137, 531
+    A = rng.uniform(-1, 1, (K, N))
+    Ab, lda = encode_b(A, layout)
+    C0 = rng.uniform(-1, 1, (N, N))
+    C = C0.copy().ravel()
+    ctx.dsyrk_host(layout, N, K, 0.5, Ab, lda, -1.25, C, N)
+    ref = 0.5 * (A.T @ A) - 1.25 * C0
+    il = np.tril_indices(N)
+    Cm = C.reshape(N, N)
+    assert rel_fro(Cm[il], ref[il]) < TOL
+    iu = np.triu_indices(N, 1)
+    assert np.array_equal(Cm[iu], C0[iu])
+
+
+@pytest.mark.parametrize("N,K", [(1, 1), (1, 1000), (2, 3), (16, 16),
+                                 (63, 65), (64, 1), (65, 64), (129, 4097),
+                                 (300, 17)])
+def test_edge_shapes(skg, ctx, orc, rng, N, K):
+    A = rng.uniform(-1, 1, (K, N))
+    ref = np.zeros(N * N)
+    orc.dsyrk(0, N, K, 1.0, np.ascontiguousarray(A.ravel()), N, 0.0, ref, N)
+    C = np.zeros(N * N)
+    ctx.dsyrk_host(0, N, K, 1.0, np.ascontiguousarray(A.ravel()), N, 0.0, C, N)
+    il = np.tril_indices(N)
+    assert rel_fro(C.reshape(N, N)[il], ref.reshape(N, N)[il]) < TOL
+
+
+def test_partition_invariance(skg, ctx, orc, rng):
+    N, K = 200, 3000
+    A = rng.uniform(-1, 1, (K, N)).ravel()
+    ref = np.zeros(N * N)
+    orc.dsyrk(0, N, K, 1.0, A, N, 0.0, ref, N, dd=True)
+    il = np.tril_indices(N)
+    for tile in range(skg.num_tiles()):
+        info = skg.tile_info(tile)
+        if info["bm"] != info["bn"]:
+            continue
+        for grid in (1, 2, 5, 33, 0):
+            ctx.set_tuning(tile, grid)
+            C = np.zeros(N * N)
+            ctx.dsyrk_host(0, N, K, 1.0, A, N, 0.0, C, N)
+            err = rel_fro(C.reshape(N, N)[il], ref.reshape(N, N)[il])
+            assert err < TOL, (tile, grid, err)
+    ctx.set_tuning(-1, 0)
+
+
+def test_config2_syrk_full_size(skg, ctx, orc):
+    """c2: Psi 32768 x 520 normal(0, 1/sqrt(Nd)) column-major.  Sampled rows
+    of Ms against the double-double oracle; diagonal against column norms;
+    agreement with the GEMM path Psi^T Psi."""
+    Nd, Ns = 32768, 520
+    psi = skg.DeviceArray(ctx, Nd * Ns)
+    ms = skg.DeviceArray(ctx, Ns * Ns, zero=True)
+    ctx.fill_normal(psi.ptr, Nd * Ns, 1, 1.0 / np.sqrt(Nd))
+    ctx.set_tuning(-1, 0)
+    ctx.dsyrk(skg.COL_MAJOR, Ns, Nd, 1.0, psi.ptr, Nd, 0.0, ms.ptr, Ns)
+    full = skg.DeviceArray(ctx, Ns * Ns, zero=True)
+    # GEMM path: A = Psi^T (ROW_MAJOR in A convention: m*ld + k with ld = Nd)
+    ctx.dgemm(skg.ROW_MAJOR, skg.COL_MAJOR, skg.ROW_MAJOR, Ns, Ns, Nd, 1.0,
+              psi.ptr, Nd, psi.ptr, Nd, 0.0, full.ptr, Ns, 0.0)
+    ctx.synchronize()
+    P = psi.download().reshape(Ns, Nd)                 # row b = column b
+    Ms = ms.download().reshape(Ns, Ns)
+    Mf = full.download().reshape(Ns, Ns)
+    rows = np.sort(np.random.default_rng(11).choice(Ns, 48, replace=False))
+    sub = np.ascontiguousarray(P[rows])                # 48 x Nd
+    ref = np.zeros(48 * Ns)
+    orc.dgemm(0, 1, 0, 48, Ns, Nd, 1.0, sub.ravel(), Nd, P.ravel().copy(), Nd,
+              0.0, ref, Ns, 0.0, dd=True)
+    ref = ref.reshape(48, Ns)
+    got = np.concatenate([Ms[r, :r + 1] for r in rows])
+    want = np.concatenate([ref[i, :r + 1] for i, r in enumerate(rows)])
+    assert rel_fro(got, want) < TOL
+    norms = np.einsum("ij,ij->i", P, P)
+    assert rel_fro(np.diag(Ms), norms) < 1e-14
+    il = np.tril_indices(Ns)
+    assert rel_fro(Ms[il], Mf[il]) < 1e-14
+
+
+def test_orthonormal_psi_gives_identity(skg, ctx, rng):
+    K, N = 20000, 96
+    Q, _ = np.linalg.qr(rng.standard_normal((K, N)))
+    Qc = np.asfortranarray(Q).ravel(order="F").copy()   # column-major
+    C = np.zeros(N * N)
+    ctx.dsyrk_host(skg.COL_MAJOR, N, K, 1.0, Qc, K, 0.0, C, N)
+    il = np.tril_indices(N)
+    assert np.max(np.abs(C.reshape(N, N)[il] - np.eye(N)[il])) < 1e-13
+
+
+def test_power_of_two_scaling_exact(skg, ctx, rng):
+    """Column scaling by 2^e scales Ms exactly (same partition, same
+    summation order)."""
+    K, N = 4096, 80
+    A = rng.uniform(-1, 1, (K, N))
+    e = rng.integers(-30, 30, N)
+    As = A * np.ldexp(1.0, e)[None, :]
+    ctx.set_tuning(1, 50)
+    C = np.zeros(N * N)
+    Cs = np.zeros(N * N)
+    ctx.dsyrk_host(0, N, K, 1.0, np.ascontiguousarray(A.ravel()), N, 0.0, C, N)
+    ctx.dsyrk_host(0, N, K, 1.0, np.ascontiguousarray(As.ravel()), N, 0.0, Cs, N)
+    ctx.set_tuning(-1, 0)
+    il = np.tril_indices(N)
+    scale = np.ldexp(1.0, e[:, None] + e[None, :])
+    assert np.array_equal(Cs.reshape(N, N)[il], (C.reshape(N, N) * scale)[il])
