@@ -214,6 +214,13 @@ int skgemm_batched(skgemm_ctx* ctx, const skgemm_problem* problems, int count,
  * autotuner's choice.  The reference's analogue is rebuilding with
  * -DMC/-DNC/-DKC (sub_gemm.sh:31-52). */
 int skgemm_set_tuning(skgemm_ctx* ctx, int tile, int grid);
+/* Force the number of K chunks (one unit = one tile over one chunk; chunk
+ * sums are reduced in chunk order); 0 restores the planner's choice, which
+ * uses one chunk per XCD when a problem has fewer tiles than CUs and a long
+ * K.  The reference's analogue is its split of K over threads
+ * (gemm_crsp.cpp:264-279). */
+int skgemm_set_kchunks(skgemm_ctx* ctx, int kchunks);
+
 /* Number of tile configurations and their shapes. */
 int skgemm_num_tiles(void);
 int skgemm_tile_info(int tile, int* bm, int* bn, int* bk, int* waves);

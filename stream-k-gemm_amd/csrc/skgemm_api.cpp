@@ -365,6 +365,23 @@ int skgemm_ctx::ensure_ws(size_t ws_doubles, int nflags) {
   return SKGEMM_OK;
 }
 
+int skgemm_ctx::ensure_part(size_t doubles) {
+  if (doubles > part_n) {
+    if (part) {
+      hipStreamSynchronize(stream);
+      hipFree(part);
+      part = nullptr;
+      part_n = 0;
+    }
+    hipError_t e = hipMalloc(&part, doubles * sizeof(double));
+    if (e != hipSuccess)
+      return set_error(SKGEMM_ERR_NOMEM, "K-chunk buffer of %zu doubles: %s",
+                       doubles, hipGetErrorString(e));
+    part_n = doubles;
+  }
+  return SKGEMM_OK;
+}
+
 double* skgemm_ctx::staging(int i, size_t bytes) {
   if (bytes > stage_n[i]) {
     if (stage[i]) {
@@ -386,6 +403,8 @@ void skgemm_ctx::free_all() {
   }
   tring.clear();
   if (ws) hipFree(ws);
+  if (part) hipFree(part);
+  part = nullptr;
   if (flags) hipFree(flags);
   if (status) hipFree(status);
   for (int i = 0; i < kStages; ++i)
@@ -436,88 +455,112 @@ int64_t tiles_for(int kind, int cfg, int M, int N) {
   return (int64_t)((M + t.bm - 1) / t.bm) * ((N + t.bn - 1) / t.bn);
 }
 
-// Default plan when the autotuner has no entry for this shape.
-void default_plan(skgemm_ctx* c, int kind, int M, int N, int K, int* cfg,
-                  int* grid) {
-  int best = 1;
-  if (kind == 1) {
-    best = (N >= 256) ? 0 : 1;
-  } else {
-    if (M >= 128 && N >= 128) {
-      const int64_t t128 = tiles_for(0, 0, M, N);
-      best = (t128 >= c->num_cus / 2) ? 0 : 1;
-    } else if (M >= 128) {
-      best = 2;
-    } else if (N >= 128) {
-      best = 3;
-    } else {
-      best = 1;
-    }
-  }
-  const TileCfg& t = tile_cfg(best);
-  const int64_t ipt = std::max<int64_t>(1, (K + t.bk - 1) / t.bk);
-  const int64_t total = tiles_for(kind, best, M, N) * ipt;
-  const int cap = c->capacity(best);
-  *cfg = best;
-  *grid = (int)std::max<int64_t>(1, std::min<int64_t>(cap, total));
+static int64_t iters_per_tile(int cfg, int K) {
+  const TileCfg& t = tile_cfg(cfg);
+  return std::max<int64_t>(1, (K + t.bk - 1) / t.bk);
 }
 
-void plan(skgemm_ctx* c, int kind, int M, int N, int K, int* cfg, int* grid) {
-  if (c->forced_cfg >= 0) {
-    *cfg = c->forced_cfg;
-    if (kind == 1 && tile_cfg(*cfg).bm != tile_cfg(*cfg).bn) *cfg = 0;
-  } else {
-    auto it = c->tuned.find(std::make_tuple(kind, M, N, K));
-    if (it != c->tuned.end()) {
-      *cfg = it->second.first;
-      *grid = it->second.second;
-    } else {
-      default_plan(c, kind, M, N, K, cfg, grid);
-    }
+// K chunks: with fewer tiles than CUs and a long K, one chunk per XCD so
+// that the workgroups of an XCD stream the same rows of the operands
+// through its L2 for all tiles; otherwise one chunk.
+static int default_kchunks(skgemm_ctx* c, int kind, int cfg, int M, int N,
+                           int K) {
+  const int64_t tiles = tiles_for(kind, cfg, M, N);
+  const int64_t ipt = iters_per_tile(cfg, K);
+  return (tiles < c->num_cus && ipt >= 64 * c->num_xcds) ? c->num_xcds : 1;
+}
+
+// Default tile when the autotuner has no entry for this shape.
+static int default_cfg(skgemm_ctx* c, int kind, int M, int N, int K) {
+  (void)K;
+  if (kind == 1) return (N >= 768) ? 4 : 1;
+  if (M >= 128 && N >= 128) {
+    const int64_t t128 = tiles_for(0, 0, M, N);
+    return (t128 >= 4 * c->num_cus) ? 4 : 1;
   }
-  const TileCfg& t = tile_cfg(*cfg);
-  const int64_t ipt = std::max<int64_t>(1, (K + t.bk - 1) / t.bk);
-  const int64_t total = tiles_for(kind, *cfg, M, N) * ipt;
-  const int cap = c->capacity(*cfg);
-  int g = (c->forced_grid > 0) ? c->forced_grid
-          : (c->forced_cfg >= 0 || c->tuned.find(std::make_tuple(kind, M, N, K)) == c->tuned.end())
-              ? cap
-              : *grid;
-  g = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)g, (int64_t)cap, total}));
-  *grid = g;
+  if (M >= 128) return 2;
+  if (N >= 128) return 3;
+  return 1;
+}
+
+static int clamp_grid(skgemm_ctx* c, int cfg, int g, int64_t total) {
+  const int cap = c->capacity(cfg);
+  g = std::min(g, cap);
+  if (g >= 16) g &= ~7;  // whole XCD rounds
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, total));
+}
+
+Plan plan(skgemm_ctx* c, int kind, int M, int N, int K) {
+  Plan pl{};
+  auto it = c->tuned.find(std::make_tuple(kind, M, N, K));
+  if (c->forced_cfg >= 0) {
+    pl.cfg = c->forced_cfg;
+    if (kind == 1 && tile_cfg(pl.cfg).bm != tile_cfg(pl.cfg).bn) pl.cfg = 1;
+    pl.grid = c->forced_grid > 0 ? c->forced_grid : c->capacity(pl.cfg);
+    pl.kchunks = c->forced_kchunks > 0
+                     ? c->forced_kchunks
+                     : default_kchunks(c, kind, pl.cfg, M, N, K);
+  } else if (it != c->tuned.end()) {
+    pl = it->second;
+  } else {
+    pl.cfg = default_cfg(c, kind, M, N, K);
+    pl.grid = c->capacity(pl.cfg);
+    pl.kchunks = default_kchunks(c, kind, pl.cfg, M, N, K);
+  }
+  if (c->forced_grid > 0) pl.grid = c->forced_grid;
+  if (c->forced_kchunks > 0) pl.kchunks = c->forced_kchunks;
+  const int64_t ipt = iters_per_tile(pl.cfg, K);
+  pl.kchunks = (int)std::max<int64_t>(1, std::min<int64_t>(pl.kchunks, ipt));
+  const int64_t kci = (ipt + pl.kchunks - 1) / pl.kchunks;
+  const int64_t total = tiles_for(kind, pl.cfg, M, N) * pl.kchunks * kci;
+  pl.grid = clamp_grid(c, pl.cfg, pl.grid, total);
+  return pl;
 }
 
 // Fill a Problem for the Stream-K kernel (tile counts depend on the cfg).
-void fill_problem(Problem* p, int kind, int cfg, int M, int N, int K) {
-  const TileCfg& t = tile_cfg(cfg);
+int fill_problem(skgemm_ctx* c, Problem* p, int kind, const Plan& pl, int M,
+                 int N, int K) {
+  const TileCfg& t = tile_cfg(pl.cfg);
   p->M = M;
   p->N = N;
   p->K = K;
   p->syrk = kind == 1;
   p->tiles_m = (M + t.bm - 1) / t.bm;
   p->tiles_n = (N + t.bn - 1) / t.bn;
-  p->iters_per_tile = (int)std::max<int64_t>(1, (K + t.bk - 1) / t.bk);
-  p->num_tiles = tiles_for(kind, cfg, M, N);
+  p->iters_per_tile = (int)iters_per_tile(pl.cfg, K);
+  // Tile order: keep the smaller operand cached and read the larger once.
+  p->row_order = (kind == 0 && (int64_t)M >= (int64_t)N) ? 1 : 0;
+  p->kchunks = pl.kchunks;
+  p->kci = (p->iters_per_tile + pl.kchunks - 1) / pl.kchunks;
+  p->num_tiles = tiles_for(kind, pl.cfg, M, N);
+  p->part = nullptr;
+  if (pl.kchunks > 1) {
+    SKG_TRY(c->ensure_part((size_t)pl.kchunks * M * N));
+    p->part = c->part;
+  }
+  return SKGEMM_OK;
 }
 
-// Launch one problem (already described) with the given cfg / grid.
-int run_single(skgemm_ctx* c, Problem& p, int cfg, int grid) {
-  const TileCfg& t = tile_cfg(cfg);
+// Launch one problem (already described) with the given plan.
+int run_single(skgemm_ctx* c, Problem& p, const Plan& pl) {
+  const TileCfg& t = tile_cfg(pl.cfg);
   p.tile_begin = 0;
   p.iter_begin = 0;
   Launch L{};
   L.problems = nullptr;
   L.single = p;
   L.num_problems = 1;
-  L.total_iters = p.num_tiles * p.iters_per_tile;
-  L.grid = grid;
-  SKG_TRY(c->ensure_ws((size_t)grid * t.bm * t.bn, grid));
+  L.total_iters = p.num_tiles * p.kchunks * p.kci;
+  L.grid = pl.grid;
+  SKG_TRY(c->ensure_ws((size_t)pl.grid * t.bm * t.bn, pl.grid));
   L.ws = c->ws;
   L.flags = c->flags;
   L.status = c->status;
   L.spin_limit = c->spin_limit;
   c->record_begin();
-  SKG_HIP(launch_dgemm_sk(cfg, L, c->stream), "Stream-K launch");
+  SKG_HIP(launch_dgemm_sk(pl.cfg, L, c->stream), "Stream-K launch");
+  if (p.kchunks > 1)
+    SKG_HIP(launch_chunk_reduce(p, c->stream), "chunk reduce launch");
   c->record_end();
   return SKGEMM_OK;
 }
@@ -573,13 +616,12 @@ extern "C" int skgemm_dgemm(skgemm_ctx* c, int la, int lb, int ld_, int M,
   SKG_TRY(prepare_gemm(la, lb, ld_, M, N, K, A, lda, B, ldb, D, ldd, &p));
   if (M == 0 || N == 0) return SKGEMM_OK;
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
-  int cfg, grid;
-  plan(c, 0, M, N, K, &cfg, &grid);
-  fill_problem(&p, 0, cfg, M, N, K);
+  const Plan pl = plan(c, 0, M, N, K);
+  SKG_TRY(fill_problem(c, &p, 0, pl, M, N, K));
   p.alpha = alpha;
   p.beta = beta;
   p.gamma = gamma;
-  return run_single(c, p, cfg, grid);
+  return run_single(c, p, pl);
 }
 
 extern "C" int skgemm_dsyrk(skgemm_ctx* c, int la, int N, int K, double alpha,
@@ -590,13 +632,12 @@ extern "C" int skgemm_dsyrk(skgemm_ctx* c, int la, int N, int K, double alpha,
   SKG_TRY(prepare_syrk(la, N, K, A, lda, C, ldc, &p));
   if (N == 0) return SKGEMM_OK;
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
-  int cfg, grid;
-  plan(c, 1, N, N, K, &cfg, &grid);
-  fill_problem(&p, 1, cfg, N, N, K);
+  const Plan pl = plan(c, 1, N, N, K);
+  SKG_TRY(fill_problem(c, &p, 1, pl, N, N, K));
   p.alpha = alpha;
   p.beta = beta;
   p.gamma = 0.0;
-  return run_single(c, p, cfg, grid);
+  return run_single(c, p, pl);
 }
 
 // ---- host-pointer wrappers ----------------------------------------------------
@@ -699,14 +740,22 @@ extern "C" int skgemm_query_plan(skgemm_ctx* c, int kind, int M, int N, int K,
   SKG_TRY(check_ctx(c));
   SKG_TRY(check_dims(M, N, K));
   if (kind != 0 && kind != 1) return set_error(SKGEMM_ERR_ARG, "kind must be 0 or 1");
-  int cfg, g;
-  plan(c, kind, M, N, K, &cfg, &g);
-  const TileCfg& t = tile_cfg(cfg);
-  const int64_t nt = tiles_for(kind, cfg, M, N);
-  if (tile) *tile = cfg;
-  if (grid) *grid = g;
+  const Plan pl = plan(c, kind, M, N, K);
+  const TileCfg& t = tile_cfg(pl.cfg);
+  const int64_t nt = tiles_for(kind, pl.cfg, M, N);
+  const int64_t ipt = std::max<int64_t>(1, (K + t.bk - 1) / t.bk);
+  const int64_t kci = (ipt + pl.kchunks - 1) / pl.kchunks;
+  if (tile) *tile = pl.cfg;
+  if (grid) *grid = pl.grid;
   if (tiles) *tiles = nt;
-  if (iters) *iters = nt * std::max<int64_t>(1, (K + t.bk - 1) / t.bk);
+  if (iters) *iters = nt * pl.kchunks * kci;
+  return SKGEMM_OK;
+}
+
+extern "C" int skgemm_set_kchunks(skgemm_ctx* c, int kchunks) {
+  SKG_TRY(check_ctx(c));
+  if (kchunks < 0) return set_error(SKGEMM_ERR_ARG, "kchunks < 0");
+  c->forced_kchunks = kchunks;
   return SKGEMM_OK;
 }
 

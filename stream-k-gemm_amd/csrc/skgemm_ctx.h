@@ -12,6 +12,15 @@
 
 #include "skgemm_internal.h"
 
+namespace skg {
+// Launch plan of one shape: tile configuration, persistent grid, K chunks.
+struct Plan {
+  int cfg;
+  int grid;
+  int kchunks;
+};
+}  // namespace skg
+
 struct skgemm_ctx {
   static constexpr int kStages = 4;
   int magic = 0;
@@ -34,6 +43,9 @@ struct skgemm_ctx {
   int flags_n = 0;
   int* status = nullptr;
   int64_t spin_limit = 1ll << 26;
+  // Per-chunk sums of K-chunked launches.
+  double* part = nullptr;
+  size_t part_n = 0;
 
   // Device staging for host-pointer calls and probes.
   void* stage[kStages] = {nullptr, nullptr, nullptr, nullptr};
@@ -43,7 +55,8 @@ struct skgemm_ctx {
   // (kind, M, N, K) -> (cfg, grid), occupancy per cfg.
   int forced_cfg = -1;
   int forced_grid = 0;
-  std::map<std::tuple<int, int, int, int>, std::pair<int, int>> tuned;
+  int forced_kchunks = 0;
+  std::map<std::tuple<int, int, int, int>, skg::Plan> tuned;
   std::map<int, int> occ_cache;
 
   // Split-form and batched state (skgemm_ozaki.hip / skgemm_batch.cpp).
@@ -52,6 +65,7 @@ struct skgemm_ctx {
 
   int check_status();
   int ensure_ws(size_t ws_doubles, int nflags);
+  int ensure_part(size_t doubles);
   double* staging(int i, size_t bytes);
   int capacity(int cfg);
   void free_all();
@@ -71,9 +85,10 @@ int64_t span_operand(int layout, bool a_convention, int64_t ld, int ext, int K);
 int64_t span_output(int layout, int64_t ld, int M, int N);
 
 int64_t tiles_for(int kind, int cfg, int M, int N);
-void plan(skgemm_ctx* c, int kind, int M, int N, int K, int* cfg, int* grid);
-void fill_problem(Problem* p, int kind, int cfg, int M, int N, int K);
-int run_single(skgemm_ctx* c, Problem& p, int cfg, int grid);
+Plan plan(skgemm_ctx* c, int kind, int M, int N, int K);
+int fill_problem(skgemm_ctx* c, Problem* p, int kind, const Plan& pl, int M,
+                 int N, int K);
+int run_single(skgemm_ctx* c, Problem& p, const Plan& pl);
 int prepare_gemm(int la, int lb, int ld_, int M, int N, int K, const double* A,
                  int64_t lda, const double* B, int64_t ldb, double* D,
                  int64_t ldd, Problem* p);

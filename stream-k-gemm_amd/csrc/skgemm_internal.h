@@ -43,6 +43,13 @@ struct Problem {
   double alpha, beta, gamma;
   int tiles_m, tiles_n;
   int iters_per_tile;  // ceil(K / BK), at least 1
+  int row_order;       // GEMM tile order: 1 = tj fastest (A panels read once)
+  // K is cut into kchunks chunks of kci BK-steps; a unit is (chunk, tile).
+  // With kchunks > 1 each unit writes its raw sum to part[c] (row-major
+  // M x N, ld = N) and a reduce pass forms D.
+  int kchunks;
+  int kci;
+  double* part;
   int64_t tile_begin;  // first flattened tile index of this problem
   int64_t iter_begin;  // first flattened MAC-iteration index
   int64_t num_tiles;
@@ -70,6 +77,9 @@ const TileCfg& tile_cfg(int i);
 
 // Launch the FP64 Stream-K kernel with tile configuration `cfg`.
 hipError_t launch_dgemm_sk(int cfg, const Launch& L, hipStream_t s);
+// D = alpha * sum_c part[c] + beta * D + gamma for a problem whose units
+// were written to part (kchunks > 1).
+hipError_t launch_chunk_reduce(const Problem& p, hipStream_t s);
 // Workgroups of configuration `cfg` that fit on one CU at once.
 int occupancy_dgemm_sk(int cfg);
 // LDS bytes one workgroup of `cfg` uses.

@@ -40,6 +40,9 @@ static const TileCfg kCfgs[] = {
     {64, 128, 16, 2, 2},   // 3: 4 waves, 32x64 per wave
     {128, 128, 16, 4, 2},  // 4: 8 waves, 32x64 per wave
     {256, 128, 16, 4, 2},  // 5: 8 waves, 64x64 per wave
+    {64, 64, 32, 2, 2},    // 6: as 1 with BK = 32 (half the barriers)
+    {128, 64, 32, 4, 2},   // 7: 8 waves, 32x32 per wave, BK = 32
+    {128, 128, 32, 4, 2},  // 8: 8 waves, 32x64 per wave, BK = 32
 };
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
@@ -82,14 +85,36 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const double* base) 
                                            kNumRecords, kRsrcWord3);
 }
 
+// LDS image of one BO x BK operand tile.  An operand contiguous along o is
+// kept as [k][BO + kPad]; one contiguous along k (KC) as [o][BK + 1], so
+// that global rows land in LDS unchanged and the odd pitch spreads the
+// MFMA reads (16 consecutive o at fixed k) over distinct banks.  Both forms
+// fit in kRegion doubles, so buffer offsets do not depend on the layout.
+template <int BO, int BK, int KC>
+struct Lds {
+  static constexpr int kPitchK = BK + 1;
+  static constexpr int kPitchO = BO + kPad;
+  static constexpr int kRegion = (BO * kPitchK > BK * kPitchO) ? BO * kPitchK
+                                                                : BK * kPitchO;
+  __device__ __forceinline__ static int idx(int o, int k) {
+    return KC ? o * kPitchK + k : k * kPitchO + o;
+  }
+};
+template <int BO, int BK>
+constexpr int lds_region() {
+  return Lds<BO, BK, 0>::kRegion > Lds<BO, BK, 1>::kRegion
+             ? Lds<BO, BK, 0>::kRegion
+             : Lds<BO, BK, 1>::kRegion;
+}
+
 // Global -> register -> LDS staging of one BO x BK operand tile.  Elements
 // move in pairs along the operand's contiguous index (k when KC, else o), one
-// 16-byte load per pair when VEC.  LDS holds the tile as [k][BO + kPad].
+// 16-byte load per pair when VEC.
 template <int BO, int BK, int NT, int KC, int VEC>
 struct Stage {
   static constexpr int kPairs = BO * BK / 2 / NT;
   static_assert(BO * BK / 2 % NT == 0, "tile not divisible over threads");
-  static constexpr int kPitch = BO + kPad;
+  using L = Lds<BO, BK, KC>;
   int rel[kPairs];  // byte offset of pair p from the tile's (o0, k0) element
   double x[2 * kPairs];
 
@@ -150,16 +175,18 @@ struct Stage {
     }
   }
 
+  // The pair is adjacent in LDS in both layouts.
   __device__ __forceinline__ void store(double* lds, int tid) {
 #pragma unroll
     for (int p = 0; p < kPairs; ++p) {
       int o, k;
       coords(p, tid, o, k);
       if (KC) {
-        lds[k * kPitch + o] = x[2 * p];
-        lds[(k + 1) * kPitch + o] = x[2 * p + 1];
+        double* d = lds + L::idx(o, k);
+        d[0] = x[2 * p];
+        d[1] = x[2 * p + 1];
       } else {
-        *reinterpret_cast<double2*>(lds + k * kPitch + o) =
+        *reinterpret_cast<double2*>(lds + L::idx(o, k)) =
             make_double2(x[2 * p], x[2 * p + 1]);
       }
     }
@@ -180,26 +207,29 @@ struct TileShape {
   static constexpr int NT = WM * WN * 64;
   static constexpr int TM = BM / WM, TN = BN / WN;
   static constexpr int MB = TM / 16, NB = TN / 16;
-  static constexpr int kPitchA = BM + kPad, kPitchB = BN + kPad;
-  static constexpr int kBufDoubles = BK * (kPitchA + kPitchB);
+  static constexpr int kRegionA = lds_region<BM, BK>();
+  static constexpr int kRegionB = lds_region<BN, BK>();
+  static constexpr int kBufDoubles = kRegionA + kRegionB;
   static_assert(TM % 16 == 0 && TN % 16 == 0 && BK % 4 == 0, "bad tile");
 };
 
 // MFMAs of one staged BK step from LDS buffer `la` (A) / `lb` (B).
-template <int BM, int BN, int BK, int WM, int WN>
-__device__ __forceinline__ void mma_step(
-    v4d (&acc)[BM / WM / 16][BN / WN / 16],
-    const double* la, const double* lb, int wm, int wn, int lq, int lr) {
+template <int BM, int BN, int BK, int WM, int WN, int AKC, int BKC>
+__device__ __forceinline__ void mma_step(v4d (&acc)[BM / WM / 16][BN / WN / 16],
+                                         const double* la, const double* lb,
+                                         int wm, int wn, int lq, int lr) {
   using S = TileShape<BM, BN, BK, WM, WN>;
+  using LA = Lds<BM, BK, AKC>;
+  using LB = Lds<BN, BK, BKC>;
 #pragma unroll
   for (int s = 0; s < BK / 4; ++s) {
     double av[S::MB], bv[S::NB];
 #pragma unroll
     for (int i = 0; i < S::MB; ++i)
-      av[i] = la[(s * 4 + lq) * S::kPitchA + wm * S::TM + i * 16 + lr];
+      av[i] = la[LA::idx(wm * S::TM + i * 16 + lr, s * 4 + lq)];
 #pragma unroll
     for (int j = 0; j < S::NB; ++j)
-      bv[j] = lb[(s * 4 + lq) * S::kPitchB + wn * S::TN + j * 16 + lr];
+      bv[j] = lb[LB::idx(wn * S::TN + j * 16 + lr, s * 4 + lq)];
 #pragma unroll
     for (int i = 0; i < S::MB; ++i)
 #pragma unroll
@@ -209,48 +239,90 @@ __device__ __forceinline__ void mma_step(
   }
 }
 
-// The MAC loop over BK steps [kb, ke) of one tile, for one layout variant:
-// AKC / BKC = operand contiguous along k, VEC = 16-byte pair loads.
-// Loads of step i+1 are issued before the MFMAs of step i; the last step is
-// peeled so the loop body has no branches.
+// The MAC loop over BK steps [kb, ke) of one unit (steps are counted from
+// the unit's first K step kbase), for one layout variant: AKC / BKC = operand
+// contiguous along k, VEC = 16-byte pair loads.  Global loads run two steps
+// ahead through two register sets, so a load has two MFMA phases to land
+// before its data is written to LDS; LDS is double-buffered with one barrier
+// per step.  Loads past the unit's range are issued unconditionally (beyond
+// K they are masked and never touch memory) to keep the loop branch-free.
 template <int BM, int BN, int BK, int WM, int WN, int AKC, int BKC, int VEC>
 __device__ __forceinline__ void mac_loop(
-    v4d (&acc)[BM / WM / 16][BN / WN / 16],
-    const Operand& A, const Operand& B, int K, int m0, int n0, int kb, int ke,
+    v4d (&acc)[BM / WM / 16][BN / WN / 16], const Operand& A,
+    const Operand& B, int K, int m0, int n0, int kbase, int kb, int ke,
     double* lds, int tid) {
   using S = TileShape<BM, BN, BK, WM, WN>;
   constexpr int NT = S::NT;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
   const int lq = lane >> 4, lr = lane & 15;
-  Stage<BM, BK, NT, AKC, VEC> sa;
-  Stage<BN, BK, NT, BKC, VEC> sb;
-  sa.init(A, tid);
-  sb.init(B, tid);
+  using SA = Stage<BM, BK, NT, AKC, VEC>;
+  using SB = Stage<BN, BK, NT, BKC, VEC>;
+  SA sa0, sa1;
+  SB sb0, sb1;
+  sa0.init(A, tid);
+  sb0.init(B, tid);
+#pragma unroll
+  for (int p = 0; p < SA::kPairs; ++p) sa1.rel[p] = sa0.rel[p];
+#pragma unroll
+  for (int p = 0; p < SB::kPairs; ++p) sb1.rel[p] = sb0.rel[p];
   const double* baseA = A.ptr + op_addr(A, m0, 0);
   const double* baseB = B.ptr + op_addr(B, n0, 0);
   const int64_t stepA = (int64_t)BK * A.ks, stepB = (int64_t)BK * B.ks;
   const int olimA = A.extent - m0, olimB = B.extent - n0;
+  auto ldA = [&](SA& st, int step) {
+    st.load(make_rsrc(baseA + (int64_t)step * stepA), olimA, K - step * BK, tid);
+  };
+  auto ldB = [&](SB& st, int step) {
+    st.load(make_rsrc(baseB + (int64_t)step * stepB), olimB, K - step * BK, tid);
+  };
 
-  sa.load(make_rsrc(baseA + kb * stepA), olimA, K - kb * BK, tid);
-  sb.load(make_rsrc(baseB + kb * stepB), olimB, K - kb * BK, tid);
-  sa.store(lds, tid);
-  sb.store(lds + BK * S::kPitchA, tid);
+  int kk = kbase + kb;
+  const int kend = kbase + ke;
+  ldA(sa0, kk);
+  ldB(sb0, kk);
+  ldA(sa1, kk + 1);
+  ldB(sb1, kk + 1);
+  sa0.store(lds, tid);
+  sb0.store(lds + S::kRegionA, tid);
   __syncthreads();
   int cur = 0;
-  for (int kk = kb; kk + 1 < ke; ++kk) {
-    sa.load(make_rsrc(baseA + (kk + 1) * stepA), olimA, K - (kk + 1) * BK, tid);
-    sb.load(make_rsrc(baseB + (kk + 1) * stepB), olimB, K - (kk + 1) * BK, tid);
-    const double* la = lds + cur * S::kBufDoubles;
-    mma_step<BM, BN, BK, WM, WN>(acc, la, la + BK * S::kPitchA, wm, wn, lq, lr);
-    double* na = lds + (cur ^ 1) * S::kBufDoubles;
-    sa.store(na, tid);
-    sb.store(na + BK * S::kPitchA, tid);
+  for (;;) {
+    // LDS[cur] holds step kk, set 1 holds step kk + 1.
+    ldA(sa0, kk + 2);
+    ldB(sb0, kk + 2);
+    {
+      const double* la = lds + cur * S::kBufDoubles;
+      mma_step<BM, BN, BK, WM, WN, AKC, BKC>(acc, la, la + S::kRegionA, wm,
+                                              wn, lq, lr);
+    }
+    if (kk + 1 >= kend) break;
+    {
+      double* na = lds + (cur ^ 1) * S::kBufDoubles;
+      sa1.store(na, tid);
+      sb1.store(na + S::kRegionA, tid);
+    }
     __syncthreads();
     cur ^= 1;
+    ++kk;
+    // LDS[cur] holds step kk, set 0 holds step kk + 1.
+    ldA(sa1, kk + 2);
+    ldB(sb1, kk + 2);
+    {
+      const double* la = lds + cur * S::kBufDoubles;
+      mma_step<BM, BN, BK, WM, WN, AKC, BKC>(acc, la, la + S::kRegionA, wm,
+                                              wn, lq, lr);
+    }
+    if (kk + 1 >= kend) break;
+    {
+      double* na = lds + (cur ^ 1) * S::kBufDoubles;
+      sa0.store(na, tid);
+      sb0.store(na + S::kRegionA, tid);
+    }
+    __syncthreads();
+    cur ^= 1;
+    ++kk;
   }
-  const double* la = lds + cur * S::kBufDoubles;
-  mma_step<BM, BN, BK, WM, WN>(acc, la, la + BK * S::kPitchA, wm, wn, lq, lr);
   __syncthreads();
 }
 
@@ -301,22 +373,34 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
     const Operand A = BATCHED ? load_uniform(&tab[pi].a) : L.single.a;
     const Operand B = BATCHED ? load_uniform(&tab[pi].b) : L.single.b;
     const int K = BATCHED ? load_uniform(&tab[pi].K) : L.single.K;
-    const int ipt = BATCHED ? load_uniform(&tab[pi].iters_per_tile)
-                            : L.single.iters_per_tile;
+    const int kci = BATCHED ? load_uniform(&tab[pi].kci) : L.single.kci;
     const int64_t ib =
         BATCHED ? load_uniform(&tab[pi].iter_begin) : L.single.iter_begin;
     const int syrk = BATCHED ? load_uniform(&tab[pi].syrk) : L.single.syrk;
     const int tiles_m =
         BATCHED ? load_uniform(&tab[pi].tiles_m) : L.single.tiles_m;
+    const int tiles_n =
+        BATCHED ? load_uniform(&tab[pi].tiles_n) : L.single.tiles_n;
+    const int64_t ntiles =
+        BATCHED ? load_uniform(&tab[pi].num_tiles) : L.single.num_tiles;
+    const int row_order =
+        BATCHED ? load_uniform(&tab[pi].row_order) : L.single.row_order;
 
-    const int64_t tile = (it - ib) / ipt;
-    const int64_t t0 = ib + tile * ipt;
+    // Unit = (K chunk c, tile); its kci steps are contiguous in the
+    // flattened iteration space.
+    const int64_t unit = (it - ib) / kci;
+    const int64_t t0 = ib + unit * kci;
+    const int c = (int)(unit / ntiles);
+    const int64_t tile = unit - (int64_t)c * ntiles;
     const int kb = (int)(it - t0);
-    const int ke = (int)((end - t0) < ipt ? (end - t0) : ipt);
+    const int ke = (int)((end - t0) < kci ? (end - t0) : kci);
 
     int ti, tj;
     if (syrk) {
       tri_tile(tile, ti, tj);
+    } else if (row_order) {
+      ti = (int)(tile / tiles_n);
+      tj = (int)(tile % tiles_n);
     } else {
       ti = (int)(tile % tiles_m);
       tj = (int)(tile / tiles_m);
@@ -335,7 +419,7 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
 #define SKG_MAC(V)                                                          \
   case V:                                                                   \
     mac_loop<BM, BN, BK, WM, WN, (V >> 2) & 1, (V >> 1) & 1, V & 1>(        \
-        acc, A, B, K, m0, n0, kb, ke, lds, tid);                            \
+        acc, A, B, K, m0, n0, c * kci, kb, ke, lds, tid);                   \
     break;
     switch (variant) {
       SKG_MAC(0) SKG_MAC(1) SKG_MAC(2) SKG_MAC(3)
@@ -344,6 +428,14 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
 #undef SKG_MAC
 
     // ---- Stream-K fix-up -------------------------------------------------
+    // Partials and flags move through device-coherent (sc1) accesses:
+    // relaxed agent-scope atomics, which bypass the XCD-private L2 state.
+    // No agent-scope fence is used on this path: on this multi-XCD part a
+    // release fence writes back the whole L2 and an acquire invalidates it,
+    // which would evict the operand panels of every workgroup on the XCD.
+    // Order: a spiller's stores complete (s_waitcnt 0) before the barrier
+    // and the flag store; the owner's data loads issue after the barrier
+    // that follows its observation of the flag.
     bool write_out = true;
     if (kb != 0) {
       // Not the owner: spill the partial to this workgroup's slot.
@@ -354,21 +446,23 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
         for (int j = 0; j < NB; ++j)
 #pragma unroll
           for (int v = 0; v < 4; ++v)
-            slot[((i * NB + j) * 4 + v) * NT + tid] = acc[i][j][v];
-      __threadfence();
+            __hip_atomic_store(slot + ((i * NB + j) * 4 + v) * NT + tid,
+                               acc[i][j][v], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+      __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
       if (tid == 0)
-        __hip_atomic_store(&L.flags[w], 1, __ATOMIC_RELEASE,
+        __hip_atomic_store(&L.flags[w], 1, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
       write_out = false;
-    } else if (ke < ipt) {
-      // Owner of a split tile: add the peers' partials in peer order.
-      const int64_t tile_end = t0 + ipt;
+    } else if (ke < kci) {
+      // Owner of a split unit: add the peers' partials in peer order.
+      const int64_t tile_end = t0 + kci;
       for (int w2 = w + 1; w2 < G && wg_start(w2, L.total_iters, G) < tile_end;
            ++w2) {
         if (tid == 0) {
           int64_t n = 0;
-          while (__hip_atomic_load(&L.flags[w2], __ATOMIC_ACQUIRE,
+          while (__hip_atomic_load(&L.flags[w2], __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT) == 0) {
             __builtin_amdgcn_s_sleep(8);
             if (++n > L.spin_limit) {
@@ -380,16 +474,16 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
                              __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
-        __threadfence();
-        const gdouble* slot = (const gdouble*)(L.ws) + (int64_t)w2 * (BM * BN);
+        gdouble* slot = (gdouble*)(L.ws) + (int64_t)w2 * (BM * BN);
 #pragma unroll
         for (int i = 0; i < MB; ++i)
 #pragma unroll
           for (int j = 0; j < NB; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v)
-              acc[i][j][v] += __builtin_nontemporal_load(
-                  slot + ((i * NB + j) * 4 + v) * NT + tid);
+              acc[i][j][v] += __hip_atomic_load(
+                  slot + ((i * NB + j) * 4 + v) * NT + tid, __ATOMIC_RELAXED,
+                  __HIP_MEMORY_SCOPE_AGENT);
       }
     }
 
@@ -403,6 +497,29 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
           BATCHED ? load_uniform(&tab[pi].gamma) : L.single.gamma;
       const int M = BATCHED ? load_uniform(&tab[pi].M) : L.single.M;
       const int N = BATCHED ? load_uniform(&tab[pi].N) : L.single.N;
+      const int kchunks =
+          BATCHED ? load_uniform(&tab[pi].kchunks) : L.single.kchunks;
+      if (kchunks > 1) {
+        // Raw unit sum to part[c] (row-major, ld = N); reduced later.
+        double* part = BATCHED ? load_uniform(&tab[pi].part) : L.single.part;
+        gdouble* pb = (gdouble*)(part) + (int64_t)c * M * N;
+#pragma unroll
+        for (int i = 0; i < MB; ++i) {
+          const int row0 = m0 + wm * TM + i * 16 + lq;
+#pragma unroll
+          for (int j = 0; j < NB; ++j) {
+            const int col = n0 + wn * TN + j * 16 + lr;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int row = row0 + 4 * v;
+              if (row < M && col < N && (!syrk || row >= col))
+                pb[(int64_t)row * N + col] = acc[i][j][v];
+            }
+          }
+        }
+        it = t0 + ke;
+        continue;
+      }
       gdouble* dbase = (gdouble*)(D.ptr);
       const int64_t rstep = 4 * D.ms;  // element v -> v + 1 moves 4 rows
 #pragma unroll
@@ -430,7 +547,7 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
 
 template <int BM, int BN, int BK, int WM, int WN>
 static size_t lds_bytes_t() {
-  return 2ull * BK * ((BM + kPad) + (BN + kPad)) * sizeof(double);
+  return 2ull * TileShape<BM, BN, BK, WM, WN>::kBufDoubles * sizeof(double);
 }
 
 template <int BM, int BN, int BK, int WM, int WN>
@@ -492,6 +609,9 @@ static int occupancy_t() {
     case 3: return FN<64, 128, 16, 2, 2>(__VA_ARGS__);  \
     case 4: return FN<128, 128, 16, 4, 2>(__VA_ARGS__); \
     case 5: return FN<256, 128, 16, 4, 2>(__VA_ARGS__); \
+    case 6: return FN<64, 64, 32, 2, 2>(__VA_ARGS__);   \
+    case 7: return FN<128, 64, 32, 4, 2>(__VA_ARGS__);  \
+    case 8: return FN<128, 128, 32, 4, 2>(__VA_ARGS__); \
     default: break;                                   \
   }
 
@@ -506,6 +626,30 @@ int occupancy_dgemm_sk(int cfg) {
 size_t lds_bytes_dgemm_sk(int cfg) {
   SKG_CFG_SWITCH(cfg, lds_bytes_t);
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// K-chunk reduction: D = alpha * sum_c part[c] + beta * D + gamma, chunks
+// added in order c = 0, 1, ... (deterministic).
+// ---------------------------------------------------------------------------
+__global__ void chunk_reduce_kernel(Problem p) {
+  const int col = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int row = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (row >= p.M || col >= p.N || (p.syrk && row < col)) return;
+  const int64_t mn = (int64_t)p.M * p.N;
+  const gdouble* src = (const gdouble*)(p.part) + (int64_t)row * p.N + col;
+  double sum = 0.0;
+  for (int c = 0; c < p.kchunks; ++c) sum += src[c * mn];
+  gdouble* dst = (gdouble*)(p.d.ptr) + out_addr(p.d, row, col);
+  double val = p.alpha * sum;
+  if (p.beta != 0.0) val = fma(p.beta, *dst, val);
+  *dst = val + p.gamma;
+}
+
+hipError_t launch_chunk_reduce(const Problem& p, hipStream_t s) {
+  dim3 grid((p.N + 63) / 64, (p.M + 3) / 4);
+  hipLaunchKernelGGL(chunk_reduce_kernel, grid, dim3(256), 0, s, p);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

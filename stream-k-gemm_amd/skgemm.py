@@ -28,7 +28,8 @@ EXPORTS = [
     "skgemm_memcpy_h2d", "skgemm_memcpy_d2h", "skgemm_dgemm",
     "skgemm_dgemm_host", "skgemm_dsyrk", "skgemm_dsyrk_host",
     "skgemm_dgemm_ozaki", "skgemm_dsyrk_ozaki", "skgemm_pack",
-    "skgemm_batched", "skgemm_set_tuning", "skgemm_num_tiles",
+    "skgemm_batched", "skgemm_set_tuning", "skgemm_set_kchunks",
+    "skgemm_num_tiles",
     "skgemm_tile_info", "skgemm_autotune", "skgemm_load_tuning",
     "skgemm_query_plan", "skgemm_timing_enable", "skgemm_timing_read",
     "skgemm_last_kernel_ms", "skgemm_peak_probe",
@@ -105,6 +106,7 @@ def lib():
                                 c_int]),
         "skgemm_batched": (c_int, [P, P, c_int, c_int, c_int, c_int]),
         "skgemm_set_tuning": (c_int, [P, c_int, c_int]),
+        "skgemm_set_kchunks": (c_int, [P, c_int]),
         "skgemm_num_tiles": (c_int, []),
         "skgemm_tile_info": (c_int, [c_int, P, P, P, P]),
         "skgemm_autotune": (c_int, [P, c_int, c_int, c_int, c_int,
@@ -204,6 +206,9 @@ class Context:
 
     def set_tuning(self, tile=-1, grid=0):
         check(lib().skgemm_set_tuning(self.h, tile, grid))
+
+    def set_kchunks(self, n=0):
+        check(lib().skgemm_set_kchunks(self.h, n))
 
     def query_plan(self, kind, M, N, K):
         t, g = c_int(), c_int()

@@ -125,13 +125,18 @@ def test_alpha_beta_gamma(skg, ctx, orc, rng):
     A = rng.uniform(-1, 1, (M, K))
     B = rng.uniform(-1, 1, (K, N))
     C0 = rng.uniform(-1, 1, (M, N))
-    for alpha, beta, gamma in [(2.0, 0.0, 0.5), (-1.0, 1.0, 0.0),
-                               (0.5, -2.0, 0.25), (0.0, 3.0, 1.0)]:
-        D = C0.copy().ravel()
-        ctx.dgemm_host(0, 0, 0, M, N, K, alpha, np.ascontiguousarray(A.ravel()),
-                       K, np.ascontiguousarray(B.ravel()), N, beta, D, N, gamma)
-        ref = alpha * (A @ B) + beta * C0 + gamma
-        assert rel_fro(D, ref) < TOL
+    for kchunks in (1, 4):
+        ctx.set_kchunks(kchunks)
+        for alpha, beta, gamma in [(2.0, 0.0, 0.5), (-1.0, 1.0, 0.0),
+                                   (0.5, -2.0, 0.25), (0.0, 3.0, 1.0)]:
+            D = C0.copy().ravel()
+            ctx.dgemm_host(0, 0, 0, M, N, K, alpha,
+                           np.ascontiguousarray(A.ravel()), K,
+                           np.ascontiguousarray(B.ravel()), N, beta, D, N,
+                           gamma)
+            ref = alpha * (A @ B) + beta * C0 + gamma
+            assert rel_fro(D, ref) < TOL
+    ctx.set_kchunks(0)
 
 
 def test_beta_zero_never_reads_d(skg, ctx, rng):
@@ -154,14 +159,17 @@ def test_streamk_partition_invariance(skg, ctx, orc, rng):
     orc.dgemm(0, 0, 0, M, N, K, 1.0, A, K, B, N, 0.0, ref, N, 0.0, dd=True)
     results = []
     for tile in range(skg.num_tiles()):
-        for grid in (1, 2, 3, 7, 64, 0):
+        for grid, kchunks in ((1, 1), (2, 1), (3, 2), (7, 1), (64, 3), (0, 1),
+                              (0, 8), (16, 8)):
             ctx.set_tuning(tile, grid)
+            ctx.set_kchunks(kchunks)
             D = np.zeros(M * N)
             ctx.dgemm_host(0, 0, 0, M, N, K, 1.0, A, K, B, N, 0.0, D, N, 0.0)
             err = rel_fro(D, ref)
-            assert err < TOL, (tile, grid, err)
+            assert err < TOL, (tile, grid, kchunks, err)
             results.append(D)
     ctx.set_tuning(-1, 0)
+    ctx.set_kchunks(0)
     for D in results[1:]:
         assert rel_fro(D, results[0]) < 1e-14
 

@@ -37,7 +37,9 @@ def test_layouts_ragged_alpha_beta(skg, ctx, orc, rng, layout):
     Ab, lda = encode_b(A, layout)
     C0 = rng.uniform(-1, 1, (N, N))
     C = C0.copy().ravel()
+    ctx.set_kchunks(1 + layout)  # K chunking with beta != 0 and a mask
     ctx.dsyrk_host(layout, N, K, 0.5, Ab, lda, -1.25, C, N)
+    ctx.set_kchunks(0)
     ref = 0.5 * (A.T @ A) - 1.25 * C0
     il = np.tril_indices(N)
     Cm = C.reshape(N, N)
@@ -69,13 +71,16 @@ def test_partition_invariance(skg, ctx, orc, rng):
         info = skg.tile_info(tile)
         if info["bm"] != info["bn"]:
             continue
-        for grid in (1, 2, 5, 33, 0):
+        for grid, kchunks in ((1, 1), (2, 1), (5, 3), (33, 1), (0, 1),
+                              (0, 8), (24, 8)):
             ctx.set_tuning(tile, grid)
+            ctx.set_kchunks(kchunks)
             C = np.zeros(N * N)
             ctx.dsyrk_host(0, N, K, 1.0, A, N, 0.0, C, N)
             err = rel_fro(C.reshape(N, N)[il], ref.reshape(N, N)[il])
-            assert err < TOL, (tile, grid, err)
+            assert err < TOL, (tile, grid, kchunks, err)
     ctx.set_tuning(-1, 0)
+    ctx.set_kchunks(0)
 
 
 def test_config2_syrk_full_size(skg, ctx, orc):
