@@ -138,6 +138,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=int, default=-1)
     ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -156,8 +157,16 @@ def main():
     rot = skg.DeviceArray(ctx, Nd * Ns, zero=True)
     ctx.fill_normal(psi.ptr, Nd * Ns, 1, 1.0 / np.sqrt(Nd))
     ctx.fill_uniform(q.ptr, Ns * Ns, 2, -1.0, 1.0)
+    plans = {}
     if args.tile >= 0 or args.grid > 0:
         ctx.set_tuning(args.tile, args.grid)
+    elif not args.no_tune:
+        # Autotuner: picks tile / grid / K chunks per shape with real
+        # launches, before (and outside) the timed region.
+        plans["syrk"] = ctx.autotune(1, Ns, Ns, Nd, la=skg.COL_MAJOR)
+        plans["rotation"] = ctx.autotune(0, Nd, Ns, Ns, la=skg.COL_MAJOR,
+                                         lb=skg.ROW_MAJOR)
+        log("autotuned plans %s" % plans)
 
     def step():
         ctx.dsyrk(skg.COL_MAJOR, Ns, Nd, 1.0, psi.ptr, Nd, 0.0, ms.ptr, Ns)
@@ -237,6 +246,9 @@ def main():
                        "step_p99": float(np.percentile(step_dev, 99)),
                        "source": "HIP events per launch, device time"},
         "kernels": kernels,
+        "plans": {k: {"tile": v["tile"], "tile_shape": skg.tile_info(v["tile"]),
+                      "grid": v["grid"], "kchunks": v["kchunks"]}
+                  for k, v in plans.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "probes": probes,

@@ -224,16 +224,23 @@ int skgemm_set_kchunks(skgemm_ctx* ctx, int kchunks);
 /* Number of tile configurations and their shapes. */
 int skgemm_num_tiles(void);
 int skgemm_tile_info(int tile, int* bm, int* bn, int* bk, int* waves);
-/* Time candidate (tile, grid) pairs on the given shape with real launches
- * and record the fastest in the context's table (and in `cache_path` if not
- * NULL, one line per shape bucket).  kind 0 GEMM / 1 SYRK. */
-int skgemm_autotune(skgemm_ctx* ctx, int kind, int M, int N, int K,
-                    const char* cache_path, int* best_tile, int* best_grid,
+/* Autotuner: times every candidate (tile configuration x persistent grid
+ * size x K-chunk count) with real launches on scratch operands of this
+ * shape and layout (layout_b is ignored for SYRK), records the fastest in
+ * the context's table, and appends it to `cache_path` when not NULL (one
+ * text line per shape: kind la lb M N K tile grid kchunks us).  The
+ * reference's analogue is its -DMC/-DNC/-DKC rebuild sweep
+ * (sub_gemm.sh:31-52, sub_syrk.sh:30-48).  kind 0 GEMM / 1 SYRK. */
+int skgemm_autotune(skgemm_ctx* ctx, int kind, int layout_a, int layout_b,
+                    int M, int N, int K, const char* cache_path,
+                    int* best_tile, int* best_grid, int* best_kchunks,
                     double* best_us);
+/* Load tuned plans written by skgemm_autotune. */
 int skgemm_load_tuning(skgemm_ctx* ctx, const char* cache_path);
-/* The tile and grid the next call on this shape would use. */
-int skgemm_query_plan(skgemm_ctx* ctx, int kind, int M, int N, int K,
-                      int* tile, int* grid, int64_t* tiles, int64_t* iters);
+/* The plan the next call on this shape and layout would use. */
+int skgemm_query_plan(skgemm_ctx* ctx, int kind, int layout_a, int layout_b,
+                      int M, int N, int K, int* tile, int* grid,
+                      int* kchunks, int64_t* tiles, int64_t* iters);
 
 /* ---- measurement -------------------------------------------------------- */
 

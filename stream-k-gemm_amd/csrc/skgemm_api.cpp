@@ -164,7 +164,7 @@ int64_t span_output(int layout, int64_t ld, int M, int N) {
 
 static const int kCtxMagic = 0x5ca1ab1e;
 
-static int check_ctx(skgemm_ctx* c) {
+int skg::check_ctx(skgemm_ctx* c) {
   if (c == nullptr || c->magic != kCtxMagic)
     return set_error(SKGEMM_ERR_STATE, "invalid or destroyed context");
   return SKGEMM_OK;
@@ -215,7 +215,8 @@ extern "C" int skgemm_create(int device, skgemm_ctx** out) {
   if ((e = hipEventCreate(&c->ev0)) != hipSuccess ||
       (e = hipEventCreate(&c->ev1)) != hipSuccess ||
       (e = hipMalloc(&c->status, sizeof(int))) != hipSuccess ||
-      (e = hipMemset(c->status, 0, sizeof(int))) != hipSuccess) {
+      (e = hipMemset(c->status, 0, sizeof(int))) != hipSuccess ||
+      (e = hipDeviceSynchronize()) != hipSuccess) {
     skgemm_destroy(c);
     return hip_error(e, "context setup");
   }
@@ -281,7 +282,16 @@ extern "C" int skgemm_dev_alloc(skgemm_ctx* c, size_t bytes, int zero,
   if (e != hipSuccess)
     return set_error(SKGEMM_ERR_NOMEM, "hipMalloc(%zu): %s", bytes,
                      hipGetErrorString(e));
-  if (zero) SKG_HIP(hipMemset(p, 0, bytes), "hipMemset");
+  if (zero) {
+    // On the context's stream: a null-stream memset is not ordered against
+    // the context's non-blocking stream.
+    hipError_t ez = hipMemsetAsync(p, 0, bytes, c->stream);
+    if (ez == hipSuccess) ez = hipStreamSynchronize(c->stream);
+    if (ez != hipSuccess) {
+      hipFree(p);
+      return hip_error(ez, "zero-fill");
+    }
+  }
   *out = p;
   return SKGEMM_OK;
 }
@@ -474,10 +484,11 @@ static int default_kchunks(skgemm_ctx* c, int kind, int cfg, int M, int N,
 static int default_cfg(skgemm_ctx* c, int kind, int M, int N, int K) {
   (void)K;
   if (kind == 1) return (N >= 768) ? 4 : 1;
-  if (M >= 128 && N >= 128) {
-    const int64_t t128 = tiles_for(0, 0, M, N);
-    return (t128 >= 4 * c->num_cus) ? 4 : 1;
+  if (M >= 768 && N >= 768) {
+    const int64_t t128 = tiles_for(0, 4, M, N);
+    return (t128 >= 4 * c->num_cus) ? 4 : 6;
   }
+  if (M >= 128 && N >= 128) return 1;
   if (M >= 128) return 2;
   if (N >= 128) return 3;
   return 1;
@@ -490,9 +501,10 @@ static int clamp_grid(skgemm_ctx* c, int cfg, int g, int64_t total) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, total));
 }
 
-Plan plan(skgemm_ctx* c, int kind, int M, int N, int K) {
+Plan plan(skgemm_ctx* c, int kind, int la, int lb, int M, int N, int K) {
   Plan pl{};
-  auto it = c->tuned.find(std::make_tuple(kind, M, N, K));
+  if (kind == 1) lb = la;
+  auto it = c->tuned.find(std::make_tuple(kind, la, lb, M, N, K));
   if (c->forced_cfg >= 0) {
     pl.cfg = c->forced_cfg;
     if (kind == 1 && tile_cfg(pl.cfg).bm != tile_cfg(pl.cfg).bn) pl.cfg = 1;
@@ -567,7 +579,7 @@ int run_single(skgemm_ctx* c, Problem& p, const Plan& pl) {
 
 }  // namespace skg
 
-static int check_dims(int M, int N, int K) {
+int skg::check_dims(int M, int N, int K) {
   if (M < 0 || N < 0 || K < 0)
     return set_error(SKGEMM_ERR_ARG, "negative dimension (M=%d N=%d K=%d)", M, N, K);
   if (M >= (1 << 30) || N >= (1 << 30) || K >= (1 << 30))
@@ -616,7 +628,7 @@ extern "C" int skgemm_dgemm(skgemm_ctx* c, int la, int lb, int ld_, int M,
   SKG_TRY(prepare_gemm(la, lb, ld_, M, N, K, A, lda, B, ldb, D, ldd, &p));
   if (M == 0 || N == 0) return SKGEMM_OK;
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
-  const Plan pl = plan(c, 0, M, N, K);
+  const Plan pl = plan(c, 0, la, lb, M, N, K);
   SKG_TRY(fill_problem(c, &p, 0, pl, M, N, K));
   p.alpha = alpha;
   p.beta = beta;
@@ -632,7 +644,7 @@ extern "C" int skgemm_dsyrk(skgemm_ctx* c, int la, int N, int K, double alpha,
   SKG_TRY(prepare_syrk(la, N, K, A, lda, C, ldc, &p));
   if (N == 0) return SKGEMM_OK;
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
-  const Plan pl = plan(c, 1, N, N, K);
+  const Plan pl = plan(c, 1, la, la, N, N, K);
   SKG_TRY(fill_problem(c, &p, 1, pl, N, N, K));
   p.alpha = alpha;
   p.beta = beta;
@@ -734,19 +746,20 @@ extern "C" int skgemm_set_tuning(skgemm_ctx* c, int tile, int grid) {
   return SKGEMM_OK;
 }
 
-extern "C" int skgemm_query_plan(skgemm_ctx* c, int kind, int M, int N, int K,
-                                 int* tile, int* grid, int64_t* tiles,
-                                 int64_t* iters) {
+extern "C" int skgemm_query_plan(skgemm_ctx* c, int kind, int la, int lb,
+                                 int M, int N, int K, int* tile, int* grid,
+                                 int* kchunks, int64_t* tiles, int64_t* iters) {
   SKG_TRY(check_ctx(c));
   SKG_TRY(check_dims(M, N, K));
   if (kind != 0 && kind != 1) return set_error(SKGEMM_ERR_ARG, "kind must be 0 or 1");
-  const Plan pl = plan(c, kind, M, N, K);
+  const Plan pl = plan(c, kind, la, lb, M, N, K);
   const TileCfg& t = tile_cfg(pl.cfg);
   const int64_t nt = tiles_for(kind, pl.cfg, M, N);
   const int64_t ipt = std::max<int64_t>(1, (K + t.bk - 1) / t.bk);
   const int64_t kci = (ipt + pl.kchunks - 1) / pl.kchunks;
   if (tile) *tile = pl.cfg;
   if (grid) *grid = pl.grid;
+  if (kchunks) *kchunks = pl.kchunks;
   if (tiles) *tiles = nt;
   if (iters) *iters = nt * pl.kchunks * kci;
   return SKGEMM_OK;

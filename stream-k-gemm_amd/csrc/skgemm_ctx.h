@@ -52,11 +52,11 @@ struct skgemm_ctx {
   size_t stage_n[kStages] = {0, 0, 0, 0};
 
   // Tuning: forced configuration, autotuned plans per exact shape
-  // (kind, M, N, K) -> (cfg, grid), occupancy per cfg.
+  // (kind, layout_a, layout_b, M, N, K) -> plan, occupancy per cfg.
   int forced_cfg = -1;
   int forced_grid = 0;
   int forced_kchunks = 0;
-  std::map<std::tuple<int, int, int, int>, skg::Plan> tuned;
+  std::map<std::tuple<int, int, int, int, int, int>, skg::Plan> tuned;
   std::map<int, int> occ_cache;
 
   // Split-form and batched state (skgemm_ozaki.hip / skgemm_batch.cpp).
@@ -85,7 +85,9 @@ int64_t span_operand(int layout, bool a_convention, int64_t ld, int ext, int K);
 int64_t span_output(int layout, int64_t ld, int M, int N);
 
 int64_t tiles_for(int kind, int cfg, int M, int N);
-Plan plan(skgemm_ctx* c, int kind, int M, int N, int K);
+Plan plan(skgemm_ctx* c, int kind, int la, int lb, int M, int N, int K);
+int check_ctx(skgemm_ctx* c);
+int check_dims(int M, int N, int K);
 int fill_problem(skgemm_ctx* c, Problem* p, int kind, const Plan& pl, int M,
                  int N, int K);
 int run_single(skgemm_ctx* c, Problem& p, const Plan& pl);

@@ -39,10 +39,8 @@ static const TileCfg kCfgs[] = {
     {128, 64, 16, 2, 2},   // 2: 4 waves, 64x32 per wave
     {64, 128, 16, 2, 2},   // 3: 4 waves, 32x64 per wave
     {128, 128, 16, 4, 2},  // 4: 8 waves, 32x64 per wave
-    {256, 128, 16, 4, 2},  // 5: 8 waves, 64x64 per wave
-    {64, 64, 32, 2, 2},    // 6: as 1 with BK = 32 (half the barriers)
-    {128, 64, 32, 4, 2},   // 7: 8 waves, 32x32 per wave, BK = 32
-    {128, 128, 32, 4, 2},  // 8: 8 waves, 32x64 per wave, BK = 32
+    {64, 64, 32, 2, 2},    // 5: as 1 with BK = 32 (half the barriers)
+    {128, 64, 32, 4, 2},   // 6: 8 waves, 32x32 per wave, BK = 32
 };
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
@@ -213,11 +211,15 @@ struct TileShape {
   static_assert(TM % 16 == 0 && TN % 16 == 0 && BK % 4 == 0, "bad tile");
 };
 
-// MFMAs of one staged BK step from LDS buffer `la` (A) / `lb` (B).
-template <int BM, int BN, int BK, int WM, int WN, int AKC, int BKC>
+// MFMAs of one staged BK step from LDS buffer `la` (A) / `lb` (B).  Bit
+// i*NB+j of the wave-uniform `mask` enables 16x16 block (i, j); blocks with
+// no output to keep (past M or N, or above the SYRK diagonal) are skipped.
+template <int BM, int BN, int BK, int WM, int WN, int AKC, int BKC,
+          bool MASKED>
 __device__ __forceinline__ void mma_step(v4d (&acc)[BM / WM / 16][BN / WN / 16],
                                          const double* la, const double* lb,
-                                         int wm, int wn, int lq, int lr) {
+                                         int wm, int wn, int lq, int lr,
+                                         uint32_t mask) {
   using S = TileShape<BM, BN, BK, WM, WN>;
   using LA = Lds<BM, BK, AKC>;
   using LB = Lds<BN, BK, BKC>;
@@ -234,8 +236,9 @@ __device__ __forceinline__ void mma_step(v4d (&acc)[BM / WM / 16][BN / WN / 16],
     for (int i = 0; i < S::MB; ++i)
 #pragma unroll
       for (int j = 0; j < S::NB; ++j)
-        acc[i][j] =
-            __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+        if (!MASKED || (mask & (1u << (i * S::NB + j))))
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j],
+                                                           acc[i][j], 0, 0, 0);
   }
 }
 
@@ -246,11 +249,12 @@ __device__ __forceinline__ void mma_step(v4d (&acc)[BM / WM / 16][BN / WN / 16],
 // before its data is written to LDS; LDS is double-buffered with one barrier
 // per step.  Loads past the unit's range are issued unconditionally (beyond
 // K they are masked and never touch memory) to keep the loop branch-free.
-template <int BM, int BN, int BK, int WM, int WN, int AKC, int BKC, int VEC>
+template <int BM, int BN, int BK, int WM, int WN, int AKC, int BKC, int VEC,
+          bool MASKED>
 __device__ __forceinline__ void mac_loop(
     v4d (&acc)[BM / WM / 16][BN / WN / 16], const Operand& A,
     const Operand& B, int K, int m0, int n0, int kbase, int kb, int ke,
-    double* lds, int tid) {
+    double* lds, int tid, uint32_t mask) {
   using S = TileShape<BM, BN, BK, WM, WN>;
   constexpr int NT = S::NT;
   const int lane = tid & 63, wave = tid >> 6;
@@ -293,8 +297,8 @@ __device__ __forceinline__ void mac_loop(
     ldB(sb0, kk + 2);
     {
       const double* la = lds + cur * S::kBufDoubles;
-      mma_step<BM, BN, BK, WM, WN, AKC, BKC>(acc, la, la + S::kRegionA, wm,
-                                              wn, lq, lr);
+      mma_step<BM, BN, BK, WM, WN, AKC, BKC, MASKED>(
+          acc, la, la + S::kRegionA, wm, wn, lq, lr, mask);
     }
     if (kk + 1 >= kend) break;
     {
@@ -310,8 +314,8 @@ __device__ __forceinline__ void mac_loop(
     ldB(sb1, kk + 2);
     {
       const double* la = lds + cur * S::kBufDoubles;
-      mma_step<BM, BN, BK, WM, WN, AKC, BKC>(acc, la, la + S::kRegionA, wm,
-                                              wn, lq, lr);
+      mma_step<BM, BN, BK, WM, WN, AKC, BKC, MASKED>(
+          acc, la, la + S::kRegionA, wm, wn, lq, lr, mask);
     }
     if (kk + 1 >= kend) break;
     {
@@ -348,7 +352,8 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
   extern __shared__ __align__(16) double lds[];
 
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave % WM, wn = wave / WM;
   const int lq = lane >> 4, lr = lane & 15;
 
@@ -414,16 +419,36 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
       for (int j = 0; j < NB; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
     // ---- MAC loop, one branch-free variant per operand layout -----------
+    const int Mx = BATCHED ? load_uniform(&tab[pi].M) : L.single.M;
+    const int Nx = BATCHED ? load_uniform(&tab[pi].N) : L.single.N;
+    uint32_t mask = 0;
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const int r0 = m0 + wm * TM + i * 16, c0 = n0 + wn * TN + j * 16;
+        if (r0 < Mx && c0 < Nx && (!syrk || r0 + 15 >= c0))
+          mask |= 1u << (i * NB + j);
+      }
+    // Tiles with skipped blocks (edges, SYRK diagonal) take the masked
+    // variant; full tiles the unmasked one.  Masked variants exist for the
+    // 16-byte-load layouts only (otherwise the unmasked loop runs, which is
+    // correct, the skipped blocks' outputs being discarded).
+    constexpr uint32_t kFull = (MB * NB >= 32) ? 0xffffffffu
+                                               : ((1u << (MB * NB)) - 1u);
+    const bool vec = A.vec2 && B.vec2;
     const int variant = (A.kcontig ? 4 : 0) | (B.kcontig ? 2 : 0) |
-                        ((A.vec2 && B.vec2) ? 1 : 0);
-#define SKG_MAC(V)                                                          \
-  case V:                                                                   \
-    mac_loop<BM, BN, BK, WM, WN, (V >> 2) & 1, (V >> 1) & 1, V & 1>(        \
-        acc, A, B, K, m0, n0, c * kci, kb, ke, lds, tid);                   \
+                        (vec ? 1 : 0) | ((mask != kFull && vec) ? 8 : 0);
+#define SKG_MAC(V)                                                           \
+  case V:                                                                    \
+    mac_loop<BM, BN, BK, WM, WN, (V >> 2) & 1, (V >> 1) & 1, V & 1,          \
+             ((V >> 3) & 1) != 0>(acc, A, B, K, m0, n0, c * kci, kb, ke,     \
+                                  lds, tid, mask);                           \
     break;
     switch (variant) {
       SKG_MAC(0) SKG_MAC(1) SKG_MAC(2) SKG_MAC(3)
       SKG_MAC(4) SKG_MAC(5) SKG_MAC(6) SKG_MAC(7)
+      SKG_MAC(9) SKG_MAC(11) SKG_MAC(13) SKG_MAC(15)
     }
 #undef SKG_MAC
 
@@ -608,10 +633,8 @@ static int occupancy_t() {
     case 2: return FN<128, 64, 16, 2, 2>(__VA_ARGS__);  \
     case 3: return FN<64, 128, 16, 2, 2>(__VA_ARGS__);  \
     case 4: return FN<128, 128, 16, 4, 2>(__VA_ARGS__); \
-    case 5: return FN<256, 128, 16, 4, 2>(__VA_ARGS__); \
-    case 6: return FN<64, 64, 32, 2, 2>(__VA_ARGS__);   \
-    case 7: return FN<128, 64, 32, 4, 2>(__VA_ARGS__);  \
-    case 8: return FN<128, 128, 32, 4, 2>(__VA_ARGS__); \
+    case 5: return FN<64, 64, 32, 2, 2>(__VA_ARGS__);   \
+    case 6: return FN<128, 64, 32, 4, 2>(__VA_ARGS__);  \
     default: break;                                   \
   }
 

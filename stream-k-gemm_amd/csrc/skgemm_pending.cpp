@@ -23,13 +23,6 @@ extern "C" int skgemm_batched(skgemm_ctx*, const skgemm_problem*, int, int,
                               int, int) {
   return set_error(SKGEMM_ERR_NOTSUP, "skgemm_batched: not built yet");
 }
-extern "C" int skgemm_autotune(skgemm_ctx*, int, int, int, int, const char*,
-                               int*, int*, double*) {
-  return set_error(SKGEMM_ERR_NOTSUP, "skgemm_autotune: not built yet");
-}
-extern "C" int skgemm_load_tuning(skgemm_ctx*, const char*) {
-  return set_error(SKGEMM_ERR_NOTSUP, "skgemm_load_tuning: not built yet");
-}
 extern "C" int skgemm_server_create(int, size_t, int, int, int, int,
                                     skgemm_server**) {
   return set_error(SKGEMM_ERR_NOTSUP, "server: not built yet");

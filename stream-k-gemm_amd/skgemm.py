@@ -109,11 +109,11 @@ def lib():
         "skgemm_set_kchunks": (c_int, [P, c_int]),
         "skgemm_num_tiles": (c_int, []),
         "skgemm_tile_info": (c_int, [c_int, P, P, P, P]),
-        "skgemm_autotune": (c_int, [P, c_int, c_int, c_int, c_int,
-                                    ctypes.c_char_p, P, P, P]),
+        "skgemm_autotune": (c_int, [P, c_int, c_int, c_int, c_int, c_int,
+                                    c_int, ctypes.c_char_p, P, P, P, P]),
         "skgemm_load_tuning": (c_int, [P, ctypes.c_char_p]),
-        "skgemm_query_plan": (c_int, [P, c_int, c_int, c_int, c_int, P, P, P,
-                                      P]),
+        "skgemm_query_plan": (c_int, [P, c_int, c_int, c_int, c_int, c_int,
+                                      c_int, P, P, P, P, P]),
         "skgemm_timing_enable": (c_int, [P, c_int]),
         "skgemm_timing_read": (c_int, [P, P, c_int, P]),
         "skgemm_last_kernel_ms": (c_int, [P, P]),
@@ -210,14 +210,15 @@ class Context:
     def set_kchunks(self, n=0):
         check(lib().skgemm_set_kchunks(self.h, n))
 
-    def query_plan(self, kind, M, N, K):
-        t, g = c_int(), c_int()
+    def query_plan(self, kind, M, N, K, la=COL_MAJOR, lb=ROW_MAJOR):
+        t, g, kc = c_int(), c_int(), c_int()
         nt, ni = c_int64(), c_int64()
-        check(lib().skgemm_query_plan(self.h, kind, M, N, K, ctypes.byref(t),
-                                      ctypes.byref(g), ctypes.byref(nt),
+        check(lib().skgemm_query_plan(self.h, kind, la, lb, M, N, K,
+                                      ctypes.byref(t), ctypes.byref(g),
+                                      ctypes.byref(kc), ctypes.byref(nt),
                                       ctypes.byref(ni)))
-        return dict(tile=t.value, grid=g.value, tiles=nt.value,
-                    iters=ni.value)
+        return dict(tile=t.value, grid=g.value, kchunks=kc.value,
+                    tiles=nt.value, iters=ni.value)
 
     def timing_enable(self, capacity):
         check(lib().skgemm_timing_enable(self.h, capacity))
@@ -307,13 +308,14 @@ class Context:
         check(lib().skgemm_copy_probe(self.h, nbytes, ctypes.byref(d)))
         return d.value
 
-    def autotune(self, kind, M, N, K, cache_path=None):
-        t, g, us = c_int(), c_int(), _out_double()
+    def autotune(self, kind, M, N, K, la=COL_MAJOR, lb=ROW_MAJOR,
+                 cache_path=None):
+        t, g, kc, us = c_int(), c_int(), c_int(), _out_double()
         check(lib().skgemm_autotune(
-            self.h, kind, M, N, K,
+            self.h, kind, la, lb, M, N, K,
             cache_path.encode() if cache_path else None, ctypes.byref(t),
-            ctypes.byref(g), ctypes.byref(us)))
-        return dict(tile=t.value, grid=g.value, us=us.value)
+            ctypes.byref(g), ctypes.byref(kc), ctypes.byref(us)))
+        return dict(tile=t.value, grid=g.value, kchunks=kc.value, us=us.value)
 
     def load_tuning(self, path):
         check(lib().skgemm_load_tuning(self.h, path.encode()))

@@ -259,3 +259,27 @@ def test_device_pack_matches_reference_layouts(skg, ctx, orc, rng):
     ctx.pack(skg.ROW_MAJOR, dB.ptr, N, skg.PACK32, dpb.ptr, K, N, K)
     ctx.synchronize()
     assert np.array_equal(dpb.download(), orc.pack_B(N, K, Bm))
+
+
+def test_autotune_save_load_and_use(skg, ctx, orc, rng, tmp_path):
+    """The autotuner's plan is recorded, persisted, reloaded by a fresh
+    context, and the tuned launch is still correct."""
+    M, N, K = 700, 130, 900
+    path = str(tmp_path / "tune.txt")
+    best = ctx.autotune(0, M, N, K, la=skg.ROW_MAJOR, lb=skg.ROW_MAJOR,
+                        cache_path=path)
+    assert best["us"] > 0 and 0 <= best["tile"] < skg.num_tiles()
+    plan = ctx.query_plan(0, M, N, K, la=skg.ROW_MAJOR, lb=skg.ROW_MAJOR)
+    assert (plan["tile"], plan["grid"], plan["kchunks"]) == (
+        best["tile"], best["grid"], best["kchunks"])
+    with skg.Context(0) as c2:
+        c2.load_tuning(path)
+        p2 = c2.query_plan(0, M, N, K, la=skg.ROW_MAJOR, lb=skg.ROW_MAJOR)
+        assert (p2["tile"], p2["grid"], p2["kchunks"]) == (
+            plan["tile"], plan["grid"], plan["kchunks"])
+        A = rng.uniform(-1, 1, (M, K))
+        B = rng.uniform(-1, 1, (K, N))
+        D = np.zeros(M * N)
+        c2.dgemm_host(0, 0, 0, M, N, K, 1.0, np.ascontiguousarray(A.ravel()),
+                      K, np.ascontiguousarray(B.ravel()), N, 0.0, D, N, 0.0)
+        assert rel_fro(D, A @ B) < TOL
