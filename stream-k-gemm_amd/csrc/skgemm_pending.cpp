@@ -5,20 +5,6 @@
 
 using namespace skg;
 
-struct skgemm_ctx::Extra {};
-void skgemm_ctx::free_extra() {}
-
-extern "C" int skgemm_dgemm_ozaki(skgemm_ctx*, int, int, int, int, int, int,
-                                  double, const double*, int64_t,
-                                  const double*, int64_t, double, double*,
-                                  int64_t, double, int, double*) {
-  return set_error(SKGEMM_ERR_NOTSUP, "skgemm_dgemm_ozaki: not built yet");
-}
-extern "C" int skgemm_dsyrk_ozaki(skgemm_ctx*, int, int, int, double,
-                                  const double*, int64_t, double, double*,
-                                  int64_t, int, double*) {
-  return set_error(SKGEMM_ERR_NOTSUP, "skgemm_dsyrk_ozaki: not built yet");
-}
 extern "C" int skgemm_batched(skgemm_ctx*, const skgemm_problem*, int, int,
                               int, int) {
   return set_error(SKGEMM_ERR_NOTSUP, "skgemm_batched: not built yet");

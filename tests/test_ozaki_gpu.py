@@ -1,0 +1,127 @@
+"""GPU parity of the split-precision (Ozaki scheme II, INT8 MFMA) GEMM and
+SYRK.  Bars: bit-for-bit equality with the CPU restatement (oracle/ozaki.py:
+exact integer product, no CRT) on the same inputs; relative Frobenius error
+<= 1e-13 against the double-double oracle and <= the returned bound on the
+c3-style inputs (columns scaled by 10^u, u ~ U(0, 6))."""
+import numpy as np
+import pytest
+
+from conftest import rel_fro
+from oracle import ozaki
+from test_oracle import decode_d, encode_a, encode_b, encode_d
+
+pytestmark = pytest.mark.gpu
+
+
+def c3_like(rng, K, N):
+    P = rng.standard_normal((K, N)) / np.sqrt(K)
+    return P * 10.0 ** rng.uniform(0, 6, N)[None, :]
+
+
+@pytest.mark.parametrize("layout", [0, 1, 2, 3])
+def test_syrk_bit_exact_vs_restatement(skg, ctx, rng, layout):
+    K, N = 300, 70
+    X = c3_like(rng, K, N)                     # K x N
+    Xb, ld = encode_b(X, layout)
+    C = np.full(N * N, 9.0)
+    bound = ctx.dsyrk_ozaki(layout, N, K, 1.0, Xb, ld, 0.0, C, N)
+    ref, _, _, _ = ozaki.product(X.T.copy(), X.T.copy(), 14)
+    Cm = C.reshape(N, N)
+    il = np.tril_indices(N)
+    assert np.array_equal(Cm[il], ref[il])
+    assert np.all(Cm[np.triu_indices(N, 1)] == 9.0)
+    assert 0 < bound < 1e-13
+
+
+def test_gemm_bit_exact_all_layouts(skg, ctx, rng):
+    M, N, K = 50, 40, 200
+    A = rng.standard_normal((M, K)) * 10.0 ** rng.uniform(0, 4, M)[:, None]
+    B = rng.standard_normal((K, N)) * 10.0 ** rng.uniform(0, 4, N)[None, :]
+    ref, _, _, _ = ozaki.product(A, B.T.copy(), 14)
+    for la in range(4):
+        Ab, lda = encode_a(A, la)
+        for lb in range(4):
+            Bb, ldb = encode_b(B, lb)
+            for ld_ in (0, 1, 2):
+                Db, ldd = encode_d(np.zeros((M, N)), ld_)
+                ctx.dgemm_ozaki(la, lb, ld_, M, N, K, 1.0, Ab, lda, Bb, ldb, 0.0,
+                                Db, ldd, 0.0, slices=14)
+                got = decode_d(Db, ld_, M, N, ldd)
+                assert np.array_equal(got, ref), (la, lb, ld_)
+
+
+def test_partition_invariance_is_bitwise(skg, ctx, rng):
+    """Integer partials: any grid gives the identical result."""
+    K, N = 5000, 200
+    X = c3_like(rng, K, N).ravel()
+    outs = []
+    for grid in (0, 1, 3, 64, 257):
+        ctx.set_tuning(-1, grid)
+        C = np.zeros(N * N)
+        ctx.dsyrk_ozaki(0, N, K, 1.0, X, N, 0.0, C, N)
+        outs.append(C)
+    ctx.set_tuning(-1, 0)
+    for C in outs[1:]:
+        assert np.array_equal(C, outs[0])
+
+
+def test_small_integer_inputs_exact(skg, ctx, rng):
+    M, N, K = 33, 65, 129
+    A = rng.integers(-1000, 1000, (M, K)).astype(np.float64)
+    B = rng.integers(-1000, 1000, (K, N)).astype(np.float64)
+    D = np.zeros(M * N)
+    ctx.dgemm_ozaki(0, 0, 0, M, N, K, 1.0, np.ascontiguousarray(A.ravel()), K,
+                    np.ascontiguousarray(B.ravel()), N, 0.0, D, N, 0.0)
+    assert np.array_equal(D.reshape(M, N), A @ B)
+
+
+def test_alpha_beta_gamma(skg, ctx, rng):
+    M, N, K = 120, 90, 333
+    A = rng.uniform(-1, 1, (M, K))
+    B = rng.uniform(-1, 1, (K, N))
+    C0 = rng.uniform(-1, 1, (M, N))
+    D = C0.copy().ravel()
+    b = ctx.dgemm_ozaki(0, 0, 0, M, N, K, 0.5, np.ascontiguousarray(A.ravel()),
+                        K, np.ascontiguousarray(B.ravel()), N, -2.0, D, N, 0.25)
+    ref = 0.5 * (A @ B) - 2.0 * C0 + 0.25
+    assert rel_fro(D, ref) < 1e-13 and b < 1e-13
+
+
+@pytest.mark.parametrize("K,N", [(16384, 256), (2048, 520)])
+def test_c3_style_accuracy_vs_double_double(skg, ctx, orc, rng, K, N):
+    X = c3_like(rng, K, N)
+    Xc = np.asfortranarray(X).ravel(order="F").copy()   # column-major
+    C = np.zeros(N * N)
+    bound = ctx.dsyrk_ozaki(skg.COL_MAJOR, N, K, 1.0, Xc, K, 0.0, C, N)
+    ref = np.zeros(N * N)
+    orc.dsyrk(0, N, K, 1.0, np.ascontiguousarray(X.ravel()), N, 0.0, ref, N,
+              dd=True)
+    il = np.tril_indices(N)
+    err = rel_fro(C.reshape(N, N)[il], ref.reshape(N, N)[il])
+    print("K", K, "N", N, "err", err, "bound", bound, flush=True)
+    assert err < 1e-13 and err <= bound and bound < 1e-13
+
+
+def test_rotation_split_accuracy(skg, ctx, orc, rng):
+    """Psi * Q in the split form: rows of Psi carry the 10^u column scales."""
+    Nd, Ns = 4096, 256
+    P = c3_like(rng, Nd, Ns)
+    Q = rng.uniform(-1, 1, (Ns, Ns))
+    Pc = np.asfortranarray(P).ravel(order="F").copy()
+    D = np.zeros(Nd * Ns)
+    bound = ctx.dgemm_ozaki(skg.COL_MAJOR, skg.ROW_MAJOR, skg.COL_MAJOR, Nd, Ns,
+                            Ns, 1.0, Pc, Nd, Q.ravel().copy(), Ns, 0.0, D, Nd,
+                            0.0)
+    ref = np.zeros(Nd * Ns)
+    orc.dgemm(1, 0, 1, Nd, Ns, Ns, 1.0, Pc, Nd, Q.ravel().copy(), Ns, 0.0, ref,
+              Nd, 0.0, dd=True)
+    err = rel_fro(D, ref)
+    assert err < 1e-13 and err <= bound
+
+
+def test_k_limit_is_reported(skg, ctx):
+    lib = skg.lib()
+    with pytest.raises(skg.SkgemmError) as e:
+        ctx.dsyrk_ozaki(0, 4, 140000, 1.0, 1, 4, 0.0, 1, 4)
+    assert e.value.code == skg.ERR_NOTSUP
+    del lib

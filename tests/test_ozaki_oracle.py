@@ -1,0 +1,77 @@
+"""Pins the split-form (Ozaki scheme II) restatement in oracle/ozaki.py.
+CPU only."""
+import numpy as np
+import pytest
+
+from conftest import rel_fro
+from oracle import ozaki
+
+
+def c3_like(rng, K, N):
+    """c3 inputs at reduced size: normal(0, 1/sqrt(K)) columns scaled by
+    10^u, u ~ U(0, 6)."""
+    P = rng.standard_normal((K, N)) / np.sqrt(K)
+    return P * 10.0 ** rng.uniform(0, 6, N)[None, :]
+
+
+def test_moduli_pairwise_coprime_and_headroom():
+    import math
+    m = ozaki.MODULI
+    for i in range(len(m)):
+        for j in range(i):
+            assert math.gcd(m[i], m[j]) == 1
+    _, h = ozaki.tables(14)
+    assert h == 54  # log2(P) = 110.16 for the first 14 moduli
+
+
+def test_crt_reconstructs_exact_product(rng):
+    """Residues of the exact product, reconstructed by CRT, give the exact
+    product back (the scaling keeps |C'| < P/2)."""
+    m, h = ozaki.tables(14)
+    X = c3_like(rng, 300, 12)
+    s = ozaki.sigma(np.einsum("ij,ij->j", X, X), h)
+    x = ozaki.scaled_integers(X.T, s)
+    cint = ozaki.exact_product(x, x)
+    res = [np.vectorize(lambda v, ml=ml: v % ml, otypes=[object])(cint)
+           for ml in m]
+    assert np.array_equal(ozaki.crt(res, m), cint)
+
+
+def test_scaling_bounds_the_integer_product(rng):
+    m, h = ozaki.tables(14)
+    P = 1
+    for x in m:
+        P *= x
+    X = c3_like(rng, 2000, 20)
+    s = ozaki.sigma(np.einsum("ij,ij->j", X, X), h)
+    x = ozaki.scaled_integers(X.T, s)
+    cint = ozaki.exact_product(x, x)
+    assert max(abs(int(v)) for v in cint.ravel()) <= P // 4
+    # and the integer rows use the headroom: ||x'_o|| > 2^(h-3)
+    norms = np.sqrt(np.einsum("ij,ij->i", x, x))
+    assert np.all(norms > 2.0 ** (h - 3)) and np.all(norms <= 2.0 ** h)
+
+
+def test_small_integer_inputs_are_exact(rng):
+    """Inputs with few significant bits are represented exactly after
+    scaling, so the split product equals the exact product."""
+    A = rng.integers(-50, 50, (7, 40)).astype(np.float64)
+    B = rng.integers(-50, 50, (9, 40)).astype(np.float64)
+    C, _, _, _ = ozaki.product(A, B, 14)
+    assert np.array_equal(C, A @ B.T)
+
+
+@pytest.mark.parametrize("L,tol", [(14, 1e-13), (16, 1e-15)])
+def test_error_against_double_double(orc, rng, L, tol):
+    K, N = 4096, 48
+    X = c3_like(rng, K, N)
+    C, _, _, _ = ozaki.product(X.T.copy(), X.T.copy(), L)
+    ref = np.zeros(N * N)
+    orc.dsyrk(0, N, K, 1.0, np.ascontiguousarray(X.ravel()), N, 0.0, ref, N,
+              dd=True)
+    ref = ref.reshape(N, N)
+    il = np.tril_indices(N)
+    assert rel_fro(C[il], ref[il]) < tol
+    # fewer moduli leave less headroom: the error grows as predicted
+    C12, _, _, _ = ozaki.product(X.T.copy(), X.T.copy(), 12)
+    assert rel_fro(C12[il], ref[il]) > rel_fro(C[il], ref[il])
