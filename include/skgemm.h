@@ -154,12 +154,16 @@ int skgemm_dsyrk_host(skgemm_ctx* ctx, int layout_a, int N, int K,
 /* ---- Ozaki split form (INT8 slices, INT32 accumulation, FP64 sum) ------ */
 
 /* Same contracts as skgemm_dgemm / skgemm_dsyrk (beta/gamma supported) with
- * the product computed by the Ozaki split: power-of-two scaling of rows of
- * A and columns of B, `slices` INT8 slices per operand (2..8; 0 = choose
- * from the requested accuracy), the triangular product set p+q <= slices+1
- * on INT8 MFMA with exact INT32 accumulation, and an FP64 cross-slice sum.
- * *err_bound (may be NULL) receives the a-priori bound on
- * ||C - C_exact||_F / ||C_exact||_F for this call's inputs.
+ * the product computed in the Ozaki split form (scheme II, CRT variant):
+ * power-of-two scaling of rows of A and columns of B from their 2-norms,
+ * each scaled operand cut into `slices` INT8 residue planes modulo
+ * pairwise-coprime moduli <= 256 (8..20; 0 = 14, which bounds the error
+ * below 1e-13 for K up to 131072 on inputs like the DFT orbitals), one INT8
+ * MFMA product per plane with exact INT32 accumulation in one persistent
+ * Stream-K launch, and an exact CRT reconstruction rounded once to FP64.
+ * *err_bound (may be NULL; non-NULL makes the call synchronous) receives
+ * the bound on ||C - AB||_F / ||AB||_F from the integer rounding of the
+ * scaled inputs.  Device pointers.  Needs K <= 131072.
  * No counterpart in the reference (its path is FP64 only). */
 int skgemm_dgemm_ozaki(skgemm_ctx* ctx, int layout_a, int layout_b,
                        int layout_d, int M, int N, int K, double alpha,
@@ -169,6 +173,16 @@ int skgemm_dgemm_ozaki(skgemm_ctx* ctx, int layout_a, int layout_b,
 int skgemm_dsyrk_ozaki(skgemm_ctx* ctx, int layout_a, int N, int K,
                        double alpha, const double* A, int64_t lda, double beta,
                        double* C, int64_t ldc, int slices, double* err_bound);
+/* Host-pointer forms of the two calls above (copy in, run, copy out). */
+int skgemm_dgemm_ozaki_host(skgemm_ctx* ctx, int layout_a, int layout_b,
+                            int layout_d, int M, int N, int K, double alpha,
+                            const double* A, int64_t lda, const double* B,
+                            int64_t ldb, double beta, double* D, int64_t ldd,
+                            double gamma, int slices, double* err_bound);
+int skgemm_dsyrk_ozaki_host(skgemm_ctx* ctx, int layout_a, int N, int K,
+                            double alpha, const double* A, int64_t lda,
+                            double beta, double* C, int64_t ldc, int slices,
+                            double* err_bound);
 
 /* ---- layout conversion (include/sme_pack.hpp, gemm_crsp.cpp:342-366) --- */
 

@@ -158,6 +158,25 @@ int64_t span_output(int layout, int64_t ld, int M, int N) {
          ((N - 1) & (P - 1)) + 1;
 }
 
+// Device-pointer entry points reject pointers the GPU cannot dereference
+// (pageable host memory) with SKGEMM_ERR_ARG instead of faulting the device.
+int check_dev(const void* p, const char* name) {
+  if (p == nullptr) return SKGEMM_OK;
+  hipPointerAttribute_t a;
+  const hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(SKGEMM_ERR_ARG,
+                     "%s is not device-accessible memory (use the _host "
+                     "entry point for host arrays)", name);
+  }
+  if (a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged &&
+      a.type != hipMemoryTypeHost)
+    return set_error(SKGEMM_ERR_ARG, "%s is not device-accessible memory",
+                     name);
+  return SKGEMM_OK;
+}
+
 }  // namespace skg
 
 // ---- context ----------------------------------------------------------------
@@ -601,6 +620,13 @@ int prepare_gemm(int la, int lb, int ld_, int M, int N, int K, const double* A,
   SKG_TRY(make_operand(la, true, A, lda, M, K, &p->a, "A"));
   SKG_TRY(make_operand(lb, false, B, ldb, N, K, &p->b, "B"));
   SKG_TRY(make_output(ld_, D, ldd, M, N, &p->d, "D"));
+  if (M > 0 && N > 0) {
+    if (K > 0) {
+      SKG_TRY(check_dev(A, "A"));
+      SKG_TRY(check_dev(B, "B"));
+    }
+    SKG_TRY(check_dev(D, "D"));
+  }
   return SKGEMM_OK;
 }
 
@@ -614,6 +640,10 @@ int prepare_syrk(int la, int N, int K, const double* A, int64_t lda, double* C,
   SKG_TRY(make_operand(la, false, A, lda, N, K, &p->a, "A"));
   p->b = p->a;
   SKG_TRY(make_output(SKGEMM_ROW_MAJOR, C, ldc, N, N, &p->d, "C"));
+  if (N > 0) {
+    if (K > 0) SKG_TRY(check_dev(A, "A"));
+    SKG_TRY(check_dev(C, "C"));
+  }
   return SKGEMM_OK;
 }
 
@@ -654,15 +684,14 @@ extern "C" int skgemm_dsyrk(skgemm_ctx* c, int la, int N, int K, double alpha,
 
 // ---- host-pointer wrappers ----------------------------------------------------
 
-extern "C" int skgemm_dgemm_host(skgemm_ctx* c, int la, int lb, int ld_,
-                                 int M, int N, int K, double alpha,
-                                 const double* A, int64_t lda, const double* B,
-                                 int64_t ldb, double beta, double* D,
-                                 int64_t ldd, double gamma) {
-  SKG_TRY(check_ctx(c));
-  Problem p{};
-  SKG_TRY(prepare_gemm(la, lb, ld_, M, N, K, A, lda, B, ldb, D, ldd, &p));
-  if (M == 0 || N == 0) return SKGEMM_OK;
+namespace {
+
+// Copy host operands of a GEMM-shaped call to device staging, run `fn` on
+// the device copies, copy D back; synchronous.
+template <typename F>
+int stage_gemm(skgemm_ctx* c, int la, int lb, int ld_, int M, int N, int K,
+               const double* A, int64_t lda, const double* B, int64_t ldb,
+               double* D, int64_t ldd, F fn) {
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
   const int64_t na = span_operand(la, true, lda, M, K);
   const int64_t nb = span_operand(lb, false, ldb, N, K);
@@ -673,21 +702,18 @@ extern "C" int skgemm_dgemm_host(skgemm_ctx* c, int la, int lb, int ld_,
   if (!dA || !dB || !dD) return set_error(SKGEMM_ERR_NOMEM, "staging buffers");
   if (na) SKG_HIP(hipMemcpyAsync(dA, A, na * 8, hipMemcpyHostToDevice, c->stream), "H2D A");
   if (nb) SKG_HIP(hipMemcpyAsync(dB, B, nb * 8, hipMemcpyHostToDevice, c->stream), "H2D B");
+  // D is copied in even when beta == 0: packed / padded layouts have
+  // elements outside the matrix that must come back unchanged.
   SKG_HIP(hipMemcpyAsync(dD, D, nd * 8, hipMemcpyHostToDevice, c->stream), "H2D D");
-  SKG_TRY(skgemm_dgemm(c, la, lb, ld_, M, N, K, alpha, dA, lda, dB, ldb, beta,
-                       dD, ldd, gamma));
+  SKG_TRY(fn(dA, dB, dD));
   SKG_HIP(hipMemcpyAsync(D, dD, nd * 8, hipMemcpyDeviceToHost, c->stream), "D2H D");
   SKG_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   return c->check_status();
 }
 
-extern "C" int skgemm_dsyrk_host(skgemm_ctx* c, int la, int N, int K,
-                                 double alpha, const double* A, int64_t lda,
-                                 double beta, double* C, int64_t ldc) {
-  SKG_TRY(check_ctx(c));
-  Problem p{};
-  SKG_TRY(prepare_syrk(la, N, K, A, lda, C, ldc, &p));
-  if (N == 0) return SKGEMM_OK;
+template <typename F>
+int stage_syrk(skgemm_ctx* c, int la, int N, int K, const double* A,
+               int64_t lda, double* C, int64_t ldc, F fn) {
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
   const int64_t na = span_operand(la, false, lda, N, K);
   const int64_t nc = span_output(SKGEMM_ROW_MAJOR, ldc, N, N);
@@ -696,10 +722,109 @@ extern "C" int skgemm_dsyrk_host(skgemm_ctx* c, int la, int N, int K,
   if (!dA || !dC) return set_error(SKGEMM_ERR_NOMEM, "staging buffers");
   if (na) SKG_HIP(hipMemcpyAsync(dA, A, na * 8, hipMemcpyHostToDevice, c->stream), "H2D A");
   SKG_HIP(hipMemcpyAsync(dC, C, nc * 8, hipMemcpyHostToDevice, c->stream), "H2D C");
-  SKG_TRY(skgemm_dsyrk(c, la, N, K, alpha, dA, lda, beta, dC, ldc));
+  SKG_TRY(fn(dA, dC));
   SKG_HIP(hipMemcpyAsync(C, dC, nc * 8, hipMemcpyDeviceToHost, c->stream), "D2H C");
   SKG_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   return c->check_status();
+}
+
+// Argument checks of the host forms (host pointers are not device memory,
+// so the device-pointer checks are skipped here).
+int host_args_gemm(int la, int lb, int ld_, int M, int N, int K,
+                   const double* A, int64_t lda, const double* B, int64_t ldb,
+                   double* D, int64_t ldd) {
+  SKG_TRY(check_dims(M, N, K));
+  if (M > 0 && N > 0) {
+    if (D == nullptr) return set_error(SKGEMM_ERR_ARG, "D is NULL");
+    if (K > 0 && (A == nullptr || B == nullptr))
+      return set_error(SKGEMM_ERR_ARG, "A or B is NULL");
+  }
+  Operand o;
+  Output d;
+  SKG_TRY(make_operand(la, true, A, lda, M, K, &o, "A"));
+  SKG_TRY(make_operand(lb, false, B, ldb, N, K, &o, "B"));
+  SKG_TRY(make_output(ld_, D, ldd, M, N, &d, "D"));
+  return SKGEMM_OK;
+}
+
+int host_args_syrk(int la, int N, int K, const double* A, int64_t lda,
+                   double* C, int64_t ldc) {
+  SKG_TRY(check_dims(N, N, K));
+  if (N > 0) {
+    if (C == nullptr) return set_error(SKGEMM_ERR_ARG, "C is NULL");
+    if (K > 0 && A == nullptr) return set_error(SKGEMM_ERR_ARG, "A is NULL");
+  }
+  Operand o;
+  Output d;
+  SKG_TRY(make_operand(la, false, A, lda, N, K, &o, "A"));
+  SKG_TRY(make_output(SKGEMM_ROW_MAJOR, C, ldc, N, N, &d, "C"));
+  return SKGEMM_OK;
+}
+
+}  // namespace
+
+extern "C" int skgemm_dgemm_host(skgemm_ctx* c, int la, int lb, int ld_,
+                                 int M, int N, int K, double alpha,
+                                 const double* A, int64_t lda, const double* B,
+                                 int64_t ldb, double beta, double* D,
+                                 int64_t ldd, double gamma) {
+  SKG_TRY(check_ctx(c));
+  SKG_TRY(host_args_gemm(la, lb, ld_, M, N, K, A, lda, B, ldb, D, ldd));
+  if (M == 0 || N == 0) return SKGEMM_OK;
+  return stage_gemm(c, la, lb, ld_, M, N, K, A, lda, B, ldb, D, ldd,
+                    [&](const double* dA, const double* dB, double* dD) {
+                      return skgemm_dgemm(c, la, lb, ld_, M, N, K, alpha, dA,
+                                          lda, dB, ldb, beta, dD, ldd, gamma);
+                    });
+}
+
+extern "C" int skgemm_dsyrk_host(skgemm_ctx* c, int la, int N, int K,
+                                 double alpha, const double* A, int64_t lda,
+                                 double beta, double* C, int64_t ldc) {
+  SKG_TRY(check_ctx(c));
+  SKG_TRY(host_args_syrk(la, N, K, A, lda, C, ldc));
+  if (N == 0) return SKGEMM_OK;
+  return stage_syrk(c, la, N, K, A, lda, C, ldc,
+                    [&](const double* dA, double* dC) {
+                      return skgemm_dsyrk(c, la, N, K, alpha, dA, lda, beta, dC,
+                                          ldc);
+                    });
+}
+
+extern "C" int skgemm_dgemm_ozaki_host(skgemm_ctx* c, int la, int lb, int ld_,
+                                       int M, int N, int K, double alpha,
+                                       const double* A, int64_t lda,
+                                       const double* B, int64_t ldb,
+                                       double beta, double* D, int64_t ldd,
+                                       double gamma, int slices,
+                                       double* err_bound) {
+  SKG_TRY(check_ctx(c));
+  SKG_TRY(host_args_gemm(la, lb, ld_, M, N, K, A, lda, B, ldb, D, ldd));
+  if (err_bound) *err_bound = 0.0;
+  if (M == 0 || N == 0) return SKGEMM_OK;
+  return stage_gemm(c, la, lb, ld_, M, N, K, A, lda, B, ldb, D, ldd,
+                    [&](const double* dA, const double* dB, double* dD) {
+                      return skgemm_dgemm_ozaki(c, la, lb, ld_, M, N, K, alpha,
+                                                dA, lda, dB, ldb, beta, dD, ldd,
+                                                gamma, slices, err_bound);
+                    });
+}
+
+extern "C" int skgemm_dsyrk_ozaki_host(skgemm_ctx* c, int la, int N, int K,
+                                       double alpha, const double* A,
+                                       int64_t lda, double beta, double* C,
+                                       int64_t ldc, int slices,
+                                       double* err_bound) {
+  SKG_TRY(check_ctx(c));
+  SKG_TRY(host_args_syrk(la, N, K, A, lda, C, ldc));
+  if (err_bound) *err_bound = 0.0;
+  if (N == 0) return SKGEMM_OK;
+  return stage_syrk(c, la, N, K, A, lda, C, ldc,
+                    [&](const double* dA, double* dC) {
+                      return skgemm_dsyrk_ozaki(c, la, N, K, alpha, dA, lda,
+                                                beta, dC, ldc, slices,
+                                                err_bound);
+                    });
 }
 
 // ---- layout conversion --------------------------------------------------------
@@ -717,6 +842,8 @@ extern "C" int skgemm_pack(skgemm_ctx* c, int src_layout, const double* src,
   SKG_TRY(make_operand(src_layout, false, src, ld_src, o_extent, r_extent, &s, "src"));
   Output d{};
   SKG_TRY(make_output(dst_layout, dst, ld_dst, r_extent, o_extent, &d, "dst"));
+  SKG_TRY(check_dev(src, "src"));
+  SKG_TRY(check_dev(dst, "dst"));
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
   SKG_HIP(launch_pack(s, r_extent, d, c->stream), "pack launch");
   return SKGEMM_OK;
@@ -881,6 +1008,7 @@ extern "C" int skgemm_fill_uniform(skgemm_ctx* c, double* x, int64_t n,
   SKG_TRY(check_ctx(c));
   if (n < 0 || (n > 0 && x == nullptr)) return set_error(SKGEMM_ERR_ARG, "bad buffer");
   if (n == 0) return SKGEMM_OK;
+  SKG_TRY(check_dev(x, "x"));
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
   SKG_HIP(launch_fill_uniform(x, n, seed, lo, hi, c->stream), "fill_uniform");
   return SKGEMM_OK;
@@ -891,6 +1019,7 @@ extern "C" int skgemm_fill_normal(skgemm_ctx* c, double* x, int64_t n,
   SKG_TRY(check_ctx(c));
   if (n < 0 || (n > 0 && x == nullptr)) return set_error(SKGEMM_ERR_ARG, "bad buffer");
   if (n == 0) return SKGEMM_OK;
+  SKG_TRY(check_dev(x, "x"));
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
   SKG_HIP(launch_fill_normal(x, n, seed, sigma, c->stream), "fill_normal");
   return SKGEMM_OK;
@@ -903,6 +1032,7 @@ extern "C" int skgemm_scale_columns_pow10(skgemm_ctx* c, double* x, int rows,
   if (rows < 0 || cols < 0 || ld < rows || (rows > 0 && cols > 0 && !x))
     return set_error(SKGEMM_ERR_ARG, "bad matrix");
   if (rows == 0 || cols == 0) return SKGEMM_OK;
+  SKG_TRY(check_dev(x, "x"));
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
   SKG_HIP(launch_scale_columns_pow10(x, rows, cols, ld, seed, umax, c->stream),
           "scale_columns");

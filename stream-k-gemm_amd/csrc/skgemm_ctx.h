@@ -83,6 +83,7 @@ int make_output(int layout, double* p, int64_t ld, int M, int N, Output* out,
                 const char* name);
 int64_t span_operand(int layout, bool a_convention, int64_t ld, int ext, int K);
 int64_t span_output(int layout, int64_t ld, int M, int N);
+int check_dev(const void* p, const char* name);
 
 int64_t tiles_for(int kind, int cfg, int M, int N);
 Plan plan(skgemm_ctx* c, int kind, int la, int lb, int M, int N, int K);

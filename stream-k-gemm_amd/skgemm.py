@@ -27,7 +27,8 @@ EXPORTS = [
     "skgemm_device_info", "skgemm_dev_alloc", "skgemm_dev_free",
     "skgemm_memcpy_h2d", "skgemm_memcpy_d2h", "skgemm_dgemm",
     "skgemm_dgemm_host", "skgemm_dsyrk", "skgemm_dsyrk_host",
-    "skgemm_dgemm_ozaki", "skgemm_dsyrk_ozaki", "skgemm_pack",
+    "skgemm_dgemm_ozaki", "skgemm_dsyrk_ozaki", "skgemm_dgemm_ozaki_host",
+    "skgemm_dsyrk_ozaki_host", "skgemm_pack",
     "skgemm_batched", "skgemm_set_tuning", "skgemm_set_kchunks",
     "skgemm_num_tiles",
     "skgemm_tile_info", "skgemm_autotune", "skgemm_load_tuning",
@@ -102,6 +103,13 @@ def lib():
         "skgemm_dsyrk_ozaki": (c_int, [P, c_int, c_int, c_int, c_double, P,
                                        c_int64, c_double, P, c_int64, c_int,
                                        P]),
+        "skgemm_dgemm_ozaki_host": (c_int, [P, c_int, c_int, c_int, c_int,
+                                            c_int, c_int, c_double, P, c_int64,
+                                            P, c_int64, c_double, P, c_int64,
+                                            c_double, c_int, P]),
+        "skgemm_dsyrk_ozaki_host": (c_int, [P, c_int, c_int, c_int, c_double,
+                                            P, c_int64, c_double, P, c_int64,
+                                            c_int, P]),
         "skgemm_pack": (c_int, [P, c_int, P, c_int64, c_int, P, c_int64, c_int,
                                 c_int]),
         "skgemm_batched": (c_int, [P, P, c_int, c_int, c_int, c_int]),
@@ -259,6 +267,27 @@ class Context:
         check(lib().skgemm_dsyrk_ozaki(self.h, la, N, K, alpha, _ptr(A), lda,
                                        beta, _ptr(C), ldc, slices,
                                        ctypes.byref(e)))
+        return e.value
+
+    def dgemm_ozaki_host(self, la, lb, ld_, M, N, K, alpha, A, lda, B, ldb,
+                         beta, D, ldd, gamma=0.0, slices=0):
+        for x in (A, B, D):
+            assert x.dtype == np.float64 and x.flags.c_contiguous
+        e = _out_double()
+        check(lib().skgemm_dgemm_ozaki_host(self.h, la, lb, ld_, M, N, K,
+                                            alpha, _ptr(A), lda, _ptr(B), ldb,
+                                            beta, _ptr(D), ldd, gamma, slices,
+                                            ctypes.byref(e)))
+        return e.value
+
+    def dsyrk_ozaki_host(self, la, N, K, alpha, A, lda, beta, C, ldc,
+                         slices=0):
+        for x in (A, C):
+            assert x.dtype == np.float64 and x.flags.c_contiguous
+        e = _out_double()
+        check(lib().skgemm_dsyrk_ozaki_host(self.h, la, N, K, alpha, _ptr(A),
+                                            lda, beta, _ptr(C), ldc, slices,
+                                            ctypes.byref(e)))
         return e.value
 
     def pack(self, src_layout, src, ld_src, dst_layout, dst, ld_dst, o_ext,

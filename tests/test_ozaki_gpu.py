@@ -24,7 +24,7 @@ def test_syrk_bit_exact_vs_restatement(skg, ctx, rng, layout):
     X = c3_like(rng, K, N)                     # K x N
     Xb, ld = encode_b(X, layout)
     C = np.full(N * N, 9.0)
-    bound = ctx.dsyrk_ozaki(layout, N, K, 1.0, Xb, ld, 0.0, C, N)
+    bound = ctx.dsyrk_ozaki_host(layout, N, K, 1.0, Xb, ld, 0.0, C, N)
     ref, _, _, _ = ozaki.product(X.T.copy(), X.T.copy(), 14)
     Cm = C.reshape(N, N)
     il = np.tril_indices(N)
@@ -44,7 +44,7 @@ def test_gemm_bit_exact_all_layouts(skg, ctx, rng):
             Bb, ldb = encode_b(B, lb)
             for ld_ in (0, 1, 2):
                 Db, ldd = encode_d(np.zeros((M, N)), ld_)
-                ctx.dgemm_ozaki(la, lb, ld_, M, N, K, 1.0, Ab, lda, Bb, ldb, 0.0,
+                ctx.dgemm_ozaki_host(la, lb, ld_, M, N, K, 1.0, Ab, lda, Bb, ldb, 0.0,
                                 Db, ldd, 0.0, slices=14)
                 got = decode_d(Db, ld_, M, N, ldd)
                 assert np.array_equal(got, ref), (la, lb, ld_)
@@ -58,7 +58,7 @@ def test_partition_invariance_is_bitwise(skg, ctx, rng):
     for grid in (0, 1, 3, 64, 257):
         ctx.set_tuning(-1, grid)
         C = np.zeros(N * N)
-        ctx.dsyrk_ozaki(0, N, K, 1.0, X, N, 0.0, C, N)
+        ctx.dsyrk_ozaki_host(0, N, K, 1.0, X, N, 0.0, C, N)
         outs.append(C)
     ctx.set_tuning(-1, 0)
     for C in outs[1:]:
@@ -70,7 +70,7 @@ def test_small_integer_inputs_exact(skg, ctx, rng):
     A = rng.integers(-1000, 1000, (M, K)).astype(np.float64)
     B = rng.integers(-1000, 1000, (K, N)).astype(np.float64)
     D = np.zeros(M * N)
-    ctx.dgemm_ozaki(0, 0, 0, M, N, K, 1.0, np.ascontiguousarray(A.ravel()), K,
+    ctx.dgemm_ozaki_host(0, 0, 0, M, N, K, 1.0, np.ascontiguousarray(A.ravel()), K,
                     np.ascontiguousarray(B.ravel()), N, 0.0, D, N, 0.0)
     assert np.array_equal(D.reshape(M, N), A @ B)
 
@@ -81,7 +81,7 @@ def test_alpha_beta_gamma(skg, ctx, rng):
     B = rng.uniform(-1, 1, (K, N))
     C0 = rng.uniform(-1, 1, (M, N))
     D = C0.copy().ravel()
-    b = ctx.dgemm_ozaki(0, 0, 0, M, N, K, 0.5, np.ascontiguousarray(A.ravel()),
+    b = ctx.dgemm_ozaki_host(0, 0, 0, M, N, K, 0.5, np.ascontiguousarray(A.ravel()),
                         K, np.ascontiguousarray(B.ravel()), N, -2.0, D, N, 0.25)
     ref = 0.5 * (A @ B) - 2.0 * C0 + 0.25
     assert rel_fro(D, ref) < 1e-13 and b < 1e-13
@@ -92,7 +92,7 @@ def test_c3_style_accuracy_vs_double_double(skg, ctx, orc, rng, K, N):
     X = c3_like(rng, K, N)
     Xc = np.asfortranarray(X).ravel(order="F").copy()   # column-major
     C = np.zeros(N * N)
-    bound = ctx.dsyrk_ozaki(skg.COL_MAJOR, N, K, 1.0, Xc, K, 0.0, C, N)
+    bound = ctx.dsyrk_ozaki_host(skg.COL_MAJOR, N, K, 1.0, Xc, K, 0.0, C, N)
     ref = np.zeros(N * N)
     orc.dsyrk(0, N, K, 1.0, np.ascontiguousarray(X.ravel()), N, 0.0, ref, N,
               dd=True)
@@ -109,7 +109,7 @@ def test_rotation_split_accuracy(skg, ctx, orc, rng):
     Q = rng.uniform(-1, 1, (Ns, Ns))
     Pc = np.asfortranarray(P).ravel(order="F").copy()
     D = np.zeros(Nd * Ns)
-    bound = ctx.dgemm_ozaki(skg.COL_MAJOR, skg.ROW_MAJOR, skg.COL_MAJOR, Nd, Ns,
+    bound = ctx.dgemm_ozaki_host(skg.COL_MAJOR, skg.ROW_MAJOR, skg.COL_MAJOR, Nd, Ns,
                             Ns, 1.0, Pc, Nd, Q.ravel().copy(), Ns, 0.0, D, Nd,
                             0.0)
     ref = np.zeros(Nd * Ns)
@@ -121,7 +121,23 @@ def test_rotation_split_accuracy(skg, ctx, orc, rng):
 
 def test_k_limit_is_reported(skg, ctx):
     lib = skg.lib()
+    A = np.zeros(140000 * 4)
+    C = np.zeros(16)
     with pytest.raises(skg.SkgemmError) as e:
-        ctx.dsyrk_ozaki(0, 4, 140000, 1.0, 1, 4, 0.0, 1, 4)
+        ctx.dsyrk_ozaki_host(0, 4, 140000, 1.0, A, 4, 0.0, C, 4)
     assert e.value.code == skg.ERR_NOTSUP
     del lib
+
+
+def test_device_entry_points_reject_host_pointers(skg, ctx):
+    """A host array handed to a device-pointer entry point is refused with
+    SKGEMM_ERR_ARG before any kernel runs."""
+    A = np.ones(64 * 64)
+    C = np.zeros(64 * 64)
+    for call in (lambda: ctx.dsyrk_ozaki(0, 64, 64, 1.0, A, 64, 0.0, C, 64),
+                 lambda: ctx.dsyrk(0, 64, 64, 1.0, A, 64, 0.0, C, 64),
+                 lambda: ctx.dgemm(0, 0, 0, 64, 64, 64, 1.0, A, 64, A, 64, 0.0,
+                                   C, 64)):
+        with pytest.raises(skg.SkgemmError) as e:
+            call()
+        assert e.value.code == skg.ERR_ARG
