@@ -573,8 +573,10 @@ int fill_problem(skgemm_ctx* c, Problem* p, int kind, const Plan& pl, int M,
 }
 
 // Launch one problem (already described) with the given plan.
-int run_single(skgemm_ctx* c, Problem& p, const Plan& pl) {
+int run_single(skgemm_ctx* c, Problem& p, const Plan& pl, const int* pred,
+               bool timed) {
   const TileCfg& t = tile_cfg(pl.cfg);
+  p.pred = pred;
   p.tile_begin = 0;
   p.iter_begin = 0;
   Launch L{};
@@ -588,11 +590,11 @@ int run_single(skgemm_ctx* c, Problem& p, const Plan& pl) {
   L.flags = c->flags;
   L.status = c->status;
   L.spin_limit = c->spin_limit;
-  c->record_begin();
+  if (timed) c->record_begin();
   SKG_HIP(launch_dgemm_sk(pl.cfg, L, c->stream), "Stream-K launch");
   if (p.kchunks > 1)
     SKG_HIP(launch_chunk_reduce(p, c->stream), "chunk reduce launch");
-  c->record_end();
+  if (timed) c->record_end();
   return SKGEMM_OK;
 }
 

@@ -91,7 +91,11 @@ int check_ctx(skgemm_ctx* c);
 int check_dims(int M, int N, int K);
 int fill_problem(skgemm_ctx* c, Problem* p, int kind, const Plan& pl, int M,
                  int N, int K);
-int run_single(skgemm_ctx* c, Problem& p, const Plan& pl);
+// pred: nullptr, or a device flag; the launch does its work only when the
+// flag is non-zero (decided on the device).  timed: record the launch in
+// the context's timing ring.
+int run_single(skgemm_ctx* c, Problem& p, const Plan& pl,
+               const int* pred = nullptr, bool timed = true);
 int prepare_gemm(int la, int lb, int ld_, int M, int N, int K, const double* A,
                  int64_t lda, const double* B, int64_t ldb, double* D,
                  int64_t ldd, Problem* p);

@@ -50,6 +50,7 @@ struct Problem {
   int kchunks;
   int kci;
   double* part;
+  const int* pred;     // single launches: run only when *pred != 0
   int64_t tile_begin;  // first flattened tile index of this problem
   int64_t iter_begin;  // first flattened MAC-iteration index
   int64_t num_tiles;

@@ -364,6 +364,10 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
   const int per = G >> 3, rem = G & 7, xcd = g & 7;
   const int w = xcd * per + (xcd < rem ? xcd : rem) + (g >> 3);
 
+  // A predicated launch (the split form's fallback) exits at once unless
+  // the predicate written by an earlier kernel on the stream is set.
+  if (!BATCHED && L.single.pred != nullptr && *L.single.pred == 0) return;
+
   int64_t it = wg_start(w, L.total_iters, G);
   const int64_t end = wg_start(w + 1, L.total_iters, G);
   int pi = 0;
@@ -659,6 +663,7 @@ __global__ void chunk_reduce_kernel(Problem p) {
   const int col = blockIdx.x * 64 + (threadIdx.x & 63);
   const int row = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (row >= p.M || col >= p.N || (p.syrk && row < col)) return;
+  if (p.pred != nullptr && *p.pred == 0) return;
   const int64_t mn = (int64_t)p.M * p.N;
   const gdouble* src = (const gdouble*)(p.part) + (int64_t)row * p.N + col;
   double sum = 0.0;

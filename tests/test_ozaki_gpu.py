@@ -3,6 +3,8 @@ SYRK.  Bars: bit-for-bit equality with the CPU restatement (oracle/ozaki.py:
 exact integer product, no CRT) on the same inputs; relative Frobenius error
 <= 1e-13 against the double-double oracle and <= the returned bound on the
 c3-style inputs (columns scaled by 10^u, u ~ U(0, 6))."""
+import os
+
 import numpy as np
 import pytest
 
@@ -141,3 +143,101 @@ def test_device_entry_points_reject_host_pointers(skg, ctx):
         with pytest.raises(skg.SkgemmError) as e:
             call()
         assert e.value.code == skg.ERR_ARG
+
+
+@pytest.mark.parametrize("L", [8, 12, 16, 17, 20])
+def test_moduli_counts_bit_exact(skg, ctx, rng, L):
+    """Every modulus count: up to 14 moduli the two-limb residue split, above
+    it the three-limb split (|x| up to 2^76); up to 16 moduli the direct CRT,
+    above it Garner's form."""
+    K, N = 257, 40
+    X = c3_like(rng, K, N)
+    C = np.zeros(N * N)
+    ctx.dsyrk_ozaki_host(0, N, K, 1.0, np.ascontiguousarray(X.ravel()), N, 0.0,
+                         C, N, slices=L)
+    ref, _, _, _ = ozaki.product(X.T.copy(), X.T.copy(), L)
+    il = np.tril_indices(N)
+    assert np.array_equal(C.reshape(N, N)[il], ref[il])
+
+
+def test_every_plan_gives_identical_bits(skg, ctx, rng):
+    """Data-parallel and Stream-K schedules of every INT8 tile configuration
+    produce the same integers, hence the same FP64 result."""
+    K, N = 1500, 600
+    X = c3_like(rng, K, N).ravel()
+    outs = []
+    try:
+        for plan in ("0,1", "1,1", "2,1", "3,1", "0,0", "1,0,77", "2,0",
+                     "3,0,300"):
+            os.environ["SKGEMM_OZ_PLAN"] = plan
+            C = np.zeros(N * N)
+            ctx.dsyrk_ozaki_host(0, N, K, 1.0, X, N, 0.0, C, N)
+            outs.append(C)
+    finally:
+        os.environ.pop("SKGEMM_OZ_PLAN", None)
+    for C in outs[1:]:
+        assert np.array_equal(C, outs[0])
+
+
+def test_rotation_plans_identical(skg, ctx, rng):
+    M, N, K = 2000, 300, 700
+    A = rng.standard_normal((M, K))
+    B = rng.uniform(-1, 1, (K, N))
+    Ab = np.ascontiguousarray(A.ravel())
+    Bb = np.ascontiguousarray(B.ravel())
+    outs = []
+    try:
+        for plan in ("0,1", "1,1", "2,0", "3,0"):
+            os.environ["SKGEMM_OZ_PLAN"] = plan
+            D = np.zeros(M * N)
+            ctx.dgemm_ozaki_host(0, 0, 0, M, N, K, 1.0, Ab, K, Bb, N, 0.0, D, N)
+            outs.append(D)
+    finally:
+        os.environ.pop("SKGEMM_OZ_PLAN", None)
+    for D in outs[1:]:
+        assert np.array_equal(D, outs[0])
+    assert rel_fro(outs[0], (A @ B).ravel()) < 1e-13
+
+
+def test_data_parallel_syrk_vs_double_double(skg, ctx, orc, rng):
+    """N = 1024 takes the data-parallel schedule (36 tiles of 128 per
+    modulus)."""
+    K, N = 2048, 1024
+    X = c3_like(rng, K, N)
+    Xc = np.asfortranarray(X).ravel(order="F").copy()
+    C = np.zeros(N * N)
+    bound = ctx.dsyrk_ozaki_host(skg.COL_MAJOR, N, K, 1.0, Xc, K, 0.0, C, N)
+    ref = np.zeros(N * N)
+    orc.dsyrk(0, N, K, 1.0, np.ascontiguousarray(X.ravel()), N, 0.0, ref, N,
+              dd=True)
+    il = np.tril_indices(N)
+    err = rel_fro(C.reshape(N, N)[il], ref.reshape(N, N)[il])
+    assert err < 1e-13 and err <= bound
+
+
+@pytest.mark.parametrize("kind", ["nan", "inf", "tiny", "huge"])
+def test_fallback_to_plain_fp64(skg, ctx, rng, kind):
+    """A row whose scale cannot be represented (non-finite, or a squared norm
+    outside [2^-1000, 2^1000]) sends the call to the plain FP64 path: the
+    result is bit-identical to skgemm_dgemm and the bound reads -1."""
+    M, N, K = 70, 50, 90
+    A = rng.uniform(-1, 1, (M, K))
+    B = rng.uniform(-1, 1, (K, N))
+    if kind == "nan":
+        A[3, 5] = np.nan
+    elif kind == "inf":
+        B[7, 2] = np.inf
+    elif kind == "tiny":
+        A[10] *= 1e-170
+    else:
+        B[:, 4] *= 1e160
+    C0 = rng.uniform(-1, 1, (M, N))
+    D1 = C0.copy().ravel()
+    D2 = C0.copy().ravel()
+    Ab = np.ascontiguousarray(A.ravel())
+    Bb = np.ascontiguousarray(B.ravel())
+    b = ctx.dgemm_ozaki_host(0, 0, 0, M, N, K, 1.5, Ab, K, Bb, N, 0.5, D1, N,
+                             0.25)
+    ctx.dgemm_host(0, 0, 0, M, N, K, 1.5, Ab, K, Bb, N, 0.5, D2, N, 0.25)
+    assert b == -1.0
+    assert np.array_equal(D1, D2, equal_nan=True)
