@@ -157,7 +157,7 @@ int skgemm_dsyrk_host(skgemm_ctx* ctx, int layout_a, int N, int K,
  * the product computed in the Ozaki split form (scheme II, CRT variant):
  * power-of-two scaling of rows of A and columns of B from their 2-norms,
  * each scaled operand cut into `slices` INT8 residue planes modulo
- * pairwise-coprime moduli <= 256 (8..20; 0 = 14, which bounds the error
+ * pairwise-coprime moduli <= 256 (8..16; 0 = 14, which bounds the error
  * below 1e-13 for K up to 131072 on inputs like the DFT orbitals), one INT8
  * MFMA product per plane with exact INT32 accumulation in one persistent
  * Stream-K launch, and an exact CRT reconstruction rounded once to FP64.

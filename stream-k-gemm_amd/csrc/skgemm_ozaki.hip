@@ -10,11 +10,12 @@
 //  2. split: for L pairwise-coprime moduli m_l <= 256, the symmetric
 //     residues of A' and B' (in [-128, 127]) as INT8 planes, K contiguous;
 //  3. L INT8 products on v_mfma_i32_32x32x32_i8 with exact INT32
-//     accumulation (|sum| <= Kpad * 2^14 < 2^31), reduced mod m_l, in one
-//     persistent Stream-K launch over (modulus, tile) units;
-//  4. CRT: Garner's mixed-radix digits (symmetric), exact 128-bit Horner,
-//     one rounding to FP64, exact power-of-two unscaling, epilogue
-//     alpha*C + beta*D + gamma.
+//     accumulation (|sum| <= Kpad * 127^2 < 2^31 for the odd moduli; for
+//     m = 256 a wrap modulo 2^32 keeps the residue), reduced mod m_l, in one
+//     launch over (modulus, tile) units, data-parallel or Stream-K;
+//  4. CRT: x = sum_l r_l W_l - q P in exact 128-bit arithmetic (q from an
+//     FP64 estimate), one rounding to FP64, exact power-of-two unscaling,
+//     epilogue alpha*C + beta*D + gamma.
 // Integer steps are exact, so a result does not depend on the partition of
 // the work (bit-identical for any grid).  The only approximation is step 1's
 // rounding to integers; its a-posteriori bound is returned with the result.
@@ -36,12 +37,13 @@
 namespace skg {
 namespace oz {
 
-constexpr int kMaxMod = 20;
+// 16 moduli keep P < 2^128, so the reconstruction is exact in 128-bit
+// integers (|x| <= P/4 < 2^126).
+constexpr int kMaxMod = 16;
 // Pairwise coprime: 256 = 2^8, 255 = 3*5*17, 253 = 11*23, 247 = 13*19,
 // 217 = 7*31, the others prime.
-static const int kModuli[kMaxMod] = {256, 255, 253, 251, 247, 241, 239,
-                                     233, 229, 227, 223, 217, 211, 199,
-                                     197, 193, 191, 181, 179, 173};
+static const int kModuli[kMaxMod] = {256, 255, 253, 251, 247, 241, 239, 233,
+                                     229, 227, 223, 217, 211, 199, 197, 193};
 constexpr int kMaxKpad = 131072;  // Kpad * 128^2 <= 2^31
 
 struct Tables {
@@ -52,14 +54,12 @@ struct Tables {
   double inv_m[kMaxMod];  // fl(1 / m_l)
   double c26[kMaxMod];    // 2^26 mod m_l, symmetric
   double c52[kMaxMod];    // 2^52 mod m_l, symmetric
-  uint8_t cinv[kMaxMod][kMaxMod];  // m_j^{-1} mod m_l (j < l), Garner form
-  // Direct CRT (L <= 16, P < 2^128): x = sum_l r_l W_l - q P (mod 2^128),
+  // CRT: x = sum_l r_l W_l - q P (mod 2^128),
   // W_l = (P / m_l) ((P / m_l)^{-1} mod m_l), q = floor(sum_l r_l w_l + 1/2)
   // with w_l = W_l / P rounded to FP64.
-  uint64_t W_lo[kMaxMod], W_hi[kMaxMod];
+  uint32_t W[kMaxMod][4];  // W_l in 32-bit limbs, least significant first
   double wfrac[kMaxMod];
-  uint64_t P_lo, P_hi;
-  int direct;
+  uint32_t P[4];
 };
 
 typedef int v4i __attribute__((ext_vector_type(4)));
@@ -266,19 +266,24 @@ __global__ __launch_bounds__(256) void split_k_kernel(
   }
 }
 
-// Operand contiguous along o: workgroup of 64 o x 128 k; thread (o, 32
-// consecutive k) loads along o (coalesced) and writes 32 bytes per plane,
-// so each workgroup writes whole 128-byte rows of every plane.
+// Operand contiguous along o: workgroup of 64 o x 128 k.  Thread (o, 32
+// consecutive k) loads along o (coalesced), then each plane's 64 x 128-byte
+// block is transposed through LDS so that every 8 lanes store one whole
+// 128-byte row segment.  LDS row pitch 144 bytes: the per-lane dword
+// writes (lanes = 64 rows) and the 16-byte row reads are conflict-free.
+constexpr int kTPitch = 144;
+
 template <bool WIDE>
 __global__ __launch_bounds__(256) void split_o_kernel(
     Operand op, int K, const int* sigma, Tables T, int8_t* planes,
     int64_t plane, int kpad, int opad) {
+  __shared__ __align__(16) uint8_t tile[2][64 * kTPitch];
   const int kblocks = kpad / 128;
   const int ob = blockIdx.x / kblocks;
   const int kb = kblocks - 1 - (blockIdx.x - ob * kblocks);
-  const int o = ob * 64 + (threadIdx.x & 63);
-  const int k0 = kb * 128 + (threadIdx.x >> 6) * 32;
-  if (o >= opad) return;
+  const int ol = threadIdx.x & 63, kq = threadIdx.x >> 6;
+  const int o = ob * 64 + ol;
+  const int k0 = kb * 128 + kq * 32;
   const bool in_o = o < op.extent;
   const int s = in_o ? sigma[o] : 0;
   Split p[32];
@@ -288,19 +293,29 @@ __global__ __launch_bounds__(256) void split_o_kernel(
     const double a = (in_o && k < K) ? op.ptr[op_addr(op, o, k)] : 0.0;
     p[j] = split26<WIDE>(rint(ldexp(a, s)));
   }
-  int8_t* dst = planes + (int64_t)o * kpad + k0;
+  // Store side: chunk q = threadIdx.x + 256 i (i = 0, 1) is row q / 8,
+  // 16-byte chunk q % 8 of the 64 x 128 block.
+  const int r0 = threadIdx.x >> 3, c0 = threadIdx.x & 7;
+  int8_t* dst0 = planes + (int64_t)(ob * 64 + r0) * kpad + kb * 128 + c0 * 16;
+  const int64_t row32 = (int64_t)32 * kpad;
   for (int l = 0; l < T.L; ++l) {
     const double m = T.md[l], inv = T.inv_m[l], c = T.c26[l], c2 = T.c52[l];
-    uint32_t w[8];
+    uint8_t* t = tile[l & 1];
+    uint32_t* trow = reinterpret_cast<uint32_t*>(t + ol * kTPitch + kq * 32);
 #pragma unroll
     for (int q = 0; q < 8; ++q)
-      w[q] = pack4(residue<WIDE>(p[4 * q], m, inv, c, c2),
-                   residue<WIDE>(p[4 * q + 1], m, inv, c, c2),
-                   residue<WIDE>(p[4 * q + 2], m, inv, c, c2),
-                   residue<WIDE>(p[4 * q + 3], m, inv, c, c2));
-    uint4* d4 = reinterpret_cast<uint4*>(dst + l * plane);
-    d4[0] = make_uint4(w[0], w[1], w[2], w[3]);
-    d4[1] = make_uint4(w[4], w[5], w[6], w[7]);
+      trow[q] = pack4(residue<WIDE>(p[4 * q], m, inv, c, c2),
+                      residue<WIDE>(p[4 * q + 1], m, inv, c, c2),
+                      residue<WIDE>(p[4 * q + 2], m, inv, c, c2),
+                      residue<WIDE>(p[4 * q + 3], m, inv, c, c2));
+    __syncthreads();
+    const uint4 v0 = *reinterpret_cast<const uint4*>(t + r0 * kTPitch + c0 * 16);
+    const uint4 v1 =
+        *reinterpret_cast<const uint4*>(t + (r0 + 32) * kTPitch + c0 * 16);
+    int8_t* d = dst0 + l * plane;
+    *reinterpret_cast<uint4*>(d) = v0;
+    *reinterpret_cast<uint4*>(d + row32) = v1;
+    // tile[l & 1] is rewritten two planes later, after the next barrier.
   }
 }
 
@@ -318,11 +333,13 @@ struct GemmLaunch {
   int L;
   int mods[kMaxMod];
   double inv_mods[kMaxMod];
-  uint8_t* R;  // [L][M][N] residues in [0, m)
+  uint8_t* R;  // [L][N][mp] residues in [0, m), column-major
   int64_t r_plane;
+  int mp;      // column pitch of R: M rounded up to 16
   int grid;
   int dp;          // 1: each workgroup takes whole units (no fix-up)
-  int64_t units;   // L * ntiles
+  int kc, kcs;     // data-parallel: K chunks per tile, MAC steps per chunk
+  int64_t units;   // L * kc * ntiles (data-parallel), L * ntiles (Stream-K)
   int64_t total;   // units * ipt (Stream-K mode)
   int* ws;
   int* flags;
@@ -344,16 +361,17 @@ __device__ __forceinline__ void tri_tile(int64_t t, int& ti, int& tj) {
 }
 
 // LDS row of BK bytes in 16-byte chunks; chunk c of row r is stored at
-// chunk c ^ f(r) so that the 8 rows one 128-byte LDS cycle serves (a
-// ds_read_b128 of 32 rows at one chunk index, or a row-wise store) fall in
-// 8 distinct 16-byte bank groups.
+// chunk c ^ f(r).  f is chosen so that a ds_read_b128 of 32 rows at one
+// chunk index and a row-wise ds_write_b128 are free of bank conflicts both
+// when 8 lanes share a 32-bank cycle and when 16 lanes share a 64-bank
+// cycle (checked exhaustively on the host for both models).
 template <int BK>
 __device__ __forceinline__ int lds_off(int r, int c) {
-  if (BK == 128) return r * 128 + ((c ^ (r & 7)) << 4);
-  return r * 64 + ((c ^ ((r >> 1) & 3)) << 4);
+  if (BK == 128) return r * 128 + ((c ^ (r & 7) ^ ((r >> 3) & 1)) << 4);
+  return r * 64 + ((c ^ (r & 3) ^ ((r >> 2) & 3)) << 4);
 }
 
-template <int BM, int BN, int WM, int WN, int BK>
+template <int BM, int BN, int WM, int WN, int BK, int PD>
 __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
   constexpr int NT = WM * WN * 64;
   constexpr int TM = BM / WM, TN = BN / WN;
@@ -386,12 +404,10 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
   }
 
   v16i acc[MB][NB];
-  v4i ra[CA], rb[CB];
 
   // MAC steps [kb, ke) of unit `unit` into acc.
-  auto mac = [&](int64_t unit, int kb, int ke, int& l, int& m0, int& n0) {
-    l = (int)(unit / G.ntiles);
-    const int64_t tile = unit - (int64_t)l * G.ntiles;
+  // MAC steps [kb, ke) of tile `tile` of plane l into acc.
+  auto mac = [&](int l, int64_t tile, int kb, int ke, int& m0, int& n0) {
     int ti, tj;
     if (G.syrk) {
       tri_tile(tile, ti, tj);
@@ -410,7 +426,12 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
 #pragma unroll
         for (int v = 0; v < 16; ++v) acc[i][j][v] = 0;
 
-    auto load = [&](int kk) {
+    // Two register sets: while step kk is multiplied out of LDS[cur], step
+    // kk + 1 waits in one set and step kk + 2 is in flight into the other.
+    // Steps past the unit's end are clamped to its last step (in bounds;
+    // never stored).
+    auto load = [&](v4i(&xa)[CA], v4i(&xb)[CB], int kk) {
+      kk = kk < ke ? kk : ke - 1;
       const __amdgpu_buffer_rsrc_t sa = __builtin_amdgcn_make_buffer_rsrc(
           const_cast<int8_t*>(baseA + (int64_t)kk * BK), (short)0,
           kNumRecords, kRsrcWord3);
@@ -419,31 +440,25 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
           kNumRecords, kRsrcWord3);
 #pragma unroll
       for (int p = 0; p < CA; ++p)
-        ra[p] = __builtin_amdgcn_raw_buffer_load_b128(
+        xa[p] = __builtin_amdgcn_raw_buffer_load_b128(
             sa, rowA[p] * G.kpad + chA[p] * 16, 0, 0);
 #pragma unroll
       for (int p = 0; p < CB; ++p)
-        rb[p] = __builtin_amdgcn_raw_buffer_load_b128(
+        xb[p] = __builtin_amdgcn_raw_buffer_load_b128(
             sb, rowB[p] * G.kpad + chB[p] * 16, 0, 0);
     };
-    auto store = [&](int buf) {
+    auto store = [&](const v4i(&xa)[CA], const v4i(&xb)[CB], int buf) {
       int8_t* base = lds + buf * (BM + BN) * BK;
 #pragma unroll
       for (int p = 0; p < CA; ++p)
-        *reinterpret_cast<v4i*>(base + lds_off<BK>(rowA[p], chA[p])) = ra[p];
+        *reinterpret_cast<v4i*>(base + lds_off<BK>(rowA[p], chA[p])) = xa[p];
 #pragma unroll
       for (int p = 0; p < CB; ++p)
         *reinterpret_cast<v4i*>(base + BM * BK + lds_off<BK>(rowB[p], chB[p])) =
-            rb[p];
+            xb[p];
     };
-    load(kb);
-    store(0);
-    __syncthreads();
-    int cur = 0;
-    for (int kk = kb; kk < ke; ++kk) {
-      const bool more = kk + 1 < ke;
-      if (more) load(kk + 1);
-      const int8_t* la = lds + cur * (BM + BN) * BK;
+    auto compute = [&](int buf) {
+      const int8_t* la = lds + buf * (BM + BN) * BK;
       const int8_t* lb = la + BM * BK;
 #pragma unroll
       for (int s = 0; s < BK / 32; ++s) {
@@ -464,49 +479,123 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
             acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j],
                                                               acc[i][j], 0, 0, 0);
       }
-      if (more) store(cur ^ 1);
+    };
+    v4i ra0[CA], rb0[CB], ra1[CA], rb1[CB];
+    int kk = kb;
+    int cur = 0;
+    if (PD == 1) {
+      // One register set: step kk + 1 is in flight while step kk is
+      // multiplied out (for tiles whose two sets would not fit).
+      load(ra0, rb0, kk);
+      store(ra0, rb0, 0);
+      __syncthreads();
+      for (;;) {
+        load(ra0, rb0, kk + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(cur);
+        if (kk + 1 >= ke) break;
+        store(ra0, rb0, cur ^ 1);
+        __syncthreads();
+        cur ^= 1;
+        ++kk;
+      }
+      __syncthreads();
+      return;
+    }
+    load(ra0, rb0, kk);
+    load(ra1, rb1, kk + 1);
+    store(ra0, rb0, 0);
+    __syncthreads();
+    // sched_barrier: keep each prefetch issued ahead of the MFMAs it is
+    // meant to overlap (the scheduler would otherwise sink it below them).
+    for (;;) {
+      load(ra0, rb0, kk + 2);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(cur);
+      if (kk + 1 >= ke) break;
+      store(ra1, rb1, cur ^ 1);
       __syncthreads();
       cur ^= 1;
+      ++kk;
+      load(ra1, rb1, kk + 2);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(cur);
+      if (kk + 1 >= ke) break;
+      store(ra0, rb0, cur ^ 1);
+      __syncthreads();
+      cur ^= 1;
+      ++kk;
     }
+    __syncthreads();  // LDS is refilled by the next unit's prologue
   };
 
-  // acc mod m_l -> R[l] (lower triangle only for SYRK).  acc is exact in
-  // FP64 and floor(acc * fl(1/m)) is off by at most one from floor(acc / m)
-  // (only when acc / m is an integer), so one correction each way suffices.
-  auto write_residues = [&](int l, int m0, int n0) {
+  // acc mod m_l -> R[l], column-major with column pitch G.mp (a multiple
+  // of 16).  The tile's residues are transposed through LDS (column-major,
+  // pitch BM + 16: dword writes of 4 rows and 16-byte reads are free of bank
+  // conflicts) and stored as 16-byte row runs of a column.  Padding rows and
+  // the upper part of diagonal SYRK tiles are stored too; the CRT pass
+  // never reads them.  acc is exact in FP64 and floor(acc * fl(1/m)) is off
+  // by at most one from floor(acc / m), so one correction each way suffices.
+  auto write_residues = [&](int l, int out_plane, int m0, int n0) {
     const int m = G.mods[l];
     const double inv = G.inv_mods[l];
-    uint8_t* rp = G.R + l * G.r_plane;
+    constexpr int TP = BM + 16;
     // Accumulator element v of lane x is row 8(v/4) + 4(x/32) + v%4,
     // column x%32 of its 32x32 block (measured on gfx950).
 #pragma unroll
     for (int i = 0; i < MB; ++i)
 #pragma unroll
       for (int j = 0; j < NB; ++j) {
-        const int col = n0 + wn * TN + j * 32 + (lane & 31);
+        const int col = wn * TN + j * 32 + (lane & 31);
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-          const int row = m0 + wm * TM + i * 32 + (v >> 2) * 8 +
-                          (lane >> 5) * 4 + (v & 3);
-          if (row < G.M && col < G.N && (!G.syrk || row >= col)) {
-            const int a = acc[i][j][v];
+        for (int v4 = 0; v4 < 4; ++v4) {
+          uint32_t word = 0;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int a = acc[i][j][v4 * 4 + u];
             int r = a - m * (int)floor((double)a * inv);
             r += (r < 0) ? m : 0;
             r -= (r >= m) ? m : 0;
-            rp[(int64_t)row * G.N + col] = (uint8_t)r;
+            word |= (uint32_t)r << (8 * u);
           }
+          const int row = wm * TM + i * 32 + v4 * 8 + (lane >> 5) * 4;
+          *reinterpret_cast<uint32_t*>(lds + col * TP + row) = word;
         }
       }
+    __syncthreads();
+    uint8_t* rp = G.R + out_plane * G.r_plane;
+    constexpr int CH = BM / 16;  // 16-byte runs per tile column
+#pragma unroll
+    for (int q0 = 0; q0 < BN * CH; q0 += NT) {
+      const int q = q0 + tid;
+      const int col = q / CH, rc = q % CH;
+      const int gc = n0 + col, gr = m0 + rc * 16;
+      if (gc < G.N && gr < G.mp) {
+        const uint4 val =
+            *reinterpret_cast<const uint4*>(lds + col * TP + rc * 16);
+        *reinterpret_cast<uint4*>(rp + (int64_t)gc * G.mp + gr) = val;
+      }
+    }
+    __syncthreads();  // LDS is refilled by the next unit's prologue
   };
 
   if (G.dp) {
-    // Whole units, round-robin: in each round the workgroups of one XCD
-    // take consecutive units, i.e. tiles of one plane that share panels in
-    // that XCD's L2 and advance through K together.
+    // Whole units (plane l, K chunk c, tile), tile fastest, round-robin: in
+    // each round the workgroups of one XCD take consecutive units, i.e.
+    // tiles of one plane and K chunk that share panels in that XCD's L2 and
+    // advance through K together.  Chunk c writes its own residue plane
+    // (c L + l); the CRT pass adds the chunks modulo m_l.
+    const int64_t per_plane = (int64_t)G.kc * G.ntiles;
     for (int64_t u = w; u < G.units; u += Gn) {
-      int l, m0, n0;
-      mac(u, 0, G.ipt, l, m0, n0);
-      write_residues(l, m0, n0);
+      const int l = (int)(u / per_plane);
+      const int64_t r = u - (int64_t)l * per_plane;
+      const int c = (int)(r / G.ntiles);
+      const int64_t tile = r - (int64_t)c * G.ntiles;
+      const int kb = c * G.kcs;
+      const int ke = min(G.ipt, kb + G.kcs);
+      int m0, n0;
+      mac(l, tile, kb, ke, m0, n0);
+      write_residues(l, c * G.L + l, m0, n0);
     }
     return;
   }
@@ -518,8 +607,9 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
     const int64_t t0 = unit * G.ipt;
     const int kb = (int)(it - t0);
     const int ke = (int)((end - t0) < G.ipt ? (end - t0) : G.ipt);
-    int l, m0, n0;
-    mac(unit, kb, ke, l, m0, n0);
+    const int l = (int)(unit / G.ntiles);
+    int m0, n0;
+    mac(l, unit - (int64_t)l * G.ntiles, kb, ke, m0, n0);
 
     // Stream-K fix-up on exact INT32 partials (device-coherent sc1 traffic,
     // no L2-wide fences; same protocol as the FP64 kernel).
@@ -573,7 +663,7 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
                   kAgentCoherent);
       }
     }
-    if (write_out) write_residues(l, m0, n0);
+    if (write_out) write_residues(l, l, m0, n0);
     it = t0 + ke;
   }
 }
@@ -581,8 +671,10 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
 // ---- 4. CRT reconstruction and epilogue ----------------------------------------
 
 struct CrtLaunch {
-  const uint8_t* R;
+  const uint8_t* R;  // [kc][L][N][mp]
   int64_t r_plane;
+  int mp;
+  int kc;            // K chunks whose residues are added modulo m_l
   Tables T;
   const int* sig_a;
   const int* sig_b;
@@ -590,61 +682,99 @@ struct CrtLaunch {
   Output d;
   int M, N, syrk;
   double alpha, beta, gamma;
-  double* fro2;  // += ||C||_F^2 (SYRK: of the full symmetric matrix)
+  double* fro2;  // per workgroup: its part of ||C||_F^2 (SYRK: both halves)
 };
 
-typedef unsigned __int128 u128;
-
-__device__ __forceinline__ __int128 crt_direct(const CrtLaunch& p, int64_t idx) {
-  u128 S = 0;
+// x = S - q P with S = sum_l r_l W_l, in 32-bit limbs held in 64-bit
+// accumulators (each stays below 2^45 before the carries are propagated).
+// Exact: the FP64 estimate f of S / P + 1/2 is within 2^-36 of the true
+// value, whose fractional part lies in [1/4, 3/4] because |x| <= P / 4.
+// r[l] holds the residue in byte e of word l.  Returns x = hi 2^64 + lo.
+__device__ __forceinline__ void crt(const Tables& T, const uint32_t* rw, int e,
+                                    int64_t& hi, uint64_t& lo) {
+  uint64_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   double f = 0.5;
-  for (int l = 0; l < p.T.L; ++l) {
-    const uint32_t r = p.R[l * p.r_plane + idx];
-    const u128 W = ((u128)p.T.W_hi[l] << 64) | p.T.W_lo[l];
-    S += W * r;
-    f = fma((double)r, p.T.wfrac[l], f);
+#pragma unroll
+  for (int l = 0; l < kMaxMod; ++l) {
+    if (l < T.L) {
+      const uint32_t r = (rw[l] >> (8 * e)) & 255u;
+      a0 += (uint64_t)T.W[l][0] * r;
+      a1 += (uint64_t)T.W[l][1] * r;
+      a2 += (uint64_t)T.W[l][2] * r;
+      a3 += (uint64_t)T.W[l][3] * r;
+      f = fma((double)r, T.wfrac[l], f);
+    }
   }
   const uint64_t q = (uint64_t)floor(f);
-  const u128 P = ((u128)p.T.P_hi << 64) | p.T.P_lo;
-  return (__int128)(S - P * q);
+  int64_t b0 = (int64_t)(a0 - (uint64_t)T.P[0] * q);
+  int64_t b1 = (int64_t)(a1 - (uint64_t)T.P[1] * q);
+  int64_t b2 = (int64_t)(a2 - (uint64_t)T.P[2] * q);
+  int64_t b3 = (int64_t)(a3 - (uint64_t)T.P[3] * q);
+  b1 += b0 >> 32;
+  b2 += b1 >> 32;
+  b3 += b2 >> 32;
+  lo = (uint64_t)(uint32_t)b0 | ((uint64_t)(uint32_t)b1 << 32);
+  hi = (int64_t)(((uint64_t)b3 << 32) | (uint64_t)(uint32_t)b2);
 }
 
-__device__ __forceinline__ __int128 crt_garner(const CrtLaunch& p, int64_t idx) {
-  int v[kMaxMod];
-  for (int l = 0; l < p.T.L; ++l) {
-    const int m = p.T.m[l];
-    int t = p.R[l * p.r_plane + idx];
-    for (int j = 0; j < l; ++j) {
-      t = (t - v[j]) % m;
-      if (t < 0) t += m;
-      t = (t * (int)p.T.cinv[j][l]) % m;
-    }
-    const int hi = (m - 1) >> 1;
-    v[l] = t > hi ? t - m : t;
-  }
-  __int128 x = v[p.T.L - 1];
-  for (int l = p.T.L - 2; l >= 0; --l) x = x * p.T.m[l] + v[l];
-  return x;
-}
-
+// Blocks of 256 rows x 4 columns; a thread takes 4 consecutive rows of one
+// column (one dword of every residue plane: all loads issued together).
+// Grid-stride; SYRK skips blocks strictly above the diagonal.  Each
+// workgroup writes its part of ||C||_F^2 to fro2[blockIdx.x] (summed on the
+// host in a fixed order).
 __global__ __launch_bounds__(256) void crt_kernel(CrtLaunch p) {
   __shared__ double red[256];
   if (*p.fallback != 0) return;
-  const int col = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int row = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int nbr = (p.M + 255) / 256, nbc = (p.N + 3) / 4;
+  const int64_t nblocks = (int64_t)nbc * nbr;
   double contrib = 0.0;
-  if (row < p.M && col < p.N && (!p.syrk || row >= col)) {
-    const int64_t idx = (int64_t)row * p.N + col;
-    const __int128 x = p.T.direct ? crt_direct(p, idx) : crt_garner(p, idx);
-    const long long xh = (long long)(x >> 64);
-    const unsigned long long xl = (unsigned long long)x;
-    const double d = (double)xh * 18446744073709551616.0 + (double)xl;
-    const double c = ldexp(d, -(p.sig_a[row] + p.sig_b[col]));
-    contrib = (p.syrk && row != col) ? 2.0 * c * c : c * c;
-    double* dst = p.d.ptr + out_addr(p.d, row, col);
-    double val = p.alpha * c;
-    if (p.beta != 0.0) val = fma(p.beta, *dst, val);
-    *dst = val + p.gamma;
+  for (int64_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    const int bc = (int)(b / nbr), br = (int)(b - (int64_t)bc * nbr);
+    if (p.syrk && br * 256 + 255 < bc * 4) continue;
+    const int row0 = br * 256 + (threadIdx.x & 63) * 4;
+    const int col = bc * 4 + (threadIdx.x >> 6);
+    if (row0 >= p.M || col >= p.N || (p.syrk && row0 + 3 < col)) continue;
+    const uint8_t* src = p.R + (int64_t)col * p.mp + row0;
+    uint32_t rw[kMaxMod];
+#pragma unroll
+    for (int l = 0; l < kMaxMod; ++l)
+      rw[l] = l < p.T.L
+                  ? *reinterpret_cast<const uint32_t*>(src + l * p.r_plane)
+                  : 0u;
+    if (p.kc > 1) {
+      // Sum the chunks' residues bytewise (each sum < 2^16), reduce mod m_l.
+      for (int l = 0; l < p.T.L; ++l) {
+        uint32_t s0 = rw[l] & 255u, s1 = (rw[l] >> 8) & 255u,
+                 s2 = (rw[l] >> 16) & 255u, s3 = rw[l] >> 24;
+        for (int c = 1; c < p.kc; ++c) {
+          const uint32_t v = *reinterpret_cast<const uint32_t*>(
+              src + ((int64_t)c * p.T.L + l) * p.r_plane);
+          s0 += v & 255u;
+          s1 += (v >> 8) & 255u;
+          s2 += (v >> 16) & 255u;
+          s3 += v >> 24;
+        }
+        const uint32_t m = (uint32_t)p.T.m[l];
+        rw[l] = (s0 % m) | ((s1 % m) << 8) | ((s2 % m) << 16) | ((s3 % m) << 24);
+      }
+    }
+    const int sb = p.sig_b[col];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int row = row0 + e;
+      if (row < p.M && (!p.syrk || row >= col)) {
+        int64_t xh;
+        uint64_t xl;
+        crt(p.T, rw, e, xh, xl);
+        const double d = (double)xh * 18446744073709551616.0 + (double)xl;
+        const double c = ldexp(d, -(p.sig_a[row] + sb));
+        contrib += (p.syrk && row != col) ? 2.0 * c * c : c * c;
+        double* dst = p.d.ptr + out_addr(p.d, row, col);
+        double val = p.alpha * c;
+        if (p.beta != 0.0) val = fma(p.beta, *dst, val);
+        *dst = val + p.gamma;
+      }
+    }
   }
   red[threadIdx.x] = contrib;
   __syncthreads();
@@ -652,45 +782,56 @@ __global__ __launch_bounds__(256) void crt_kernel(CrtLaunch p) {
     if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0 && red[0] != 0.0) atomicAdd(p.fro2, red[0]);
+  if (threadIdx.x == 0) p.fro2[blockIdx.x] = red[0];
 }
+
+constexpr int kCrtGrid = 4096;
 
 // ---- host ----------------------------------------------------------------------
 
 struct GemmCfg {
-  int bm, bn, wm, wn, bk;
+  int bm, bn, wm, wn, bk, pd;
 };
-// 0: 256x256, 8 waves of 64x128, BK 128 (128 KB LDS, one workgroup per CU)
-// 1: 128x128, 4 waves of 64x64,  BK 128 (64 KB LDS)
-// 2: 128x128, 2 waves of 64x128, BK 128 (64 KB LDS)
-// 3: 256x256, 8 waves of 64x128, BK 64  (64 KB LDS)
-static const GemmCfg kGemmCfgs[] = {{256, 256, 4, 2, 128},
-                                    {128, 128, 2, 2, 128},
-                                    {128, 128, 2, 1, 128},
-                                    {256, 256, 4, 2, 64}};
-constexpr int kNumGemmCfgs = 4;
+// 0: 256x256, 8 waves of 64x128, BK 64,  2-deep prefetch (64 KB LDS)
+// 1: 128x128, 4 waves of 64x64,  BK 128, 2-deep (64 KB LDS)
+// 2: 128x128, 2 waves of 64x128, BK 128, 2-deep (64 KB LDS)
+// 3: 128x128, 4 waves of 64x64,  BK 64,  2-deep (32 KB LDS)
+// 4: 256x256, 8 waves of 64x128, BK 128, 1-deep (128 KB LDS)
+static const GemmCfg kGemmCfgs[] = {{256, 256, 4, 2, 64, 2},
+                                    {128, 128, 2, 2, 128, 2},
+                                    {128, 128, 2, 1, 128, 2},
+                                    {128, 128, 2, 2, 64, 2},
+                                    {256, 256, 4, 2, 128, 1}};
+constexpr int kNumGemmCfgs = 5;
 
-template <int BM, int BN, int WM, int WN, int BK>
-static const void* kfn() {
-  return reinterpret_cast<const void*>(gemm_i8_kernel<BM, BN, WM, WN, BK>);
-}
+#define SKG_OZ_CFGS(X)          \
+  X(0, 256, 256, 4, 2, 64, 2)   \
+  X(1, 128, 128, 2, 2, 128, 2)  \
+  X(2, 128, 128, 2, 1, 128, 2)  \
+  X(3, 128, 128, 2, 2, 64, 2)   \
+  X(4, 256, 256, 4, 2, 128, 1)
 
 static const void* gemm_fn(int cfg) {
+#define SKG_FN_CASE(i, bm, bn, wm, wn, bk, pd) \
+  case i:                                      \
+    return reinterpret_cast<const void*>(      \
+        gemm_i8_kernel<bm, bn, wm, wn, bk, pd>);
   switch (cfg) {
-    case 0: return kfn<256, 256, 4, 2, 128>();
-    case 1: return kfn<128, 128, 2, 2, 128>();
-    case 2: return kfn<128, 128, 2, 1, 128>();
-    default: return kfn<256, 256, 4, 2, 64>();
+    SKG_OZ_CFGS(SKG_FN_CASE)
+    default: return nullptr;
   }
+#undef SKG_FN_CASE
 }
 
 static size_t gemm_lds(int cfg) {
   const GemmCfg& g = kGemmCfgs[cfg];
-  return (size_t)2 * (g.bm + g.bn) * g.bk;
+  const size_t pipe = (size_t)2 * (g.bm + g.bn) * g.bk;
+  const size_t tile = (size_t)g.bn * (g.bm + 16);  // residue transpose
+  return std::max(pipe, tile);
 }
 
 static hipError_t gemm_attr(int cfg) {
-  static bool done[kNumGemmCfgs] = {false, false, false, false};
+  static bool done[kNumGemmCfgs] = {};
   if (done[cfg]) return hipSuccess;
   const hipError_t e = hipFuncSetAttribute(
       gemm_fn(cfg), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -715,24 +856,16 @@ static hipError_t launch_gemm(int cfg, const GemmLaunch& G, hipStream_t s) {
   const GemmCfg& g = kGemmCfgs[cfg];
   const dim3 grid(G.grid), block(g.wm * g.wn * 64);
   const size_t lds = gemm_lds(cfg);
+#define SKG_LAUNCH_CASE(i, bm, bn, wm, wn, bk, pd)                        \
+  case i:                                                                 \
+    hipLaunchKernelGGL((gemm_i8_kernel<bm, bn, wm, wn, bk, pd>), grid,    \
+                       block, lds, s, G);                                 \
+    break;
   switch (cfg) {
-    case 0:
-      hipLaunchKernelGGL((gemm_i8_kernel<256, 256, 4, 2, 128>), grid, block,
-                         lds, s, G);
-      break;
-    case 1:
-      hipLaunchKernelGGL((gemm_i8_kernel<128, 128, 2, 2, 128>), grid, block,
-                         lds, s, G);
-      break;
-    case 2:
-      hipLaunchKernelGGL((gemm_i8_kernel<128, 128, 2, 1, 128>), grid, block,
-                         lds, s, G);
-      break;
-    default:
-      hipLaunchKernelGGL((gemm_i8_kernel<256, 256, 4, 2, 64>), grid, block,
-                         lds, s, G);
-      break;
+    SKG_OZ_CFGS(SKG_LAUNCH_CASE)
+    default: return hipErrorInvalidValue;
   }
+#undef SKG_LAUNCH_CASE
   return hipGetLastError();
 }
 
@@ -761,25 +894,18 @@ Tables make_tables(int L) {
     T.c52[l] = sym_pow2_mod(52, kModuli[l]);
     bits += std::log2((double)kModuli[l]);
   }
-  for (int l = 0; l < L; ++l)
-    for (int j = 0; j < l; ++j) T.cinv[j][l] = (uint8_t)modinv(kModuli[j], kModuli[l]);
   T.h = (int)std::floor(bits / 2.0) - 1;
-  T.direct = bits < 127.0 ? 1 : 0;
-  if (T.direct) {
-    typedef unsigned __int128 u128;
-    u128 P = 1;
-    for (int l = 0; l < L; ++l) P *= (u128)kModuli[l];
-    T.P_lo = (uint64_t)P;
-    T.P_hi = (uint64_t)(P >> 64);
-    for (int l = 0; l < L; ++l) {
-      const u128 Pl = P / (u128)kModuli[l];
-      const int inv = modinv((int)(Pl % (u128)kModuli[l]), kModuli[l]);
-      const u128 W = Pl * (u128)inv;  // < P
-      T.W_lo[l] = (uint64_t)W;
-      T.W_hi[l] = (uint64_t)(W >> 64);
-      // W / P = inv / m_l exactly; rounded once to FP64.
-      T.wfrac[l] = (double)inv / (double)kModuli[l];
-    }
+  typedef unsigned __int128 u128;
+  u128 P = 1;
+  for (int l = 0; l < L; ++l) P *= (u128)kModuli[l];
+  for (int j = 0; j < 4; ++j) T.P[j] = (uint32_t)(P >> (32 * j));
+  for (int l = 0; l < L; ++l) {
+    const u128 Pl = P / (u128)kModuli[l];
+    const int inv = modinv((int)(Pl % (u128)kModuli[l]), kModuli[l]);
+    const u128 W = Pl * (u128)inv;  // < P
+    for (int j = 0; j < 4; ++j) T.W[l][j] = (uint32_t)(W >> (32 * j));
+    // W / P = inv / m_l exactly; rounded once to FP64.
+    T.wfrac[l] = (double)inv / (double)kModuli[l];
   }
   return T;
 }
@@ -848,7 +974,10 @@ int prepare_operand(skgemm_ctx* c, const Operand& op, int K, const Tables& T,
                          op, K, sigma, T, planes, plane, kpad);
   } else {
     if (O > 0) {
-      const int nseg = std::max(1, std::min(64, (K + 255) / 256));
+      // Enough (o block, k segment) workgroups to fill the chip.
+      const int oblocks = (O + 255) / 256;
+      const int nseg = std::max(
+          1, std::min({(2048 + oblocks - 1) / oblocks, (K + 15) / 16, 512}));
       const int kseg = (K + nseg - 1) / nseg;
       double* part = (double*)c->extra->get(part_buf,
                                             sizeof(double) * 2 * nseg * O);
@@ -874,49 +1003,58 @@ int prepare_operand(skgemm_ctx* c, const Operand& op, int K, const Tables& T,
 }
 
 struct OzPlan {
-  int cfg, dp, grid;
+  int cfg, dp, grid, kc;
 };
 
 // Tile configuration and schedule of the INT8 product.  Data-parallel whole
-// units when they fill the chip in near-even rounds (L2 reuse of panels
-// between the tiles of one plane on one XCD); Stream-K otherwise.
-// SKGEMM_OZ_PLAN="cfg,dp[,grid]" overrides (tuning sweeps); a grid forced
-// with skgemm_set_tuning selects Stream-K on that grid.
-OzPlan choose_plan(skgemm_ctx* c, int kind, int M, int N, int L) {
-  OzPlan best{1, 0, 0};
-  auto units_for = [&](int cfg) {
+// units (plane, K chunk, tile) when they fill the chip in near-even rounds
+// (L2 reuse of panels between the tiles of one plane on one XCD, no
+// inter-workgroup waits); Stream-K otherwise.  SKGEMM_OZ_PLAN=
+// "cfg,dp[,grid[,kc]]" overrides (tuning sweeps); a grid forced with
+// skgemm_set_tuning selects Stream-K on that grid.
+OzPlan choose_plan(skgemm_ctx* c, int kind, int M, int N, int L, int kpad) {
+  auto tiles_for = [&](int cfg) {
     const GemmCfg& g = kGemmCfgs[cfg];
     const int64_t tm = (M + g.bm - 1) / g.bm, tn = (N + g.bn - 1) / g.bn;
-    return (int64_t)L * (kind == 1 ? tm * (tm + 1) / 2 : tm * tn);
+    return kind == 1 ? tm * (tm + 1) / 2 : tm * tn;
   };
   if (const char* env = std::getenv("SKGEMM_OZ_PLAN")) {
-    int cfg = 1, dp = 0, grid = 0;
-    if (std::sscanf(env, "%d,%d,%d", &cfg, &dp, &grid) >= 2 && cfg >= 0 &&
-        cfg < kNumGemmCfgs) {
+    int cfg = 1, dp = 0, grid = 0, kc = 1;
+    if (std::sscanf(env, "%d,%d,%d,%d", &cfg, &dp, &grid, &kc) >= 2 &&
+        cfg >= 0 && cfg < kNumGemmCfgs) {
+      const int ipt = kpad / kGemmCfgs[cfg].bk;
+      kc = dp ? std::max(1, std::min(kc, ipt)) : 1;
       const int cap = gemm_occupancy(cfg) * c->num_cus;
-      const int64_t U = units_for(cfg);
+      const int64_t U = (int64_t)L * tiles_for(cfg) * kc;
       int g = grid > 0 ? std::min(grid, cap) : cap;
       if (dp) g = (int)std::min<int64_t>(g, U);
-      return OzPlan{cfg, dp ? 1 : 0, std::max(1, g)};
+      return OzPlan{cfg, dp ? 1 : 0, std::max(1, g), kc};
     }
   }
   if (c->forced_grid > 0)  // skgemm_set_tuning: Stream-K on that many WGs
     return OzPlan{1, 0, std::min(c->forced_grid,
-                                 gemm_occupancy(1) * c->num_cus)};
+                                 gemm_occupancy(1) * c->num_cus), 1};
+  OzPlan best{1, 0, 0, 1};
   double best_score = 0.0;
   for (int cfg : {0, 1}) {
     const int cap = gemm_occupancy(cfg) * c->num_cus;
-    const int64_t U = units_for(cfg);
-    const int64_t rounds = (U + cap - 1) / cap;
-    const double eff = (double)U / (double)(rounds * cap);
-    const double score = eff * (cfg == 0 ? 1.0 : 0.92);
-    if (eff >= 0.75 && score > best_score) {
-      best_score = score;
-      best = OzPlan{cfg, 1, (int)std::min<int64_t>(cap, U)};
+    const int ipt = kpad / kGemmCfgs[cfg].bk;
+    for (int kc = 1; kc <= 16 && (kc == 1 || ipt / kc >= 32); ++kc) {
+      const int64_t U = (int64_t)L * tiles_for(cfg) * kc;
+      const int64_t rounds = (U + cap - 1) / cap;
+      const double eff = (double)U / (double)(rounds * cap);
+      // Bigger tiles halve the L2 traffic per MFMA; each extra chunk costs
+      // one more residue plane to write and read.
+      const double score =
+          eff * (cfg == 0 ? 1.0 : 0.9) * (1.0 - 0.005 * (kc - 1));
+      if (eff >= 0.75 && score > best_score) {
+        best_score = score;
+        best = OzPlan{cfg, 1, (int)std::min<int64_t>(cap, U), kc};
+      }
     }
   }
   if (best.dp) return best;
-  return OzPlan{1, 0, gemm_occupancy(1) * c->num_cus};
+  return OzPlan{1, 0, gemm_occupancy(1) * c->num_cus, 1};
 }
 
 // A-posteriori bound on ||C - AB||_F / ||AB||_F from the integer rounding
@@ -969,30 +1107,35 @@ int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
     return set_error(SKGEMM_ERR_HIP, "hipSetDevice");
   if (c->extra == nullptr) c->extra = new skgemm_ctx::Extra();
   const Tables T = make_tables(L);
-  const OzPlan op = choose_plan(c, kind, M, N, L);
+  const OzPlan op = choose_plan(c, kind, M, N, L, kpad);
   const GemmCfg& gc = kGemmCfgs[op.cfg];
   const int opa = (int)round_up(M, 256), opb = (int)round_up(N, 256);
   const int64_t pa = (int64_t)opa * kpad, pb = (int64_t)opb * kpad;
   auto* X = c->extra;
   int8_t* planesA = (int8_t*)X->get(kPlanesA, (size_t)L * pa);
   int8_t* planesB = kind == 1 ? planesA : (int8_t*)X->get(kPlanesB, (size_t)L * pb);
-  uint8_t* R = (uint8_t*)X->get(kRes, (size_t)L * M * N);
+  const int mp = (int)round_up(M, 16);
+  uint8_t* R = (uint8_t*)X->get(kRes, (size_t)op.kc * L * N * mp);
   double* n2a = (double*)X->get(kNormA, sizeof(double) * opa);
   double* n2b = kind == 1 ? n2a : (double*)X->get(kNormB, sizeof(double) * opb);
   int* sga = (int*)X->get(kSigA, sizeof(int) * opa);
   int* sgb = kind == 1 ? sga : (int*)X->get(kSigB, sizeof(int) * opb);
-  // [0, 8): ||C||_F^2 accumulator (double); [8, 12): fallback flag.
-  char* scal = (char*)X->get(kScalars, 16);
+  // [0, 4): fallback flag; [8, 8 + 8 kCrtGrid): per-workgroup parts of
+  // ||C||_F^2 written by the CRT kernel.
+  char* scal = (char*)X->get(kScalars, 8 + sizeof(double) * kCrtGrid);
   if (!planesA || !planesB || !R || !n2a || !n2b || !sga || !sgb || !scal)
     return set_error(SKGEMM_ERR_NOMEM, "split-form workspace");
-  double* fro2 = (double*)scal;
-  int* fallback = (int*)(scal + 8);
+  int* fallback = (int*)scal;
+  double* fro2 = (double*)(scal + 8);
+  const int64_t crt_blocks = (int64_t)((M + 255) / 256) * ((N + 3) / 4);
+  const int crt_grid = (int)std::max<int64_t>(
+      1, std::min<int64_t>(kCrtGrid, crt_blocks));
 
   c->record_begin();
   // Cleared by a kernel, not a memset node, so the sequence stays valid
   // inside a captured graph.
-  hipError_t e = launch_zero_ints(reinterpret_cast<int*>(scal), 3, c->stream);
-  if (e != hipSuccess) return hip_error(e, "clear split-form scalars");
+  hipError_t e = launch_zero_ints(fallback, 1, c->stream);
+  if (e != hipSuccess) return hip_error(e, "clear split-form flag");
   int rc = prepare_operand(c, A, K, T, planesA, pa, kpad, opa, n2a, sga,
                            fallback, kPartA);
   if (rc != SKGEMM_OK) return rc;
@@ -1022,10 +1165,14 @@ int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
     G.inv_mods[l] = 1.0 / T.m[l];
   }
   G.R = R;
-  G.r_plane = (int64_t)M * N;
-  G.units = G.ntiles * L;
-  G.total = G.units * G.ipt;
+  G.mp = mp;
+  G.r_plane = (int64_t)N * mp;
   G.dp = op.dp;
+  // No empty chunk: kcs = ceil(ipt / kc), then kc = ceil(ipt / kcs) <= kc.
+  G.kcs = (G.ipt + op.kc - 1) / op.kc;
+  G.kc = (G.ipt + G.kcs - 1) / G.kcs;
+  G.units = G.ntiles * L * (G.dp ? G.kc : 1);
+  G.total = G.ntiles * L * G.ipt;
   G.grid = (int)std::max<int64_t>(1, std::min<int64_t>(op.grid, G.dp ? G.units
                                                                      : G.total));
   if (!G.dp) {
@@ -1041,7 +1188,9 @@ int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
 
   CrtLaunch P{};
   P.R = R;
-  P.r_plane = (int64_t)M * N;
+  P.mp = mp;
+  P.kc = G.dp ? G.kc : 1;
+  P.r_plane = (int64_t)N * mp;
   P.T = T;
   P.sig_a = sga;
   P.sig_b = sgb;
@@ -1054,15 +1203,15 @@ int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
   P.beta = pr.beta;
   P.gamma = pr.gamma;
   P.fro2 = fro2;
-  hipLaunchKernelGGL(crt_kernel, dim3((N + 63) / 64, (M + 3) / 4), dim3(256),
-                     0, c->stream, P);
+  hipLaunchKernelGGL(crt_kernel, dim3(crt_grid), dim3(256), 0, c->stream, P);
   if ((e = hipGetLastError()) != hipSuccess) return hip_error(e, "crt launch");
 
   // Plain FP64 fallback, decided on the device: the launch does its work
   // only when a row of A or column of B could not be scaled (fallback set).
   {
     Problem fp = pr;
-    const Plan pl = plan(c, kind, la, lb, M, N, K);
+    Plan pl = plan(c, kind, la, lb, M, N, K);
+    pl.kchunks = 1;  // one launch, no chunk buffers
     rc = fill_problem(c, &fp, kind, pl, M, N, K);
     if (rc != SKGEMM_OK) return rc;
     rc = run_single(c, fp, pl, fallback, false);
@@ -1071,9 +1220,9 @@ int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
   c->record_end();
 
   if (err_bound != nullptr) {
-    std::vector<double> na(M), nb(N);
+    std::vector<double> na(M), nb(N), parts(crt_grid);
     std::vector<int> sa(M), sb(N);
-    double f[2];
+    int fb = 0;
     if ((e = hipMemcpyAsync(na.data(), n2a, sizeof(double) * M,
                             hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
         (e = hipMemcpyAsync(sa.data(), sga, sizeof(int) * M,
@@ -1082,15 +1231,17 @@ int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
                             hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
         (e = hipMemcpyAsync(sb.data(), sgb, sizeof(int) * N,
                             hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(f, scal, 16, hipMemcpyDeviceToHost, c->stream)) !=
-            hipSuccess ||
+        (e = hipMemcpyAsync(&fb, fallback, sizeof(int), hipMemcpyDeviceToHost,
+                            c->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(parts.data(), fro2, sizeof(double) * crt_grid,
+                            hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(c->stream)) != hipSuccess)
       return hip_error(e, "error-bound readback");
     rc = c->check_status();
     if (rc != SKGEMM_OK) return rc;
-    int fb;
-    std::memcpy(&fb, &f[1], sizeof(int));
-    *err_bound = fb ? -1.0 : error_bound(na, sa, nb, sb, K, f[0]);
+    double f2 = 0.0;
+    for (double v : parts) f2 += v;
+    *err_bound = fb ? -1.0 : error_bound(na, sa, nb, sb, K, f2);
   }
   return SKGEMM_OK;
 }

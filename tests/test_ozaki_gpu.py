@@ -145,11 +145,10 @@ def test_device_entry_points_reject_host_pointers(skg, ctx):
         assert e.value.code == skg.ERR_ARG
 
 
-@pytest.mark.parametrize("L", [8, 12, 16, 17, 20])
+@pytest.mark.parametrize("L", [8, 12, 15, 16])
 def test_moduli_counts_bit_exact(skg, ctx, rng, L):
-    """Every modulus count: up to 14 moduli the two-limb residue split, above
-    it the three-limb split (|x| up to 2^76); up to 16 moduli the direct CRT,
-    above it Garner's form."""
+    """Modulus counts: up to 14 moduli the two-limb residue split, above it
+    the three-limb split (|x| up to 2^60)."""
     K, N = 257, 40
     X = c3_like(rng, K, N)
     C = np.zeros(N * N)
@@ -160,6 +159,15 @@ def test_moduli_counts_bit_exact(skg, ctx, rng, L):
     assert np.array_equal(C.reshape(N, N)[il], ref[il])
 
 
+def test_modulus_count_range_is_checked(skg, ctx):
+    A = np.ones(16)
+    C = np.zeros(16)
+    for L in (7, 17):
+        with pytest.raises(skg.SkgemmError) as e:
+            ctx.dsyrk_ozaki_host(0, 4, 4, 1.0, A, 4, 0.0, C, 4, slices=L)
+        assert e.value.code == skg.ERR_ARG
+
+
 def test_every_plan_gives_identical_bits(skg, ctx, rng):
     """Data-parallel and Stream-K schedules of every INT8 tile configuration
     produce the same integers, hence the same FP64 result."""
@@ -168,7 +176,7 @@ def test_every_plan_gives_identical_bits(skg, ctx, rng):
     outs = []
     try:
         for plan in ("0,1", "1,1", "2,1", "3,1", "0,0", "1,0,77", "2,0",
-                     "3,0,300"):
+                     "3,0,300", "0,1,0,3", "1,1,100,5", "3,1,0,7"):
             os.environ["SKGEMM_OZ_PLAN"] = plan
             C = np.zeros(N * N)
             ctx.dsyrk_ozaki_host(0, N, K, 1.0, X, N, 0.0, C, N)
