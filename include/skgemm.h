@@ -198,7 +198,8 @@ int skgemm_pack(skgemm_ctx* ctx, int src_layout, const double* src,
 /* ---- batched / grouped launch ------------------------------------------ */
 
 /* One problem of a batch.  kind: 0 = GEMM D = alpha*A*B + beta*D + gamma,
- * 1 = SYRK C = alpha*A^T*A + beta*C (lower).  Device pointers. */
+ * 1 = SYRK C = alpha*A^T*A + beta*C (lower; M = N, C in D/ldd, layout_a;
+ * B, ldb, layout_b, layout_d and gamma unused).  Device pointers. */
 typedef struct skgemm_problem {
   int kind;
   int layout_a, layout_b, layout_d;
@@ -212,12 +213,16 @@ typedef struct skgemm_problem {
   int64_t ldd;
 } skgemm_problem;
 
-/* Run `count` independent problems in ONE persistent Stream-K launch over
- * the flattened iteration space of all of them (plain FP64), or, for
- * precision == SKGEMM_OZAKI_INT8, one split pass per stage over all of them.
- * use_graph != 0 captures the launch sequence in a hipGraph on first use of
- * a batch shape and replays it (descriptors are refreshed in device memory
- * before each replay).  No counterpart in the reference. */
+/* Run `count` independent problems (outputs must not overlap) on ctx's
+ * stream.  Plain FP64: ONE persistent Stream-K launch over the flattened
+ * MAC-iteration space of all of them, one tile shape for the batch; the
+ * problem table lives in device memory and is refreshed before each
+ * launch, so with use_graph != 0 one captured graph per (tile, grid) is
+ * replayed for any batch.  SKGEMM_OZAKI_INT8: the split-form sequence of
+ * each problem in turn; with use_graph != 0 the whole sequence is one
+ * graph, cached per exact problem list (pointers included) and recaptured
+ * when a workspace buffer was reallocated.  Asynchronous.  No counterpart
+ * in the reference. */
 int skgemm_batched(skgemm_ctx* ctx, const skgemm_problem* problems, int count,
                    int precision, int slices, int use_graph);
 

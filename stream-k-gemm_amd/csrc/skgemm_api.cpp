@@ -189,17 +189,6 @@ int skg::check_ctx(skgemm_ctx* c) {
   return SKGEMM_OK;
 }
 
-#define SKG_TRY(expr)              \
-  do {                             \
-    int _rc = (expr);              \
-    if (_rc != SKGEMM_OK) return _rc; \
-  } while (0)
-#define SKG_HIP(expr, what)                             \
-  do {                                                  \
-    hipError_t _e = (expr);                             \
-    if (_e != hipSuccess) return hip_error(_e, what);   \
-  } while (0)
-
 extern "C" const char* skgemm_last_error(void) { return g_last_error.c_str(); }
 extern "C" const char* skgemm_version(void) { return "skgemm 0.1 (gfx950)"; }
 
@@ -366,6 +355,7 @@ int skgemm_ctx::ensure_ws(size_t ws_doubles, int nflags) {
       hipFree(ws);
       ws = nullptr;
     }
+    ++mem_gen;
     hipError_t e = hipMalloc(&ws, ws_doubles * sizeof(double));
     if (e != hipSuccess) {
       ws_n = 0;
@@ -380,6 +370,7 @@ int skgemm_ctx::ensure_ws(size_t ws_doubles, int nflags) {
       hipFree(flags);
       flags = nullptr;
     }
+    ++mem_gen;
     hipError_t e = hipMalloc(&flags, sizeof(int) * nflags);
     if (e != hipSuccess) {
       flags_n = 0;
@@ -442,15 +433,18 @@ void skgemm_ctx::free_all() {
   flags = nullptr;
   status = nullptr;
   free_extra();
+  free_batch();
 }
 
 void skgemm_ctx::record_begin() {
+  if (capturing) return;
   hipEventRecord(ev0, stream);
   if (tring_used < (int)tring.size())
     hipEventRecord(tring[tring_used].first, stream);
 }
 
 void skgemm_ctx::record_end() {
+  if (capturing) return;
   hipEventRecord(ev1, stream);
   timed = true;
   if (tring_used < (int)tring.size()) {

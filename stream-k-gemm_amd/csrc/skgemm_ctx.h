@@ -59,9 +59,17 @@ struct skgemm_ctx {
   std::map<std::tuple<int, int, int, int, int, int>, skg::Plan> tuned;
   std::map<int, int> occ_cache;
 
-  // Split-form and batched state (skgemm_ozaki.hip / skgemm_batch.cpp).
+  // Split-form workspace (skgemm_ozaki.hip) and batched-launch state
+  // (skgemm_batch.cpp).  While a launch sequence is being captured into a
+  // hipGraph, `capturing` suppresses timing-ring records.
   struct Extra;
   Extra* extra = nullptr;
+  struct Batch;
+  Batch* batch = nullptr;
+  bool capturing = false;
+  // Bumped whenever a workspace buffer is reallocated: captured graphs that
+  // hold the old addresses are recaptured.
+  uint64_t mem_gen = 0;
 
   int check_status();
   int ensure_ws(size_t ws_doubles, int nflags);
@@ -70,7 +78,20 @@ struct skgemm_ctx {
   int capacity(int cfg);
   void free_all();
   void free_extra();
+  void free_batch();
 };
+
+// Early return of an error code / of a failed HIP call as SKGEMM_ERR_HIP.
+#define SKG_TRY(expr)                 \
+  do {                                \
+    int _rc = (expr);                 \
+    if (_rc != SKGEMM_OK) return _rc; \
+  } while (0)
+#define SKG_HIP(expr, what)                                 \
+  do {                                                      \
+    hipError_t _e = (expr);                                 \
+    if (_e != hipSuccess) return skg::hip_error(_e, what);  \
+  } while (0)
 
 namespace skg {
 
@@ -84,6 +105,14 @@ int make_output(int layout, double* p, int64_t ld, int M, int N, Output* out,
 int64_t span_operand(int layout, bool a_convention, int64_t ld, int ext, int K);
 int64_t span_output(int layout, int64_t ld, int M, int N);
 int check_dev(const void* p, const char* name);
+// Split-form driver (skgemm_ozaki.hip).  kind 0: GEMM, 1: SYRK; pr holds
+// operands, output, sizes and alpha/beta/gamma.  ozaki_reserve grows every
+// buffer the call would use, so that a later call allocates nothing (for
+// graph capture).
+int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
+              int slices, double* err_bound);
+int ozaki_reserve(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
+                  int slices);
 
 int64_t tiles_for(int kind, int cfg, int M, int N);
 Plan plan(skgemm_ctx* c, int kind, int la, int lb, int M, int N, int K);

@@ -59,8 +59,9 @@ struct Problem {
 // Launch-wide state of one Stream-K launch.
 struct Launch {
   const Problem* problems;  // device table (batched) or nullptr
+  const int64_t* hdr;       // batched: device {num_problems, total_iters}
   Problem single;           // used when problems == nullptr
-  int num_problems;
+  int num_problems;         // single launches (batched: from hdr)
   int64_t total_iters;
   int grid;             // persistent workgroups
   double* ws;           // grid partial tiles of BM*BN doubles

@@ -368,15 +368,31 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
   // the predicate written by an earlier kernel on the stream is set.
   if (!BATCHED && L.single.pred != nullptr && *L.single.pred == 0) return;
 
-  int64_t it = wg_start(w, L.total_iters, G);
-  const int64_t end = wg_start(w + 1, L.total_iters, G);
+  // Batched launches read the problem count and the flattened iteration
+  // count from device memory, so a captured graph replays any batch.
+  const int64_t total =
+      BATCHED ? load_uniform(&L.hdr[1]) : L.total_iters;
+  const int nprob = BATCHED ? (int)load_uniform(&L.hdr[0]) : 1;
+  int64_t it = wg_start(w, total, G);
+  const int64_t end = wg_start(w + 1, total, G);
   int pi = 0;
   const Problem* tab = L.problems;
+  if (BATCHED && it < end) {
+    // First problem containing `it`: binary search on iter_begin.
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (load_uniform(&tab[mid].iter_begin) <= it)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    pi = lo;
+  }
 
   while (it < end) {
     if (BATCHED) {
-      while (pi + 1 < L.num_problems &&
-             load_uniform(&tab[pi + 1].iter_begin) <= it)
+      while (pi + 1 < nprob && load_uniform(&tab[pi + 1].iter_begin) <= it)
         ++pi;
     }
     const Operand A = BATCHED ? load_uniform(&tab[pi].a) : L.single.a;
@@ -487,7 +503,7 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
     } else if (ke < kci) {
       // Owner of a split unit: add the peers' partials in peer order.
       const int64_t tile_end = t0 + kci;
-      for (int w2 = w + 1; w2 < G && wg_start(w2, L.total_iters, G) < tile_end;
+      for (int w2 = w + 1; w2 < G && wg_start(w2, total, G) < tile_end;
            ++w2) {
         if (tid == 0) {
           int64_t n = 0;

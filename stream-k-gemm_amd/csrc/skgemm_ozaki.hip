@@ -919,8 +919,10 @@ struct skgemm_ctx::Extra {
   static constexpr int kBufs = 12;
   void* buf[kBufs] = {nullptr};
   size_t n[kBufs] = {0};
+  uint64_t* gen = nullptr;  // the context's mem_gen
   void* get(int i, size_t bytes) {
     if (bytes > n[i]) {
+      if (gen) ++*gen;
       if (buf[i]) {
         hipDeviceSynchronize();
         hipFree(buf[i]);
@@ -954,6 +956,14 @@ enum {
 
 int64_t round_up(int64_t x, int64_t q) { return (x + q - 1) / q * q; }
 
+// K segments of the o-contiguous norm pass: enough (o block, k segment)
+// workgroups to fill the chip.
+int norm_segments(int O, int K) {
+  const int oblocks = (O + 255) / 256;
+  return std::max(1, std::min({(2048 + oblocks - 1) / oblocks, (K + 15) / 16,
+                               512}));
+}
+
 // Scale + split one operand (free index o, contraction k) into planes.
 int prepare_operand(skgemm_ctx* c, const Operand& op, int K, const Tables& T,
                     int8_t* planes, int64_t plane, int kpad, int opad,
@@ -974,10 +984,7 @@ int prepare_operand(skgemm_ctx* c, const Operand& op, int K, const Tables& T,
                          op, K, sigma, T, planes, plane, kpad);
   } else {
     if (O > 0) {
-      // Enough (o block, k segment) workgroups to fill the chip.
-      const int oblocks = (O + 255) / 256;
-      const int nseg = std::max(
-          1, std::min({(2048 + oblocks - 1) / oblocks, (K + 15) / 16, 512}));
+      const int nseg = norm_segments(O, K);
       const int kseg = (K + nseg - 1) / nseg;
       double* part = (double*)c->extra->get(part_buf,
                                             sizeof(double) * 2 * nseg * O);
@@ -1089,6 +1096,53 @@ double error_bound(const std::vector<double>& na, const std::vector<int>& sa,
   return (double)(num / den) + 4.5e-16;  // + final roundings
 }
 
+}  // namespace
+
+namespace skg {
+
+int ozaki_reserve(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
+                  int slices) {
+  const int M = pr.M, N = pr.N, K = pr.K;
+  const int L = slices == 0 ? 14 : slices;
+  if (L < 8 || L > kMaxMod)
+    return set_error(SKGEMM_ERR_ARG, "slices (moduli) must be 0 or 8..%d",
+                     kMaxMod);
+  const int kpad = (int)round_up(std::max(K, 1), 128);
+  if (kpad > kMaxKpad)
+    return set_error(SKGEMM_ERR_NOTSUP,
+                     "split form needs K <= %d for exact INT32 sums", kMaxKpad);
+  if (c->extra == nullptr) {
+    c->extra = new skgemm_ctx::Extra();
+    c->extra->gen = &c->mem_gen;
+  }
+  auto* X = c->extra;
+  const OzPlan op = choose_plan(c, kind, M, N, L, kpad);
+  const GemmCfg& gc = kGemmCfgs[op.cfg];
+  const int opa = (int)round_up(M, 256), opb = (int)round_up(N, 256);
+  const int mp = (int)round_up(M, 16);
+  bool ok = X->get(kPlanesA, (size_t)L * opa * kpad) &&
+            X->get(kRes, (size_t)op.kc * L * N * mp) &&
+            X->get(kNormA, sizeof(double) * opa) &&
+            X->get(kSigA, sizeof(int) * opa) &&
+            X->get(kScalars, 8 + sizeof(double) * kCrtGrid);
+  if (kind == 0)
+    ok = ok && X->get(kPlanesB, (size_t)L * opb * kpad) &&
+         X->get(kNormB, sizeof(double) * opb) &&
+         X->get(kSigB, sizeof(int) * opb);
+  if (!pr.a.kcontig && M > 0)
+    ok = ok && X->get(kPartA, sizeof(double) * 2 * norm_segments(M, K) * M);
+  if (kind == 0 && !pr.b.kcontig && N > 0)
+    ok = ok && X->get(kPartB, sizeof(double) * 2 * norm_segments(N, K) * N);
+  if (!ok) return set_error(SKGEMM_ERR_NOMEM, "split-form workspace");
+  if (!op.dp) {
+    const int rc = c->ensure_ws((size_t)op.grid * gc.bm * gc.bn / 2 + 1, op.grid);
+    if (rc != SKGEMM_OK) return rc;
+  }
+  Plan pl = plan(c, kind, la, lb, M, N, K);
+  const TileCfg& t = tile_cfg(pl.cfg);
+  return c->ensure_ws((size_t)pl.grid * t.bm * t.bn, pl.grid);
+}
+
 // Shared driver.  kind 0: C = A B; kind 1: C = Psi^T Psi (B == A).
 int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
               int slices, double* err_bound) {
@@ -1105,7 +1159,10 @@ int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
                      "split form needs K <= %d for exact INT32 sums", kMaxKpad);
   if (hipSetDevice(c->device) != hipSuccess)
     return set_error(SKGEMM_ERR_HIP, "hipSetDevice");
-  if (c->extra == nullptr) c->extra = new skgemm_ctx::Extra();
+  if (c->extra == nullptr) {
+    c->extra = new skgemm_ctx::Extra();
+    c->extra->gen = &c->mem_gen;
+  }
   const Tables T = make_tables(L);
   const OzPlan op = choose_plan(c, kind, M, N, L, kpad);
   const GemmCfg& gc = kGemmCfgs[op.cfg];
@@ -1246,7 +1303,7 @@ int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
   return SKGEMM_OK;
 }
 
-}  // namespace
+}  // namespace skg
 
 extern "C" int skgemm_dgemm_ozaki(skgemm_ctx* c, int la, int lb, int ld_,
                                   int M, int N, int K, double alpha,

@@ -5,10 +5,6 @@
 
 using namespace skg;
 
-extern "C" int skgemm_batched(skgemm_ctx*, const skgemm_problem*, int, int,
-                              int, int) {
-  return set_error(SKGEMM_ERR_NOTSUP, "skgemm_batched: not built yet");
-}
 extern "C" int skgemm_server_create(int, size_t, int, int, int, int,
                                     skgemm_server**) {
   return set_error(SKGEMM_ERR_NOTSUP, "server: not built yet");
