@@ -70,6 +70,9 @@ struct skgemm_ctx {
   // Bumped whenever a workspace buffer is reallocated: captured graphs that
   // hold the old addresses are recaptured.
   uint64_t mem_gen = 0;
+  // Split form: never pick a schedule whose workgroups wait on each other
+  // (set on serving slots, whose launches run concurrently).
+  bool oz_no_wait = false;
 
   int check_status();
   int ensure_ws(size_t ws_doubles, int nflags);

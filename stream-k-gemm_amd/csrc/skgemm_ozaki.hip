@@ -1018,7 +1018,8 @@ struct OzPlan {
 // (L2 reuse of panels between the tiles of one plane on one XCD, no
 // inter-workgroup waits); Stream-K otherwise.  SKGEMM_OZ_PLAN=
 // "cfg,dp[,grid[,kc]]" overrides (tuning sweeps); a grid forced with
-// skgemm_set_tuning selects Stream-K on that grid.
+// skgemm_set_tuning selects Stream-K on that grid, except on contexts that
+// must not wait (oz_no_wait), where it caps the data-parallel grid.
 OzPlan choose_plan(skgemm_ctx* c, int kind, int M, int N, int L, int kpad) {
   auto tiles_for = [&](int cfg) {
     const GemmCfg& g = kGemmCfgs[cfg];
@@ -1038,7 +1039,7 @@ OzPlan choose_plan(skgemm_ctx* c, int kind, int M, int N, int L, int kpad) {
       return OzPlan{cfg, dp ? 1 : 0, std::max(1, g), kc};
     }
   }
-  if (c->forced_grid > 0)  // skgemm_set_tuning: Stream-K on that many WGs
+  if (c->forced_grid > 0 && !c->oz_no_wait)  // Stream-K on that many WGs
     return OzPlan{1, 0, std::min(c->forced_grid,
                                  gemm_occupancy(1) * c->num_cus), 1};
   OzPlan best{1, 0, 0, 1};
@@ -1060,7 +1061,15 @@ OzPlan choose_plan(skgemm_ctx* c, int kind, int M, int N, int L, int kpad) {
       }
     }
   }
-  if (best.dp) return best;
+  if (!best.dp && c->oz_no_wait) {
+    const int64_t U = (int64_t)L * tiles_for(1);
+    best = OzPlan{1, 1, (int)std::min<int64_t>(gemm_occupancy(1) * c->num_cus, U),
+                  1};
+  }
+  if (best.dp) {
+    if (c->forced_grid > 0) best.grid = std::min(best.grid, c->forced_grid);
+    return best;
+  }
   return OzPlan{1, 0, gemm_occupancy(1) * c->num_cus, 1};
 }
 

@@ -401,3 +401,54 @@ def tile_info(i):
     check(lib().skgemm_tile_info(i, *[ctypes.byref(x) for x in v]))
     return dict(bm=v[0].value, bn=v[1].value, bk=v[2].value,
                 waves=v[3].value)
+
+
+class Server:
+    """Continuous-batching request loop (skgemm_server_*) over systems whose
+    Psi stays resident in device memory."""
+
+    def __init__(self, device=0, hbm_budget=64 << 30, max_batch=16,
+                 max_inflight=2, precision=FP64, slices=0):
+        h = c_void_p()
+        check(lib().skgemm_server_create(device, hbm_budget, max_batch,
+                                         max_inflight, precision, slices,
+                                         ctypes.byref(h)))
+        self.h = h
+
+    def add_system(self, psi_ptr, Nd, Ns):
+        i = ctypes.c_int()
+        check(lib().skgemm_server_add_system(self.h, psi_ptr, Nd, Ns,
+                                             ctypes.byref(i)))
+        return i.value
+
+    def submit(self, system, kind, out_ptr, q_ptr=None):
+        t = ctypes.c_int64()
+        check(lib().skgemm_server_submit(self.h, system, kind, q_ptr, out_ptr,
+                                         ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        """Latency of the request in microseconds (submit -> completion)."""
+        lat = _out_double()
+        check(lib().skgemm_server_wait(self.h, ticket, ctypes.byref(lat)))
+        return lat.value
+
+    def stats(self):
+        c, l, pk = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        f, b = _out_double(), _out_double()
+        check(lib().skgemm_server_stats(self.h, ctypes.byref(c),
+                                        ctypes.byref(l), ctypes.byref(f),
+                                        ctypes.byref(b), ctypes.byref(pk)))
+        return {"completed": c.value, "launches": l.value, "flops": f.value,
+                "busy_s": b.value, "peak_inflight": pk.value}
+
+    def close(self):
+        if self.h:
+            check(lib().skgemm_server_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -66,6 +66,22 @@ def test_errors_without_device(skg):
         assert info["bk"] % 4 == 0 and info["waves"] in (4, 8)
 
 
+def test_batch_and_server_argument_checks_without_device(skg):
+    """Argument errors are reported before any device work."""
+    import ctypes
+    lib = skg.lib()
+    assert lib.skgemm_batched(None, None, 1, 0, 0, 0) == skg.ERR_STATE
+    out = ctypes.c_void_p()
+    for args in ((0, 1 << 30, 0, 1, 0, 0), (0, 1 << 30, 4, 0, 0, 0),
+                 (0, 1 << 30, 4, 9, 0, 0), (0, 1 << 30, 4, 2, 5, 0),
+                 (0, 1 << 30, 4, 2, 1, 3)):
+        assert lib.skgemm_server_create(*args, ctypes.byref(out)) == skg.ERR_ARG
+        assert out.value is None
+    assert lib.skgemm_server_wait(None, 1, None) == skg.ERR_ARG
+    assert lib.skgemm_server_stats(None, None, None, None, None,
+                                   None) == skg.ERR_ARG
+
+
 # ---- Stream-K work split (restates sk_dgemm_kernel's scheduling) ---------
 
 def wg_start(w, total, G):
