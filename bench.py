@@ -130,6 +130,198 @@ def cpu_baseline(psi_host, q_host, Nd, Ns, gpu_ms, gpu_rot):
     }
 
 
+def median_ms(ctx, fn, reps):
+    """Median device time of `reps` calls from the context's timing ring
+    (HIP events on the launch stream), after one warm-up call."""
+    fn()
+    ctx.synchronize()
+    ctx.timing_enable(reps)
+    for _ in range(reps):
+        fn()
+    ctx.synchronize()
+    return float(np.median(ctx.timing_read(reps)))
+
+
+def lower_rel(a, b, n):
+    il = np.tril_indices(n)
+    x = a.reshape(n, n)[il]
+    y = b.reshape(n, n)[il]
+    return float(np.linalg.norm(x - y) / np.linalg.norm(y))
+
+
+def bench_split(skg, ctx, reps):
+    """Split-form SYRK on Psi 65536 x 1024 (columns scaled by 10^u, u ~
+    U(0, 6), seed 3) against the autotuned plain FP64 SYRK on the same
+    input; error of the split result against the plain FP64 result and the
+    bound the library returns."""
+    Nd, Ns = 65536, 1024
+    psi = skg.DeviceArray(ctx, Nd * Ns)
+    ctx.fill_normal(psi.ptr, Nd * Ns, 3, 1.0 / np.sqrt(Nd))
+    ctx.scale_columns_pow10(psi.ptr, Nd, Ns, Nd, 4, 6.0)
+    plain = skg.DeviceArray(ctx, Ns * Ns, zero=True)
+    split = skg.DeviceArray(ctx, Ns * Ns, zero=True)
+    plan = ctx.autotune(1, Ns, Ns, Nd, la=skg.COL_MAJOR)
+    t_plain = median_ms(ctx, lambda: ctx.dsyrk(
+        skg.COL_MAJOR, Ns, Nd, 1.0, psi.ptr, Nd, 0.0, plain.ptr, Ns), reps)
+    out = {"workload": "SYRK Ms = Psi^T Psi (lower), Psi 65536x1024 FP64 "
+                       "col-major, columns scaled by 10^U(0,6)",
+           "plain_fp64": {"ms": t_plain, "plan": plan}}
+    f = float(Ns) * (Ns + 1) * Nd
+    out["plain_fp64"]["tflops"] = f / (t_plain * 1e-3) / 1e12
+    for L in (14, 13):
+        bound = ctx.dsyrk_ozaki(skg.COL_MAJOR, Ns, Nd, 1.0, psi.ptr, Nd, 0.0,
+                                split.ptr, Ns, slices=L)
+        t = median_ms(ctx, lambda: ctx.dsyrk_ozaki(
+            skg.COL_MAJOR, Ns, Nd, 1.0, psi.ptr, Nd, 0.0, split.ptr, Ns,
+            slices=L), reps)
+        err = lower_rel(split.download(), plain.download(), Ns)
+        out["split_L%d" % L] = {
+            "ms": t, "tflops_fp64_equivalent": f / (t * 1e-3) / 1e12,
+            "speedup_vs_plain": t_plain / t, "rel_fro_vs_plain_fp64": err,
+            "stated_bound": bound, "moduli": L,
+            "int8_tops": L * f / (t * 1e-3) / 1e12}
+    psi.free()
+    return out
+
+
+def c4_batch(skg, ctx, seed=5, count=64):
+    """64 systems: Ns ~ U{128, 136, ..., 1536}, Nd ~ U{4096, 13824, 32768,
+    65536}, Psi ~ N(0, 1/sqrt(Nd)) (seeded per system), Q ~ U(-1, 1); one
+    overlap SYRK and one rotation GEMM per system."""
+    rng = np.random.default_rng(seed)
+    probs, keep, flops = [], [], 0.0
+    for i in range(count):
+        Ns = int(rng.choice(np.arange(128, 1537, 8)))
+        Nd = int(rng.choice([4096, 13824, 32768, 65536]))
+        psi = skg.DeviceArray(ctx, Nd * Ns)
+        ctx.fill_normal(psi.ptr, Nd * Ns, 1000 + i, 1.0 / np.sqrt(Nd))
+        q = skg.DeviceArray(ctx, Ns * Ns)
+        ctx.fill_uniform(q.ptr, Ns * Ns, 2000 + i, -1.0, 1.0)
+        ms = skg.DeviceArray(ctx, Ns * Ns, zero=True)
+        rot = skg.DeviceArray(ctx, Nd * Ns, zero=True)
+        probs.append(skg.Problem(kind=1, layout_a=skg.COL_MAJOR, M=Ns, N=Ns,
+                                 K=Nd, alpha=1.0, A=psi.ptr, lda=Nd,
+                                 D=ms.ptr, ldd=Ns))
+        probs.append(skg.Problem(kind=0, layout_a=skg.COL_MAJOR,
+                                 layout_b=skg.ROW_MAJOR,
+                                 layout_d=skg.COL_MAJOR, M=Nd, N=Ns, K=Ns,
+                                 alpha=1.0, A=psi.ptr, lda=Nd, B=q.ptr,
+                                 ldb=Ns, D=rot.ptr, ldd=Nd))
+        flops += float(Ns) * (Ns + 1) * Nd + 2.0 * Nd * Ns * Ns
+        keep += [psi, q, ms, rot]
+    ctx.synchronize()
+    return probs, keep, flops
+
+
+def bench_batched(skg, ctx, reps):
+    probs, keep, flops = c4_batch(skg, ctx)
+    out = {"workload": "64 systems x (overlap SYRK + rotation GEMM), Ns ~ "
+                       "U{128..1536 step 8}, Nd ~ U{4096,13824,32768,65536}, "
+                       "seed 5; one persistent launch per batch",
+           "problems": len(probs), "flop": flops}
+    for name, prec, graph, r in (("plain_fp64_graph", skg.FP64, 1, reps),
+                                 ("plain_fp64_eager", skg.FP64, 0, reps),
+                                 ("split_graph", skg.OZAKI_INT8, 1,
+                                  max(2, reps // 2))):
+        t = median_ms(ctx, lambda: ctx.batched(probs, precision=prec,
+                                               use_graph=graph), r)
+        out[name] = {"ms": t, "tflops": flops / (t * 1e-3) / 1e12}
+    for a in keep:
+        a.free()
+    return out
+
+
+def bench_serving(skg, ctx, c4_tflops, seconds):
+    """200 resident systems (Ns ~ U[128, 2048], Nd ~ U[8192, 65536], seed 6,
+    ~64 GB of Psi); overlap and rotation requests arrive as a Poisson
+    process at ~70% of the batched throughput; latency = submit ->
+    completion observed by the server."""
+    import threading
+    rng = np.random.default_rng(6)
+    systems = []
+    for i in range(200):
+        Ns = int(rng.integers(128, 2049))
+        Nd = int(rng.integers(8192, 65537))
+        psi = skg.DeviceArray(ctx, Nd * Ns)
+        ctx.fill_normal(psi.ptr, Nd * Ns, 3000 + i, 1.0 / np.sqrt(Nd))
+        q = skg.DeviceArray(ctx, Ns * Ns)
+        ctx.fill_uniform(q.ptr, Ns * Ns, 4000 + i, -1.0, 1.0)
+        systems.append((psi, q, Nd, Ns))
+    ctx.synchronize()
+    pool_gb = sum(s[2] * s[3] * 8 for s in systems) / 1e9
+    mean_flop = np.mean([s[3] * (s[3] + 1) * s[2] + 2.0 * s[2] * s[3] ** 2
+                         for s in systems]) / 2.0
+    lam = 0.7 * c4_tflops * 1e12 / mean_flop           # requests per second
+    n_req = max(50, int(lam * seconds))
+    kinds = rng.integers(0, 2, n_req)
+    picks = rng.integers(0, len(systems), n_req)
+    gaps = rng.exponential(1.0 / lam, n_req)
+    # Outputs: a ring of buffers sized for the largest result, longer than
+    # the number of requests that can be in flight (2 launches x 16).
+    ring = 48
+    max_out = max(s[2] * s[3] for s in systems)
+    outs = [skg.DeviceArray(ctx, max_out) for _ in range(ring)]
+    srv = skg.Server(0, hbm_budget=int(pool_gb * 1e9) + (16 << 30),
+                     max_batch=16, max_inflight=2, precision=skg.FP64)
+    lat = []
+    try:
+        ids = [srv.add_system(s[0].ptr, s[2], s[3]) for s in systems]
+        pending = []
+        lock = threading.Lock()
+        done = threading.Event()
+
+        def waiter():
+            while True:
+                with lock:
+                    item = pending.pop(0) if pending else None
+                if item is None:
+                    if done.is_set():
+                        with lock:
+                            if not pending:
+                                return
+                    time.sleep(0.0002)
+                    continue
+                lat.append(srv.wait(item))
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        t_next = t0
+        for j in range(n_req):
+            t_next += gaps[j]
+            while time.perf_counter() < t_next:
+                pass
+            si, kind = int(picks[j]), int(kinds[j])
+            psi, q, Nd, Ns = systems[si]
+            tk = srv.submit(ids[si], kind, outs[j % ring].ptr,
+                            q.ptr if kind == 1 else None)
+            with lock:
+                pending.append(tk)
+        done.set()
+        th.join()
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    finally:
+        srv.close()
+    lat = np.array(lat)
+    out = {"workload": "200 resident systems (Ns ~ U[128,2048], Nd ~ "
+                       "U[8192,65536], seed 6), Poisson arrivals, overlap or "
+                       "rotation per request, plain FP64",
+           "resident_psi_gb": pool_gb, "arrival_rate_per_s": lam,
+           "requests": int(n_req), "wall_s": wall,
+           "latency_ms_p50": float(np.percentile(lat, 50)) / 1e3,
+           "latency_ms_p99": float(np.percentile(lat, 99)) / 1e3,
+           "tflops": st["flops"] / wall / 1e12,
+           "launches": st["launches"], "peak_inflight": st["peak_inflight"],
+           "requests_per_launch": st["completed"] / max(1, st["launches"])}
+    for s in systems:
+        s[0].free()
+        s[1].free()
+    for o in outs:
+        o.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +331,9 @@ def main():
     ap.add_argument("--tile", type=int, default=-1)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the split-form, batched and serving sections")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -217,6 +412,17 @@ def main():
     probes = {"fp64_mfma_tflops": ctx.peak_probe(0),
               "int8_mfma_tops": ctx.peak_probe(1)}
 
+    extra = {}
+    if not args.no_extra:
+        log("split form (SYRK 65536x1024) ...")
+        extra["split_syrk"] = bench_split(skg, ctx, 10)
+        log("batched (64 systems) ...")
+        extra["batched"] = bench_batched(skg, ctx, 5)
+        log("serving (200 systems, Poisson) ...")
+        extra["serving"] = bench_serving(
+            skg, ctx, extra["batched"]["plain_fp64_graph"]["tflops"],
+            args.serve_seconds)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -253,6 +459,7 @@ def main():
         "cpu_baseline": cpu,
         "probes": probes,
     }
+    out.update(extra)
     print(json.dumps(out))
 
 
