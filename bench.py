@@ -333,6 +333,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the split-form, batched and serving sections")
+    ap.add_argument("--tuning-cache", default=None,
+                    help="autotuner cache file: read if it exists (no "
+                         "tuning runs), else written by the tuning runs")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -355,12 +358,23 @@ def main():
     plans = {}
     if args.tile >= 0 or args.grid > 0:
         ctx.set_tuning(args.tile, args.grid)
+    elif args.tuning_cache and os.path.exists(args.tuning_cache):
+        ctx.load_tuning(args.tuning_cache)
+        for name, kind, M, N, K in (("syrk", 1, Ns, Ns, Nd),
+                                    ("rotation", 0, Nd, Ns, Ns)):
+            p = ctx.query_plan(kind, M, N, K, la=skg.COL_MAJOR,
+                               lb=skg.ROW_MAJOR)
+            plans[name] = {"tile": p["tile"], "grid": p["grid"],
+                           "kchunks": p["kchunks"], "us": None}
+        log("plans from %s: %s" % (args.tuning_cache, plans))
     elif not args.no_tune:
         # Autotuner: picks tile / grid / K chunks per shape with real
         # launches, before (and outside) the timed region.
-        plans["syrk"] = ctx.autotune(1, Ns, Ns, Nd, la=skg.COL_MAJOR)
+        plans["syrk"] = ctx.autotune(1, Ns, Ns, Nd, la=skg.COL_MAJOR,
+                                     cache_path=args.tuning_cache)
         plans["rotation"] = ctx.autotune(0, Nd, Ns, Ns, la=skg.COL_MAJOR,
-                                         lb=skg.ROW_MAJOR)
+                                         lb=skg.ROW_MAJOR,
+                                         cache_path=args.tuning_cache)
         log("autotuned plans %s" % plans)
 
     def step():
@@ -410,7 +424,8 @@ def main():
     }
 
     probes = {"fp64_mfma_tflops": ctx.peak_probe(0),
-              "int8_mfma_tops": ctx.peak_probe(1)}
+              "int8_mfma_tops": ctx.peak_probe(1),
+              "hbm_copy_gbps_1GiB": ctx.copy_probe(1 << 30)}
 
     extra = {}
     if not args.no_extra:
