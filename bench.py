@@ -149,6 +149,28 @@ def lower_rel(a, b, n):
     return float(np.linalg.norm(x - y) / np.linalg.norm(y))
 
 
+def bench_c1(skg, ctx, reps):
+    """c1: A, B 1024 x 1024 uniform(-1, 1) (seeds 42, 43), row-major, D = A B,
+    autotuned plan."""
+    n = 1024
+    a = skg.DeviceArray(ctx, n * n)
+    b = skg.DeviceArray(ctx, n * n)
+    d = skg.DeviceArray(ctx, n * n, zero=True)
+    ctx.fill_uniform(a.ptr, n * n, 42, -1.0, 1.0)
+    ctx.fill_uniform(b.ptr, n * n, 43, -1.0, 1.0)
+    plan = ctx.autotune(0, n, n, n, la=skg.ROW_MAJOR, lb=skg.ROW_MAJOR)
+    t = median_ms(ctx, lambda: ctx.dgemm(
+        skg.ROW_MAJOR, skg.ROW_MAJOR, skg.ROW_MAJOR, n, n, n, 1.0, a.ptr, n,
+        b.ptr, n, 0.0, d.ptr, n, 0.0), reps)
+    for x in (a, b, d):
+        x.free()
+    f = 2.0 * n ** 3
+    return {"workload": "GEMM 1024^3 FP64 row-major, D = A B", "ms": t,
+            "tflops": f / (t * 1e-3) / 1e12,
+            "frac_fp64_peak": f / (t * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+            "plan": plan}
+
+
 def bench_split(skg, ctx, reps):
     """Split-form SYRK on Psi 65536 x 1024 (columns scaled by 10^u, u ~
     U(0, 6), seed 3) against the autotuned plain FP64 SYRK on the same
@@ -332,7 +354,10 @@ def main():
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the split-form, batched and serving sections")
+                    help="skip the c1, split-form, batched and serving "
+                         "sections")
+    ap.add_argument("--extra", default="c1,split,batched,serving",
+                    help="comma list of extra sections to run")
     ap.add_argument("--tuning-cache", default=None,
                     help="autotuner cache file: read if it exists (no "
                          "tuning runs), else written by the tuning runs")
@@ -428,11 +453,18 @@ def main():
               "hbm_copy_gbps_1GiB": ctx.copy_probe(1 << 30)}
 
     extra = {}
-    if not args.no_extra:
+    sections = set() if args.no_extra else set(
+        x for x in args.extra.split(",") if x)
+    if "c1" in sections:
+        log("c1 GEMM 1024^3 ...")
+        extra["c1_gemm"] = bench_c1(skg, ctx, 20)
+    if "split" in sections:
         log("split form (SYRK 65536x1024) ...")
         extra["split_syrk"] = bench_split(skg, ctx, 10)
+    if "batched" in sections or "serving" in sections:
         log("batched (64 systems) ...")
         extra["batched"] = bench_batched(skg, ctx, 5)
+    if "serving" in sections:
         log("serving (200 systems, Poisson) ...")
         extra["serving"] = bench_serving(
             skg, ctx, extra["batched"]["plain_fp64_graph"]["tflops"],

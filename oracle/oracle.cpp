@@ -26,9 +26,8 @@
 //       exact small-integer products, and agreement with numpy to 1e-15.
 //   pack / unpack
 //       round trips and element-wise layout checks (pack.cpp:37-258).
-//   oracle_ozaki_syrk (CPU restatement of the split form)
-//       exact-integer inputs reproduce the FP64 product bit-for-bit, and the
-//       error vs the double-double oracle is below the stated bound.
+// The split form (Ozaki scheme II) is restated separately in oracle/ozaki.py
+// and pinned in tests/test_ozaki_oracle.py.
 
 #include <omp.h>
 

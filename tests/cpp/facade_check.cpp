@@ -1,0 +1,111 @@
+// Host program written against the C++ facade exactly as a caller of the
+// reference classes would be (gemm<double>, sme_syrk<double>,
+// sme_gemm_trp_3d<double>, linalg_pack_c16_f64): it checks each call against
+// closed forms or naive loops computed here and exits non-zero on a
+// mismatch.  Built with g++ against libskgemm.so (no HIP headers needed).
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "skgemm.hpp"
+
+static int failures = 0;
+
+static void expect(bool ok, const char* what, double v) {
+  std::printf("%-44s %s (%.3e)\n", what, ok ? "ok" : "FAIL", v);
+  if (!ok) ++failures;
+}
+
+int main() {
+  // gemm_crsp.cpp:451-460 inputs; closed form
+  // D_ij = alpha (1e-5 (I+J) S1 + 1e-4 I J K + 1e-6 S2) + gamma.
+  {
+    const int M = 32, N = 64, K = 4096;
+    const double alpha = 2.0, gamma = 0.5;
+    std::vector<double> A((size_t)M * K), B((size_t)K * N), D((size_t)M * N);
+    for (int k = 0; k < K; ++k)
+      for (int i = 0; i < M; ++i) A[i + (size_t)k * M] = 0.01 * (i + 1) + 0.001 * (k + 1);
+    for (int k = 0; k < K; ++k)
+      for (int j = 0; j < N; ++j) B[(size_t)k * N + j] = 0.001 * (k + 1) + 0.01 * (j + 1);
+    gemm<double>::compute_test(M, N, K, A.data(), B.data(), D.data(), alpha, gamma);
+    const long double S1 = (long double)K * (K + 1) / 2,
+                      S2 = (long double)K * (K + 1) * (2 * K + 1) / 6;
+    double worst = 0.0;
+    for (int i = 0; i < M; ++i)
+      for (int j = 0; j < N; ++j) {
+        const long double I = i + 1, J = j + 1;
+        const long double want =
+            alpha * (1e-5L * (I + J) * S1 + 1e-4L * I * J * K + 1e-6L * S2) + gamma;
+        worst = std::fmax(worst, (double)std::fabs((D[(size_t)i * N + j] - want) / want));
+      }
+    expect(worst < 1e-13, "gemm::compute_test vs closed form", worst);
+
+    std::vector<double> PA((size_t)M * K), PB((size_t)K * N), D2((size_t)M * N);
+    gemm<double>::pack_A(M, K, A.data(), PA.data());
+    gemm<double>::pack_B(N, K, B.data(), PB.data());
+    gemm<double>::compute_streamk_packed(M, N, K, PA.data(), PB.data(), D2.data(),
+                                         alpha, gamma);
+    double diff = 0.0;
+    for (size_t e = 0; e < D.size(); ++e)
+      diff = std::fmax(diff, std::fabs(D2[e] - D[e]) / std::fabs(D[e]));
+    expect(diff < 1e-13, "gemm::compute_streamk_packed vs compute_test", diff);
+    const double ns = gemm<double>::compute_benchmark(M, N, K, A.data(), B.data(),
+                                                      D2.data(), alpha, gamma);
+    expect(ns > 0.0, "gemm::compute_benchmark ns per call", ns);
+  }
+  // sme_syrk<double>::compute: P16 input, C = alpha A^T A + beta C on the
+  // column-major upper triangle; entries below it untouched.
+  {
+    const int K = 3000, M = 70, t = (M + 15) / 16;
+    std::vector<double> Ad((size_t)K * M), P((size_t)t * K * 16, 0.0);
+    unsigned s = 12345;
+    for (auto& x : Ad) {
+      s = s * 1103515245u + 12345u;
+      x = ((s >> 8) & 0xFFFF) / 65536.0 - 0.5;
+    }
+    for (int m = 0; m < M; ++m)
+      for (int k = 0; k < K; ++k) P[((size_t)(m / 16) * K + k) * 16 + m % 16] = Ad[(size_t)k * M + m];
+    std::vector<double> C((size_t)M * M, 0.25);
+    sme_syrk<double>::compute(K, M, P.data(), C.data(), 1.5, 2.0);
+    double worst = 0.0, untouched = 0.0;
+    for (int m2 = 0; m2 < M; ++m2)
+      for (int m1 = 0; m1 < M; ++m1) {
+        const double got = C[m1 + (size_t)m2 * M];
+        if (m1 <= m2) {
+          long double acc = 0;
+          for (int k = 0; k < K; ++k) acc += (long double)Ad[(size_t)k * M + m1] * Ad[(size_t)k * M + m2];
+          const double want = (double)(1.5L * acc + 0.5L);
+          worst = std::fmax(worst, std::fabs(got - want) / std::fmax(1.0, std::fabs(want)));
+        } else {
+          untouched = std::fmax(untouched, std::fabs(got - 0.25));
+        }
+      }
+    expect(worst < 1e-13, "sme_syrk::compute vs naive (long double)", worst);
+    expect(untouched == 0.0, "sme_syrk::compute leaves m1 > m2 alone", untouched);
+  }
+  // sme_gemm_trp_3d<double>::compute with Q = I: the band-packed output
+  // holds the grid-packed input, re-laid out.
+  {
+    const int Nd = 256, nb = 48;
+    std::vector<double> col((size_t)Nd * nb), grid((size_t)Nd * nb), Q((size_t)nb * nb, 0.0),
+        D((size_t)Nd * nb);
+    for (size_t e = 0; e < col.size(); ++e) col[e] = std::sin(0.001 * (double)e);
+    linalg_pack_c16_f64(Nd, nb, col.data(), grid.data());
+    for (int b = 0; b < nb; ++b) Q[(size_t)b * nb + b] = 1.0;
+    sme_gemm_trp_3d<double>::compute(Nd, nb, grid.data(), Q.data(), D.data());
+    double worst = 0.0;
+    for (int g = 0; g < Nd; ++g)
+      for (int b = 0; b < nb; ++b)
+        worst = std::fmax(worst, std::fabs(D[((size_t)(b / 16) * Nd + g) * 16 + b % 16] -
+                                           col[g + (size_t)b * Nd]));
+    expect(worst == 0.0, "sme_gemm_trp_3d::compute with Q = I", worst);
+    std::vector<double> back((size_t)Nd * nb);
+    linalg_unpack_c16_f64(Nd, nb, grid.data(), back.data());
+    double rt = 0.0;
+    for (size_t e = 0; e < col.size(); ++e) rt = std::fmax(rt, std::fabs(back[e] - col[e]));
+    expect(rt == 0.0, "linalg_pack/unpack_c16_f64 round trip", rt);
+  }
+  std::printf("%s\n", failures ? "FAILED" : "all facade checks passed");
+  return failures ? 1 : 0;
+}
