@@ -58,8 +58,10 @@ def build(verbose=False):
             raise RuntimeError("hipcc failed on %s:\n%s" % (src, r.stderr))
         objs.append(obj)
     tmp = LIB + ".tmp"
+    # --no-undefined: a kernel template whose host stub was not emitted
+    # must fail the build, not the first dlopen.
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-pthread",
-           "-o", tmp] + objs
+           "-Wl,--no-undefined", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s" % r.stderr)

@@ -347,6 +347,16 @@ struct GemmLaunch {
   int64_t spin_limit;
 };
 
+// 16 bytes per lane from a buffer resource straight into LDS at
+// lds_dst + 16 * lane (a non-template helper, so the host pass of the
+// kernel templates never sees the target builtin).
+__device__ __forceinline__ void lds_dma16(__amdgpu_buffer_rsrc_t r,
+                                          int8_t* lds_dst, int voffset) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(
+      r, (__attribute__((address_space(3))) void*)lds_dst, 16, voffset, 0, 0,
+      0);
+}
+
 __device__ __forceinline__ int64_t wg_start(int w, int64_t total, int G) {
   const int64_t base = total / G, extra = total % G;
   return (int64_t)w * base + ((int64_t)w < extra ? (int64_t)w : extra);
@@ -403,6 +413,27 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
     chB[p] = q % CPR;
   }
 
+  // PD == 0: operands go straight from memory into LDS (16-byte LDS DMA:
+  // lane l of a wave writes LDS base + 16 l, measured on gfx950), one wave
+  // instruction per 1 KB = 1024 / BK rows; the swizzle is applied on the
+  // global side (lane l fetches logical chunk (l % CPR) ^ f(row)).
+  constexpr int RPD = 1024 / BK;                       // rows per DMA
+  constexpr int NDA = BM / RPD, NDB = BN / RPD;        // DMAs per stage
+  constexpr int DPW = (NDA + NDB) / (WM * WN);         // DMAs per wave
+  static_assert(PD != 0 || (NDA + NDB) % (WM * WN) == 0, "DMA split");
+  int dma_off[PD == 0 ? DPW : 1];
+  if (PD == 0) {
+#pragma unroll
+    for (int d = 0; d < DPW; ++d) {
+      const int q = wave * DPW + d;
+      const int r = (q < NDA ? q : q - NDA) * RPD + lane / CPR;
+      const int pc = lane % CPR;
+      const int c = BK == 128 ? (pc ^ (r & 7) ^ ((r >> 3) & 1))
+                              : (pc ^ (r & 3) ^ ((r >> 2) & 3));
+      dma_off[d] = r * G.kpad + c * 16;
+    }
+  }
+
   v16i acc[MB][NB];
 
   // MAC steps [kb, ke) of unit `unit` into acc.
@@ -417,6 +448,10 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
     }
     m0 = ti * BM;
     n0 = tj * BN;
+    // SYRK diagonal tile: a wave whose sub-tile lies entirely above the
+    // diagonal does no MFMA work (its residues are never read); the other
+    // waves on its SIMD get the matrix core to themselves.
+    const bool idle = G.syrk && ti == tj && wm * TM + TM <= wn * TN;
     const int8_t* baseA = G.A + l * G.a_plane + (int64_t)m0 * G.kpad;
     const int8_t* baseB = G.B + l * G.b_plane + (int64_t)n0 * G.kpad;
 #pragma unroll
@@ -480,9 +515,40 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
                                                               acc[i][j], 0, 0, 0);
       }
     };
-    v4i ra0[CA], rb0[CB], ra1[CA], rb1[CB];
     int kk = kb;
     int cur = 0;
+    if (PD == 0) {
+      auto issue = [&](int buf, int step) {
+        const __amdgpu_buffer_rsrc_t sa = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<int8_t*>(baseA + (int64_t)step * BK), (short)0,
+            kNumRecords, kRsrcWord3);
+        const __amdgpu_buffer_rsrc_t sb = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<int8_t*>(baseB + (int64_t)step * BK), (short)0,
+            kNumRecords, kRsrcWord3);
+        int8_t* base = lds + buf * (BM + BN) * BK;
+#pragma unroll
+        for (int d = 0; d < DPW; ++d) {
+          const int q = wave * DPW + d;  // wave-uniform
+          if (q < NDA)
+            lds_dma16(sa, base + q * 1024, dma_off[d]);
+          else
+            lds_dma16(sb, base + BM * BK + (q - NDA) * 1024, dma_off[d]);
+        }
+      };
+      issue(0, kk);
+      for (; kk < ke; ++kk) {
+        // This wave's copies into LDS[cur] are done; after the barrier
+        // everyone's are, and everyone has finished reading LDS[cur ^ 1].
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (kk + 1 < ke) issue(cur ^ 1, kk + 1);
+        if (!idle) compute(cur);
+        cur ^= 1;
+      }
+      __syncthreads();
+      return;
+    }
+    v4i ra0[CA], rb0[CB], ra1[CA], rb1[CB];
     if (PD == 1) {
       // One register set: step kk + 1 is in flight while step kk is
       // multiplied out (for tiles whose two sets would not fit).
@@ -492,7 +558,7 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
       for (;;) {
         load(ra0, rb0, kk + 1);
         __builtin_amdgcn_sched_barrier(0);
-        compute(cur);
+        if (!idle) compute(cur);
         if (kk + 1 >= ke) break;
         store(ra0, rb0, cur ^ 1);
         __syncthreads();
@@ -511,7 +577,7 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
     for (;;) {
       load(ra0, rb0, kk + 2);
       __builtin_amdgcn_sched_barrier(0);
-      compute(cur);
+      if (!idle) compute(cur);
       if (kk + 1 >= ke) break;
       store(ra1, rb1, cur ^ 1);
       __syncthreads();
@@ -519,7 +585,7 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
       ++kk;
       load(ra1, rb1, kk + 2);
       __builtin_amdgcn_sched_barrier(0);
-      compute(cur);
+      if (!idle) compute(cur);
       if (kk + 1 >= ke) break;
       store(ra0, rb0, cur ^ 1);
       __syncthreads();
@@ -797,19 +863,25 @@ struct GemmCfg {
 // 2: 128x128, 2 waves of 64x128, BK 128, 2-deep (64 KB LDS)
 // 3: 128x128, 4 waves of 64x64,  BK 64,  2-deep (32 KB LDS)
 // 4: 256x256, 8 waves of 64x128, BK 128, 1-deep (128 KB LDS)
+// 5: 256x256, 8 waves of 64x128, BK 128, LDS DMA, 2 stages (128 KB LDS)
+// 6: 128x128, 4 waves of 64x64,  BK 128, LDS DMA, 2 stages (64 KB LDS)
 static const GemmCfg kGemmCfgs[] = {{256, 256, 4, 2, 64, 2},
                                     {128, 128, 2, 2, 128, 2},
                                     {128, 128, 2, 1, 128, 2},
                                     {128, 128, 2, 2, 64, 2},
-                                    {256, 256, 4, 2, 128, 1}};
-constexpr int kNumGemmCfgs = 5;
+                                    {256, 256, 4, 2, 128, 1},
+                                    {256, 256, 4, 2, 128, 0},
+                                    {128, 128, 2, 2, 128, 0}};
+constexpr int kNumGemmCfgs = 7;
 
 #define SKG_OZ_CFGS(X)          \
   X(0, 256, 256, 4, 2, 64, 2)   \
   X(1, 128, 128, 2, 2, 128, 2)  \
   X(2, 128, 128, 2, 1, 128, 2)  \
   X(3, 128, 128, 2, 2, 64, 2)   \
-  X(4, 256, 256, 4, 2, 128, 1)
+  X(4, 256, 256, 4, 2, 128, 1)  \
+  X(5, 256, 256, 4, 2, 128, 0)  \
+  X(6, 128, 128, 2, 2, 128, 0)
 
 static const void* gemm_fn(int cfg) {
 #define SKG_FN_CASE(i, bm, bn, wm, wn, bk, pd) \
@@ -1044,7 +1116,7 @@ OzPlan choose_plan(skgemm_ctx* c, int kind, int M, int N, int L, int kpad) {
                                  gemm_occupancy(1) * c->num_cus), 1};
   OzPlan best{1, 0, 0, 1};
   double best_score = 0.0;
-  for (int cfg : {0, 1}) {
+  for (int cfg : {5, 6}) {
     const int cap = gemm_occupancy(cfg) * c->num_cus;
     const int ipt = kpad / kGemmCfgs[cfg].bk;
     for (int kc = 1; kc <= 16 && (kc == 1 || ipt / kc >= 32); ++kc) {
@@ -1054,7 +1126,8 @@ OzPlan choose_plan(skgemm_ctx* c, int kind, int M, int N, int L, int kpad) {
       // Bigger tiles halve the L2 traffic per MFMA; each extra chunk costs
       // one more residue plane to write and read.
       const double score =
-          eff * (cfg == 0 ? 1.0 : 0.9) * (1.0 - 0.005 * (kc - 1));
+          eff * (kGemmCfgs[cfg].bm == 256 ? 1.0 : 0.97) *
+          (1.0 - 0.01 * (kc - 1));
       if (eff >= 0.75 && score > best_score) {
         best_score = score;
         best = OzPlan{cfg, 1, (int)std::min<int64_t>(cap, U), kc};
@@ -1062,8 +1135,8 @@ OzPlan choose_plan(skgemm_ctx* c, int kind, int M, int N, int L, int kpad) {
     }
   }
   if (!best.dp && c->oz_no_wait) {
-    const int64_t U = (int64_t)L * tiles_for(1);
-    best = OzPlan{1, 1, (int)std::min<int64_t>(gemm_occupancy(1) * c->num_cus, U),
+    const int64_t U = (int64_t)L * tiles_for(6);
+    best = OzPlan{6, 1, (int)std::min<int64_t>(gemm_occupancy(6) * c->num_cus, U),
                   1};
   }
   if (best.dp) {
