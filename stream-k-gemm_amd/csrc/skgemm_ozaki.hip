@@ -420,9 +420,12 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
   constexpr int RPD = 1024 / BK;                       // rows per DMA
   constexpr int NDA = BM / RPD, NDB = BN / RPD;        // DMAs per stage
   constexpr int DPW = (NDA + NDB) / (WM * WN);         // DMAs per wave
-  static_assert(PD != 0 || (NDA + NDB) % (WM * WN) == 0, "DMA split");
-  int dma_off[PD == 0 ? DPW : 1];
-  if (PD == 0) {
+  // PD >= 3: the same with PD LDS stages (PD - 1 steps in flight).
+  constexpr bool DMA = PD == 0 || PD >= 3;
+  constexpr int NS = PD == 0 ? 2 : PD;  // LDS stages of the DMA path
+  static_assert(!DMA || (NDA + NDB) % (WM * WN) == 0, "DMA split");
+  int dma_off[DMA ? DPW : 1];
+  if (DMA) {
 #pragma unroll
     for (int d = 0; d < DPW; ++d) {
       const int q = wave * DPW + d;
@@ -545,6 +548,53 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
         if (!idle) compute(cur);
         cur ^= 1;
       }
+      __syncthreads();
+      return;
+    }
+    if constexpr (PD >= 3) {
+      // NS stages: steps kk + 1 .. kk + NS - 1 are in flight while step kk
+      // is multiplied out.  Steps past the unit's end are clamped to its
+      // last step (in bounds, never read), so exactly (NS - 2) DPW copies
+      // are younger than step kk's and vmcnt <= (NS - 2) DPW means step
+      // kk's copies have landed.
+      constexpr int kAhead = (NS > 2 ? NS - 2 : 0) * DPW;
+      constexpr int kWait = (kAhead & 15) | ((kAhead >> 4) << 14) |
+                            (7 << 4) | (15 << 8);  // vmcnt only
+      auto issue = [&](int buf, int step) {
+        step = step < ke ? step : ke - 1;
+        const __amdgpu_buffer_rsrc_t sa = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<int8_t*>(baseA + (int64_t)step * BK), (short)0,
+            kNumRecords, kRsrcWord3);
+        const __amdgpu_buffer_rsrc_t sb = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<int8_t*>(baseB + (int64_t)step * BK), (short)0,
+            kNumRecords, kRsrcWord3);
+        int8_t* base = lds + buf * (BM + BN) * BK;
+#pragma unroll
+        for (int d = 0; d < DPW; ++d) {
+          const int q = wave * DPW + d;  // wave-uniform
+          if (q < NDA)
+            lds_dma16(sa, base + q * 1024, dma_off[d]);
+          else
+            lds_dma16(sb, base + BM * BK + (q - NDA) * 1024, dma_off[d]);
+        }
+      };
+#pragma unroll
+      for (int s0 = 0; s0 < NS - 1; ++s0) issue(s0, kk + s0);
+      int stage = 0;
+      for (; kk < ke; ++kk) {
+        // A bare s_barrier: a workgroup fence would also wait for the
+        // younger copies still in flight (they write LDS).  The empty asm
+        // statements keep the compiler from moving LDS accesses across it.
+        __builtin_amdgcn_s_waitcnt(kWait);
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();  // all of `stage` landed; stage-1 free
+        asm volatile("" ::: "memory");
+        const int free_stage = stage == 0 ? NS - 1 : stage - 1;
+        issue(free_stage, kk + NS - 1);
+        if (!idle) compute(stage);
+        stage = stage + 1 == NS ? 0 : stage + 1;
+      }
+      __builtin_amdgcn_s_waitcnt(0);  // clamped tail copies
       __syncthreads();
       return;
     }
@@ -865,14 +915,20 @@ struct GemmCfg {
 // 4: 256x256, 8 waves of 64x128, BK 128, 1-deep (128 KB LDS)
 // 5: 256x256, 8 waves of 64x128, BK 128, LDS DMA, 2 stages (128 KB LDS)
 // 6: 128x128, 4 waves of 64x64,  BK 128, LDS DMA, 2 stages (64 KB LDS)
+// 7: 128x128, 4 waves of 64x64,  BK 64,  LDS DMA, 4 stages (64 KB LDS)
+// 8: 128x128, 4 waves of 64x64,  BK 128, LDS DMA, 3 stages (96 KB LDS)
+// 9: 256x256, 8 waves of 64x128, BK 64,  LDS DMA, 3 stages (96 KB LDS)
 static const GemmCfg kGemmCfgs[] = {{256, 256, 4, 2, 64, 2},
                                     {128, 128, 2, 2, 128, 2},
                                     {128, 128, 2, 1, 128, 2},
                                     {128, 128, 2, 2, 64, 2},
                                     {256, 256, 4, 2, 128, 1},
                                     {256, 256, 4, 2, 128, 0},
-                                    {128, 128, 2, 2, 128, 0}};
-constexpr int kNumGemmCfgs = 7;
+                                    {128, 128, 2, 2, 128, 0},
+                                    {128, 128, 2, 2, 64, 4},
+                                    {128, 128, 2, 2, 128, 3},
+                                    {256, 256, 4, 2, 64, 3}};
+constexpr int kNumGemmCfgs = 10;
 
 #define SKG_OZ_CFGS(X)          \
   X(0, 256, 256, 4, 2, 64, 2)   \
@@ -881,7 +937,10 @@ constexpr int kNumGemmCfgs = 7;
   X(3, 128, 128, 2, 2, 64, 2)   \
   X(4, 256, 256, 4, 2, 128, 1)  \
   X(5, 256, 256, 4, 2, 128, 0)  \
-  X(6, 128, 128, 2, 2, 128, 0)
+  X(6, 128, 128, 2, 2, 128, 0)  \
+  X(7, 128, 128, 2, 2, 64, 4)   \
+  X(8, 128, 128, 2, 2, 128, 3)  \
+  X(9, 256, 256, 4, 2, 64, 3)
 
 static const void* gemm_fn(int cfg) {
 #define SKG_FN_CASE(i, bm, bn, wm, wn, bk, pd) \
@@ -897,7 +956,8 @@ static const void* gemm_fn(int cfg) {
 
 static size_t gemm_lds(int cfg) {
   const GemmCfg& g = kGemmCfgs[cfg];
-  const size_t pipe = (size_t)2 * (g.bm + g.bn) * g.bk;
+  const size_t stages = g.pd >= 3 ? g.pd : 2;
+  const size_t pipe = stages * (g.bm + g.bn) * g.bk;
   const size_t tile = (size_t)g.bn * (g.bm + 16);  // residue transpose
   return std::max(pipe, tile);
 }

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <set>
+#include <utility>
 #include <tuple>
 #include <vector>
 
@@ -97,28 +98,40 @@ extern "C" int skgemm_autotune(skgemm_ctx* c, int kind, int la, int lb, int M,
     const int64_t ipt = std::max<int64_t>(1, (K + t.bk - 1) / t.bk);
     std::set<int> chunks = {1};
     if (ipt >= 16 * c->num_xcds) chunks.insert(c->num_xcds);
+    std::set<std::pair<int, int>> cands;
     for (int g : grids)
-      for (int kc : chunks) {
-        c->forced_cfg = cfg;
-        c->forced_grid = g;
-        c->forced_kchunks = kc;
-        rc = SKGEMM_OK;
-        for (int i = 0; i < 2 && rc == SKGEMM_OK; ++i) rc = run();
-        if (rc != SKGEMM_OK) break;
-        const int reps = 5;
-        hipEventRecord(e0, c->stream);
-        for (int i = 0; i < reps && rc == SKGEMM_OK; ++i) rc = run();
-        hipEventRecord(e1, c->stream);
-        if (rc != SKGEMM_OK) break;
-        hipEventSynchronize(e1);
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, e0, e1);
-        const double us = 1e3 * ms / reps;
-        if (us < best) {
-          best = us;
-          bp = plan(c, kind, la, lb, M, N, K);
-        }
+      for (int kc : chunks) cands.insert({g, kc});
+    // Data-parallel candidates: one (K chunk, tile) unit per workgroup, so
+    // no fix-up and the workgroups of one XCD run tiles of one K chunk in
+    // step (panel reuse in L2).
+    const int64_t tiles = tiles_for(kind, cfg, M, N);
+    for (int kc = 1; kc <= 16 && ipt / kc >= 8; ++kc) {
+      const int64_t units = tiles * kc;
+      if (units <= cap && 10 * units >= 7 * (int64_t)cap)
+        cands.insert({(int)units, kc});
+    }
+    for (const auto& gk : cands) {
+      const int g = gk.first, kc = gk.second;
+      c->forced_cfg = cfg;
+      c->forced_grid = g;
+      c->forced_kchunks = kc;
+      rc = SKGEMM_OK;
+      for (int i = 0; i < 2 && rc == SKGEMM_OK; ++i) rc = run();
+      if (rc != SKGEMM_OK) break;
+      const int reps = 5;
+      hipEventRecord(e0, c->stream);
+      for (int i = 0; i < reps && rc == SKGEMM_OK; ++i) rc = run();
+      hipEventRecord(e1, c->stream);
+      if (rc != SKGEMM_OK) break;
+      hipEventSynchronize(e1);
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      const double us = 1e3 * ms / reps;
+      if (us < best) {
+        best = us;
+        bp = plan(c, kind, la, lb, M, N, K);
       }
+    }
     if (rc != SKGEMM_OK) break;
   }
   c->forced_cfg = saved.cfg;

@@ -177,7 +177,8 @@ def test_every_plan_gives_identical_bits(skg, ctx, rng):
     try:
         for plan in ("0,1", "1,1", "2,1", "3,1", "0,0", "1,0,77", "2,0",
                      "3,0,300", "0,1,0,3", "1,1,100,5", "3,1,0,7",
-                     "5,1,0,3", "6,1", "5,0", "6,0,200", "4,1,0,2"):
+                     "5,1,0,3", "6,1", "5,0", "6,0,200", "4,1,0,2",
+                     "7,1", "8,1,0,2", "9,1,0,3", "7,0", "9,0,150"):
             os.environ["SKGEMM_OZ_PLAN"] = plan
             C = np.zeros(N * N)
             ctx.dsyrk_ozaki_host(0, N, K, 1.0, X, N, 0.0, C, N)
@@ -196,7 +197,8 @@ def test_rotation_plans_identical(skg, ctx, rng):
     Bb = np.ascontiguousarray(B.ravel())
     outs = []
     try:
-        for plan in ("0,1", "1,1", "2,0", "3,0", "5,1", "6,1", "5,0"):
+        for plan in ("0,1", "1,1", "2,0", "3,0", "5,1", "6,1", "5,0", "7,1",
+                     "8,1", "9,1", "9,0"):
             os.environ["SKGEMM_OZ_PLAN"] = plan
             D = np.zeros(M * N)
             ctx.dgemm_ozaki_host(0, 0, 0, M, N, K, 1.0, Ab, K, Bb, N, 0.0, D, N)
