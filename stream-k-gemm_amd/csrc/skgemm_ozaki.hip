@@ -57,9 +57,9 @@ struct Tables {
   // CRT: x = sum_l r_l W_l - q P (mod 2^128),
   // W_l = (P / m_l) ((P / m_l)^{-1} mod m_l), q = floor(sum_l r_l w_l + 1/2)
   // with w_l = W_l / P rounded to FP64.
-  uint32_t W[kMaxMod][4];  // W_l in 32-bit limbs, least significant first
+  double W[kMaxMod][4];  // W_l in 32-bit limbs (exact), least significant first
   double wfrac[kMaxMod];
-  uint32_t P[4];
+  double P[4];
 };
 
 typedef int v4i __attribute__((ext_vector_type(4)));
@@ -801,36 +801,43 @@ struct CrtLaunch {
   double* fro2;  // per workgroup: its part of ||C||_F^2 (SYRK: both halves)
 };
 
-// x = S - q P with S = sum_l r_l W_l, in 32-bit limbs held in 64-bit
-// accumulators (each stays below 2^45 before the carries are propagated).
-// Exact: the FP64 estimate f of S / P + 1/2 is within 2^-36 of the true
-// value, whose fractional part lies in [1/4, 3/4] because |x| <= P / 4.
-// r[l] holds the residue in byte e of word l.  Returns x = hi 2^64 + lo.
+// x = S - q P with S = sum_l r_l W_l, in 32-bit limbs held in FP64
+// accumulators: every product r_l W_lj < 2^40 and every limb sum < 2^45, so
+// all of it is exact FMA arithmetic (full-rate FP64 instead of quarter-rate
+// 64-bit integer multiplies).  q = floor(S / P + 1/2) comes from the FP64
+// estimate f, within 2^-36 of the true value, whose fractional part lies in
+// [1/4, 3/4] because |x| <= P / 4.  Carries are propagated exactly with
+// floor(b 2^-32).  r[l] holds the residue in byte e of word l.  Returns
+// x = hi 2^64 + lo with lo unsigned.
 __device__ __forceinline__ void crt(const Tables& T, const uint32_t* rw, int e,
                                     int64_t& hi, uint64_t& lo) {
-  uint64_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-  double f = 0.5;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, f = 0.5;
 #pragma unroll
   for (int l = 0; l < kMaxMod; ++l) {
     if (l < T.L) {
-      const uint32_t r = (rw[l] >> (8 * e)) & 255u;
-      a0 += (uint64_t)T.W[l][0] * r;
-      a1 += (uint64_t)T.W[l][1] * r;
-      a2 += (uint64_t)T.W[l][2] * r;
-      a3 += (uint64_t)T.W[l][3] * r;
-      f = fma((double)r, T.wfrac[l], f);
+      const double r = (double)((rw[l] >> (8 * e)) & 255u);
+      a0 = fma(r, T.W[l][0], a0);
+      a1 = fma(r, T.W[l][1], a1);
+      a2 = fma(r, T.W[l][2], a2);
+      a3 = fma(r, T.W[l][3], a3);
+      f = fma(r, T.wfrac[l], f);
     }
   }
-  const uint64_t q = (uint64_t)floor(f);
-  int64_t b0 = (int64_t)(a0 - (uint64_t)T.P[0] * q);
-  int64_t b1 = (int64_t)(a1 - (uint64_t)T.P[1] * q);
-  int64_t b2 = (int64_t)(a2 - (uint64_t)T.P[2] * q);
-  int64_t b3 = (int64_t)(a3 - (uint64_t)T.P[3] * q);
-  b1 += b0 >> 32;
-  b2 += b1 >> 32;
-  b3 += b2 >> 32;
+  const double q = floor(f);
+  const double two32 = 4294967296.0, inv32 = 2.3283064365386963e-10;
+  double b0 = fma(-q, T.P[0], a0), b1 = fma(-q, T.P[1], a1),
+         b2 = fma(-q, T.P[2], a2), b3 = fma(-q, T.P[3], a3);
+  double c = floor(b0 * inv32);
+  b0 = fma(-c, two32, b0);
+  b1 += c;
+  c = floor(b1 * inv32);
+  b1 = fma(-c, two32, b1);
+  b2 += c;
+  c = floor(b2 * inv32);
+  b2 = fma(-c, two32, b2);
+  b3 += c;
   lo = (uint64_t)(uint32_t)b0 | ((uint64_t)(uint32_t)b1 << 32);
-  hi = (int64_t)(((uint64_t)b3 << 32) | (uint64_t)(uint32_t)b2);
+  hi = (int64_t)(((uint64_t)(int64_t)b3 << 32) | (uint64_t)(uint32_t)b2);
 }
 
 // Blocks of 256 rows x 4 columns; a thread takes 4 consecutive rows of one
@@ -1030,12 +1037,12 @@ Tables make_tables(int L) {
   typedef unsigned __int128 u128;
   u128 P = 1;
   for (int l = 0; l < L; ++l) P *= (u128)kModuli[l];
-  for (int j = 0; j < 4; ++j) T.P[j] = (uint32_t)(P >> (32 * j));
+  for (int j = 0; j < 4; ++j) T.P[j] = (double)(uint32_t)(P >> (32 * j));
   for (int l = 0; l < L; ++l) {
     const u128 Pl = P / (u128)kModuli[l];
     const int inv = modinv((int)(Pl % (u128)kModuli[l]), kModuli[l]);
     const u128 W = Pl * (u128)inv;  // < P
-    for (int j = 0; j < 4; ++j) T.W[l][j] = (uint32_t)(W >> (32 * j));
+    for (int j = 0; j < 4; ++j) T.W[l][j] = (double)(uint32_t)(W >> (32 * j));
     // W / P = inv / m_l exactly; rounded once to FP64.
     T.wfrac[l] = (double)inv / (double)kModuli[l];
   }
