@@ -206,6 +206,39 @@ def bench_split(skg, ctx, reps):
     return out
 
 
+def bench_layouts(skg, ctx, reps):
+    """Layout conversions of the SYRK / rotation operand Psi, Nd x nb =
+    65536 x 1024 FP64 (512 MiB): column-major -> grid-packed (pack_c16),
+    grid-packed -> band-packed and band-packed -> grid-packed (the
+    band/grid tile conversions).  Algorithmic traffic: 8 bytes read + 8
+    written per element."""
+    Nd, nb = 65536, 1024
+    n = Nd * nb
+    col = skg.DeviceArray(ctx, n)
+    grid = skg.DeviceArray(ctx, n, zero=True)
+    band = skg.DeviceArray(ctx, n, zero=True)
+    ctx.fill_normal(col.ptr, n, 7, 1.0)
+    runs = {
+        "pack_c16": lambda: ctx.pack(skg.ROW_MAJOR, col.ptr, Nd, skg.PACK16,
+                                     grid.ptr, nb, Nd, nb),
+        "grid_to_band": lambda: ctx.transpose(skg.PACK16, grid.ptr, nb,
+                                              skg.PACK16, band.ptr, Nd, Nd,
+                                              nb),
+        "band_to_grid": lambda: ctx.transpose(skg.PACK16, band.ptr, Nd,
+                                              skg.PACK16, grid.ptr, nb, nb,
+                                              Nd),
+    }
+    out = {"workload": "Psi 65536 x 1024 FP64 layout conversions",
+           "bytes_per_launch": 16 * n}
+    for name, fn in runs.items():
+        t = median_ms(ctx, fn, reps)
+        gbps = 16.0 * n / (t * 1e-3) / 1e9
+        out[name] = {"ms": t, "gbps": gbps, "frac_hbm_peak": gbps / PEAK_HBM_GBS}
+    for x in (col, grid, band):
+        x.free()
+    return out
+
+
 def c4_batch(skg, ctx, seed=5, count=64):
     """64 systems: Ns ~ U{128, 136, ..., 1536}, Nd ~ U{4096, 13824, 32768,
     65536}, Psi ~ N(0, 1/sqrt(Nd)) (seeded per system), Q ~ U(-1, 1); one
@@ -356,7 +389,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the c1, split-form, batched and serving "
                          "sections")
-    ap.add_argument("--extra", default="c1,split,batched,serving",
+    ap.add_argument("--extra", default="c1,split,layouts,batched,serving",
                     help="comma list of extra sections to run")
     ap.add_argument("--tuning-cache", default=None,
                     help="autotuner cache file: read if it exists (no "
@@ -461,6 +494,9 @@ def main():
     if "split" in sections:
         log("split form (SYRK 65536x1024) ...")
         extra["split_syrk"] = bench_split(skg, ctx, 10)
+    if "layouts" in sections:
+        log("layout conversions (65536x1024) ...")
+        extra["layouts"] = bench_layouts(skg, ctx, 10)
     if "batched" in sections or "serving" in sections:
         log("batched (64 systems) ...")
         extra["batched"] = bench_batched(skg, ctx, 5)

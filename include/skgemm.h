@@ -19,6 +19,8 @@
  *                                                         -> skgemm_dgemm(_host)
  *   linalg_pack_c16_f64 / unpack_c16      include/sme_pack.hpp:319-352
  *                                                         -> skgemm_pack
+ *   sme_pack<T>::band_to_grid_tile /      include/sme_pack.hpp:178-313
+ *     grid_to_band_tile                                   -> skgemm_transpose
  *   hbm_alloc / hbm_calloc / hbm_free     include/hbm_alloc.hpp:21-50
  *                                                         -> skgemm_dev_alloc/free
  *   run_fmopa_loop peak probe             sme_peak_microbench.cpp:16-60
@@ -201,6 +203,19 @@ int skgemm_dsyrk_ozaki_host(skgemm_ctx* ctx, int layout_a, int N, int K,
 int skgemm_pack(skgemm_ctx* ctx, int src_layout, const double* src,
                 int64_t ld_src, int dst_layout, double* dst, int64_t ld_dst,
                 int o_extent, int r_extent);
+
+/* Copy with the roles of the two indices exchanged: dst element (r = o0,
+ * o = r0) = src element (o0, r0), so dst has o extent r_extent and r extent
+ * o_extent (ld_dst is checked against those).  With src PACK16 of (o = band
+ * b, r = grid g, ld_src = Nd) and dst PACK16 (ld_dst = nb) this is
+ * band_to_grid_tile: band-packed [nb/16][Nd][16] -> grid-packed
+ * [Nd/16][nb][16] (include/sme_pack.hpp:178-244); the same call with the
+ * extents exchanged is grid_to_band_tile (include/sme_pack.hpp:246-313).
+ * Plain layouts give an ordinary out-of-place transpose.  Device pointers,
+ * asynchronous; src and dst must not overlap. */
+int skgemm_transpose(skgemm_ctx* ctx, int src_layout, const double* src,
+                     int64_t ld_src, int dst_layout, double* dst,
+                     int64_t ld_dst, int o_extent, int r_extent);
 
 /* ---- batched / grouped launch ------------------------------------------ */
 

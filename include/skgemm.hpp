@@ -6,7 +6,7 @@
 //   gemm<T>                 gemm_crsp.cpp:57-431, gemm_opt.cpp:55-426
 //   sme_syrk<T>             include/sme_syrk.hpp:22-150, syrk_crsp.cpp:51-349
 //   sme_gemm_trp_3d<T>      include/sme_trp_3d.hpp:48-220
-//   sme_pack<T> C wrappers  include/sme_pack.hpp:316-352
+//   sme_pack<T>             include/sme_pack.hpp:30-352
 //   hbm_init/hbm_alloc/...  include/hbm_alloc.hpp:14-50
 // Only T = double is provided (the reference instantiates double only).
 //
@@ -234,43 +234,115 @@ class sme_gemm_trp_3d<double> {
 template <typename T>
 using sme_gemm_trp = sme_gemm_trp_3d<T>;
 
-// ---- include/sme_pack.hpp:316-352 (C wrappers) ------------------------------
-// Column-major [Nd, nb] <-> grid-packed [Nd/16][nb][16], on the device.
-inline void linalg_pack_c16_f64(const int Nd, const int nb, const double* src,
-                                double* dst) {
-  skgemm_ctx* c = skgemm_facade::ctx();
-  const size_t n = (size_t)Nd * nb, np = (size_t)(Nd / 16) * nb * 16;
+// ---- include/sme_pack.hpp ----------------------------------------------------
+namespace skgemm_facade {
+
+// Host-pointer layout conversion on the device: copies src (n_src doubles)
+// and dst (n_dst doubles, so elements the conversion does not write keep
+// their values, as in the reference) to HBM, runs `op`, copies dst back.
+template <typename Op>
+inline void convert_host(const double* src, size_t n_src, double* dst,
+                         size_t n_dst, Op op, const char* what) {
+  skgemm_ctx* c = ctx();
   void *ds, *dd;
-  skgemm_facade::check(skgemm_dev_alloc(c, n * 8, 0, &ds), "alloc");
-  skgemm_facade::check(skgemm_dev_alloc(c, np * 8 + 8, 0, &dd), "alloc");
-  skgemm_memcpy_h2d(c, ds, src, n * 8);
-  // skgemm_pack addresses element (r, o) = (band b, grid g); the source
-  // src[g + b*Nd] is r*ld + o with ld = Nd, i.e. ROW_MAJOR in that convention.
-  skgemm_facade::check(skgemm_pack(c, SKGEMM_ROW_MAJOR, (const double*)ds, Nd,
-                                   SKGEMM_PACK16, (double*)dd, nb,
-                                   (Nd / 16) * 16, nb),
-                       "linalg_pack_c16_f64");
-  skgemm_facade::check(skgemm_synchronize(c), "synchronize");
-  skgemm_memcpy_d2h(c, dst, dd, np * 8);
+  check(skgemm_dev_alloc(c, n_src * 8 + 8, 0, &ds), "alloc");
+  check(skgemm_dev_alloc(c, n_dst * 8 + 8, 0, &dd), "alloc");
+  if (n_src) skgemm_memcpy_h2d(c, ds, src, n_src * 8);
+  if (n_dst) skgemm_memcpy_h2d(c, dd, dst, n_dst * 8);
+  check(op(c, (const double*)ds, (double*)dd), what);
+  check(skgemm_synchronize(c), "synchronize");
+  if (n_dst) skgemm_memcpy_d2h(c, dst, dd, n_dst * 8);
   skgemm_dev_free(c, ds);
   skgemm_dev_free(c, dd);
 }
 
+}  // namespace skgemm_facade
+
+template <typename T>
+class sme_pack;
+
+// Layouts (include/sme_pack.hpp:12-28): column-major [Nd, nb] (g + b*Nd;
+// the reference's row-major [nb][Nd] is the same memory), grid-packed
+// [Nd/16][nb][16] ((g/16)*nb*16 + b*16 + g%16), band-packed [nb/16][Nd][16]
+// ((b/16)*Nd*16 + g*16 + b%16).  As in the reference, only whole 16-row
+// (16-band) panels are converted: Nd/16 grid tiles, nb/16 band tiles.
+template <>
+class sme_pack<double> {
+ public:
+  using T = double;
+  static constexpr int PACK_G = 16;
+
+  // include/sme_pack.hpp:85-95: column-major [Nd, nb] -> grid-packed.
+  // skgemm_pack addresses element (r, o) = (band b, grid g); src[g + b*Nd]
+  // is r*ld + o with ld = Nd, i.e. ROW_MAJOR in that convention.
+  static void pack_c16(const int Nd, const int nb, const T* src, T* dst) {
+    const int g16 = (Nd / 16) * 16;
+    skgemm_facade::convert_host(
+        src, (size_t)Nd * nb, dst, (size_t)g16 * nb,
+        [&](skgemm_ctx* c, const double* s, double* d) {
+          return skgemm_pack(c, SKGEMM_ROW_MAJOR, s, Nd > 0 ? Nd : 1,
+                             SKGEMM_PACK16, d, nb > 0 ? nb : 1, g16, nb);
+        },
+        "sme_pack::pack_c16");
+  }
+  // include/sme_pack.hpp:97-107: row-major [nb][Nd] holds the same memory.
+  static void pack_r16(const int Nd, const int nb, const T* src, T* dst) {
+    pack_c16(Nd, nb, src, dst);
+  }
+  // include/sme_pack.hpp:130-140: grid-packed -> column-major.
+  static void unpack_c16(const int Nd, const int nb, const T* src, T* dst) {
+    const int g16 = (Nd / 16) * 16;
+    skgemm_facade::convert_host(
+        src, (size_t)g16 * nb, dst, (size_t)Nd * nb,
+        [&](skgemm_ctx* c, const double* s, double* d) {
+          return skgemm_pack(c, SKGEMM_PACK16, s, nb > 0 ? nb : 1,
+                             SKGEMM_ROW_MAJOR, d, Nd > 0 ? Nd : 1, g16, nb);
+        },
+        "sme_pack::unpack_c16");
+  }
+  static void unpack_r16(const int Nd, const int nb, const T* src, T* dst) {
+    unpack_c16(Nd, nb, src, dst);
+  }
+  // include/sme_pack.hpp:232-244: band-packed -> grid-packed.
+  static void band_to_grid_tile(const int Nd, const int nb, const T* src,
+                                T* dst) {
+    const int g16 = (Nd / 16) * 16, b16 = (nb / 16) * 16;
+    skgemm_facade::convert_host(
+        src, (size_t)b16 * Nd, dst, (size_t)g16 * nb,
+        [&](skgemm_ctx* c, const double* s, double* d) {
+          return skgemm_transpose(c, SKGEMM_PACK16, s, Nd > 0 ? Nd : 1,
+                                  SKGEMM_PACK16, d, nb > 0 ? nb : 1, b16, g16);
+        },
+        "sme_pack::band_to_grid_tile");
+  }
+  // include/sme_pack.hpp:300-312: grid-packed -> band-packed.
+  static void grid_to_band_tile(const int Nd, const int nb, const T* src,
+                                T* dst) {
+    const int g16 = (Nd / 16) * 16, b16 = (nb / 16) * 16;
+    skgemm_facade::convert_host(
+        src, (size_t)g16 * nb, dst, (size_t)b16 * Nd,
+        [&](skgemm_ctx* c, const double* s, double* d) {
+          return skgemm_transpose(c, SKGEMM_PACK16, s, nb > 0 ? nb : 1,
+                                  SKGEMM_PACK16, d, Nd > 0 ? Nd : 1, g16, b16);
+        },
+        "sme_pack::grid_to_band_tile");
+  }
+};
+
+// include/sme_pack.hpp:316-352 (C wrappers).
+inline void linalg_pack_c16_f64(const int Nd, const int nb, const double* src,
+                                double* dst) {
+  sme_pack<double>::pack_c16(Nd, nb, src, dst);
+}
+inline void linalg_pack_r16_f64(const int Nd, const int nb, const double* src,
+                                double* dst) {
+  sme_pack<double>::pack_r16(Nd, nb, src, dst);
+}
 inline void linalg_unpack_c16_f64(const int Nd, const int nb,
                                   const double* src, double* dst) {
-  skgemm_ctx* c = skgemm_facade::ctx();
-  const size_t n = (size_t)Nd * nb, np = (size_t)(Nd / 16) * nb * 16;
-  void *ds, *dd;
-  skgemm_facade::check(skgemm_dev_alloc(c, np * 8 + 8, 0, &ds), "alloc");
-  skgemm_facade::check(skgemm_dev_alloc(c, n * 8, 0, &dd), "alloc");
-  skgemm_memcpy_h2d(c, ds, src, np * 8);
-  skgemm_memcpy_h2d(c, dd, dst, n * 8);
-  skgemm_facade::check(skgemm_pack(c, SKGEMM_PACK16, (const double*)ds, nb,
-                                   SKGEMM_ROW_MAJOR, (double*)dd, Nd,
-                                   (Nd / 16) * 16, nb),
-                       "linalg_unpack_c16_f64");
-  skgemm_facade::check(skgemm_synchronize(c), "synchronize");
-  skgemm_memcpy_d2h(c, dst, dd, n * 8);
-  skgemm_dev_free(c, ds);
-  skgemm_dev_free(c, dd);
+  sme_pack<double>::unpack_c16(Nd, nb, src, dst);
+}
+inline void linalg_unpack_r16_f64(const int Nd, const int nb,
+                                  const double* src, double* dst) {
+  sme_pack<double>::unpack_r16(Nd, nb, src, dst);
 }

@@ -24,8 +24,11 @@
 //       closed form of trp_3d_test.cpp:125-138 inputs, Q = I, Q = permutation.
 //   oracle_dgemm_dd / oracle_dsyrk_dd (double-double accumulation)
 //       exact small-integer products, and agreement with numpy to 1e-15.
-//   pack / unpack
-//       round trips and element-wise layout checks (pack.cpp:37-258).
+//   pack / unpack, band_to_grid / grid_to_band
+//       round trips, element-wise layout checks (pack.cpp:37-258), the
+//       reference's band<->grid known-answer pattern g + 1000 b
+//       (pack.cpp:190-258), and numpy reshape/transpose restatements of
+//       both packed layouts.
 // The split form (Ozaki scheme II) is restated separately in oracle/ozaki.py
 // and pinned in tests/test_ozaki_oracle.py.
 
@@ -142,6 +145,24 @@ void oracle_unpack_c16(int Nd, int nb, const double* src, double* dst) {
   for (int g = 0; g < (Nd / 16) * 16; ++g)
     for (int b = 0; b < nb; ++b)
       dst[g + (long)b * Nd] =
+          src[(long)(g / 16) * nb * 16 + (long)b * 16 + g % 16];
+}
+
+// sme_pack.hpp:178-244 (band_to_grid_tile): band-packed [nb/16][Nd][16]
+// (src[(b/16)*Nd*16 + g*16 + b%16]) -> grid-packed [Nd/16][nb][16]
+// (dst[(g/16)*nb*16 + b*16 + g%16]); whole 16 x 16 blocks only
+// (g_tiles = Nd/16, b_tiles = nb/16), as the reference's loops.
+void oracle_band_to_grid(int Nd, int nb, const double* src, double* dst) {
+  for (int g = 0; g < (Nd / 16) * 16; ++g)
+    for (int b = 0; b < (nb / 16) * 16; ++b)
+      dst[(long)(g / 16) * nb * 16 + (long)b * 16 + g % 16] =
+          src[(long)(b / 16) * Nd * 16 + (long)g * 16 + b % 16];
+}
+// sme_pack.hpp:246-313 (grid_to_band_tile): the inverse map.
+void oracle_grid_to_band(int Nd, int nb, const double* src, double* dst) {
+  for (int g = 0; g < (Nd / 16) * 16; ++g)
+    for (int b = 0; b < (nb / 16) * 16; ++b)
+      dst[(long)(b / 16) * Nd * 16 + (long)g * 16 + b % 16] =
           src[(long)(g / 16) * nb * 16 + (long)b * 16 + g % 16];
 }
 

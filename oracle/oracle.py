@@ -50,6 +50,8 @@ def load(path=None):
         "oracle_pack_B": [I, I, P, P],
         "oracle_pack_c16": [I, I, P, P],
         "oracle_unpack_c16": [I, I, P, P],
+        "oracle_band_to_grid": [I, I, P, P],
+        "oracle_grid_to_band": [I, I, P, P],
         "oracle_gemm_crsp_packed": [I, I, I, P, P, P, D, D, I],
         "oracle_gemm_opt_packed": [I, I, I, P, P, P, D, D, I],
         "oracle_gemm_naive": [I, I, I, P, P, P, D, D],
@@ -124,6 +126,18 @@ def pack_c16(Nd, nb, src):
 def unpack_c16(Nd, nb, src):
     out = np.zeros(Nd * nb, np.float64)
     load().oracle_unpack_c16(Nd, nb, _p(src), _p(out))
+    return out
+
+
+def band_to_grid(Nd, nb, src):
+    out = np.zeros((Nd // 16) * nb * 16, np.float64)
+    load().oracle_band_to_grid(Nd, nb, _p(src), _p(out))
+    return out
+
+
+def grid_to_band(Nd, nb, src):
+    out = np.zeros((nb // 16) * Nd * 16, np.float64)
+    load().oracle_grid_to_band(Nd, nb, _p(src), _p(out))
     return out
 
 

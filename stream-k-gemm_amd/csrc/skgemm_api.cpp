@@ -825,24 +825,65 @@ extern "C" int skgemm_dsyrk_ozaki_host(skgemm_ctx* c, int la, int N, int K,
 
 // ---- layout conversion --------------------------------------------------------
 
-extern "C" int skgemm_pack(skgemm_ctx* c, int src_layout, const double* src,
-                           int64_t ld_src, int dst_layout, double* dst,
-                           int64_t ld_dst, int o_extent, int r_extent) {
+// Lane order for one side of a layout conversion: along the index with unit
+// stride, in groups of the panel width when that index is a packed one.
+static void pack_side(bool o_unit, bool r_unit, int pshift, bool* fast_o,
+                      int* wshift) {
+  *fast_o = o_unit || !r_unit;
+  *wshift = (*fast_o && pshift != kNoPanelShift) ? pshift : 6;
+}
+
+static int convert(skgemm_ctx* c, bool swap, int src_layout, const double* src,
+                   int64_t ld_src, int dst_layout, double* dst, int64_t ld_dst,
+                   int o_extent, int r_extent, const char* name) {
   SKG_TRY(check_ctx(c));
   SKG_TRY(check_dims(o_extent, r_extent, 1));
   if (o_extent == 0 || r_extent == 0) return SKGEMM_OK;
   if (src == nullptr || dst == nullptr)
-    return set_error(SKGEMM_ERR_ARG, "src or dst is NULL");
+    return set_error(SKGEMM_ERR_ARG, "%s: src or dst is NULL", name);
   // Both sides in the B convention: element (r, o), ROW_MAJOR = r*ld + o.
   Operand s{};
   SKG_TRY(make_operand(src_layout, false, src, ld_src, o_extent, r_extent, &s, "src"));
   Output d{};
-  SKG_TRY(make_output(dst_layout, dst, ld_dst, r_extent, o_extent, &d, "dst"));
+  if (swap)  // dst element (r' = o, o' = r)
+    SKG_TRY(make_output(dst_layout, dst, ld_dst, o_extent, r_extent, &d, "dst"));
+  else
+    SKG_TRY(make_output(dst_layout, dst, ld_dst, r_extent, o_extent, &d, "dst"));
   SKG_TRY(check_dev(src, "src"));
   SKG_TRY(check_dev(dst, "dst"));
+  PackMap map{};
+  map.swap = swap ? 1 : 0;
+  bool fo;
+  pack_side(s.os == 1, s.ks == 1, s.pshift, &fo, &map.src_wshift);
+  map.src_fast_o = fo ? 1 : 0;
+  // dst: n is the packed / unit-stride index of an Output; n = o, or r
+  // when swapped.
+  const bool n_unit = d.ns == 1, m_unit = d.ms == 1;
+  bool n_fast;
+  int w;
+  pack_side(n_unit, m_unit, d.pshift, &n_fast, &w);
+  map.dst_fast_o = (n_fast != swap) ? 1 : 0;
+  map.dst_wshift = w;
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
-  SKG_HIP(launch_pack(s, r_extent, d, c->stream), "pack launch");
+  c->record_begin();
+  SKG_HIP(launch_pack(s, r_extent, d, map, c->stream), name);
+  c->record_end();
   return SKGEMM_OK;
+}
+
+extern "C" int skgemm_pack(skgemm_ctx* c, int src_layout, const double* src,
+                           int64_t ld_src, int dst_layout, double* dst,
+                           int64_t ld_dst, int o_extent, int r_extent) {
+  return convert(c, false, src_layout, src, ld_src, dst_layout, dst, ld_dst,
+                 o_extent, r_extent, "pack launch");
+}
+
+extern "C" int skgemm_transpose(skgemm_ctx* c, int src_layout,
+                                const double* src, int64_t ld_src,
+                                int dst_layout, double* dst, int64_t ld_dst,
+                                int o_extent, int r_extent) {
+  return convert(c, true, src_layout, src, ld_src, dst_layout, dst, ld_dst,
+                 o_extent, r_extent, "transpose launch");
 }
 
 // ---- tuning -------------------------------------------------------------------

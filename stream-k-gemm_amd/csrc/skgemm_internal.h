@@ -95,8 +95,15 @@ hipError_t launch_fill_normal(double* x, int64_t n, uint64_t seed,
 hipError_t launch_scale_columns_pow10(double* x, int rows, int cols,
                                       int64_t ld, uint64_t seed, double umax,
                                       hipStream_t s);
+// Lane order of the layout-conversion kernel on each side: lanes run along
+// o (fast_o) or r in groups of 2^wshift (4 <= wshift <= 6).
+struct PackMap {
+  int swap;  // dst element (o, r) is out_addr(dst, o, r), else (r, o)
+  int src_fast_o, src_wshift;
+  int dst_fast_o, dst_wshift;
+};
 hipError_t launch_pack(const Operand& src, int r_extent, const Output& dst,
-                       hipStream_t s);
+                       const PackMap& map, hipStream_t s);
 hipError_t launch_zero_ints(int* p, int n, hipStream_t s);
 hipError_t launch_peak_probe(int which, int blocks, int iters, double* sink,
                              hipStream_t s);

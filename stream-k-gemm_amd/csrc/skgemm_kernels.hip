@@ -772,26 +772,72 @@ hipError_t launch_scale_columns_pow10(double* x, int rows, int cols,
 }
 
 // ---------------------------------------------------------------------------
-// Layout conversion: dst(o, r) = src(o, r) for o < o_extent (src.extent),
-// r < r_extent.  One thread per element, r fastest across a block row so
-// that the side contiguous in r is coalesced.
+// Layout conversion.  Element (o, r), o < src.extent, r < r_extent, moves
+// from op_addr(src, o, r) to out_addr(dst, r, o) (a copy between layouts)
+// or to out_addr(dst, o, r) (`swap`: the two indices exchange roles, e.g.
+// band-packed <-> grid-packed).  A block moves a 64 x 64 tile through LDS:
+// it reads with the lanes running along the index that is contiguous in
+// src and writes with them running along the one contiguous in dst, each
+// in groups of 2^wshift (the panel width of a packed layout, else 64), so
+// a wave touches whole 512-byte runs on both sides.
 // ---------------------------------------------------------------------------
-__global__ void pack_kernel(Operand src, int r_extent, Output dst) {
-  const int o = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (o >= src.extent) return;
-  for (int r = blockIdx.x * 64 + (threadIdx.x & 63); r < r_extent;
-       r += gridDim.x * 64)
-    dst.ptr[out_addr(dst, r, o)] = src.ptr[op_addr(src, o, r)];
+constexpr int kPackTile = 64;
+constexpr int kPackPitch = kPackTile + 1;  // doubles; spreads a column over banks
+
+__device__ __forceinline__ void pack_coords(int e, int fast_o, int wshift,
+                                            int* o, int* r) {
+  const int f = ((e >> (wshift + 6)) << wshift) | (e & ((1 << wshift) - 1));
+  const int sl = (e >> wshift) & (kPackTile - 1);
+  *o = fast_o ? f : sl;
+  *r = fast_o ? sl : f;
+}
+
+__global__ __launch_bounds__(256) void pack_kernel(Operand src, int r_extent,
+                                                   Output dst, PackMap map) {
+  __shared__ double tile[kPackTile * kPackPitch];
+  const int t = threadIdx.x;
+  const int r0 = blockIdx.x * kPackTile;
+  const int o_blocks = (src.extent + kPackTile - 1) / kPackTile;
+  for (int ob = blockIdx.y; ob < o_blocks; ob += gridDim.y) {
+    const int o0 = ob * kPackTile;
+    double v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      int o, r;
+      pack_coords(t + 256 * i, map.src_fast_o, map.src_wshift, &o, &r);
+      v[i] = (o0 + o < src.extent && r0 + r < r_extent)
+                 ? src.ptr[op_addr(src, o0 + o, r0 + r)]
+                 : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      int o, r;
+      pack_coords(t + 256 * i, map.src_fast_o, map.src_wshift, &o, &r);
+      tile[o * kPackPitch + r] = v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      int o, r;
+      pack_coords(t + 256 * i, map.dst_fast_o, map.dst_wshift, &o, &r);
+      if (o0 + o < src.extent && r0 + r < r_extent) {
+        const int64_t a = map.swap ? out_addr(dst, o0 + o, r0 + r)
+                                   : out_addr(dst, r0 + r, o0 + o);
+        dst.ptr[a] = tile[o * kPackPitch + r];
+      }
+    }
+    __syncthreads();
+  }
 }
 
 hipError_t launch_pack(const Operand& src, int r_extent, const Output& dst,
-                       hipStream_t s) {
+                       const PackMap& map, hipStream_t s) {
   if (src.extent <= 0 || r_extent <= 0) return hipSuccess;
-  int bx = (r_extent + 63) / 64;
-  if (bx > 1024) bx = 1024;
-  const int by = (src.extent + 3) / 4;
+  const int bx = (r_extent + kPackTile - 1) / kPackTile;
+  const int ob = (src.extent + kPackTile - 1) / kPackTile;
+  const int by = ob < 65535 ? ob : 65535;
   hipLaunchKernelGGL(pack_kernel, dim3(bx, by), dim3(256), 0, s, src,
-                     r_extent, dst);
+                     r_extent, dst, map);
   return hipGetLastError();
 }
 

@@ -28,7 +28,7 @@ EXPORTS = [
     "skgemm_memcpy_h2d", "skgemm_memcpy_d2h", "skgemm_dgemm",
     "skgemm_dgemm_host", "skgemm_dsyrk", "skgemm_dsyrk_host",
     "skgemm_dgemm_ozaki", "skgemm_dsyrk_ozaki", "skgemm_dgemm_ozaki_host",
-    "skgemm_dsyrk_ozaki_host", "skgemm_pack",
+    "skgemm_dsyrk_ozaki_host", "skgemm_pack", "skgemm_transpose",
     "skgemm_batched", "skgemm_set_tuning", "skgemm_set_kchunks",
     "skgemm_num_tiles",
     "skgemm_tile_info", "skgemm_autotune", "skgemm_load_tuning",
@@ -112,6 +112,8 @@ def lib():
                                             c_int, P]),
         "skgemm_pack": (c_int, [P, c_int, P, c_int64, c_int, P, c_int64, c_int,
                                 c_int]),
+        "skgemm_transpose": (c_int, [P, c_int, P, c_int64, c_int, P, c_int64,
+                                     c_int, c_int]),
         "skgemm_batched": (c_int, [P, P, c_int, c_int, c_int, c_int]),
         "skgemm_set_tuning": (c_int, [P, c_int, c_int]),
         "skgemm_set_kchunks": (c_int, [P, c_int]),
@@ -294,6 +296,12 @@ class Context:
              r_ext):
         check(lib().skgemm_pack(self.h, src_layout, _ptr(src), ld_src,
                                 dst_layout, _ptr(dst), ld_dst, o_ext, r_ext))
+
+    def transpose(self, src_layout, src, ld_src, dst_layout, dst, ld_dst,
+                  o_ext, r_ext):
+        check(lib().skgemm_transpose(self.h, src_layout, _ptr(src), ld_src,
+                                     dst_layout, _ptr(dst), ld_dst, o_ext,
+                                     r_ext))
 
     def batched(self, problems, precision=FP64, slices=0, use_graph=0):
         arr = (Problem * len(problems))(*problems)

@@ -1,6 +1,6 @@
 // Host program written against the C++ facade exactly as a caller of the
 // reference classes would be (gemm<double>, sme_syrk<double>,
-// sme_gemm_trp_3d<double>, linalg_pack_c16_f64): it checks each call against
+// sme_gemm_trp_3d<double>, sme_pack<double>, linalg_pack_*): it checks each call against
 // closed forms or naive loops computed here and exits non-zero on a
 // mismatch.  Built with g++ against libskgemm.so (no HIP headers needed).
 #include <cmath>
@@ -105,6 +105,42 @@ int main() {
     double rt = 0.0;
     for (size_t e = 0; e < col.size(); ++e) rt = std::fmax(rt, std::fabs(back[e] - col[e]));
     expect(rt == 0.0, "linalg_pack/unpack_c16_f64 round trip", rt);
+    // the rotation's band-packed output converted back to grid-packed
+    std::vector<double> grid2((size_t)Nd * nb, -1.0);
+    sme_pack<double>::band_to_grid_tile(Nd, nb, D.data(), grid2.data());
+    double bg = 0.0;
+    for (size_t e = 0; e < grid.size(); ++e) bg = std::fmax(bg, std::fabs(grid2[e] - grid[e]));
+    expect(bg == 0.0, "sme_pack::band_to_grid_tile of the rotation output", bg);
+  }
+  // pack.cpp:190-258: value[g, b] = g + 1000 b through band -> grid -> band,
+  // and the r16 wrappers (same memory as c16).
+  {
+    const int Nd = 80, nb = 48;
+    std::vector<double> band((size_t)Nd * nb), grid((size_t)Nd * nb, 0.0),
+        band2((size_t)Nd * nb, 0.0);
+    for (int g = 0; g < Nd; ++g)
+      for (int b = 0; b < nb; ++b)
+        band[((size_t)(b / 16) * Nd + g) * 16 + b % 16] = g + b * 1000.0;
+    sme_pack<double>::band_to_grid_tile(Nd, nb, band.data(), grid.data());
+    sme_pack<double>::grid_to_band_tile(Nd, nb, grid.data(), band2.data());
+    double bad = 0.0;
+    for (int g = 0; g < Nd; ++g)
+      for (int b = 0; b < nb; ++b)
+        bad = std::fmax(bad, std::fabs(grid[((size_t)(g / 16) * nb + b) * 16 + g % 16] -
+                                       (g + b * 1000.0)));
+    for (size_t e = 0; e < band.size(); ++e) bad = std::fmax(bad, std::fabs(band2[e] - band[e]));
+    expect(bad == 0.0, "sme_pack band <-> grid (pack.cpp pattern)", bad);
+    std::vector<double> rows((size_t)Nd * nb), packed((size_t)Nd * nb), back((size_t)Nd * nb);
+    for (size_t e = 0; e < rows.size(); ++e) rows[e] = std::cos(0.01 * (double)e);
+    linalg_pack_r16_f64(Nd, nb, rows.data(), packed.data());
+    linalg_unpack_r16_f64(Nd, nb, packed.data(), back.data());
+    double rt = 0.0;
+    for (int g = 0; g < Nd; ++g)
+      for (int b = 0; b < nb; ++b)
+        rt = std::fmax(rt, std::fabs(packed[((size_t)(g / 16) * nb + b) * 16 + g % 16] -
+                                     rows[(size_t)b * Nd + g]));
+    for (size_t e = 0; e < rows.size(); ++e) rt = std::fmax(rt, std::fabs(back[e] - rows[e]));
+    expect(rt == 0.0, "linalg_pack/unpack_r16_f64", rt);
   }
   std::printf("%s\n", failures ? "FAILED" : "all facade checks passed");
   return failures ? 1 : 0;

@@ -258,6 +258,48 @@ def test_pack_roundtrip_and_layout(orc, rng):
     assert pb[(1 * K + 7) * 32 + 9] == B[7 * N + 32 + 9]
 
 
+def band_pack(X):
+    """Column-major dense Nd x nb -> band-packed [nb/16][Nd][16] by numpy
+    reshapes (element (g, b) at (b/16, g, b%16))."""
+    Nd, nb = X.shape
+    return X.reshape(Nd, nb // 16, 16).transpose(1, 0, 2).ravel().copy()
+
+
+def grid_pack(X):
+    """Dense Nd x nb -> grid-packed [Nd/16][nb][16] (element (g, b) at
+    (g/16, b, g%16))."""
+    Nd, nb = X.shape
+    return X.reshape(Nd // 16, 16, nb).transpose(0, 2, 1).ravel().copy()
+
+
+def test_band_grid_convert_known_answer(orc):
+    # pack.cpp:190-258: value[g, b] = g + 1000 b, band -> grid -> band.
+    Nd, nb = 64, 48
+    band = np.zeros((nb // 16) * Nd * 16)
+    for g in range(Nd):
+        for b in range(nb):
+            band[(b // 16) * Nd * 16 + g * 16 + b % 16] = g + b * 1000
+    grid = orc.band_to_grid(Nd, nb, band)
+    for g in range(Nd):
+        for b in range(nb):
+            assert grid[(g // 16) * nb * 16 + b * 16 + g % 16] == g + b * 1000
+    assert np.array_equal(orc.grid_to_band(Nd, nb, grid), band)
+
+
+def test_band_grid_convert_matches_reshapes(orc, rng):
+    Nd, nb = 96, 80
+    X = rng.uniform(-1, 1, (Nd, nb))
+    band, grid = band_pack(X), grid_pack(X)
+    assert np.array_equal(orc.band_to_grid(Nd, nb, band), grid)
+    assert np.array_equal(orc.grid_to_band(Nd, nb, grid), band)
+    # grid-packed is pack_c16 of the column-major matrix
+    assert np.array_equal(orc.pack_c16(Nd, nb, X.ravel(order="F").copy()), grid)
+    # a swapped pair of indices is caught
+    Y = X.copy()
+    Y[[3, 4]] = Y[[4, 3]]
+    assert not np.array_equal(orc.band_to_grid(Nd, nb, band_pack(Y)), grid)
+
+
 # ---- planted mistakes are caught -----------------------------------------------
 
 def test_oracle_mutations_are_detected(orc, rng):
