@@ -206,6 +206,36 @@ def bench_split(skg, ctx, reps):
     return out
 
 
+def bench_projection(skg, ctx, reps):
+    """Projection GEMM hp = Psi^T (H Psi), Ns = 616, Nd = 175616 (the
+    reference's sweep shape, sub_gemm.sh:26-30), Psi band-packed 16 (PA)
+    and H Psi band-packed 32 (PB), D row-major: the gemm_crsp packed entry
+    point's layouts.  Default plan, then the autotuned one."""
+    Ns, Nd = 616, 175616
+    tA, tB = (Ns + 15) // 16, (Ns + 31) // 32
+    psi = skg.DeviceArray(ctx, tA * Nd * 16)
+    hpsi = skg.DeviceArray(ctx, tB * Nd * 32)
+    hp = skg.DeviceArray(ctx, Ns * Ns, zero=True)
+    ctx.fill_normal(psi.ptr, tA * Nd * 16, 11, 1.0 / np.sqrt(Nd))
+    ctx.fill_normal(hpsi.ptr, tB * Nd * 32, 12, 1.0 / np.sqrt(Nd))
+
+    def run():
+        ctx.dgemm(skg.PACK16, skg.PACK32, skg.ROW_MAJOR, Ns, Ns, Nd, 1.0,
+                  psi.ptr, Nd, hpsi.ptr, Nd, 0.0, hp.ptr, Ns, 0.0)
+    f = 2.0 * Ns * Ns * Nd
+    t0 = median_ms(ctx, run, reps)
+    plan = ctx.autotune(0, Ns, Ns, Nd, la=skg.PACK16, lb=skg.PACK32)
+    t1 = median_ms(ctx, run, reps)
+    for x in (psi, hpsi, hp):
+        x.free()
+    return {"workload": "projection GEMM hp = Psi^T (H Psi), Ns 616, Nd "
+                        "175616, FP64, PA / PB packed",
+            "default_plan": {"ms": t0, "tflops": f / (t0 * 1e-3) / 1e12},
+            "tuned": {"ms": t1, "tflops": f / (t1 * 1e-3) / 1e12,
+                      "frac_fp64_peak": f / (t1 * 1e-3) / 1e12 /
+                      PEAK_FP64_TFLOPS, "plan": plan}}
+
+
 def bench_layouts(skg, ctx, reps):
     """Layout conversions of the SYRK / rotation operand Psi, Nd x nb =
     65536 x 1024 FP64 (512 MiB): column-major -> grid-packed (pack_c16),
@@ -389,7 +419,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the c1, split-form, batched and serving "
                          "sections")
-    ap.add_argument("--extra", default="c1,split,layouts,batched,serving",
+    ap.add_argument("--extra", default="c1,split,projection,layouts,batched,serving",
                     help="comma list of extra sections to run")
     ap.add_argument("--tuning-cache", default=None,
                     help="autotuner cache file: read if it exists (no "
@@ -494,6 +524,9 @@ def main():
     if "split" in sections:
         log("split form (SYRK 65536x1024) ...")
         extra["split_syrk"] = bench_split(skg, ctx, 10)
+    if "projection" in sections:
+        log("projection GEMM (616x616x175616) ...")
+        extra["projection"] = bench_projection(skg, ctx, 10)
     if "layouts" in sections:
         log("layout conversions (65536x1024) ...")
         extra["layouts"] = bench_layouts(skg, ctx, 10)

@@ -283,3 +283,65 @@ def test_autotune_save_load_and_use(skg, ctx, orc, rng, tmp_path):
         c2.dgemm_host(0, 0, 0, M, N, K, 1.0, np.ascontiguousarray(A.ravel()),
                       K, np.ascontiguousarray(B.ravel()), N, 0.0, D, N, 0.0)
         assert rel_fro(D, A @ B) < TOL
+
+
+# ---- projection GEMM hp = Psi^T (H Psi) (sub_gemm.sh:26-30) -------------------
+
+def test_projection_packed_layouts(skg, ctx, orc, rng):
+    """hp = Psi^T HPsi with Psi band-packed 16 (PA [Ns/16][Nd][16]) and HPsi
+    band-packed 32 (PB [Ns/32][Nd][32], the H*Psi output layout of
+    include/nloc_packed_opt2.hpp:1-8), Ns not a multiple of 16 or 32."""
+    Ns, Nd = 72, 12288
+    P = rng.standard_normal((Nd, Ns)) / np.sqrt(Nd)
+    H = rng.standard_normal((Nd, Ns)) / np.sqrt(Nd)
+    pa, lda = encode_a(np.ascontiguousarray(P.T), 2, 0)
+    pb, ldb = encode_b(H, 3, 0)
+    hp = skg.DeviceArray(ctx, Ns * Ns, zero=True)
+    dA, dB = dev(skg, ctx, pa), dev(skg, ctx, pb)
+    ctx.dgemm(skg.PACK16, skg.PACK32, skg.ROW_MAJOR, Ns, Ns, Nd, 1.0, dA.ptr,
+              lda, dB.ptr, ldb, 0.0, hp.ptr, Ns, 0.0)
+    ctx.synchronize()
+    ref = np.zeros(Ns * Ns)
+    # A = Psi^T: element (m, k) = P[k, m] at m*Nd + k (ROW_MAJOR, ld = Nd)
+    orc.dgemm(0, 0, 0, Ns, Ns, Nd, 1.0, P.ravel(order="F").copy(), Nd,
+              H.ravel().copy(), Ns, 0.0, ref, Ns, 0.0, dd=True)
+    assert rel_fro(hp.download(), ref) < TOL
+
+
+def test_projection_full_size(skg, ctx, orc):
+    """Ns = 616, Nd = 175616 (the reference's projection sweep shape): a
+    16 x 16 sample of hp against the double-double oracle, and with
+    HPsi = Psi (repacked on the device) hp is the SYRK of Psi."""
+    Ns, Nd = 616, 175616
+    tA, tB = (Ns + 15) // 16, (Ns + 31) // 32
+    psi = skg.DeviceArray(ctx, tA * Nd * 16)
+    hpsi = skg.DeviceArray(ctx, tB * Nd * 32)
+    ctx.fill_normal(psi.ptr, tA * Nd * 16, 11, 1.0 / np.sqrt(Nd))
+    ctx.fill_normal(hpsi.ptr, tB * Nd * 32, 12, 1.0 / np.sqrt(Nd))
+    hp = skg.DeviceArray(ctx, Ns * Ns, zero=True)
+    ctx.dgemm(skg.PACK16, skg.PACK32, skg.ROW_MAJOR, Ns, Ns, Nd, 1.0, psi.ptr,
+              Nd, hpsi.ptr, Nd, 0.0, hp.ptr, Ns, 0.0)
+    ctx.synchronize()
+    got = hp.download().reshape(Ns, Ns)
+    pa = psi.download().reshape(tA, Nd, 16)
+    pb = hpsi.download().reshape(tB, Nd, 32)
+    g = np.random.default_rng(5)
+    rows = np.sort(g.choice(Ns, 16, replace=False))
+    cols = np.sort(g.choice(Ns, 16, replace=False))
+    A_sub = np.stack([pa[i // 16, :, i % 16] for i in rows])      # 16 x Nd
+    B_sub = np.stack([pb[j // 32, :, j % 32] for j in cols], 1)   # Nd x 16
+    ref = np.zeros(256)
+    orc.dgemm(0, 0, 0, 16, 16, Nd, 1.0, A_sub.ravel().copy(), Nd,
+              B_sub.ravel().copy(), 16, 0.0, ref, 16, 0.0, dd=True)
+    assert rel_fro(got[np.ix_(rows, cols)], ref.reshape(16, 16)) < TOL
+    # HPsi = Psi: repack P16 -> P32 on the device, then hp == SYRK(Psi)
+    ctx.pack(skg.PACK16, psi.ptr, Nd, skg.PACK32, hpsi.ptr, Nd, Ns, Nd)
+    ctx.dgemm(skg.PACK16, skg.PACK32, skg.ROW_MAJOR, Ns, Ns, Nd, 1.0, psi.ptr,
+              Nd, hpsi.ptr, Nd, 0.0, hp.ptr, Ns, 0.0)
+    ms = skg.DeviceArray(ctx, Ns * Ns, zero=True)
+    ctx.dsyrk(skg.PACK16, Ns, Nd, 1.0, psi.ptr, Nd, 0.0, ms.ptr, Ns)
+    ctx.synchronize()
+    full = hp.download().reshape(Ns, Ns)
+    assert rel_fro(full, full.T) < TOL
+    il = np.tril_indices(Ns)
+    assert rel_fro(full[il], ms.download().reshape(Ns, Ns)[il]) < TOL
