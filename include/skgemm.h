@@ -271,7 +271,11 @@ int skgemm_tile_info(int tile, int* bm, int* bn, int* bk, int* waves);
  * the context's table, and appends it to `cache_path` when not NULL (one
  * text line per shape: kind la lb M N K tile grid kchunks us).  The
  * reference's analogue is its -DMC/-DNC/-DKC rebuild sweep
- * (sub_gemm.sh:31-52, sub_syrk.sh:30-48).  kind 0 GEMM / 1 SYRK. */
+ * (sub_gemm.sh:31-52, sub_syrk.sh:30-48).  kind 0 GEMM / 1 SYRK.  A GEMM
+ * is tuned with a column-major D when layout_a is column-major (else a
+ * row-major D); a GEMM with a column-major D is launched as its transpose
+ * D^T = B^T A^T (row-major D^T, the same memory), and tuned plans, the cache
+ * file and skgemm_query_plan describe that launched problem. */
 int skgemm_autotune(skgemm_ctx* ctx, int kind, int layout_a, int layout_b,
                     int M, int N, int K, const char* cache_path,
                     int* best_tile, int* best_grid, int* best_kchunks,

@@ -337,6 +337,12 @@ int skgemm_ctx::check_status() {
   int h = 0;
   SKG_HIP(hipMemcpy(&h, status, sizeof(int), hipMemcpyDeviceToHost),
           "status readback");
+  if (h & 2) {
+    hipMemset(status, 0, sizeof(int));
+    return set_error(SKGEMM_ERR_NOTSUP,
+                     "internal: an LDS-DMA tile configuration was given an "
+                     "operand without 16-byte aligned pairs");
+  }
   if (h != 0) {
     hipMemset(status, 0, sizeof(int));
     if (flags) hipMemset(flags, 0, sizeof(int) * flags_n);
@@ -494,12 +500,14 @@ static int default_kchunks(skgemm_ctx* c, int kind, int cfg, int M, int N,
 }
 
 // Default tile when the autotuner has no entry for this shape.
+// 128 x 128 tiles use the 4-stage LDS-DMA variant (cfg 8; the host takes
+// its register-staged sibling, cfg 4, for unaligned operands).
 static int default_cfg(skgemm_ctx* c, int kind, int M, int N, int K) {
   (void)K;
-  if (kind == 1) return (N >= 768) ? 4 : 1;
+  if (kind == 1) return (N >= 768) ? 8 : 1;
   if (M >= 768 && N >= 768) {
     const int64_t t128 = tiles_for(0, 4, M, N);
-    return (t128 >= 4 * c->num_cus) ? 4 : 6;
+    return (t128 >= 4 * c->num_cus) ? 8 : 6;
   }
   if (M >= 128 && N >= 128) return 1;
   if (M >= 128) return 2;
@@ -531,6 +539,17 @@ Plan plan(skgemm_ctx* c, int kind, int la, int lb, int M, int N, int K) {
     pl.cfg = default_cfg(c, kind, M, N, K);
     pl.grid = c->capacity(pl.cfg);
     pl.kchunks = default_kchunks(c, kind, pl.cfg, M, N, K);
+    // Inner-product GEMM (few output tiles, long K, e.g. the projection
+    // Psi^T (H Psi)): 128 x 128 tiles, as many K chunks as fill the CUs
+    // once, one workgroup per (tile, chunk).
+    const int64_t t4 = tiles_for(0, 4, M, N);
+    if (kind == 0 && M >= 256 && N >= 256 && 2 * t4 <= c->num_cus &&
+        iters_per_tile(4, K) >= 64 * c->num_xcds) {
+      pl.cfg = 8;
+      pl.kchunks = (int)std::min<int64_t>(16, std::max<int64_t>(
+                                                  c->num_xcds, c->num_cus / t4));
+      pl.grid = (int)std::min<int64_t>(t4 * pl.kchunks, c->capacity(8));
+    }
   }
   if (c->forced_grid > 0) pl.grid = c->forced_grid;
   if (c->forced_kchunks > 0) pl.kchunks = c->forced_kchunks;
@@ -567,8 +586,15 @@ int fill_problem(skgemm_ctx* c, Problem* p, int kind, const Plan& pl, int M,
 }
 
 // Launch one problem (already described) with the given plan.
-int run_single(skgemm_ctx* c, Problem& p, const Plan& pl, const int* pred,
-               bool timed) {
+int run_single(skgemm_ctx* c, Problem& p, const Plan& plan_in,
+               const int* pred, bool timed) {
+  Plan pl = plan_in;
+  if (tile_cfg(pl.cfg).ns >= 3 && !(p.a.vec2 && p.b.vec2)) {
+    // LDS-DMA tiles move 16-byte pairs: unaligned operands take the
+    // register-staged tile of the same shape (same tiles and units).
+    pl.cfg = register_sibling(pl.cfg);
+    pl.grid = std::max(1, std::min(pl.grid, c->capacity(pl.cfg)));
+  }
   const TileCfg& t = tile_cfg(pl.cfg);
   p.pred = pred;
   p.tile_begin = 0;
@@ -603,6 +629,29 @@ int skg::check_dims(int M, int N, int K) {
 }
 
 namespace skg {
+
+// Operand layout of B in the A convention (element (n, k)) and of A in the
+// B convention: plain layouts exchange names, panel layouts keep them.
+static int flip_layout(int l) {
+  if (l == SKGEMM_ROW_MAJOR) return SKGEMM_COL_MAJOR;
+  if (l == SKGEMM_COL_MAJOR) return SKGEMM_ROW_MAJOR;
+  return l;
+}
+
+bool transpose_launch(int& la, int& lb, int& ld_, int& M, int& N,
+                      const double*& A, int64_t& lda, const double*& B,
+                      int64_t& ldb) {
+  if (ld_ != SKGEMM_COL_MAJOR || !valid_layout(la) || !valid_layout(lb))
+    return false;
+  const int la2 = flip_layout(lb), lb2 = flip_layout(la);
+  la = la2;
+  lb = lb2;
+  ld_ = SKGEMM_ROW_MAJOR;
+  std::swap(M, N);
+  std::swap(A, B);
+  std::swap(lda, ldb);
+  return true;
+}
 
 int prepare_gemm(int la, int lb, int ld_, int M, int N, int K, const double* A,
                  int64_t lda, const double* B, int64_t ldb, double* D,
@@ -650,6 +699,7 @@ extern "C" int skgemm_dgemm(skgemm_ctx* c, int la, int lb, int ld_, int M,
                             int64_t lda, const double* B, int64_t ldb,
                             double beta, double* D, int64_t ldd, double gamma) {
   SKG_TRY(check_ctx(c));
+  transpose_launch(la, lb, ld_, M, N, A, lda, B, ldb);
   Problem p{};
   SKG_TRY(prepare_gemm(la, lb, ld_, M, N, K, A, lda, B, ldb, D, ldd, &p));
   if (M == 0 || N == 0) return SKGEMM_OK;
@@ -916,6 +966,14 @@ extern "C" int skgemm_query_plan(skgemm_ctx* c, int kind, int la, int lb,
   SKG_TRY(check_ctx(c));
   SKG_TRY(check_dims(M, N, K));
   if (kind != 0 && kind != 1) return set_error(SKGEMM_ERR_ARG, "kind must be 0 or 1");
+  if (kind == 0 && la == SKGEMM_COL_MAJOR) {
+    // The autotuner's convention: a column-major A comes with a
+    // column-major D, which is launched transposed.
+    int ld_ = SKGEMM_COL_MAJOR;
+    const double *a = nullptr, *b = nullptr;
+    int64_t x = 0, y = 0;
+    transpose_launch(la, lb, ld_, M, N, a, x, b, y);
+  }
   const Plan pl = plan(c, kind, la, lb, M, N, K);
   const TileCfg& t = tile_cfg(pl.cfg);
   const int64_t nt = tiles_for(kind, pl.cfg, M, N);

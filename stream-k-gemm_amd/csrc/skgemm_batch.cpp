@@ -99,14 +99,24 @@ int batch_state(skgemm_ctx* c, int count) {
   return SKGEMM_OK;
 }
 
-int describe(const skgemm_problem& q, Problem* p) {
-  if (q.kind == 0)
-    return prepare_gemm(q.layout_a, q.layout_b, q.layout_d, q.M, q.N, q.K,
-                        q.A, q.lda, q.B, q.ldb, q.D, q.ldd, p);
+// Problem descriptor of q and the (M, N) it is launched with (with
+// `transpose`, a GEMM with a column-major D runs transposed).
+int describe(const skgemm_problem& q, Problem* p, int* Ml, int* Nl,
+             bool transpose) {
+  if (q.kind == 0) {
+    int la = q.layout_a, lb = q.layout_b, ld_ = q.layout_d, M = q.M, N = q.N;
+    const double *A = q.A, *B = q.B;
+    int64_t lda = q.lda, ldb = q.ldb;
+    if (transpose) transpose_launch(la, lb, ld_, M, N, A, lda, B, ldb);
+    *Ml = M;
+    *Nl = N;
+    return prepare_gemm(la, lb, ld_, M, N, q.K, A, lda, B, ldb, q.D, q.ldd, p);
+  }
   if (q.kind == 1) {
     if (q.M != q.N)
       return set_error(SKGEMM_ERR_ARG, "SYRK problem needs M == N (got %d, %d)",
                        q.M, q.N);
+    *Ml = *Nl = q.N;
     return prepare_syrk(q.layout_a, q.N, q.K, q.A, q.lda, q.D, q.ldd, p);
   }
   return set_error(SKGEMM_ERR_ARG, "problem kind must be 0 (GEMM) or 1 (SYRK)");
@@ -114,16 +124,29 @@ int describe(const skgemm_problem& q, Problem* p) {
 
 // One tile configuration for the whole batch (square tiles: SYRK needs
 // them).  The 128x128 / 8-wave tile unless every problem is small.
+// The 128 x 128 tiles run their LDS-DMA variant (cfg 8) when every operand
+// of the batch has 16-byte aligned pairs (checked by run_plain).
 int batch_cfg(skgemm_ctx* c, const skgemm_problem* q, int n) {
   if (c->forced_cfg >= 0 && tile_cfg(c->forced_cfg).bm == tile_cfg(c->forced_cfg).bn)
     return c->forced_cfg;
   int big = 0;
   for (int i = 0; i < n; ++i) big = std::max(big, std::min(q[i].M, q[i].N));
-  return big >= 256 ? 4 : 1;
+  return big >= 256 ? 8 : 1;
 }
 
 int run_plain(skgemm_ctx* c, const skgemm_problem* q, int n, int use_graph) {
-  const int cfg = batch_cfg(c, q, n);
+  int cfg = batch_cfg(c, q, n);
+  if (tile_cfg(cfg).ns >= 3) {
+    for (int i = 0; i < n; ++i) {
+      Problem p{};
+      int M = 0, N = 0;
+      SKG_TRY(describe(q[i], &p, &M, &N, true));
+      if (M > 0 && N > 0 && !(p.a.vec2 && p.b.vec2)) {
+        cfg = register_sibling(cfg);  // same tiles, register-staged loads
+        break;
+      }
+    }
+  }
   const TileCfg& t = tile_cfg(cfg);
   int grid = c->capacity(cfg);
   if (c->forced_grid > 0) grid = std::min(grid, c->forced_grid);
@@ -138,11 +161,11 @@ int run_plain(skgemm_ctx* c, const skgemm_problem* q, int n, int use_graph) {
   int used = 0;
   for (int i = 0; i < n; ++i) {
     Problem p{};
-    SKG_TRY(describe(q[i], &p));
-    const int M = q[i].kind == 1 ? q[i].N : q[i].M;
-    if (M == 0 || q[i].N == 0) continue;
+    int M = 0, N = 0;
+    SKG_TRY(describe(q[i], &p, &M, &N, true));
+    if (M == 0 || N == 0) continue;
     const Plan pl{cfg, grid, 1};
-    SKG_TRY(fill_problem(c, &p, q[i].kind, pl, M, q[i].N, q[i].K));
+    SKG_TRY(fill_problem(c, &p, q[i].kind, pl, M, N, q[i].K));
     p.alpha = q[i].alpha;
     p.beta = q[i].beta;
     p.gamma = q[i].kind == 1 ? 0.0 : q[i].gamma;
@@ -213,7 +236,8 @@ int run_split(skgemm_ctx* c, const skgemm_problem* q, int n, int slices,
               int use_graph) {
   std::vector<Problem> ps(n);
   for (int i = 0; i < n; ++i) {
-    SKG_TRY(describe(q[i], &ps[i]));
+    int Ml = 0, Nl = 0;
+    SKG_TRY(describe(q[i], &ps[i], &Ml, &Nl, false));
     Problem& p = ps[i];
     p.M = q[i].kind == 1 ? q[i].N : q[i].M;
     p.N = q[i].N;

@@ -119,6 +119,13 @@ int ozaki_reserve(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
 
 int64_t tiles_for(int kind, int cfg, int M, int N);
 Plan plan(skgemm_ctx* c, int kind, int la, int lb, int M, int N, int K);
+// A GEMM whose D is column-major is launched as its transpose, D^T = B^T A^T
+// with D^T row-major (the same memory): the epilogue then stores contiguous
+// 128-byte rows instead of one 8-byte element per column.  Rewrites the
+// arguments in place and returns true when it did.
+bool transpose_launch(int& la, int& lb, int& ld_, int& M, int& N,
+                      const double*& A, int64_t& lda, const double*& B,
+                      int64_t& ldb);
 int check_ctx(skgemm_ctx* c);
 int check_dims(int M, int N, int K);
 int fill_problem(skgemm_ctx* c, Problem* p, int kind, const Plan& pl, int M,

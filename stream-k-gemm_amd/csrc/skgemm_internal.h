@@ -73,9 +73,13 @@ struct Launch {
 // Tile configurations (index = kernel table entry).
 struct TileCfg {
   int bm, bn, bk, wm, wn;
+  int ns;  // 0: register-staged LDS double buffer; >= 3: LDS-DMA stages
 };
 int num_tile_cfgs();
 const TileCfg& tile_cfg(int i);
+// The register-staged configuration with the same tile shape (cfg itself
+// when it has no DMA stages).
+int register_sibling(int cfg);
 
 // Launch the FP64 Stream-K kernel with tile configuration `cfg`.
 hipError_t launch_dgemm_sk(int cfg, const Launch& L, hipStream_t s);

@@ -41,11 +41,24 @@ static const TileCfg kCfgs[] = {
     {128, 128, 16, 4, 2},  // 4: 8 waves, 32x64 per wave
     {64, 64, 32, 2, 2},    // 5: as 1 with BK = 32 (half the barriers)
     {128, 64, 32, 4, 2},   // 6: 8 waves, 32x32 per wave, BK = 32
+    {128, 128, 16, 4, 2, 3},  // 7: as 4, operands by LDS DMA, 3 stages
+    {128, 128, 16, 4, 2, 4},  // 8: as 4, 4 DMA stages
+    {128, 64, 16, 2, 2, 4},   // 9: as 2, 4 DMA stages
+    {64, 64, 16, 2, 2, 4},    // 10: as 1, 4 DMA stages
 };
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
 int num_tile_cfgs() { return kNumCfgs; }
 const TileCfg& tile_cfg(int i) { return kCfgs[i]; }
+int register_sibling(int cfg) {
+  const TileCfg& t = kCfgs[cfg];
+  if (t.ns == 0) return cfg;
+  for (int i = 0; i < kNumCfgs; ++i)
+    if (kCfgs[i].ns == 0 && kCfgs[i].bm == t.bm && kCfgs[i].bn == t.bn &&
+        kCfgs[i].bk == t.bk && kCfgs[i].wm == t.wm && kCfgs[i].wn == t.wn)
+      return i;
+  return cfg;
+}
 
 constexpr int kPad = 2;  // LDS row padding in doubles (keeps 16 B alignment)
 
@@ -330,6 +343,223 @@ __device__ __forceinline__ void mac_loop(
   __syncthreads();
 }
 
+// ---------------------------------------------------------------------------
+// LDS-DMA pipeline (tile configurations with ns >= 3).  Operand tiles go
+// from memory straight into LDS with 16-byte buffer loads to LDS (lane l of
+// a wave instruction writes LDS base + 16 l), ns stages deep, so no
+// registers or LDS-store instructions are spent on staging and ns - 1 steps
+// are in flight.  An operand contiguous along o keeps the layout of the
+// register path ([k][BO + 16] when a row is one instruction, else [k][BO]);
+// one contiguous along k is kept as [o][16] with 16-byte chunk c of row o
+// at slot c ^ (o & 7) ^ ((o >> 3) & 1) (the swizzle is applied on the
+// global side: a lane fetches the chunk that belongs at its slot), which
+// makes the fragment reads below conflict-free for both 8- and 16-lane
+// bank cycles.  Within a BK = 16 step, lane quarter lq takes k = 4 lq + s in
+// MFMA sub-step s (the same for A and B, so every k is summed once); a
+// k-contiguous row then yields all four values with two 16-byte reads.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void dma16(__amdgpu_buffer_rsrc_t r, double* lds_dst,
+                                      int voffset) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(
+      r, (__attribute__((address_space(3))) void*)lds_dst, 16, voffset, 0, 0,
+      0);
+}
+
+template <int BO, int BK, int KC>
+struct DmaTile {
+  static_assert(BK == 16, "the DMA pipeline stages BK = 16 steps");
+  static constexpr int kCpr = KC ? BK / 2 : BO / 2;  // 16-byte chunks per row
+  static constexpr int kRpd = 64 / kCpr;             // rows per instruction
+  static constexpr int kRows = KC ? BO : BK;
+  static constexpr int kPitch = KC ? BK : (kRpd == 1 ? BO + 16 : BO);
+  static constexpr int kNumDma = kRows / kRpd;
+  static constexpr int kDoubles = kRows * kPitch;
+  static_assert(kRpd >= 1 && kRows % kRpd == 0, "tile vs instruction");
+  __device__ __forceinline__ static int slot(int r, int c) {
+    return KC ? (c ^ (r & 7) ^ ((r >> 3) & 1)) : c;
+  }
+  // (o, k) of the first element of the chunk lane `lane` of instruction q
+  // fetches (slot() is an involution).
+  __device__ __forceinline__ static void coords(int q, int lane, int& o,
+                                                int& k) {
+    const int r = q * kRpd + lane / kCpr;
+    const int sl = lane % kCpr;
+    if (KC) {
+      o = r;
+      k = 2 * slot(r, sl);
+    } else {
+      k = r;
+      o = 2 * sl;
+    }
+  }
+  // Fragment of rows o (this lane's), k = 4 lq + s, s = 0..3.
+  __device__ __forceinline__ static void frag(const double* l, int o, int lq,
+                                              double (&f)[4]) {
+    if (KC) {
+      const double2 x = *reinterpret_cast<const double2*>(
+          l + o * kPitch + 2 * slot(o, 2 * lq));
+      const double2 y = *reinterpret_cast<const double2*>(
+          l + o * kPitch + 2 * slot(o, 2 * lq + 1));
+      f[0] = x.x;
+      f[1] = x.y;
+      f[2] = y.x;
+      f[3] = y.y;
+    } else {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) f[s] = l[(4 * lq + s) * kPitch + o];
+    }
+  }
+};
+
+template <int BO, int BK>
+constexpr int dma_region() {
+  return DmaTile<BO, BK, 0>::kDoubles > DmaTile<BO, BK, 1>::kDoubles
+             ? DmaTile<BO, BK, 0>::kDoubles
+             : DmaTile<BO, BK, 1>::kDoubles;
+}
+
+template <int BM, int BN, int BK, int WM, int WN>
+struct DmaFrag {
+  double a[BM / WM / 16][4];
+  double b[BN / WN / 16][4];
+};
+
+template <int BM, int BN, int BK, int WM, int WN, bool MASKED, int S0, int S1>
+__device__ __forceinline__ void dma_mma(
+    v4d (&acc)[BM / WM / 16][BN / WN / 16],
+    const DmaFrag<BM, BN, BK, WM, WN>& f, uint32_t mask) {
+  constexpr int MB = BM / WM / 16, NB = BN / WN / 16;
+#pragma unroll
+  for (int s = S0; s < S1; ++s)
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+        if (!MASKED || (mask & (1u << (i * NB + j))))
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(f.a[i][s], f.b[j][s],
+                                                           acc[i][j], 0, 0, 0);
+}
+
+// The MAC loop of the DMA configurations.  Step kk's fragments are read into
+// registers before its MFMAs; the barrier that publishes step kk + 1 sits
+// before the last MFMA sub-step of kk, and kk + 1's fragments are read
+// while that sub-step runs, so the matrix core is not left waiting on LDS.
+// Steps past the unit's end are clamped to its last step (in bounds, never
+// read), so exactly (NS - 2) steps of copies are younger than the one
+// waited for at each barrier; the last step issues no copy.
+template <int BM, int BN, int BK, int WM, int WN, int AKC, int BKC,
+          bool MASKED, int NS>
+__device__ __forceinline__ void mac_loop_dma(
+    v4d (&acc)[BM / WM / 16][BN / WN / 16], const Operand& A,
+    const Operand& B, int K, int m0, int n0, int kbase, int kb, int ke,
+    double* lds, int tid, uint32_t mask) {
+  using S = TileShape<BM, BN, BK, WM, WN>;
+  using TA = DmaTile<BM, BK, AKC>;
+  using TB = DmaTile<BN, BK, BKC>;
+  using F = DmaFrag<BM, BN, BK, WM, WN>;
+  constexpr int NW = WM * WN;
+  constexpr int NDA = TA::kNumDma, NDB = TB::kNumDma;
+  static_assert((NDA + NDB) % NW == 0, "DMA instructions vs waves");
+  constexpr int DPW = (NDA + NDB) / NW;
+  constexpr int kRegA = dma_region<BM, BK>();
+  constexpr int kStage = kRegA + dma_region<BN, BK>();
+  constexpr int kAhead = (NS - 2) * DPW;
+  static_assert(kAhead < 64, "vmcnt range");
+  // vmcnt: this step's copies have landed; lgkmcnt 0: this wave's fragment
+  // reads are complete, so after the barrier no wave still reads a stage
+  // that the next issue overwrites.
+  constexpr int kWait = (kAhead & 15) | ((kAhead >> 4) << 14) | (7 << 4);
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave % WM, wn = wave / WM;
+  const int lq = lane >> 4, lr = lane & 15;
+
+  // Per-instruction byte offset of this lane's chunk from the tile origin,
+  // and its coordinates for the bounds test.
+  const int olimA = A.extent - m0, olimB = B.extent - n0;
+  int rel[DPW], ko[DPW];
+  bool oin[DPW];
+#pragma unroll
+  for (int d = 0; d < DPW; ++d) {
+    const int q = wave * DPW + d;
+    int o, k;
+    if (q < NDA) {
+      TA::coords(q, lane, o, k);
+      const int pm = (1 << A.pshift) - 1;
+      rel[d] = (int)(((int64_t)(o >> A.pshift) * A.pstride +
+                      (int64_t)(o & pm) * A.os + (int64_t)k * A.ks) * 8);
+      oin[d] = o < olimA;
+    } else {
+      TB::coords(q - NDA, lane, o, k);
+      const int pm = (1 << B.pshift) - 1;
+      rel[d] = (int)(((int64_t)(o >> B.pshift) * B.pstride +
+                      (int64_t)(o & pm) * B.os + (int64_t)k * B.ks) * 8);
+      oin[d] = o < olimB;
+    }
+    ko[d] = k;
+  }
+  const double* baseA = A.ptr + op_addr(A, m0, 0);
+  const double* baseB = B.ptr + op_addr(B, n0, 0);
+  const int64_t stepA = (int64_t)BK * A.ks, stepB = (int64_t)BK * B.ks;
+  int kk = kbase + kb;
+  const int kend = kbase + ke;
+
+  auto issue = [&](int stage, int step) {
+    step = step < kend ? step : kend - 1;
+    const __amdgpu_buffer_rsrc_t ra = make_rsrc(baseA + (int64_t)step * stepA);
+    const __amdgpu_buffer_rsrc_t rb = make_rsrc(baseB + (int64_t)step * stepB);
+    const int klim = K - step * BK;
+    double* base = lds + stage * kStage;
+#pragma unroll
+    for (int d = 0; d < DPW; ++d) {
+      const int q = wave * DPW + d;  // wave-uniform
+      const int off = (oin[d] && ko[d] < klim) ? rel[d] : kOOB;
+      if (q < NDA)
+        dma16(ra, base + q * TA::kRpd * TA::kPitch, off);
+      else
+        dma16(rb, base + kRegA + (q - NDA) * TB::kRpd * TB::kPitch, off);
+    }
+  };
+  auto read = [&](F& f, int stage) {
+    const double* la = lds + stage * kStage;
+    const double* lb = la + kRegA;
+#pragma unroll
+    for (int i = 0; i < S::MB; ++i)
+      TA::frag(la, wm * S::TM + i * 16 + lr, lq, f.a[i]);
+#pragma unroll
+    for (int j = 0; j < S::NB; ++j)
+      TB::frag(lb, wn * S::TN + j * 16 + lr, lq, f.b[j]);
+  };
+  auto publish = [&]() {
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(kWait);
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+  };
+
+#pragma unroll
+  for (int s0 = 0; s0 < NS - 1; ++s0) issue(s0, kk + s0);
+  publish();
+  F f0, f1;
+  read(f0, 0);
+  int st = 0;  // stage of step kk
+#pragma unroll 1
+  for (; kk + 1 < kend; ++kk) {
+    issue(st == 0 ? NS - 1 : st - 1, kk + NS - 1);
+    dma_mma<BM, BN, BK, WM, WN, MASKED, 0, 3>(acc, f0, mask);
+    publish();
+    st = st + 1 == NS ? 0 : st + 1;
+    read(f1, st);
+    __builtin_amdgcn_sched_barrier(0);  // reads issue before the MFMAs
+    dma_mma<BM, BN, BK, WM, WN, MASKED, 3, 4>(acc, f0, mask);
+    f0 = f1;
+  }
+  dma_mma<BM, BN, BK, WM, WN, MASKED, 0, 4>(acc, f0, mask);
+  __builtin_amdgcn_s_waitcnt(0);  // clamped tail copies
+  __syncthreads();
+}
+
 // Uniform read of a descriptor held in global memory: through the constant
 // address space the loads become scalar (s_load) and land in SGPRs.
 template <typename T>
@@ -344,7 +574,7 @@ __device__ __forceinline__ T load_uniform(const T* p) {
   return out;
 }
 
-template <int BM, int BN, int BK, int WM, int WN, bool BATCHED>
+template <int BM, int BN, int BK, int WM, int WN, int NS, bool BATCHED>
 __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
   using S = TileShape<BM, BN, BK, WM, WN>;
   constexpr int NT = S::NT, TM = S::TM, TN = S::TN, MB = S::MB, NB = S::NB;
@@ -459,11 +689,21 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
     const bool vec = A.vec2 && B.vec2;
     const int variant = (A.kcontig ? 4 : 0) | (B.kcontig ? 2 : 0) |
                         (vec ? 1 : 0) | ((mask != kFull && vec) ? 8 : 0);
+    // DMA configurations take 16-byte aligned operands only (the host
+    // launches the register-staged sibling configuration otherwise); an
+    // unaligned operand reaching one is reported through the status word.
 #define SKG_MAC(V)                                                           \
   case V:                                                                    \
-    mac_loop<BM, BN, BK, WM, WN, (V >> 2) & 1, (V >> 1) & 1, V & 1,          \
-             ((V >> 3) & 1) != 0>(acc, A, B, K, m0, n0, c * kci, kb, ke,     \
-                                  lds, tid, mask);                           \
+    if constexpr (NS >= 3 && (V & 1))                                        \
+      mac_loop_dma<BM, BN, BK, WM, WN, (V >> 2) & 1, (V >> 1) & 1,           \
+                   ((V >> 3) & 1) != 0, NS>(acc, A, B, K, m0, n0, c * kci,   \
+                                            kb, ke, lds, tid, mask);         \
+    else if constexpr (NS >= 3)                                              \
+      if (tid == 0) atomicOr(L.status, 2);                                   \
+    if constexpr (NS == 0)                                                   \
+      mac_loop<BM, BN, BK, WM, WN, (V >> 2) & 1, (V >> 1) & 1, V & 1,        \
+               ((V >> 3) & 1) != 0>(acc, A, B, K, m0, n0, c * kci, kb, ke,   \
+                                    lds, tid, mask);                         \
     break;
     switch (variant) {
       SKG_MAC(0) SKG_MAC(1) SKG_MAC(2) SKG_MAC(3)
@@ -590,29 +830,21 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
   }
 }
 
-template <int BM, int BN, int BK, int WM, int WN>
+template <int BM, int BN, int BK, int WM, int WN, int NS>
 static size_t lds_bytes_t() {
-  return 2ull * TileShape<BM, BN, BK, WM, WN>::kBufDoubles * sizeof(double);
+  size_t d = 2ull * TileShape<BM, BN, BK, WM, WN>::kBufDoubles;
+  if constexpr (NS >= 3) {
+    const size_t dma =
+        (size_t)NS * (dma_region<BM, BK>() + dma_region<BN, BK>());
+    if (dma > d) d = dma;
+  }
+  return d * sizeof(double);
 }
 
-template <int BM, int BN, int BK, int WM, int WN>
-static hipError_t launch_t(const Launch& L, hipStream_t s) {
-  if (L.problems != nullptr) {
-    auto kb = sk_dgemm_kernel<BM, BN, BK, WM, WN, true>;
-    const size_t lds = lds_bytes_t<BM, BN, BK, WM, WN>();
-    static bool attr_b = false;
-    if (!attr_b) {
-      hipError_t e = hipFuncSetAttribute(
-          reinterpret_cast<const void*>(kb),
-          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      if (e != hipSuccess) return e;
-      attr_b = true;
-    }
-    hipLaunchKernelGGL(kb, dim3(L.grid), dim3(WM * WN * 64), lds, s, L);
-    return hipGetLastError();
-  }
-  auto kfn = sk_dgemm_kernel<BM, BN, BK, WM, WN, false>;
-  const size_t lds = lds_bytes_t<BM, BN, BK, WM, WN>();
+template <int BM, int BN, int BK, int WM, int WN, int NS, bool BATCHED>
+static hipError_t launch_v(const Launch& L, hipStream_t s) {
+  auto kfn = sk_dgemm_kernel<BM, BN, BK, WM, WN, NS, BATCHED>;
+  const size_t lds = lds_bytes_t<BM, BN, BK, WM, WN, NS>();
   static bool attr_set = false;
   if (!attr_set) {
     hipError_t e = hipFuncSetAttribute(
@@ -625,14 +857,22 @@ static hipError_t launch_t(const Launch& L, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int BM, int BN, int BK, int WM, int WN>
+template <int BM, int BN, int BK, int WM, int WN, int NS>
+static hipError_t launch_t(const Launch& L, hipStream_t s) {
+  if (L.problems != nullptr) return launch_v<BM, BN, BK, WM, WN, NS, true>(L, s);
+  return launch_v<BM, BN, BK, WM, WN, NS, false>(L, s);
+}
+
+template <int BM, int BN, int BK, int WM, int WN, int NS>
 static int occupancy_t() {
   // Both variants (single problem / batched table) must fit: the persistent
   // grid is sized from the smaller of the two.
   const void* fns[2] = {
-      reinterpret_cast<const void*>(sk_dgemm_kernel<BM, BN, BK, WM, WN, true>),
-      reinterpret_cast<const void*>(sk_dgemm_kernel<BM, BN, BK, WM, WN, false>)};
-  const size_t lds = lds_bytes_t<BM, BN, BK, WM, WN>();
+      reinterpret_cast<const void*>(
+          sk_dgemm_kernel<BM, BN, BK, WM, WN, NS, true>),
+      reinterpret_cast<const void*>(
+          sk_dgemm_kernel<BM, BN, BK, WM, WN, NS, false>)};
+  const size_t lds = lds_bytes_t<BM, BN, BK, WM, WN, NS>();
   int best = 1 << 30;
   for (const void* f : fns) {
     hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -646,16 +886,20 @@ static int occupancy_t() {
   return best;
 }
 
-#define SKG_CFG_SWITCH(cfg, FN, ...)                  \
-  switch (cfg) {                                      \
-    case 0: return FN<128, 128, 16, 2, 2>(__VA_ARGS__); \
-    case 1: return FN<64, 64, 16, 2, 2>(__VA_ARGS__);   \
-    case 2: return FN<128, 64, 16, 2, 2>(__VA_ARGS__);  \
-    case 3: return FN<64, 128, 16, 2, 2>(__VA_ARGS__);  \
-    case 4: return FN<128, 128, 16, 4, 2>(__VA_ARGS__); \
-    case 5: return FN<64, 64, 32, 2, 2>(__VA_ARGS__);   \
-    case 6: return FN<128, 64, 32, 4, 2>(__VA_ARGS__);  \
-    default: break;                                   \
+#define SKG_CFG_SWITCH(cfg, FN, ...)                       \
+  switch (cfg) {                                           \
+    case 0: return FN<128, 128, 16, 2, 2, 0>(__VA_ARGS__);  \
+    case 1: return FN<64, 64, 16, 2, 2, 0>(__VA_ARGS__);    \
+    case 2: return FN<128, 64, 16, 2, 2, 0>(__VA_ARGS__);   \
+    case 3: return FN<64, 128, 16, 2, 2, 0>(__VA_ARGS__);   \
+    case 4: return FN<128, 128, 16, 4, 2, 0>(__VA_ARGS__);  \
+    case 5: return FN<64, 64, 32, 2, 2, 0>(__VA_ARGS__);    \
+    case 6: return FN<128, 64, 32, 4, 2, 0>(__VA_ARGS__);   \
+    case 7: return FN<128, 128, 16, 4, 2, 3>(__VA_ARGS__);  \
+    case 8: return FN<128, 128, 16, 4, 2, 4>(__VA_ARGS__);  \
+    case 9: return FN<128, 64, 16, 2, 2, 4>(__VA_ARGS__);   \
+    case 10: return FN<64, 64, 16, 2, 2, 4>(__VA_ARGS__);   \
+    default: break;                                        \
   }
 
 hipError_t launch_dgemm_sk(int cfg, const Launch& L, hipStream_t s) {

@@ -24,6 +24,7 @@
 #include <cstdint>
 #include <deque>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -35,7 +36,7 @@ using namespace skg;
 
 namespace {
 
-constexpr int kServeCfg = 4;  // 128 x 128 tiles, 8 waves (square: SYRK)
+constexpr int kServeCfg = 8;  // 128 x 128 tiles, 8 waves, LDS-DMA (square: SYRK)
 constexpr int kMaxInflight = 8;
 
 double now_s() {
@@ -58,6 +59,7 @@ struct Request {
   double flops = 0.0;
   bool done = false;
   int rc = SKGEMM_OK;
+  std::string err;  // error text, set in the thread that saw the error
 };
 
 struct Slot {
@@ -217,6 +219,7 @@ void skgemm_server::dispatch_loop() {
         Request& r = reqs[tk];
         r.done = true;
         r.rc = rc;
+        r.err = skgemm_last_error();
         r.t_done = t;
       }
       sl.busy = false;
@@ -254,6 +257,7 @@ void skgemm_server::complete_loop() {
         Request& r = reqs[tk];
         r.done = true;
         r.rc = rc;
+        if (rc != SKGEMM_OK) r.err = skgemm_last_error();
         r.t_done = t;
       }
       completed += (int64_t)s.tickets.size();
@@ -415,6 +419,7 @@ extern "C" int skgemm_server_wait(skgemm_server* s, int64_t ticket,
   const Request r = s->reqs[ticket];
   s->reqs.erase(ticket);
   if (latency_us) *latency_us = (r.t_done - r.t_submit) * 1e6;
+  if (r.rc != SKGEMM_OK) return set_error(r.rc, "%s", r.err.c_str());
   return r.rc;
 }
 

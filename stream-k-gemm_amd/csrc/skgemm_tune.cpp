@@ -72,6 +72,15 @@ extern "C" int skgemm_autotune(skgemm_ctx* c, int kind, int la, int lb, int M,
   double* B = kind == 0 ? sc.alloc(span(lb, false, ldb, N, K)) : A;
   const int ld_layout = (la == SKGEMM_COL_MAJOR && kind == 0) ? SKGEMM_COL_MAJOR
                                                              : SKGEMM_ROW_MAJOR;
+  // Plans are keyed by the problem as launched (a column-major D runs
+  // transposed).
+  int kla = la, klb = lb, kM = M, kN = N;
+  if (kind == 0) {
+    int kld = ld_layout;
+    const double *pa = nullptr, *pb = nullptr;
+    int64_t x = 0, y = 0;
+    transpose_launch(kla, klb, kld, kM, kN, pa, x, pb, y);
+  }
   const int64_t ldd = ld_layout == SKGEMM_COL_MAJOR ? M : N;
   double* D = sc.alloc((size_t)M * N);
   if (!A || !B || !D) return set_error(SKGEMM_ERR_NOMEM, "autotune scratch");
@@ -104,7 +113,7 @@ extern "C" int skgemm_autotune(skgemm_ctx* c, int kind, int la, int lb, int M,
     // Data-parallel candidates: one (K chunk, tile) unit per workgroup, so
     // no fix-up and the workgroups of one XCD run tiles of one K chunk in
     // step (panel reuse in L2).
-    const int64_t tiles = tiles_for(kind, cfg, M, N);
+    const int64_t tiles = tiles_for(kind, cfg, kM, kN);
     for (int kc = 1; kc <= 16 && ipt / kc >= 8; ++kc) {
       const int64_t units = tiles * kc;
       if (units <= cap && 10 * units >= 7 * (int64_t)cap)
@@ -129,7 +138,7 @@ extern "C" int skgemm_autotune(skgemm_ctx* c, int kind, int la, int lb, int M,
       const double us = 1e3 * ms / reps;
       if (us < best) {
         best = us;
-        bp = plan(c, kind, la, lb, M, N, K);
+        bp = plan(c, kind, kla, klb, kM, kN, K);
       }
     }
     if (rc != SKGEMM_OK) break;
@@ -142,12 +151,12 @@ extern "C" int skgemm_autotune(skgemm_ctx* c, int kind, int la, int lb, int M,
   if (rc != SKGEMM_OK) return rc;
   const int st = c->check_status();
   if (st != SKGEMM_OK) return st;
-  c->tuned[std::make_tuple(kind, la, lb, M, N, K)] = bp;
+  c->tuned[std::make_tuple(kind, kla, klb, kM, kN, K)] = bp;
   if (cache_path != nullptr) {
     FILE* f = std::fopen(cache_path, "a");
     if (f == nullptr) return set_error(SKGEMM_ERR_ARG, "cannot open %s", cache_path);
-    std::fprintf(f, "%d %d %d %d %d %d %d %d %d %.3f\n", kind, la, lb, M, N, K,
-                 bp.cfg, bp.grid, bp.kchunks, best);
+    std::fprintf(f, "%d %d %d %d %d %d %d %d %d %.3f\n", kind, kla, klb, kM,
+                 kN, K, bp.cfg, bp.grid, bp.kchunks, best);
     std::fclose(f);
   }
   if (best_tile) *best_tile = bp.cfg;

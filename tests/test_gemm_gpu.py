@@ -345,3 +345,27 @@ def test_projection_full_size(skg, ctx, orc):
     assert rel_fro(full, full.T) < TOL
     il = np.tril_indices(Ns)
     assert rel_fro(full[il], ms.download().reshape(Ns, Ns)[il]) < TOL
+
+
+@pytest.mark.parametrize("pad", [2, 3])
+def test_every_tile_every_layout(skg, ctx, orc, rng, pad):
+    """Every tile configuration (register-staged and LDS-DMA) on every A / B
+    layout, ragged extents; even leading dimensions give 16-byte aligned
+    pairs (the DMA tiles run), odd ones take the register-staged sibling."""
+    M, N, K = 78, 46, 134
+    A = rng.uniform(-1, 1, (M, K))
+    B = rng.uniform(-1, 1, (K, N))
+    ref = np.zeros(M * N)
+    orc.dgemm(0, 0, 0, M, N, K, 1.0, np.ascontiguousarray(A.ravel()), K,
+              np.ascontiguousarray(B.ravel()), N, 0.0, ref, N, 0.0, dd=True)
+    for tile in range(skg.num_tiles()):
+        ctx.set_tuning(tile, 0)
+        for la in range(4):
+            Ab, lda = encode_a(A, la, pad)
+            for lb in range(4):
+                Bb, ldb = encode_b(B, lb, pad)
+                D = np.zeros(M * N)
+                ctx.dgemm_host(la, lb, 0, M, N, K, 1.0, Ab, lda, Bb, ldb, 0.0,
+                               D, N, 0.0)
+                assert rel_fro(D, ref) < TOL, (tile, la, lb, pad)
+    ctx.set_tuning(-1, 0)

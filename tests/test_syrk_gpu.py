@@ -142,3 +142,28 @@ def test_power_of_two_scaling_exact(skg, ctx, rng):
     il = np.tril_indices(N)
     scale = np.ldexp(1.0, e[:, None] + e[None, :])
     assert np.array_equal(Cs.reshape(N, N)[il], (C.reshape(N, N) * scale)[il])
+
+
+def test_every_square_tile_column_major(skg, ctx, orc, rng):
+    """SYRK on column-major Psi (contiguous along K: the k-contiguous LDS
+    image of the DMA tiles) with every square tile configuration, split and
+    unsplit K."""
+    N, K = 136, 2000
+    P = rng.standard_normal((K, N))
+    colmajor = P.ravel(order="F").copy()        # element (k, n) at k + n*K
+    ref = np.zeros(N * N)
+    orc.dsyrk(0, N, K, 1.0, P.ravel().copy(), N, 0.0, ref, N, dd=True)
+    il = np.tril_indices(N)
+    for tile in range(skg.num_tiles()):
+        info = skg.tile_info(tile)
+        if info["bm"] != info["bn"]:
+            continue
+        for grid, kchunks in ((0, 1), (0, 4), (37, 1)):
+            ctx.set_tuning(tile, grid)
+            ctx.set_kchunks(kchunks)
+            C = np.zeros(N * N)
+            ctx.dsyrk_host(1, N, K, 1.0, colmajor, K, 0.0, C, N)
+            err = rel_fro(C.reshape(N, N)[il], ref.reshape(N, N)[il])
+            assert err < TOL, (tile, grid, kchunks, err)
+    ctx.set_tuning(-1, 0)
+    ctx.set_kchunks(0)
