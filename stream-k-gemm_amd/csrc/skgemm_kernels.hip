@@ -60,7 +60,11 @@ int register_sibling(int cfg) {
   return cfg;
 }
 
-constexpr int kPad = 2;  // LDS row padding in doubles (keeps 16 B alignment)
+// LDS row padding (doubles) of an operand image kept as [k][BO + kPad]: the
+// MFMA fragment reads of one instruction touch rows k..k+3 (16 consecutive
+// doubles each); a 128-byte pad puts consecutive rows in opposite halves of
+// the 64 banks, so 32 lanes read 256 distinct bytes.
+constexpr int kPad = 16;
 
 __device__ __forceinline__ int64_t wg_start(int w, int64_t total, int G) {
   const int64_t base = total / G, extra = total % G;
@@ -348,8 +352,8 @@ __device__ __forceinline__ void mac_loop(
 // from memory straight into LDS with 16-byte buffer loads to LDS (lane l of
 // a wave instruction writes LDS base + 16 l), ns stages deep, so no
 // registers or LDS-store instructions are spent on staging and ns - 1 steps
-// are in flight.  An operand contiguous along o keeps the layout of the
-// register path ([k][BO + 16] when a row is one instruction, else [k][BO]);
+// are in flight.  An operand contiguous along o is kept as [k][BO + 4]
+// when a row is one instruction (else [k][BO]);
 // one contiguous along k is kept as [o][16] with 16-byte chunk c of row o
 // at slot c ^ (o & 7) ^ ((o >> 3) & 1) (the swizzle is applied on the
 // global side: a lane fetches the chunk that belongs at its slot), which
@@ -371,7 +375,9 @@ struct DmaTile {
   static constexpr int kCpr = KC ? BK / 2 : BO / 2;  // 16-byte chunks per row
   static constexpr int kRpd = 64 / kCpr;             // rows per instruction
   static constexpr int kRows = KC ? BO : BK;
-  static constexpr int kPitch = KC ? BK : (kRpd == 1 ? BO + 16 : BO);
+  // o-contiguous rows: a fragment read touches rows 4 lq + s (lq = 0..3),
+  // 4 rows apart; a 32-byte pad per row puts those in opposite bank halves.
+  static constexpr int kPitch = KC ? BK : (kRpd == 1 ? BO + 4 : BO);
   static constexpr int kNumDma = kRows / kRpd;
   static constexpr int kDoubles = kRows * kPitch;
   static_assert(kRpd >= 1 && kRows % kRpd == 0, "tile vs instruction");
