@@ -186,7 +186,8 @@ def bench_split(skg, ctx, reps):
     t_plain = median_ms(ctx, lambda: ctx.dsyrk(
         skg.COL_MAJOR, Ns, Nd, 1.0, psi.ptr, Nd, 0.0, plain.ptr, Ns), reps)
     out = {"workload": "SYRK Ms = Psi^T Psi (lower), Psi 65536x1024 FP64 "
-                       "col-major, columns scaled by 10^U(0,6)",
+                       "col-major, columns scaled by 10^U(0,6); split form "
+                       "vs plain FP64, and the plain FP64 rotation Psi*Q",
            "plain_fp64": {"ms": t_plain, "plan": plan}}
     f = float(Ns) * (Ns + 1) * Nd
     out["plain_fp64"]["tflops"] = f / (t_plain * 1e-3) / 1e12
@@ -202,7 +203,23 @@ def bench_split(skg, ctx, reps):
             "speedup_vs_plain": t_plain / t, "rel_fro_vs_plain_fp64": err,
             "stated_bound": bound, "moduli": L,
             "int8_tops": L * f / (t * 1e-3) / 1e12}
-    psi.free()
+    # plain FP64 rotation Psi * Q on the same Psi (Q uniform(-1, 1), seed 4)
+    q = skg.DeviceArray(ctx, Ns * Ns)
+    rot = skg.DeviceArray(ctx, Nd * Ns)
+    ctx.fill_uniform(q.ptr, Ns * Ns, 4, -1.0, 1.0)
+    rplan = ctx.autotune(0, Nd, Ns, Ns, la=skg.COL_MAJOR, lb=skg.ROW_MAJOR)
+    t_rot = median_ms(ctx, lambda: ctx.dgemm(
+        skg.COL_MAJOR, skg.ROW_MAJOR, skg.COL_MAJOR, Nd, Ns, Ns, 1.0, psi.ptr,
+        Nd, q.ptr, Ns, 0.0, rot.ptr, Nd, 0.0), reps)
+    fr = 2.0 * Nd * Ns * Ns
+    out["plain_fp64_rotation"] = {
+        "ms": t_rot, "tflops": fr / (t_rot * 1e-3) / 1e12,
+        "frac_fp64_peak": fr / (t_rot * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+        "plan": rplan}
+    out["plain_fp64"]["frac_fp64_peak"] = (out["plain_fp64"]["tflops"] /
+                                           PEAK_FP64_TFLOPS)
+    for x in (psi, q, rot):
+        x.free()
     return out
 
 
@@ -234,6 +251,56 @@ def bench_projection(skg, ctx, reps):
             "tuned": {"ms": t1, "tflops": f / (t1 * 1e-3) / 1e12,
                       "frac_fp64_peak": f / (t1 * 1e-3) / 1e12 /
                       PEAK_FP64_TFLOPS, "plan": plan}}
+
+
+def bench_nloc(skg, ctx, reps):
+    """Non-local projector operator result += V_nloc psi on the c2 grid
+    (Nd 32768, 520 bands, psi 16-band-packed, result 32-band-packed), 64
+    projectors of 512..1024 grid points (compact spheres) and 9, 13 or 16
+    columns, seeded.  HBM-bound: the algorithmic traffic is psi read at the
+    touched points plus result read and written there (3 x 8 x points x
+    nb bytes); the implementation also writes and reads the gathered rows
+    and the backward products (4 x 8 x rows x nb more)."""
+    import time
+    K, nb, count = 32768, 520, 64
+    rng = np.random.default_rng(6)
+    nrows = [int(rng.integers(512, 1025)) for _ in range(count)]
+    ncols = [int(rng.choice([9, 13, 16])) for _ in range(count)]
+    idx = []
+    for n in nrows:
+        c = int(rng.integers(0, K))
+        idx.append(np.unique((c + np.arange(-(n // 2), n - n // 2) * 3) % K)[:n])
+    nrows = [len(x) for x in idx]
+    chi = rng.uniform(-1, 1, sum(r * c for r, c in zip(nrows, ncols)))
+    gam = rng.uniform(-1, 1, sum(ncols))
+    idx_all = np.concatenate(idx)
+    points = len(np.unique(idx_all))
+    rows = int(sum(nrows))
+    h = skg.Nloc(ctx, K, nrows, ncols, list(range(count)), idx_all, chi, gam)
+    psi = skg.DeviceArray(ctx, ((nb + 15) // 16) * K * 16)
+    res = skg.DeviceArray(ctx, ((nb + 31) // 32) * K * 32, zero=True)
+    ctx.fill_normal(psi.ptr, ((nb + 15) // 16) * K * 16, 7, 1.0 / np.sqrt(K))
+    h.apply(nb, 0.02, psi.ptr, res.ptr, 32)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        h.apply(nb, 0.02, psi.ptr, res.ptr, 32)
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    h.close()
+    for x in (psi, res):
+        x.free()
+    alg = 3.0 * 8 * points * nb
+    moved = alg + 4.0 * 8 * rows * nb
+    flop = 4.0 * nb * sum(r * c for r, c in zip(nrows, ncols))
+    return {"workload": "nloc projectors: Nd 32768, 520 bands, 64 projectors "
+                        "(512..1024 points, 9/13/16 columns), 16->32 packed",
+            "ms": ms, "timing": "host wall clock over %d applications" % reps,
+            "grid_points_touched": points, "projector_rows": rows,
+            "algorithmic_gbps": alg / (ms * 1e-3) / 1e9,
+            "moved_gbps": moved / (ms * 1e-3) / 1e9,
+            "frac_hbm_peak_algorithmic": alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            "gflops": flop / (ms * 1e-3) / 1e9}
 
 
 def bench_layouts(skg, ctx, reps):
@@ -419,7 +486,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the c1, split-form, batched and serving "
                          "sections")
-    ap.add_argument("--extra", default="c1,split,projection,layouts,batched,serving",
+    ap.add_argument("--extra", default="c1,split,projection,nloc,layouts,batched,serving",
                     help="comma list of extra sections to run")
     ap.add_argument("--tuning-cache", default=None,
                     help="autotuner cache file: read if it exists (no "
@@ -527,6 +594,9 @@ def main():
     if "projection" in sections:
         log("projection GEMM (616x616x175616) ...")
         extra["projection"] = bench_projection(skg, ctx, 10)
+    if "nloc" in sections:
+        log("nloc projectors (c2 grid) ...")
+        extra["nloc"] = bench_nloc(skg, ctx, 20)
     if "layouts" in sections:
         log("layout conversions (65536x1024) ...")
         extra["layouts"] = bench_layouts(skg, ctx, 10)

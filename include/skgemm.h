@@ -21,6 +21,9 @@
  *                                                         -> skgemm_pack
  *   sme_pack<T>::band_to_grid_tile /      include/sme_pack.hpp:178-313
  *     grid_to_band_tile                                   -> skgemm_transpose
+ *   Hamiltonian::nloc_project_vectors_packed(_16to16)
+ *                                         include/nloc_packed_opt2.hpp:229,453
+ *                                                         -> skgemm_nloc_*
  *   hbm_alloc / hbm_calloc / hbm_free     include/hbm_alloc.hpp:21-50
  *                                                         -> skgemm_dev_alloc/free
  *   run_fmopa_loop peak probe             sme_peak_microbench.cpp:16-60
@@ -247,6 +250,36 @@ typedef struct skgemm_problem {
  * in the reference. */
 int skgemm_batched(skgemm_ctx* ctx, const skgemm_problem* problems, int count,
                    int precision, int slices, int use_graph);
+
+/* ---- non-local projectors (include/nloc_packed_opt2.hpp:229-655) -------- */
+
+/* result += V_nloc psi on band-packed orbitals, the reference's
+ * nloc_project_vectors_packed (16-band-packed psi -> 32-band-packed
+ * result, :229-447) and nloc_project_vectors_packed_16to16 (:453-655), as a
+ * client of the batched scheduler: per projector p, X_p = dv chi_p^T
+ * psi[idx_p, :], X_p *= gamma_p, result[idx_p, :] += chi_p X_p.
+ * A projector set on Nd grid points is uploaded once.  Host arrays,
+ * projectors concatenated in order: nrow[p] grid indices in
+ * index_all[sum_{q<p} nrow_q ..] (in [0, Nd)), chi_p column-major
+ * nrow x ncol in chi_all[sum_{q<p} nrow_q ncol_q ..], gamma_p (ncol) in
+ * gamma_all[sum_{q<p} ncol_q ..].  As in the reference, gamma scales a
+ * projector only when its atom_index differs from the previous
+ * projector's (the first projector always); projectors with nrow or ncol
+ * 0 contribute nothing; a grid point may belong to several projectors (the
+ * contributions are added in projector order, without atomics).  The
+ * reference's MPI_Allreduce of X over domain ranks has no counterpart (one
+ * GPU holds the whole grid). */
+typedef struct skgemm_nloc skgemm_nloc;
+int skgemm_nloc_create(skgemm_ctx* ctx, int Nd, int num_proj, const int* nrow,
+                       const int* ncol, const int* atom_index,
+                       const int* index_all, const double* chi_all,
+                       const double* gamma_all, skgemm_nloc** out);
+/* Device pointers, asynchronous on the context's stream: psi16
+ * [ceil(nb/16)][Nd][16], result [ceil(nb/T)][Nd][T] with T = result_tile
+ * (32 or 16).  The handle's workspace grows with nb. */
+int skgemm_nloc_apply(skgemm_nloc* h, int nb, double dv, const double* psi16,
+                      double* result, int result_tile);
+int skgemm_nloc_destroy(skgemm_nloc* h);
 
 /* ---- tuning ------------------------------------------------------------- */
 

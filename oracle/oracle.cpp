@@ -29,6 +29,10 @@
 //       reference's band<->grid known-answer pattern g + 1000 b
 //       (pack.cpp:190-258), and numpy reshape/transpose restatements of
 //       both packed layouts.
+//   oracle_nloc_apply
+//       the reference test's unpacked formulation (nloc_packed_opt2_test.cpp:
+//       84-165) restated in numpy, closed forms (one projector, one column,
+//       identity chi), the gamma-once-per-atom rule, overlapping indices.
 // The split form (Ozaki scheme II) is restated separately in oracle/ozaki.py
 // and pinned in tests/test_ozaki_oracle.py.
 
@@ -515,6 +519,89 @@ void oracle_dsyrk(int la, int N, int K, double alpha, const double* A, long lda,
       if (beta != 0.0) out += beta * C[(long)i * ldc + j];
       C[(long)i * ldc + j] = out;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Non-local projector application (include/nloc_packed_opt2.hpp:229-447,
+// 16-band-packed psi -> 32-band-packed result; :453-655, -> 16-band-packed),
+// the sequential branch, restated step by step.  Projector p has nrow[p]
+// grid indices index_all[roff[p] ..], chi_p column-major nrow x ncol at
+// chi_all[choff[p] ..] and gamma_p at gamma_all[coff[p] ..].
+//   1. gather (:55-79): filtered[r + b nrow] = psi16[(b/16 K + index[r]) 16
+//      + b%16];
+//   2. forward GEMM (:145-178 with beta = 1 on a zeroed chi_vector):
+//      X[(coff + c) + b total_ncol] = 1 * X + dv * sum_r chi[r + c nrow]
+//      filtered[r + b nrow], r ascending;
+//   3. scale (:329-353): X rows of projector p times gamma_p, only when p's
+//      atom_index differs from the previous projector's (starting from the
+//      first projector's atom_index - 1);
+//   4. backward GEMM (:185-224, beta = 0): temp[r + b nrow] = sum_c
+//      chi[r + c nrow] * X[(coff + c) + b total_ncol], c ascending, each term
+//      added to the running sum;
+//   5. scatter (:88-137): result[(b/T K + index[r]) T + b%T] += temp[r + b
+//      nrow], projectors in order, T = 32 or 16.
+// Projectors with nrow = 0 or ncol = 0 are skipped (:259, :398).
+void oracle_nloc_apply(int K, int nb, double dv, int num_proj,
+                       const int* nrow, const int* ncol, const int* atom_index,
+                       const int* index_all, const double* chi_all,
+                       const double* gamma_all, const double* psi16,
+                       double* result, int out_tile) {
+  if (num_proj <= 0) return;
+  std::vector<long> roff(num_proj + 1, 0), coff(num_proj + 1, 0),
+      choff(num_proj + 1, 0);
+  for (int p = 0; p < num_proj; ++p) {
+    roff[p + 1] = roff[p] + nrow[p];
+    coff[p + 1] = coff[p] + ncol[p];
+    choff[p + 1] = choff[p] + (long)nrow[p] * ncol[p];
+  }
+  const long total_ncol = coff[num_proj];
+  if (total_ncol == 0) return;
+  std::vector<double> X((size_t)total_ncol * nb, 0.0);
+  for (int p = 0; p < num_proj; ++p) {
+    const int nr = nrow[p], nc = ncol[p];
+    if (nr == 0 || nc == 0) continue;
+    const int* idx = index_all + roff[p];
+    const double* chi = chi_all + choff[p];
+    std::vector<double> filtered((size_t)nr * nb);
+    for (int b = 0; b < nb; ++b)
+      for (int r = 0; r < nr; ++r)
+        filtered[r + (size_t)b * nr] =
+            psi16[((long)(b / 16) * K + idx[r]) * 16 + b % 16];
+    for (int b = 0; b < nb; ++b)
+      for (int c = 0; c < nc; ++c) {
+        double sum = 0.0;
+        for (int r = 0; r < nr; ++r)
+          sum += chi[r + (size_t)c * nr] * filtered[r + (size_t)b * nr];
+        double& x = X[(coff[p] + c) + (size_t)b * total_ncol];
+        x = 1.0 * x + dv * sum;
+      }
+  }
+  int atom = atom_index[0] - 1;
+  for (int p = 0; p < num_proj; ++p) {
+    if (atom == atom_index[p]) continue;
+    atom = atom_index[p];
+    const double* gamma = gamma_all + coff[p];
+    for (int b = 0; b < nb; ++b)
+      for (int c = 0; c < ncol[p]; ++c)
+        X[(coff[p] + c) + (size_t)b * total_ncol] *= gamma[c];
+  }
+  for (int p = 0; p < num_proj; ++p) {
+    const int nr = nrow[p], nc = ncol[p];
+    if (nr == 0 || nc == 0) continue;
+    const int* idx = index_all + roff[p];
+    const double* chi = chi_all + choff[p];
+    std::vector<double> temp((size_t)nr * nb, 0.0);
+    for (int b = 0; b < nb; ++b)
+      for (int c = 0; c < nc; ++c) {
+        const double bv = 1.0 * X[(coff[p] + c) + (size_t)b * total_ncol];
+        for (int r = 0; r < nr; ++r)
+          temp[r + (size_t)b * nr] += chi[r + (size_t)c * nr] * bv;
+      }
+    for (int b = 0; b < nb; ++b)
+      for (int r = 0; r < nr; ++r)
+        result[((long)(b / out_tile) * K + idx[r]) * out_tile + b % out_tile] +=
+            temp[r + (size_t)b * nr];
+  }
 }
 
 }  // extern "C"

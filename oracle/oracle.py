@@ -51,6 +51,7 @@ def load(path=None):
         "oracle_pack_c16": [I, I, P, P],
         "oracle_unpack_c16": [I, I, P, P],
         "oracle_band_to_grid": [I, I, P, P],
+        "oracle_nloc_apply": [I, I, D, I, P, P, P, P, P, P, P, P, I],
         "oracle_grid_to_band": [I, I, P, P],
         "oracle_gemm_crsp_packed": [I, I, I, P, P, P, D, D, I],
         "oracle_gemm_opt_packed": [I, I, I, P, P, P, D, D, I],
@@ -127,6 +128,21 @@ def unpack_c16(Nd, nb, src):
     out = np.zeros(Nd * nb, np.float64)
     load().oracle_unpack_c16(Nd, nb, _p(src), _p(out))
     return out
+
+
+def nloc_apply(K, nb, dv, nrow, ncol, atom_index, index_all, chi_all,
+               gamma_all, psi16, result, out_tile):
+    """In place on result (out_tile = 32 or 16 band packing)."""
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    nrow, ncol, atom_index, index_all = (i32(nrow), i32(ncol), i32(atom_index),
+                                         i32(index_all))
+    chi_all, gamma_all, psi16 = f64(chi_all), f64(gamma_all), f64(psi16)
+    pi = lambda a: a.ctypes.data
+    load().oracle_nloc_apply(K, nb, dv, len(nrow), pi(nrow), pi(ncol),
+                             pi(atom_index), pi(index_all), _p(chi_all),
+                             _p(gamma_all), _p(psi16), _p(result), out_tile)
+    return result
 
 
 def band_to_grid(Nd, nb, src):

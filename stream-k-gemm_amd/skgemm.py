@@ -29,6 +29,7 @@ EXPORTS = [
     "skgemm_dgemm_host", "skgemm_dsyrk", "skgemm_dsyrk_host",
     "skgemm_dgemm_ozaki", "skgemm_dsyrk_ozaki", "skgemm_dgemm_ozaki_host",
     "skgemm_dsyrk_ozaki_host", "skgemm_pack", "skgemm_transpose",
+    "skgemm_nloc_create", "skgemm_nloc_apply", "skgemm_nloc_destroy",
     "skgemm_batched", "skgemm_set_tuning", "skgemm_set_kchunks",
     "skgemm_num_tiles",
     "skgemm_tile_info", "skgemm_autotune", "skgemm_load_tuning",
@@ -114,6 +115,10 @@ def lib():
                                 c_int]),
         "skgemm_transpose": (c_int, [P, c_int, P, c_int64, c_int, P, c_int64,
                                      c_int, c_int]),
+        "skgemm_nloc_create": (c_int, [P, c_int, c_int, P, P, P, P, P, P,
+                                       ctypes.POINTER(c_void_p)]),
+        "skgemm_nloc_apply": (c_int, [P, c_int, c_double, P, P, c_int]),
+        "skgemm_nloc_destroy": (c_int, [P]),
         "skgemm_batched": (c_int, [P, P, c_int, c_int, c_int, c_int]),
         "skgemm_set_tuning": (c_int, [P, c_int, c_int]),
         "skgemm_set_kchunks": (c_int, [P, c_int]),
@@ -453,6 +458,40 @@ class Server:
     def close(self):
         if self.h:
             check(lib().skgemm_server_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Nloc:
+    """Non-local projector set (skgemm_nloc_*): projectors concatenated in
+    order (nrow[p] grid indices, chi_p column-major nrow x ncol, gamma_p)."""
+
+    def __init__(self, ctx, Nd, nrow, ncol, atom_index, index_all, chi_all,
+                 gamma_all):
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        self._keep = (i32(nrow), i32(ncol), i32(atom_index), i32(index_all),
+                      f64(chi_all), f64(gamma_all))
+        nr, nc, at, ix, ch, ga = self._keep
+        h = c_void_p()
+        check(lib().skgemm_nloc_create(ctx.h, Nd, len(nr), nr.ctypes.data,
+                                       nc.ctypes.data, at.ctypes.data,
+                                       ix.ctypes.data, ch.ctypes.data,
+                                       ga.ctypes.data, ctypes.byref(h)))
+        self.h = h
+
+    def apply(self, nb, dv, psi16, result, result_tile=32):
+        check(lib().skgemm_nloc_apply(self.h, nb, dv, _ptr(psi16),
+                                      _ptr(result), result_tile))
+
+    def close(self):
+        if self.h:
+            check(lib().skgemm_nloc_destroy(self.h))
             self.h = None
 
     def __del__(self):

@@ -2,6 +2,7 @@
 include/skgemm.h declares, headers compile, argument errors are reported
 without touching a device, and the Stream-K work split (restated from the
 kernel) covers every MAC iteration once with deadlock-free fix-up order."""
+import ctypes
 import os
 import re
 import subprocess
@@ -157,3 +158,12 @@ def test_triangle_tile_map():
             i += 1
         seen.append((i, t - i * (i + 1) // 2))
     assert seen == [(i, j) for i in range(100) for j in range(i + 1)]
+
+
+def test_nloc_argument_checks_without_device(skg):
+    lib = skg.lib()
+    h = ctypes.c_void_p()
+    assert lib.skgemm_nloc_create(None, 100, 0, None, None, None, None, None,
+                                  None, ctypes.byref(h)) == skg.ERR_STATE
+    assert lib.skgemm_nloc_apply(None, 16, 1.0, None, None, 32) == skg.ERR_ARG
+    assert lib.skgemm_nloc_destroy(None) == skg.OK

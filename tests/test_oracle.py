@@ -300,6 +300,121 @@ def test_band_grid_convert_matches_reshapes(orc, rng):
     assert not np.array_equal(orc.band_to_grid(Nd, nb, band_pack(Y)), grid)
 
 
+# ---- non-local projectors (include/nloc_packed_opt2.hpp) ---------------------
+
+NLOC_CONFIGS = [(100, 16, 20, 5, 1), (200, 32, 50, 8, 2), (500, 48, 100, 12, 3),
+                (1000, 64, 200, 16, 2), (2000, 128, 300, 20, 4)]
+
+
+def nloc_projectors(rng, K, nrow, ncol, nproj, atoms=None):
+    """Random projectors as in nloc_packed_opt2_test.cpp:239-266: chi and
+    gamma uniform, nrow distinct grid indices per projector (projectors may
+    share grid points)."""
+    atoms = list(range(1, nproj + 1)) if atoms is None else atoms
+    nrows = [nrow] * nproj
+    ncols = [ncol] * nproj
+    idx = np.concatenate([rng.permutation(K)[:nrow] for _ in range(nproj)])
+    chi = np.concatenate([rng.uniform(-1, 1, nrow * ncol) for _ in range(nproj)])
+    gam = np.concatenate([rng.uniform(-1, 1, ncol) * 0.5 for _ in range(nproj)])
+    return nrows, ncols, atoms, idx, chi, gam
+
+
+def nloc_unpacked(K, nb, dv, nrows, ncols, atoms, idx, chi, gam, psi, result,
+                  scale_all=False):
+    """The reference test's unpacked formulation (nloc_packed_opt2_test.cpp:
+    84-165) with numpy products: psi and result column-major K x nb."""
+    P = psi.reshape(nb, K).T
+    R = result.reshape(nb, K).T.copy()
+    coff = np.concatenate([[0], np.cumsum(ncols)])
+    roff = np.concatenate([[0], np.cumsum(nrows)])
+    choff = np.concatenate([[0], np.cumsum(np.array(nrows) * np.array(ncols))])
+    X = np.zeros((coff[-1], nb))
+    chis = []
+    for p in range(len(nrows)):
+        C = chi[choff[p]:choff[p + 1]].reshape(ncols[p], nrows[p]).T
+        chis.append(C)
+        X[coff[p]:coff[p + 1]] = dv * (C.T @ P[idx[roff[p]:roff[p + 1]]])
+    atom = atoms[0] - 1
+    for p in range(len(nrows)):
+        if scale_all or atom != atoms[p]:
+            atom = atoms[p]
+            X[coff[p]:coff[p + 1]] *= gam[coff[p]:coff[p + 1], None]
+    for p in range(len(nrows)):
+        R[idx[roff[p]:roff[p + 1]]] += chis[p] @ X[coff[p]:coff[p + 1]]
+    return R.T.ravel().copy()
+
+
+def band_pack_cm(v, K, nb, T):
+    """Column-major K x nb -> T-band-packed [ceil(nb/T)][K][T] (zero pad)."""
+    t = (nb + T - 1) // T
+    out = np.zeros((t, K, T))
+    M = v.reshape(nb, K).T
+    for b in range(nb):
+        out[b // T, :, b % T] = M[:, b]
+    return out.ravel()
+
+
+def band_unpack_cm(v, K, nb, T):
+    t = (nb + T - 1) // T
+    A = v.reshape(t, K, T)
+    return np.stack([A[b // T, :, b % T] for b in range(nb)]).ravel()
+
+
+@pytest.mark.parametrize("cfg", NLOC_CONFIGS)
+@pytest.mark.parametrize("tile", [32, 16])
+def test_nloc_matches_unpacked_formulation(orc, cfg, tile):
+    K, nb, nrow, ncol, nproj = cfg
+    rng = np.random.default_rng(K + nb)
+    nrows, ncols, atoms, idx, chi, gam = nloc_projectors(rng, K, nrow, ncol, nproj)
+    psi = rng.uniform(-1, 1, K * nb)
+    res0 = rng.uniform(-1, 1, K * nb)
+    dv = 0.01
+    want = nloc_unpacked(K, nb, dv, nrows, ncols, atoms, idx, chi, gam, psi, res0)
+    out = band_pack_cm(res0, K, nb, tile)
+    orc.nloc_apply(K, nb, dv, nrows, ncols, atoms, idx, chi, gam,
+                   band_pack_cm(psi, K, nb, 16), out, tile)
+    got = band_unpack_cm(out, K, nb, tile)
+    assert np.max(np.abs(got - want)) < 1e-12
+
+
+def test_nloc_closed_form_and_gamma_once_per_atom(orc):
+    """One column of ones: every touched point gets gamma dv sum(psi[idx]);
+    two projectors of one atom: only the first is scaled."""
+    K, nb = 64, 16
+    psi = np.arange(K * nb, dtype=np.float64) % 7          # small integers
+    idx = np.array([3, 9, 17, 40])
+    for atoms, scales in (([5, 6], (2.0, 3.0)), ([5, 5], (2.0, 1.0))):
+        res = np.zeros(K * 32)                              # 32-packed, nb = 16
+        orc.nloc_apply(K, nb, 0.5, [4, 4], [1, 1], atoms,
+                       np.concatenate([idx, idx]), np.ones(8), [2.0, 3.0],
+                       band_pack_cm(psi, K, nb, 16), res, 32)
+        got = band_unpack_cm(res, K, nb, 32).reshape(nb, K).T
+        P = psi.reshape(nb, K).T
+        s = 0.5 * P[idx].sum(axis=0)                        # dv sum, per band
+        want = np.zeros((K, nb))
+        want[idx] = (scales[0] + scales[1]) * s
+        assert np.array_equal(got, want)
+
+
+def test_nloc_planted_mistake_is_caught(orc):
+    """Scaling every projector (ignoring the once-per-atom rule) differs."""
+    K, nb = 300, 32
+    rng = np.random.default_rng(9)
+    nrows, ncols, atoms, idx, chi, gam = nloc_projectors(rng, K, 40, 6, 3,
+                                                         atoms=[1, 1, 2])
+    psi = rng.uniform(-1, 1, K * nb)
+    res0 = np.zeros(K * nb)
+    good = nloc_unpacked(K, nb, 0.1, nrows, ncols, atoms, idx, chi, gam, psi, res0)
+    bad = nloc_unpacked(K, nb, 0.1, nrows, ncols, atoms, idx, chi, gam, psi, res0,
+                        scale_all=True)
+    out = band_pack_cm(res0, K, nb, 32)
+    orc.nloc_apply(K, nb, 0.1, nrows, ncols, atoms, idx, chi, gam,
+                   band_pack_cm(psi, K, nb, 16), out, 32)
+    got = band_unpack_cm(out, K, nb, 32)
+    assert np.max(np.abs(got - good)) < 1e-12
+    assert np.max(np.abs(got - bad)) > 1e-3
+
+
 # ---- planted mistakes are caught -----------------------------------------------
 
 def test_oracle_mutations_are_detected(orc, rng):
