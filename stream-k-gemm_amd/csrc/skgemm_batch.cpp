@@ -134,49 +134,94 @@ int batch_cfg(skgemm_ctx* c, const skgemm_problem* q, int n) {
   return big >= 256 ? 8 : 1;
 }
 
-int run_plain(skgemm_ctx* c, const skgemm_problem* q, int n, int use_graph) {
+}  // namespace
+
+namespace skg {
+
+int build_plain_table(skgemm_ctx* c, const skgemm_problem* q, int n,
+                      PlainTable* out) {
+  PlainTable& tb = *out;
+  tb.rows.clear();
   int cfg = batch_cfg(c, q, n);
-  if (tile_cfg(cfg).ns >= 3) {
-    for (int i = 0; i < n; ++i) {
-      Problem p{};
-      int M = 0, N = 0;
-      SKG_TRY(describe(q[i], &p, &M, &N, true));
-      if (M > 0 && N > 0 && !(p.a.vec2 && p.b.vec2)) {
-        cfg = register_sibling(cfg);  // same tiles, register-staged loads
-        break;
-      }
-    }
+  struct Row {
+    Problem p;
+    int M, N, i;
+  };
+  std::vector<Row> rows;
+  rows.reserve(n);
+  bool aligned = true;
+  for (int i = 0; i < n; ++i) {
+    Row r{};
+    SKG_TRY(describe(q[i], &r.p, &r.M, &r.N, true));
+    if (r.M == 0 || r.N == 0) continue;
+    aligned = aligned && r.p.a.vec2 && r.p.b.vec2;
+    r.i = i;
+    rows.push_back(r);
   }
-  const TileCfg& t = tile_cfg(cfg);
+  // DMA tiles move 16-byte pairs; same tiles, register-staged loads else.
+  if (tile_cfg(cfg).ns >= 3 && !aligned) cfg = register_sibling(cfg);
   int grid = c->capacity(cfg);
   if (c->forced_grid > 0) grid = std::min(grid, c->forced_grid);
   grid = std::max(8, grid - grid % 8);
+  int64_t iters = 0, tiles = 0;
+  for (Row& r : rows) {
+    const skgemm_problem& x = q[r.i];
+    const Plan pl{cfg, grid, 1};
+    SKG_TRY(fill_problem(c, &r.p, x.kind, pl, r.M, r.N, x.K));
+    r.p.alpha = x.alpha;
+    r.p.beta = x.beta;
+    r.p.gamma = x.kind == 1 ? 0.0 : x.gamma;
+    r.p.pred = nullptr;
+    r.p.tile_begin = tiles;
+    r.p.iter_begin = iters;
+    tiles += r.p.num_tiles;
+    iters += r.p.num_tiles * (int64_t)r.p.kci;
+    tb.rows.push_back(r.p);
+  }
+  tb.cfg = cfg;
+  tb.grid = grid;
+  tb.iters = iters;
+  return SKGEMM_OK;
+}
+
+int launch_plain_table(skgemm_ctx* c, const PlainTable& tb,
+                       const Problem* dtab, const int64_t* dhdr) {
+  if (tb.rows.empty()) return SKGEMM_OK;
+  const TileCfg& t = tile_cfg(tb.cfg);
+  SKG_TRY(c->ensure_ws((size_t)tb.grid * t.bm * t.bn, tb.grid));
+  Launch L{};
+  L.problems = dtab;
+  L.hdr = dhdr;
+  L.num_problems = (int)tb.rows.size();
+  L.total_iters = tb.iters;
+  L.grid = tb.grid;
+  L.ws = c->ws;
+  L.flags = c->flags;
+  L.status = c->status;
+  L.spin_limit = c->spin_limit;
+  SKG_HIP(launch_dgemm_sk(tb.cfg, L, c->stream), "batched launch");
+  return SKGEMM_OK;
+}
+
+}  // namespace skg
+
+namespace {
+
+int run_plain(skgemm_ctx* c, const skgemm_problem* q, int n, int use_graph) {
+  PlainTable tb;
+  SKG_TRY(build_plain_table(c, q, n, &tb));
+  const int cfg = tb.cfg, grid = tb.grid;
+  const int used = (int)tb.rows.size();
+  const int64_t iters = tb.iters;
+  const TileCfg& t = tile_cfg(cfg);
+  if (used == 0) return SKGEMM_OK;
   SKG_TRY(batch_state(c, n));
   auto* b = c->batch;
   if (b->pending) {
     SKG_HIP(hipEventSynchronize(b->copied), "batch staging wait");
     b->pending = false;
   }
-  int64_t iters = 0, tiles = 0;
-  int used = 0;
-  for (int i = 0; i < n; ++i) {
-    Problem p{};
-    int M = 0, N = 0;
-    SKG_TRY(describe(q[i], &p, &M, &N, true));
-    if (M == 0 || N == 0) continue;
-    const Plan pl{cfg, grid, 1};
-    SKG_TRY(fill_problem(c, &p, q[i].kind, pl, M, N, q[i].K));
-    p.alpha = q[i].alpha;
-    p.beta = q[i].beta;
-    p.gamma = q[i].kind == 1 ? 0.0 : q[i].gamma;
-    p.pred = nullptr;
-    p.tile_begin = tiles;
-    p.iter_begin = iters;
-    tiles += p.num_tiles;
-    iters += p.num_tiles * (int64_t)p.kci;
-    b->htab[used++] = p;
-  }
-  if (used == 0) return SKGEMM_OK;
+  std::memcpy(b->htab, tb.rows.data(), sizeof(Problem) * used);
   SKG_TRY(c->ensure_ws((size_t)grid * t.bm * t.bn, grid));
   b->hhdr[0] = used;
   b->hhdr[1] = iters;

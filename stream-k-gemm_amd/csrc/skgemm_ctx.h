@@ -10,6 +10,7 @@
 #include <utility>
 #include <vector>
 
+#include "../../include/skgemm.h"
 #include "skgemm_internal.h"
 
 namespace skg {
@@ -133,6 +134,19 @@ int fill_problem(skgemm_ctx* c, Problem* p, int kind, const Plan& pl, int M,
 // pred: nullptr, or a device flag; the launch does its work only when the
 // flag is non-zero (decided on the device).  timed: record the launch in
 // the context's timing ring.
+// A plain FP64 batch described once (tile configuration, persistent grid,
+// problem rows with their iteration ranges) for callers that keep the rows
+// in device memory of their own and launch them many times.
+struct PlainTable {
+  int cfg = 0, grid = 0;
+  int64_t iters = 0;
+  std::vector<Problem> rows;
+};
+int build_plain_table(skgemm_ctx* c, const skgemm_problem* q, int n,
+                      PlainTable* out);
+// dtab: the rows in device memory; dhdr: device {row count, iterations}.
+int launch_plain_table(skgemm_ctx* c, const PlainTable& t, const Problem* dtab,
+                       const int64_t* dhdr);
 int run_single(skgemm_ctx* c, Problem& p, const Plan& pl,
                const int* pred = nullptr, bool timed = true);
 int prepare_gemm(int la, int lb, int ld_, int M, int N, int K, const double* A,

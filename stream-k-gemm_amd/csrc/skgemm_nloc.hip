@@ -8,11 +8,13 @@
 // Device sequence, all on the context's stream:
 //   1. gather: filtered[R, b] = psi16[idx(R), b] for every projector row R
 //      (rows of all projectors stacked, row-major, ld = nb);
-//   2. forward products: one skgemm_batched launch, one problem per
-//      projector (X_p = dv chi_p^T filtered_p);
+//   2. forward products: one persistent batched Stream-K launch, one
+//      problem per projector (X_p = dv chi_p^T filtered_p);
 //   3. scale: X rows times gamma where the atom rule applies (times 1.0,
 //      exact, elsewhere);
-//   4. backward products: one skgemm_batched launch (temp_p = chi_p X_p);
+//   4. backward products: one batched launch (temp_p = chi_p X_p);
+//   The two problem tables are described once per band count and kept in
+//   device memory, so an application is five launches and no host work.
 //   5. scatter: result[g, b] += temp[R, b] for every row R with grid index
 //      g, each grid point by one thread in projector order (the rows that
 //      touch g are listed in a CSR table built at create time), so shared
@@ -108,12 +110,21 @@ struct skgemm_nloc {
   double* X = nullptr;
   double* temp = nullptr;
   int ws_nb = 0;
+  // The two grouped products, described for `table_nb` bands and kept in
+  // device memory (their rows depend on nb and the workspace only).
+  PlainTable fw, bw;
+  Problem* d_fw = nullptr;
+  Problem* d_bw = nullptr;
+  int64_t* d_hdr = nullptr;  // {rows, iterations} of fw, then of bw
+  int table_nb = -1, table_cfg = -2, table_grid = -2;
+  double fw_dv = 0.0;
 };
 
 static void nloc_free(skgemm_nloc* h) {
   for (void* p : {(void*)h->index, (void*)h->chi, (void*)h->factor,
                   (void*)h->point, (void*)h->ptr, (void*)h->rows_of,
-                  (void*)h->filtered, (void*)h->X, (void*)h->temp})
+                  (void*)h->filtered, (void*)h->X, (void*)h->temp,
+                  (void*)h->d_fw, (void*)h->d_bw, (void*)h->d_hdr})
     if (p) hipFree(p);
 }
 
@@ -237,43 +248,11 @@ extern "C" int skgemm_nloc_create(skgemm_ctx* c, int Nd, int num_proj,
   return SKGEMM_OK;
 }
 
-extern "C" int skgemm_nloc_apply(skgemm_nloc* h, int nb, double dv,
-                                 const double* psi16, double* result,
-                                 int result_tile) {
-  if (h == nullptr) return set_error(SKGEMM_ERR_ARG, "nloc handle is NULL");
+// Describe the forward products X_p (ncol x nb, row-major) = dv chi_p^T
+// filtered_p, chi_p^T being chi_p (column-major nrow x ncol) read row-major,
+// and the backward products temp_p = chi_p X_p; upload both tables.
+static int nloc_tables(skgemm_nloc* h, int nb, double dv) {
   skgemm_ctx* c = h->ctx;
-  SKG_TRY(check_ctx(c));
-  if (nb < 0) return set_error(SKGEMM_ERR_ARG, "nb < 0");
-  if (result_tile != 16 && result_tile != 32)
-    return set_error(SKGEMM_ERR_ARG, "result_tile must be 16 or 32");
-  if (nb == 0 || h->rows == 0 || h->cols == 0 || h->npoints == 0)
-    return SKGEMM_OK;
-  if (psi16 == nullptr || result == nullptr)
-    return set_error(SKGEMM_ERR_ARG, "psi16 or result is NULL");
-  SKG_TRY(check_dev(psi16, "psi16"));
-  SKG_TRY(check_dev(result, "result"));
-  SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
-  if (nb > h->ws_nb) {
-    SKG_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-    for (double** p : {&h->filtered, &h->X, &h->temp}) {
-      if (*p) hipFree(*p);
-      *p = nullptr;
-    }
-    h->ws_nb = 0;
-    if (hipMalloc(&h->filtered, (size_t)h->rows * nb * sizeof(double)) != hipSuccess ||
-        hipMalloc(&h->X, (size_t)h->cols * nb * sizeof(double)) != hipSuccess ||
-        hipMalloc(&h->temp, (size_t)h->rows * nb * sizeof(double)) != hipSuccess)
-      return set_error(SKGEMM_ERR_NOMEM, "nloc workspace for %d bands", nb);
-    h->ws_nb = nb;
-    ++c->mem_gen;
-  }
-  hipLaunchKernelGGL(nloc_gather_kernel, rows_grid(nb, h->rows), dim3(256), 0,
-                     c->stream, psi16, h->Nd, nb, h->index, h->rows,
-                     h->filtered);
-  SKG_HIP(hipGetLastError(), "nloc gather launch");
-
-  // Forward: X_p (ncol x nb, row-major) = dv chi_p^T filtered_p, where
-  // chi_p^T is chi_p (column-major nrow x ncol) read row-major.
   std::vector<skgemm_problem> fw, bw;
   for (int p = 0; p < h->num_proj; ++p) {
     const int nr = h->nrow[p], nc = h->ncol[p];
@@ -312,13 +291,84 @@ extern "C" int skgemm_nloc_apply(skgemm_nloc* h, int nb, double dv,
     g.ldd = nb;
     bw.push_back(g);
   }
-  SKG_TRY(skgemm_batched(c, fw.data(), (int)fw.size(), SKGEMM_FP64, 0, 0));
+  h->table_nb = -1;
+  SKG_TRY(build_plain_table(c, fw.data(), (int)fw.size(), &h->fw));
+  SKG_TRY(build_plain_table(c, bw.data(), (int)bw.size(), &h->bw));
+  SKG_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  for (Problem** d : {&h->d_fw, &h->d_bw}) {
+    if (*d) hipFree(*d);
+    *d = nullptr;
+  }
+  const size_t nf = std::max<size_t>(h->fw.rows.size(), 1);
+  const size_t nbk = std::max<size_t>(h->bw.rows.size(), 1);
+  if (hipMalloc(&h->d_fw, nf * sizeof(Problem)) != hipSuccess ||
+      hipMalloc(&h->d_bw, nbk * sizeof(Problem)) != hipSuccess ||
+      (h->d_hdr == nullptr &&
+       hipMalloc(&h->d_hdr, 4 * sizeof(int64_t)) != hipSuccess))
+    return set_error(SKGEMM_ERR_NOMEM, "nloc problem tables");
+  const int64_t hdr[4] = {(int64_t)h->fw.rows.size(), h->fw.iters,
+                          (int64_t)h->bw.rows.size(), h->bw.iters};
+  SKG_HIP(hipMemcpy(h->d_fw, h->fw.rows.data(),
+                    h->fw.rows.size() * sizeof(Problem), hipMemcpyHostToDevice),
+          "nloc table upload");
+  SKG_HIP(hipMemcpy(h->d_bw, h->bw.rows.data(),
+                    h->bw.rows.size() * sizeof(Problem), hipMemcpyHostToDevice),
+          "nloc table upload");
+  SKG_HIP(hipMemcpy(h->d_hdr, hdr, sizeof(hdr), hipMemcpyHostToDevice),
+          "nloc table upload");
+  h->table_nb = nb;
+  h->fw_dv = dv;
+  h->table_cfg = c->forced_cfg;
+  h->table_grid = c->forced_grid;
+  return SKGEMM_OK;
+}
+
+extern "C" int skgemm_nloc_apply(skgemm_nloc* h, int nb, double dv,
+                                 const double* psi16, double* result,
+                                 int result_tile) {
+  if (h == nullptr) return set_error(SKGEMM_ERR_ARG, "nloc handle is NULL");
+  skgemm_ctx* c = h->ctx;
+  SKG_TRY(check_ctx(c));
+  if (nb < 0) return set_error(SKGEMM_ERR_ARG, "nb < 0");
+  if (result_tile != 16 && result_tile != 32)
+    return set_error(SKGEMM_ERR_ARG, "result_tile must be 16 or 32");
+  if (nb == 0 || h->rows == 0 || h->cols == 0 || h->npoints == 0)
+    return SKGEMM_OK;
+  if (psi16 == nullptr || result == nullptr)
+    return set_error(SKGEMM_ERR_ARG, "psi16 or result is NULL");
+  SKG_TRY(check_dev(psi16, "psi16"));
+  SKG_TRY(check_dev(result, "result"));
+  SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
+  if (nb > h->ws_nb) {
+    SKG_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    for (double** p : {&h->filtered, &h->X, &h->temp}) {
+      if (*p) hipFree(*p);
+      *p = nullptr;
+    }
+    h->ws_nb = 0;
+    if (hipMalloc(&h->filtered, (size_t)h->rows * nb * sizeof(double)) != hipSuccess ||
+        hipMalloc(&h->X, (size_t)h->cols * nb * sizeof(double)) != hipSuccess ||
+        hipMalloc(&h->temp, (size_t)h->rows * nb * sizeof(double)) != hipSuccess)
+      return set_error(SKGEMM_ERR_NOMEM, "nloc workspace for %d bands", nb);
+    h->ws_nb = nb;
+    ++c->mem_gen;
+  }
+  hipLaunchKernelGGL(nloc_gather_kernel, rows_grid(nb, h->rows), dim3(256), 0,
+                     c->stream, psi16, h->Nd, nb, h->index, h->rows,
+                     h->filtered);
+  SKG_HIP(hipGetLastError(), "nloc gather launch");
+
+  if (nb != h->table_nb || h->ws_nb != nb || c->forced_cfg != h->table_cfg ||
+      c->forced_grid != h->table_grid || dv != h->fw_dv) {
+    SKG_TRY(nloc_tables(h, nb, dv));
+  }
+  SKG_TRY(launch_plain_table(c, h->fw, h->d_fw, h->d_hdr));
   if (h->any_scale) {
     hipLaunchKernelGGL(nloc_scale_kernel, rows_grid(nb, h->cols), dim3(256), 0,
                        c->stream, h->X, nb, h->factor, h->cols);
     SKG_HIP(hipGetLastError(), "nloc scale launch");
   }
-  SKG_TRY(skgemm_batched(c, bw.data(), (int)bw.size(), SKGEMM_FP64, 0, 0));
+  SKG_TRY(launch_plain_table(c, h->bw, h->d_bw, h->d_hdr + 2));
   hipLaunchKernelGGL(nloc_scatter_kernel, rows_grid(nb, h->npoints), dim3(256),
                      0, c->stream, h->temp, nb, h->point, h->ptr, h->rows_of,
                      h->npoints, h->Nd, result_tile == 32 ? 5 : 4, result);
