@@ -130,6 +130,25 @@ def cpu_baseline(psi_host, q_host, Nd, Ns, gpu_ms, gpu_rot):
     }
 
 
+TUNING_CACHE = None  # set from --tuning-cache
+
+
+def tuned(ctx, kind, M, N, K, la, lb=0):
+    """The plan for this shape: from the tuning cache when it holds the
+    shape (the cache was loaded into the context at start-up), else by the
+    autotuner, which appends it to the cache."""
+    if TUNING_CACHE and os.path.exists(TUNING_CACHE):
+        with open(TUNING_CACHE) as f:
+            for line in f:
+                v = line.split()
+                if (len(v) == 10 and int(v[0]) == kind and int(v[5]) == K and
+                        sorted((int(v[3]), int(v[4]))) == sorted((M, N))):
+                    p = ctx.query_plan(kind, M, N, K, la=la, lb=lb)
+                    return {"tile": p["tile"], "grid": p["grid"],
+                            "kchunks": p["kchunks"], "us": float(v[9])}
+    return ctx.autotune(kind, M, N, K, la=la, lb=lb, cache_path=TUNING_CACHE)
+
+
 def median_ms(ctx, fn, reps):
     """Median device time of `reps` calls from the context's timing ring
     (HIP events on the launch stream), after one warm-up call."""
@@ -158,7 +177,7 @@ def bench_c1(skg, ctx, reps):
     d = skg.DeviceArray(ctx, n * n, zero=True)
     ctx.fill_uniform(a.ptr, n * n, 42, -1.0, 1.0)
     ctx.fill_uniform(b.ptr, n * n, 43, -1.0, 1.0)
-    plan = ctx.autotune(0, n, n, n, la=skg.ROW_MAJOR, lb=skg.ROW_MAJOR)
+    plan = tuned(ctx, 0, n, n, n, la=skg.ROW_MAJOR, lb=skg.ROW_MAJOR)
     t = median_ms(ctx, lambda: ctx.dgemm(
         skg.ROW_MAJOR, skg.ROW_MAJOR, skg.ROW_MAJOR, n, n, n, 1.0, a.ptr, n,
         b.ptr, n, 0.0, d.ptr, n, 0.0), reps)
@@ -182,7 +201,7 @@ def bench_split(skg, ctx, reps):
     ctx.scale_columns_pow10(psi.ptr, Nd, Ns, Nd, 4, 6.0)
     plain = skg.DeviceArray(ctx, Ns * Ns, zero=True)
     split = skg.DeviceArray(ctx, Ns * Ns, zero=True)
-    plan = ctx.autotune(1, Ns, Ns, Nd, la=skg.COL_MAJOR)
+    plan = tuned(ctx, 1, Ns, Ns, Nd, la=skg.COL_MAJOR)
     t_plain = median_ms(ctx, lambda: ctx.dsyrk(
         skg.COL_MAJOR, Ns, Nd, 1.0, psi.ptr, Nd, 0.0, plain.ptr, Ns), reps)
     out = {"workload": "SYRK Ms = Psi^T Psi (lower), Psi 65536x1024 FP64 "
@@ -207,7 +226,7 @@ def bench_split(skg, ctx, reps):
     q = skg.DeviceArray(ctx, Ns * Ns)
     rot = skg.DeviceArray(ctx, Nd * Ns)
     ctx.fill_uniform(q.ptr, Ns * Ns, 4, -1.0, 1.0)
-    rplan = ctx.autotune(0, Nd, Ns, Ns, la=skg.COL_MAJOR, lb=skg.ROW_MAJOR)
+    rplan = tuned(ctx, 0, Nd, Ns, Ns, la=skg.COL_MAJOR, lb=skg.ROW_MAJOR)
     t_rot = median_ms(ctx, lambda: ctx.dgemm(
         skg.COL_MAJOR, skg.ROW_MAJOR, skg.COL_MAJOR, Nd, Ns, Ns, 1.0, psi.ptr,
         Nd, q.ptr, Ns, 0.0, rot.ptr, Nd, 0.0), reps)
@@ -241,7 +260,7 @@ def bench_projection(skg, ctx, reps):
                   psi.ptr, Nd, hpsi.ptr, Nd, 0.0, hp.ptr, Ns, 0.0)
     f = 2.0 * Ns * Ns * Nd
     t0 = median_ms(ctx, run, reps)
-    plan = ctx.autotune(0, Ns, Ns, Nd, la=skg.PACK16, lb=skg.PACK32)
+    plan = tuned(ctx, 0, Ns, Ns, Nd, la=skg.PACK16, lb=skg.PACK32)
     t1 = median_ms(ctx, run, reps)
     for x in (psi, hpsi, hp):
         x.free()
@@ -510,6 +529,8 @@ def main():
     rot = skg.DeviceArray(ctx, Nd * Ns, zero=True)
     ctx.fill_normal(psi.ptr, Nd * Ns, 1, 1.0 / np.sqrt(Nd))
     ctx.fill_uniform(q.ptr, Ns * Ns, 2, -1.0, 1.0)
+    global TUNING_CACHE
+    TUNING_CACHE = args.tuning_cache
     plans = {}
     if args.tile >= 0 or args.grid > 0:
         ctx.set_tuning(args.tile, args.grid)
