@@ -7,6 +7,8 @@
 //   sme_syrk<T>             include/sme_syrk.hpp:22-150, syrk_crsp.cpp:51-349
 //   sme_gemm_trp_3d<T>      include/sme_trp_3d.hpp:48-220
 //   sme_pack<T>             include/sme_pack.hpp:30-352
+//   Hamiltonian::nloc_project_vectors_packed(_16to16)
+//                           include/nloc_packed_opt2.hpp:229-655
 //   hbm_init/hbm_alloc/...  include/hbm_alloc.hpp:14-50
 // Only T = double is provided (the reference instantiates double only).
 //
@@ -30,6 +32,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "skgemm.h"
 
@@ -346,3 +349,79 @@ inline void linalg_unpack_r16_f64(const int Nd, const int nb,
                                   const double* src, double* dst) {
   sme_pack<double>::unpack_r16(Nd, nb, src, dst);
 }
+
+// ---- include/nloc_packed_opt2.hpp:229-655 -----------------------------------
+// Hamiltonian::nloc_project_vectors_packed(_16to16) over the caller's own
+// projector structures, read the way the reference reads them (a
+// `nloc_projectors` vector whose elements have nrow, ncol, atom_index,
+// chi.data (column-major nrow x ncol), gamma.data and index[], and an
+// `offsets` vector), with host arrays: psi 16-band-packed
+// [ceil(nb/16)][K][16], result [ceil(nb/T)][K][T], T = 32 or 16.  Each call
+// uploads the projector set and the arrays, runs skgemm_nloc_apply and
+// copies the result back.  The communicator is not used (one GPU holds the
+// whole grid; the reference's MPI_Allreduce over ranks of one has no effect).
+namespace skgemm_facade {
+
+template <typename V>
+inline void nloc_apply_host(double* result, int tile, const double* psi16,
+                            const V& Vnloc, unsigned nb, unsigned K,
+                            double dv) {
+  const size_t np = Vnloc.nloc_projectors.size();
+  if (np == 0 || nb == 0) return;
+  std::vector<int> nrow(np), ncol(np), atom(np), index;
+  std::vector<double> chi, gamma;
+  for (size_t p = 0; p < np; ++p) {
+    const auto& pr = Vnloc.nloc_projectors[p];
+    nrow[p] = (int)pr.nrow;
+    ncol[p] = (int)pr.ncol;
+    atom[p] = (int)pr.atom_index;
+    for (unsigned r = 0; r < pr.nrow; ++r) index.push_back((int)pr.index[r]);
+    chi.insert(chi.end(), pr.chi.data, pr.chi.data + (size_t)pr.nrow * pr.ncol);
+    gamma.insert(gamma.end(), pr.gamma.data, pr.gamma.data + pr.ncol);
+  }
+  skgemm_ctx* c = ctx();
+  skgemm_nloc* h = nullptr;
+  check(skgemm_nloc_create(c, (int)K, (int)np, nrow.data(), ncol.data(),
+                           atom.data(), index.data(), chi.data(), gamma.data(),
+                           &h),
+        "nloc_project_vectors_packed");
+  const size_t n16 = (size_t)((nb + 15) / 16) * K * 16;
+  const size_t nt = (size_t)((nb + tile - 1) / tile) * K * tile;
+  void *dp, *dr;
+  check(skgemm_dev_alloc(c, n16 * 8, 0, &dp), "alloc");
+  check(skgemm_dev_alloc(c, nt * 8, 0, &dr), "alloc");
+  skgemm_memcpy_h2d(c, dp, psi16, n16 * 8);
+  skgemm_memcpy_h2d(c, dr, result, nt * 8);
+  check(skgemm_nloc_apply(h, (int)nb, dv, (const double*)dp, (double*)dr, tile),
+        "nloc_project_vectors_packed");
+  check(skgemm_synchronize(c), "synchronize");
+  skgemm_memcpy_d2h(c, result, dr, nt * 8);
+  skgemm_dev_free(c, dp);
+  skgemm_dev_free(c, dr);
+  skgemm_nloc_destroy(h);
+}
+
+}  // namespace skgemm_facade
+
+namespace Hamiltonian {
+
+template <typename T, typename V, typename Comm>
+void nloc_project_vectors_packed(T* result_32packed, const T* psi_16packed,
+                                 const V& Vnloc, unsigned nb, unsigned K, T dv,
+                                 const Comm&) {
+  static_assert(sizeof(T) == 8, "double only");
+  skgemm_facade::nloc_apply_host(result_32packed, 32, psi_16packed, Vnloc, nb,
+                                 K, dv);
+}
+
+template <typename T, typename V, typename Comm>
+void nloc_project_vectors_packed_16to16(T* result_16packed,
+                                        const T* psi_16packed, const V& Vnloc,
+                                        unsigned nb, unsigned K, T dv,
+                                        const Comm&) {
+  static_assert(sizeof(T) == 8, "double only");
+  skgemm_facade::nloc_apply_host(result_16packed, 16, psi_16packed, Vnloc, nb,
+                                 K, dv);
+}
+
+}  // namespace Hamiltonian

@@ -1,6 +1,7 @@
 // Host program written against the C++ facade exactly as a caller of the
 // reference classes would be (gemm<double>, sme_syrk<double>,
-// sme_gemm_trp_3d<double>, sme_pack<double>, linalg_pack_*): it checks each call against
+// sme_gemm_trp_3d<double>, sme_pack<double>, linalg_pack_*,
+// Hamiltonian::nloc_project_vectors_packed): it checks each call against
 // closed forms or naive loops computed here and exits non-zero on a
 // mismatch.  Built with g++ against libskgemm.so (no HIP headers needed).
 #include <cmath>
@@ -16,6 +17,27 @@ static void expect(bool ok, const char* what, double v) {
   std::printf("%-44s %s (%.3e)\n", what, ok ? "ok" : "FAIL", v);
   if (!ok) ++failures;
 }
+
+// Projector structures shaped like the reference's (only the members the
+// facade reads).
+struct NlocArray2 {
+  double* data = nullptr;
+};
+struct NlocArray1 {
+  double* data = nullptr;
+};
+struct NlocProj {
+  unsigned nrow = 0, ncol = 0;
+  int atom_index = 0;
+  NlocArray2 chi;
+  NlocArray1 gamma;
+  std::vector<unsigned> index;
+  std::vector<double> chi_store, gamma_store;
+};
+struct NlocV {
+  std::vector<NlocProj> nloc_projectors;
+  std::vector<unsigned> offsets;
+};
 
 int main() {
   // gemm_crsp.cpp:451-460 inputs; closed form
@@ -141,6 +163,68 @@ int main() {
                                      rows[(size_t)b * Nd + g]));
     for (size_t e = 0; e < rows.size(); ++e) rt = std::fmax(rt, std::fabs(back[e] - rows[e]));
     expect(rt == 0.0, "linalg_pack/unpack_r16_f64", rt);
+  }
+  // Hamiltonian::nloc_project_vectors_packed on projector structures shaped
+  // like the reference's (nloc_packed_opt2_test.cpp:38-78), against the
+  // unpacked formulation computed here with long double sums.
+  {
+    const unsigned K = 500, nb = 48;
+    NlocV V;
+    std::vector<unsigned> perm(K);
+    for (int p = 0; p < 3; ++p) {
+      NlocProj pr;
+      pr.nrow = 60 + 10 * p;
+      pr.ncol = 4 + p;
+      pr.atom_index = p == 2 ? 2 : 1;  // projectors 0 and 1 share an atom
+      pr.chi_store.resize((size_t)pr.nrow * pr.ncol);
+      pr.gamma_store.resize(pr.ncol);
+      for (size_t e = 0; e < pr.chi_store.size(); ++e)
+        pr.chi_store[e] = std::sin(0.37 * (double)e + p);
+      for (unsigned c = 0; c < pr.ncol; ++c) pr.gamma_store[c] = 0.5 + 0.1 * c;
+      for (unsigned g = 0; g < K; ++g) perm[g] = (g * 37u + 11u * (unsigned)p) % K;
+      pr.index.assign(perm.begin(), perm.begin() + pr.nrow);
+      pr.chi.data = pr.chi_store.data();
+      pr.gamma.data = pr.gamma_store.data();
+      V.nloc_projectors.push_back(pr);
+    }
+    for (auto& pr : V.nloc_projectors) {  // re-point after the vector settled
+      pr.chi.data = pr.chi_store.data();
+      pr.gamma.data = pr.gamma_store.data();
+    }
+    std::vector<double> psi((size_t)K * nb), psi16(((nb + 15) / 16) * K * 16, 0.0),
+        res32(((nb + 31) / 32) * K * 32, 0.0);
+    for (size_t e = 0; e < psi.size(); ++e) psi[e] = std::cos(0.013 * (double)e);
+    for (unsigned b = 0; b < nb; ++b)
+      for (unsigned g = 0; g < K; ++g)
+        psi16[((size_t)(b / 16) * K + g) * 16 + b % 16] = psi[g + (size_t)b * K];
+    Hamiltonian::nloc_project_vectors_packed<double>(res32.data(), psi16.data(),
+                                                     V, nb, K, 0.01, 0);
+    std::vector<long double> want((size_t)K * nb, 0.0L);
+    int atom = V.nloc_projectors[0].atom_index - 1;
+    for (const auto& pr : V.nloc_projectors) {
+      const bool scale = atom != pr.atom_index;
+      atom = pr.atom_index;
+      for (unsigned b = 0; b < nb; ++b) {
+        std::vector<long double> x(pr.ncol, 0.0L);
+        for (unsigned c = 0; c < pr.ncol; ++c) {
+          for (unsigned r = 0; r < pr.nrow; ++r)
+            x[c] += (long double)pr.chi.data[r + (size_t)c * pr.nrow] *
+                    psi[pr.index[r] + (size_t)b * K];
+          x[c] *= 0.01L;
+          if (scale) x[c] *= pr.gamma.data[c];
+        }
+        for (unsigned r = 0; r < pr.nrow; ++r)
+          for (unsigned c = 0; c < pr.ncol; ++c)
+            want[pr.index[r] + (size_t)b * K] +=
+                (long double)pr.chi.data[r + (size_t)c * pr.nrow] * x[c];
+      }
+    }
+    double worst = 0.0;
+    for (unsigned b = 0; b < nb; ++b)
+      for (unsigned g = 0; g < K; ++g)
+        worst = std::fmax(worst, std::fabs(res32[((size_t)(b / 32) * K + g) * 32 + b % 32] -
+                                           (double)want[g + (size_t)b * K]));
+    expect(worst < 1e-12, "Hamiltonian::nloc_project_vectors_packed", worst);
   }
   std::printf("%s\n", failures ? "FAILED" : "all facade checks passed");
   return failures ? 1 : 0;
