@@ -322,6 +322,41 @@ def bench_nloc(skg, ctx, reps):
             "gflops": flop / (ms * 1e-3) / 1e9}
 
 
+def bench_stencil(skg, ctx, reps):
+    """Real-space stencil on the c2 grid: 32^3 points, 520 bands, FDn 6
+    (12th order), periodic halo.  HBM-bound; algorithmic bytes: halo fill
+    reads the local grid and writes the extended one, the stencil reads the
+    extended grid and the diagonal once and writes the 32-band output."""
+    n, nb, F = 32, 520, 6
+    K, K_ex = n ** 3, (n + 2 * F) ** 3
+    t16, t32 = (nb + 15) // 16, (nb + 31) // 32
+    loc = skg.DeviceArray(ctx, t16 * K * 16)
+    ext = skg.DeviceArray(ctx, t16 * K_ex * 16, zero=True)
+    A = skg.DeviceArray(ctx, K)
+    out = skg.DeviceArray(ctx, t32 * K * 32, zero=True)
+    ctx.fill_normal(loc.ptr, t16 * K * 16, 8, 1.0 / np.sqrt(K))
+    ctx.fill_uniform(A.ptr, K, 9, -1.0, 1.0)
+    c = [0, -1.0 / 12, 2.0 / 3, -2.5, 8.0 / 3, -1.0 / 4, 1.0 / 30]
+    th = median_ms(ctx, lambda: ctx.halo_extend(n, n, n, nb, F, loc.ptr,
+                                                ext.ptr, True), reps)
+    ts = median_ms(ctx, lambda: ctx.stencil_apply(n, n, n, nb, F, ext.ptr,
+                                                  A.ptr, c, c, c, 0.5,
+                                                  out.ptr, 32), reps)
+    for x in (loc, ext, A, out):
+        x.free()
+    bh = 8.0 * 16 * t16 * (K + K_ex)
+    bs = 8.0 * (16 * t16 * K_ex + K + nb * K)
+    return {"workload": "stencil H psi, 32^3 grid, 520 bands, FDn 6, "
+                        "16-band-packed extended in, 32-band-packed out",
+            "halo_extend": {"ms": th, "gbps": bh / (th * 1e-3) / 1e9,
+                            "frac_hbm_peak": bh / (th * 1e-3) / 1e9 /
+                            PEAK_HBM_GBS},
+            "stencil": {"ms": ts, "gbps": bs / (ts * 1e-3) / 1e9,
+                        "frac_hbm_peak": bs / (ts * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                        "gflops": (1 + 6 * F) * 2.0 * K * nb / (ts * 1e-3) /
+                        1e9}}
+
+
 def bench_layouts(skg, ctx, reps):
     """Layout conversions of the SYRK / rotation operand Psi, Nd x nb =
     65536 x 1024 FP64 (512 MiB): column-major -> grid-packed (pack_c16),
@@ -505,7 +540,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the c1, split-form, batched and serving "
                          "sections")
-    ap.add_argument("--extra", default="c1,split,projection,nloc,layouts,batched,serving",
+    ap.add_argument("--extra", default="c1,split,projection,nloc,stencil,layouts,batched,serving",
                     help="comma list of extra sections to run")
     ap.add_argument("--tuning-cache", default=None,
                     help="autotuner cache file: read if it exists (no "
@@ -618,6 +653,9 @@ def main():
     if "nloc" in sections:
         log("nloc projectors (c2 grid) ...")
         extra["nloc"] = bench_nloc(skg, ctx, 20)
+    if "stencil" in sections:
+        log("stencil (32^3, 520 bands) ...")
+        extra["stencil"] = bench_stencil(skg, ctx, 10)
     if "layouts" in sections:
         log("layout conversions (65536x1024) ...")
         extra["layouts"] = bench_layouts(skg, ctx, 10)

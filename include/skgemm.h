@@ -24,6 +24,12 @@
  *   Hamiltonian::nloc_project_vectors_packed(_16to16)
  *                                         include/nloc_packed_opt2.hpp:229,453
  *                                                         -> skgemm_nloc_*
+ *   Stencil_method::Packed::calc_laplacian_extended_packed_16to32 / 16to16
+ *                                         include/stencil_extended_packed.hpp:47,276
+ *                                                         -> skgemm_stencil_apply
+ *   copy_interior_to_extended_packed + single-domain halo
+ *                                         include/halo_packed.hpp:86,250
+ *                                                         -> skgemm_halo_extend
  *   hbm_alloc / hbm_calloc / hbm_free     include/hbm_alloc.hpp:21-50
  *                                                         -> skgemm_dev_alloc/free
  *   run_fmopa_loop peak probe             sme_peak_microbench.cpp:16-60
@@ -280,6 +286,38 @@ int skgemm_nloc_create(skgemm_ctx* ctx, int Nd, int num_proj, const int* nrow,
 int skgemm_nloc_apply(skgemm_nloc* h, int nb, double dv, const double* psi16,
                       double* result, int result_tile);
 int skgemm_nloc_destroy(skgemm_nloc* h);
+
+/* ---- real-space stencil (include/stencil_extended_packed.hpp,
+ *      include/halo_packed.hpp) ------------------------------------------- */
+
+/* Local grid: 16-band-packed [ceil(nb/16)][K][16], K = ni nj nk, grid point
+ * g = i + j ni + k ni nj.  Extended grid: the same with
+ * K_ex = (ni + 2 FDn)(nj + 2 FDn)(nk + 2 FDn) and the interior at offset
+ * (FDn, FDn, FDn).  1 <= FDn <= 8. */
+enum {
+  SKGEMM_HALO_ZERO = 0,     /* ghost points are 0 (isolated domain) */
+  SKGEMM_HALO_PERIODIC = 1  /* ghost points wrap around (one periodic domain) */
+};
+/* Interior copy (copy_interior_to_extended_packed, halo_packed.hpp:86-123)
+ * and the halo of one domain: what fill_domain_par_ex_arr_packed
+ * (halo_packed.hpp:250-398) receives from the neighbouring ranks is, for a
+ * single periodic domain, the wrapped interior (every ghost point, corners
+ * included).  Whole 16-band rows move.  Device pointers, asynchronous. */
+int skgemm_halo_extend(skgemm_ctx* ctx, int ni, int nj, int nk, int nb,
+                       int FDn, const double* local16, double* ext16,
+                       int boundary);
+/* H psi on the interior (calc_laplacian_extended_packed_16to32 /
+ * _16to16, stencil_extended_packed.hpp:47-197, :276-402):
+ *   out[g, b] = psi[g, b] (A_diag[g] + coef_0)
+ *             + sum_{r=1..FDn} coef_x[r] (psi[g + r x] + psi[g - r x]) + y + z
+ * read from the extended grid, written band-packed by out_tile (32 or 16),
+ * bands b < nb only.  coef_x/y/z: host arrays of FDn + 1 (index 0 unused);
+ * ext16, A_diag (K) and out: device pointers.  Asynchronous. */
+int skgemm_stencil_apply(skgemm_ctx* ctx, int ni, int nj, int nk, int nb,
+                         int FDn, const double* ext16, const double* A_diag,
+                         const double* coef_x, const double* coef_y,
+                         const double* coef_z, double coef_0, double* out,
+                         int out_tile);
 
 /* ---- tuning ------------------------------------------------------------- */
 

@@ -33,6 +33,11 @@
 //       the reference test's unpacked formulation (nloc_packed_opt2_test.cpp:
 //       84-165) restated in numpy, closed forms (one projector, one column,
 //       identity chi), the gamma-once-per-atom rule, overlapping indices.
+//   oracle_halo_extend / oracle_stencil_apply
+//       the reference test's naive unpacked stencil (stencil_extended_test.cpp:
+//       18-52) restated with numpy slices, polynomial closed forms, the
+//       reference's periodic-halo spot checks (halo_packed_test.cpp:
+//       151-200), and numpy np.pad(mode="wrap") of the local grid.
 // The split form (Ozaki scheme II) is restated separately in oracle/ozaki.py
 // and pinned in tests/test_ozaki_oracle.py.
 
@@ -602,6 +607,79 @@ void oracle_nloc_apply(int K, int nb, double dv, int num_proj,
         result[((long)(b / out_tile) * K + idx[r]) * out_tile + b % out_tile] +=
             temp[r + (size_t)b * nr];
   }
+}
+
+// ---------------------------------------------------------------------------
+// Real-space stencil on the packed extended grid
+// (include/stencil_extended_packed.hpp, include/halo_packed.hpp).
+// Local grid [ceil(nb/16)][K][16], K = ni nj nk, g = i + j ni + k ni nj;
+// extended grid [ceil(nb/16)][K_ex][16], (ni+2F)(nj+2F)(nk+2F), interior at
+// offset (F, F, F).
+// ---------------------------------------------------------------------------
+
+// halo_packed.hpp:86-123 (copy_interior_to_extended_packed), then the halo
+// of a single periodic domain (every ghost point holds the interior point at
+// the wrapped coordinates; the reference's single-rank test fills the x
+// faces this way, halo_packed_test.cpp:84-150) or zero (boundary 0).  Whole
+// 16-band rows move, band lanes past nb included, as in the reference.
+void oracle_halo_extend(int ni, int nj, int nk, int nb, int F,
+                        const double* local16, double* ext16, int periodic) {
+  const long K = (long)ni * nj * nk;
+  const int nie = ni + 2 * F, nje = nj + 2 * F, nke = nk + 2 * F;
+  const long Kex = (long)nie * nje * nke;
+  const int tiles = (nb + 15) / 16;
+  for (int t = 0; t < tiles; ++t)
+    for (int ke = 0; ke < nke; ++ke)
+      for (int je = 0; je < nje; ++je)
+        for (int ie = 0; ie < nie; ++ie) {
+          const long gex = ie + (long)je * nie + (long)ke * nie * nje;
+          const int i = ie - F, j = je - F, k = ke - F;
+          const bool inside = i >= 0 && i < ni && j >= 0 && j < nj && k >= 0 &&
+                              k < nk;
+          for (int bl = 0; bl < 16; ++bl) {
+            double v = 0.0;
+            if (inside || periodic) {
+              const int ii = ((i % ni) + ni) % ni, jj = ((j % nj) + nj) % nj,
+                        kk = ((k % nk) + nk) % nk;
+              const long g = ii + (long)jj * ni + (long)kk * ni * nj;
+              v = local16[(t * K + g) * 16 + bl];
+            }
+            ext16[(t * Kex + gex) * 16 + bl] = v;
+          }
+        }
+}
+
+// stencil_extended_packed.hpp:199-253 (the scalar version, any FDn):
+// out[g, b] = psi_ex[g_ex, b] (A[g] + c0), then + cx[r] (psi_ex[g_ex + r] +
+// psi_ex[g_ex - r]) for r = 1..F, then the y terms (offsets r ni_ex), then
+// the z terms (r ni_ex nj_ex); out packed by out_tile (32: :47-197,
+// 16: :276-), bands b < nb only.
+void oracle_stencil_apply(int ni, int nj, int nk, int nb, int F,
+                          const double* ext16, const double* A_diag,
+                          const double* cx, const double* cy, const double* cz,
+                          double c0, double* out, int out_tile) {
+  const long K = (long)ni * nj * nk;
+  const long nie = ni + 2 * F, nje = nj + 2 * F;
+  const long Kex = nie * nje * (nk + 2 * F);
+  const long sj = nie, sk = nie * nje;
+  for (int k = 0; k < nk; ++k)
+    for (int j = 0; j < nj; ++j)
+      for (int i = 0; i < ni; ++i) {
+        const long g = i + (long)j * ni + (long)k * ni * nj;
+        const long gex = (i + F) + (j + F) * sj + (k + F) * sk;
+        const double a = A_diag[g] + c0;
+        for (int b = 0; b < nb; ++b) {
+          const double* s = ext16 + (long)(b / 16) * Kex * 16 + b % 16;
+          double res = s[gex * 16] * a;
+          for (int r = 1; r <= F; ++r)
+            res += cx[r] * (s[(gex + r) * 16] + s[(gex - r) * 16]);
+          for (int r = 1; r <= F; ++r)
+            res += cy[r] * (s[(gex + r * sj) * 16] + s[(gex - r * sj) * 16]);
+          for (int r = 1; r <= F; ++r)
+            res += cz[r] * (s[(gex + r * sk) * 16] + s[(gex - r * sk) * 16]);
+          out[((long)(b / out_tile) * K + g) * out_tile + b % out_tile] = res;
+        }
+      }
 }
 
 }  // extern "C"

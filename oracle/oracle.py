@@ -52,6 +52,8 @@ def load(path=None):
         "oracle_unpack_c16": [I, I, P, P],
         "oracle_band_to_grid": [I, I, P, P],
         "oracle_nloc_apply": [I, I, D, I, P, P, P, P, P, P, P, P, I],
+        "oracle_halo_extend": [I, I, I, I, I, P, P, I],
+        "oracle_stencil_apply": [I, I, I, I, I, P, P, P, P, P, D, P, I],
         "oracle_grid_to_band": [I, I, P, P],
         "oracle_gemm_crsp_packed": [I, I, I, P, P, P, D, D, I],
         "oracle_gemm_opt_packed": [I, I, I, P, P, P, D, D, I],
@@ -143,6 +145,26 @@ def nloc_apply(K, nb, dv, nrow, ncol, atom_index, index_all, chi_all,
                              pi(atom_index), pi(index_all), _p(chi_all),
                              _p(gamma_all), _p(psi16), _p(result), out_tile)
     return result
+
+
+def halo_extend(ni, nj, nk, nb, F, local16, periodic):
+    t = (nb + 15) // 16
+    out = np.zeros(t * (ni + 2 * F) * (nj + 2 * F) * (nk + 2 * F) * 16)
+    load().oracle_halo_extend(ni, nj, nk, nb, F, _p(local16), _p(out),
+                              1 if periodic else 0)
+    return out
+
+
+def stencil_apply(ni, nj, nk, nb, F, ext16, A_diag, cx, cy, cz, c0, out_tile,
+                  out=None):
+    t = (nb + out_tile - 1) // out_tile
+    if out is None:
+        out = np.zeros(t * ni * nj * nk * out_tile)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    load().oracle_stencil_apply(ni, nj, nk, nb, F, _p(f64(ext16)),
+                                _p(f64(A_diag)), _p(f64(cx)), _p(f64(cy)),
+                                _p(f64(cz)), c0, _p(out), out_tile)
+    return out
 
 
 def band_to_grid(Nd, nb, src):

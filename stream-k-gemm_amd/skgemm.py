@@ -30,6 +30,7 @@ EXPORTS = [
     "skgemm_dgemm_ozaki", "skgemm_dsyrk_ozaki", "skgemm_dgemm_ozaki_host",
     "skgemm_dsyrk_ozaki_host", "skgemm_pack", "skgemm_transpose",
     "skgemm_nloc_create", "skgemm_nloc_apply", "skgemm_nloc_destroy",
+    "skgemm_halo_extend", "skgemm_stencil_apply",
     "skgemm_batched", "skgemm_set_tuning", "skgemm_set_kchunks",
     "skgemm_num_tiles",
     "skgemm_tile_info", "skgemm_autotune", "skgemm_load_tuning",
@@ -119,6 +120,11 @@ def lib():
                                        ctypes.POINTER(c_void_p)]),
         "skgemm_nloc_apply": (c_int, [P, c_int, c_double, P, P, c_int]),
         "skgemm_nloc_destroy": (c_int, [P]),
+        "skgemm_halo_extend": (c_int, [P, c_int, c_int, c_int, c_int, c_int,
+                                       P, P, c_int]),
+        "skgemm_stencil_apply": (c_int, [P, c_int, c_int, c_int, c_int,
+                                         c_int, P, P, P, P, P, c_double, P,
+                                         c_int]),
         "skgemm_batched": (c_int, [P, P, c_int, c_int, c_int, c_int]),
         "skgemm_set_tuning": (c_int, [P, c_int, c_int]),
         "skgemm_set_kchunks": (c_int, [P, c_int]),
@@ -301,6 +307,21 @@ class Context:
              r_ext):
         check(lib().skgemm_pack(self.h, src_layout, _ptr(src), ld_src,
                                 dst_layout, _ptr(dst), ld_dst, o_ext, r_ext))
+
+    def halo_extend(self, ni, nj, nk, nb, F, local16, ext16, periodic=True):
+        check(lib().skgemm_halo_extend(self.h, ni, nj, nk, nb, F,
+                                       _ptr(local16), _ptr(ext16),
+                                       1 if periodic else 0))
+
+    def stencil_apply(self, ni, nj, nk, nb, F, ext16, A_diag, cx, cy, cz, c0,
+                      out, out_tile=32):
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        cx, cy, cz = f(cx), f(cy), f(cz)
+        check(lib().skgemm_stencil_apply(self.h, ni, nj, nk, nb, F,
+                                         _ptr(ext16), _ptr(A_diag),
+                                         cx.ctypes.data, cy.ctypes.data,
+                                         cz.ctypes.data, c0, _ptr(out),
+                                         out_tile))
 
     def transpose(self, src_layout, src, ld_src, dst_layout, dst, ld_dst,
                   o_ext, r_ext):

@@ -167,3 +167,12 @@ def test_nloc_argument_checks_without_device(skg):
                                   None, ctypes.byref(h)) == skg.ERR_STATE
     assert lib.skgemm_nloc_apply(None, 16, 1.0, None, None, 32) == skg.ERR_ARG
     assert lib.skgemm_nloc_destroy(None) == skg.OK
+
+
+def test_stencil_argument_checks_without_device(skg):
+    lib = skg.lib()
+    c = (ctypes.c_double * 7)()
+    assert lib.skgemm_halo_extend(None, 4, 4, 4, 16, 2, None, None,
+                                  1) == skg.ERR_STATE
+    assert lib.skgemm_stencil_apply(None, 4, 4, 4, 16, 2, None, None, c, c, c,
+                                    0.0, None, 32) == skg.ERR_STATE

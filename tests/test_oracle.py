@@ -415,6 +415,116 @@ def test_nloc_planted_mistake_is_caught(orc):
     assert np.max(np.abs(got - bad)) > 1e-3
 
 
+# ---- real-space stencil (include/stencil_extended_packed.hpp, halo_packed.hpp)
+
+# stencil_extended_test.cpp:151-153: 12th-order coefficients used by the test
+STENCIL_COEF = np.array([0, -1.0 / 12, 2.0 / 3, -2.5, 8.0 / 3, -1.0 / 4, 1.0 / 30])
+
+
+def grid_pack16(V, nb):
+    """[nb][nk][nj][ni] -> 16-band-packed [ceil(nb/16)][K][16] (zero pad)."""
+    t = (nb + 15) // 16
+    out = np.zeros((t, V[0].size, 16))
+    for b in range(nb):
+        out[b // 16, :, b % 16] = V[b].ravel()
+    return out.ravel()
+
+
+def grid_unpack(v, nb, K, T):
+    t = (nb + T - 1) // T
+    A = v.reshape(t, K, T)
+    return np.stack([A[b // T, :, b % T] for b in range(nb)])
+
+
+def stencil_numpy(Sx, A, F, cx, cy, cz, c0):
+    """The reference test's naive stencil (stencil_extended_test.cpp:18-52)
+    with numpy slices: Sx [nb][nk+2F][nj+2F][ni+2F] extended, A [nk][nj][ni]."""
+    nb = Sx.shape[0]
+    nk, nj, ni = A.shape
+    C = Sx[:, F:F + nk, F:F + nj, F:F + ni]
+    out = C * (A + c0)
+    for r in range(1, F + 1):
+        out = out + cx[r] * (Sx[:, F:F + nk, F:F + nj, F + r:F + r + ni] +
+                             Sx[:, F:F + nk, F:F + nj, F - r:F - r + ni])
+    for r in range(1, F + 1):
+        out = out + cy[r] * (Sx[:, F:F + nk, F + r:F + r + nj, F:F + ni] +
+                             Sx[:, F:F + nk, F - r:F - r + nj, F:F + ni])
+    for r in range(1, F + 1):
+        out = out + cz[r] * (Sx[:, F + r:F + r + nk, F:F + nj, F:F + ni] +
+                             Sx[:, F - r:F - r + nk, F:F + nj, F:F + ni])
+    return out
+
+
+@pytest.mark.parametrize("shape", [(12, 10, 8, 48, 6), (7, 5, 9, 40, 2),
+                                   (4, 4, 4, 16, 1)])
+@pytest.mark.parametrize("tile", [32, 16])
+def test_stencil_matches_unpacked_formulation(orc, rng, shape, tile):
+    ni, nj, nk, nb, F = shape
+    Sx = rng.uniform(-1, 1, (nb, nk + 2 * F, nj + 2 * F, ni + 2 * F))
+    A = rng.uniform(-1, 1, (nk, nj, ni))
+    c = STENCIL_COEF if F == 6 else rng.uniform(-1, 1, F + 1)
+    ext = grid_pack16(Sx, nb)
+    out = orc.stencil_apply(ni, nj, nk, nb, F, ext, A.ravel(), c, c * 0.5,
+                            c * 0.25, 1.5, tile)
+    want = stencil_numpy(Sx, A, F, c, c * 0.5, c * 0.25, 1.5)
+    got = grid_unpack(out, nb, ni * nj * nk, tile)
+    assert np.max(np.abs(got - want.reshape(nb, -1))) < 1e-12
+
+
+def test_stencil_closed_forms(orc):
+    """Constant input: (A + c0 + 2 sum(cx + cy + cz)) psi; input linear in i:
+    the symmetric x terms give 2 i sum(cx); all exact in binary fractions."""
+    ni, nj, nk, nb, F = 6, 5, 4, 16, 3
+    c = np.array([0, 0.5, -0.25, 0.125])
+    A = np.full((nk, nj, ni), 2.0)
+    Sx = np.full((nb, nk + 2 * F, nj + 2 * F, ni + 2 * F), 3.0)
+    out = grid_unpack(orc.stencil_apply(ni, nj, nk, nb, F, grid_pack16(Sx, nb),
+                                        A.ravel(), c, c, c, 1.0, 16),
+                      nb, ni * nj * nk, 16)
+    assert np.all(out == 3.0 * (2.0 + 1.0 + 3 * 2 * c.sum()))
+    ie = np.arange(ni + 2 * F, dtype=np.float64)
+    Sx = np.broadcast_to(ie, (nb, nk + 2 * F, nj + 2 * F, ni + 2 * F)).copy()
+    out = grid_unpack(orc.stencil_apply(ni, nj, nk, nb, F, grid_pack16(Sx, nb),
+                                        A.ravel(), c, 0 * c, 0 * c, 0.0, 32),
+                      nb, ni * nj * nk, 32).reshape(nb, nk, nj, ni)
+    i_ex = np.arange(ni) + F
+    assert np.all(out == (i_ex * 2.0 + 2 * i_ex * c.sum())[None, None, None, :])
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_halo_extend(orc, rng, periodic):
+    """Interior copy (halo_packed.hpp:86-123) and the single-domain halo:
+    np.pad(mode='wrap') for a periodic domain, zeros otherwise; the
+    reference's spot checks (halo_packed_test.cpp:177-196)."""
+    ni, nj, nk, nb, F = 6, 5, 4, 20, 2
+    L = rng.uniform(-1, 1, (nb, nk, nj, ni))
+    ext = orc.halo_extend(ni, nj, nk, nb, F, grid_pack16(L, nb), periodic)
+    mode = "wrap" if periodic else "constant"
+    want = np.pad(L, ((0, 0), (F, F), (F, F), (F, F)), mode=mode)
+    K_ex = (ni + 2 * F) * (nj + 2 * F) * (nk + 2 * F)
+    assert np.array_equal(grid_unpack(ext, nb, K_ex, 16), want.reshape(nb, -1))
+    if periodic:
+        nie, nje = ni + 2 * F, nj + 2 * F
+        e = ext.reshape(-1, K_ex, 16)
+        assert e[0, F * nie + F * nie * nje, 0] == L[0, 0, 0, ni - F]
+        assert e[0, (ni + F) + F * nie + F * nie * nje, 0] == L[0, 0, 0, 0]
+
+
+def test_stencil_planted_mistake_is_caught(orc, rng):
+    """Dropping the outermost x term (r = F) is seen."""
+    ni, nj, nk, nb, F = 8, 6, 5, 16, 3
+    Sx = rng.uniform(-1, 1, (nb, nk + 2 * F, nj + 2 * F, ni + 2 * F))
+    A = rng.uniform(-1, 1, (nk, nj, ni))
+    c = rng.uniform(-1, 1, F + 1)
+    got = grid_unpack(orc.stencil_apply(ni, nj, nk, nb, F, grid_pack16(Sx, nb),
+                                        A.ravel(), c, c, c, 0.0, 16),
+                      nb, ni * nj * nk, 16)
+    bad_c = c.copy()
+    bad_c[F] = 0.0
+    bad = stencil_numpy(Sx, A, F, bad_c, c, c, 0.0).reshape(nb, -1)
+    assert np.max(np.abs(got - bad)) > 1e-3
+
+
 # ---- planted mistakes are caught -----------------------------------------------
 
 def test_oracle_mutations_are_detected(orc, rng):
