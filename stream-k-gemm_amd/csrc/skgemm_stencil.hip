@@ -29,8 +29,14 @@ struct StencilCoef {
   double c0;
 };
 
-// One workgroup: 16 band lanes x 16 consecutive i of one (j, k) row of one
-// 16-band tile; grid (ceil(ni/16), nj, nk * tiles16).
+// One workgroup: 16 band lanes x 16 consecutive i of one j row of one
+// 16-band tile, marching through kStencilPlanes planes of k; grid
+// (ceil(ni/16), nj, tiles16 * ceil(nk / kStencilPlanes)).  Each lane keeps
+// its own column's 2F + 1 values along k in registers, so a plane step loads
+// one new z value and the 4F x / y neighbours (lines that the lanes and the
+// neighbouring rows' workgroups of the same plane share in L1 / L2), not 6F.
+constexpr int kStencilPlanes = 8;
+
 template <int F>
 __global__ __launch_bounds__(256) void stencil_kernel(
     const double* __restrict__ ext, const double* __restrict__ A_diag, int ni,
@@ -39,31 +45,44 @@ __global__ __launch_bounds__(256) void stencil_kernel(
   const int bl = threadIdx.x & 15;
   const int i = blockIdx.x * 16 + (threadIdx.x >> 4);
   const int j = blockIdx.y;
-  const int k = blockIdx.z % nk;
-  const int t16 = blockIdx.z / nk;
+  const int kchunks = (nk + kStencilPlanes - 1) / kStencilPlanes;
+  const int t16 = blockIdx.z / kchunks;
+  const int k0 = (blockIdx.z % kchunks) * kStencilPlanes;
+  const int k1 = k0 + kStencilPlanes < nk ? k0 + kStencilPlanes : nk;
   const int b = t16 * 16 + bl;
   if (i >= ni || b >= nb) return;
   const int64_t K = (int64_t)ni * nj * nk;
   const int64_t sj = ni + 2 * F;
   const int64_t sk = sj * (nj + 2 * F);
   const int64_t Kex = sk * (nk + 2 * F);
-  const int64_t g = i + (int64_t)j * ni + (int64_t)k * ni * nj;
-  const int64_t gex = (i + F) + (j + F) * sj + (k + F) * sk;
   const double* s = ext + (int64_t)t16 * Kex * 16 + bl;
-  // Same order as the reference's scalar version: centre, x terms, y terms,
-  // z terms, r ascending in each (each term one fused multiply-add).
-  double res = s[gex * 16] * (A_diag[g] + c.c0);
-#pragma unroll
-  for (int r = 1; r <= F; ++r)
-    res = fma(c.x[r], s[(gex + r) * 16] + s[(gex - r) * 16], res);
-#pragma unroll
-  for (int r = 1; r <= F; ++r)
-    res = fma(c.y[r], s[(gex + r * sj) * 16] + s[(gex - r * sj) * 16], res);
-#pragma unroll
-  for (int r = 1; r <= F; ++r)
-    res = fma(c.z[r], s[(gex + r * sk) * 16] + s[(gex - r * sk) * 16], res);
   const int T = 1 << tshift;
-  out[((int64_t)(b >> tshift) * K + g) * T + (b & (T - 1))] = res;
+  double* o = out + ((int64_t)(b >> tshift) * K) * T + (b & (T - 1));
+  // Column (i, j): extended index of plane k is col + (k + F) sk.
+  const int64_t col = (i + F) + (j + F) * sj;
+  double q[2 * F + 1];  // q[F + d] = psi at plane k + d
+#pragma unroll
+  for (int d = 0; d < 2 * F; ++d) q[d] = s[(col + (k0 + d) * sk) * 16];
+  for (int k = k0; k < k1; ++k) {
+    q[2 * F] = s[(col + (k + 2 * F) * sk) * 16];
+    const int64_t gex = col + (k + F) * sk;
+    const int64_t g = i + (int64_t)j * ni + (int64_t)k * ni * nj;
+    // Same order as the reference's scalar version: centre, x terms, y
+    // terms, z terms, r ascending in each (each term one fused
+    // multiply-add).
+    double res = q[F] * (A_diag[g] + c.c0);
+#pragma unroll
+    for (int r = 1; r <= F; ++r)
+      res = fma(c.x[r], s[(gex + r) * 16] + s[(gex - r) * 16], res);
+#pragma unroll
+    for (int r = 1; r <= F; ++r)
+      res = fma(c.y[r], s[(gex + r * sj) * 16] + s[(gex - r * sj) * 16], res);
+#pragma unroll
+    for (int r = 1; r <= F; ++r) res = fma(c.z[r], q[F + r] + q[F - r], res);
+    o[g * T] = res;
+#pragma unroll
+    for (int d = 0; d < 2 * F; ++d) q[d] = q[d + 1];
+  }
 }
 
 // Every extended point of every tile: the interior copies the local point,
@@ -165,7 +184,8 @@ extern "C" int skgemm_stencil_apply(skgemm_ctx* c, int ni, int nj, int nk,
   }
   cf.c0 = coef_0;
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
-  const dim3 grid((ni + 15) / 16, nj, nk * ((nb + 15) / 16));
+  const int kchunks = (nk + kStencilPlanes - 1) / kStencilPlanes;
+  const dim3 grid((ni + 15) / 16, nj, kchunks * ((nb + 15) / 16));
   const int ts = out_tile == 32 ? 5 : 4;
   c->record_begin();
 #define SKG_STENCIL(F)                                                     \
