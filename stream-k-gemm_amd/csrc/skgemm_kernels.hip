@@ -217,6 +217,17 @@ __device__ __forceinline__ void tri_tile(int64_t t, int& ti, int& tj) {
   tj = (int)(t - i * (i + 1) / 2);
 }
 
+// Wave w -> sub-tile (wm, wn).  Hardware places wave w on SIMD w % 4.
+// wn = w / WM keeps the WM waves of one sub-tile column on distinct SIMDs;
+// the rotation by wn (WM / 2) puts the WN waves of one sub-tile row on
+// distinct SIMDs too.  An edge tile with a single valid row or column of
+// sub-tiles then still spreads its MFMA work over the SIMDs.
+template <int WM>
+__device__ __forceinline__ void wave_subtile(int wave, int& wm, int& wn) {
+  wn = wave / WM;
+  wm = (wave + wn * (WM / 2)) % WM;
+}
+
 template <int BM, int BN, int BK, int WM, int WN>
 struct TileShape {
   static constexpr int NT = WM * WN * 64;
@@ -275,7 +286,8 @@ __device__ __forceinline__ void mac_loop(
   using S = TileShape<BM, BN, BK, WM, WN>;
   constexpr int NT = S::NT;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wm = wave % WM, wn = wave / WM;
+  int wm, wn;
+  wave_subtile<WM>(wave, wm, wn);
   const int lq = lane >> 4, lr = lane & 15;
   using SA = Stage<BM, BK, NT, AKC, VEC>;
   using SB = Stage<BN, BK, NT, BKC, VEC>;
@@ -477,7 +489,8 @@ __device__ __forceinline__ void mac_loop_dma(
   constexpr int kWait = (kAhead & 15) | ((kAhead >> 4) << 14) | (7 << 4);
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wm = wave % WM, wn = wave / WM;
+  int wm, wn;
+  wave_subtile<WM>(wave, wm, wn);
   const int lq = lane >> 4, lr = lane & 15;
 
   // Per-instruction byte offset of this lane's chunk from the tile origin,
@@ -590,7 +603,8 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wm = wave % WM, wn = wave / WM;
+  int wm, wn;
+  wave_subtile<WM>(wave, wm, wn);
   const int lq = lane >> 4, lr = lane & 15;
 
   // XCD-aware logical workgroup id: hardware places workgroup g on XCD g % 8;
