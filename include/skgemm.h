@@ -296,7 +296,9 @@ int skgemm_nloc_destroy(skgemm_nloc* h);
  * (FDn, FDn, FDn).  1 <= FDn <= 8. */
 enum {
   SKGEMM_HALO_ZERO = 0,     /* ghost points are 0 (isolated domain) */
-  SKGEMM_HALO_PERIODIC = 1  /* ghost points wrap around (one periodic domain) */
+  SKGEMM_HALO_PERIODIC = 1, /* ghost points wrap around (one periodic domain) */
+  SKGEMM_HALO_KEEP = 2      /* interior copy only, ghost points untouched
+                               (copy_interior_to_extended_packed) */
 };
 /* Interior copy (copy_interior_to_extended_packed, halo_packed.hpp:86-123)
  * and the halo of one domain: what fill_domain_par_ex_arr_packed

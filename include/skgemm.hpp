@@ -9,6 +9,8 @@
 //   sme_pack<T>             include/sme_pack.hpp:30-352
 //   Hamiltonian::nloc_project_vectors_packed(_16to16)
 //                           include/nloc_packed_opt2.hpp:229-655
+//   Stencil_method::Packed  include/stencil_extended_packed.hpp:47-402,
+//                           include/halo_packed.hpp:86-123
 //   hbm_init/hbm_alloc/...  include/hbm_alloc.hpp:14-50
 // Only T = double is provided (the reference instantiates double only).
 //
@@ -425,3 +427,98 @@ void nloc_project_vectors_packed_16to16(T* result_16packed,
 }
 
 }  // namespace Hamiltonian
+
+// ---- include/stencil_extended_packed.hpp, include/halo_packed.hpp ----------
+// The reference's stencil and interior copy with host arrays: extended
+// psi [ceil(nb/16)][K_ex][16], output [ceil(nb/T)][K][T], A_diag [K].  Each
+// call copies its operands to the GPU and the result back (elements the
+// kernel does not write keep their values).
+namespace skgemm_facade {
+
+inline void stencil_host(const double* ext, double* out, int tile,
+                         const double* A, unsigned ni, unsigned nj,
+                         unsigned nk, unsigned nb, int FDn, const double* cx,
+                         const double* cy, const double* cz, double c0) {
+  if (nb == 0) return;
+  const size_t K = (size_t)ni * nj * nk;
+  const size_t Kex = (size_t)(ni + 2 * FDn) * (nj + 2 * FDn) * (nk + 2 * FDn);
+  const size_t ne = (size_t)((nb + 15) / 16) * Kex * 16;
+  const size_t no = (size_t)((nb + tile - 1) / tile) * K * tile;
+  skgemm_ctx* c = ctx();
+  void *de, *da, *dout;
+  check(skgemm_dev_alloc(c, ne * 8, 0, &de), "alloc");
+  check(skgemm_dev_alloc(c, K * 8, 0, &da), "alloc");
+  check(skgemm_dev_alloc(c, no * 8, 0, &dout), "alloc");
+  skgemm_memcpy_h2d(c, de, ext, ne * 8);
+  skgemm_memcpy_h2d(c, da, A, K * 8);
+  skgemm_memcpy_h2d(c, dout, out, no * 8);
+  check(skgemm_stencil_apply(c, (int)ni, (int)nj, (int)nk, (int)nb, FDn,
+                             (const double*)de, (const double*)da, cx, cy, cz,
+                             c0, (double*)dout, tile),
+        "calc_laplacian_extended_packed");
+  check(skgemm_synchronize(c), "synchronize");
+  skgemm_memcpy_d2h(c, out, dout, no * 8);
+  skgemm_dev_free(c, de);
+  skgemm_dev_free(c, da);
+  skgemm_dev_free(c, dout);
+}
+
+}  // namespace skgemm_facade
+
+namespace Stencil_method {
+namespace Packed {
+
+// include/stencil_extended_packed.hpp:47-197 (any FDn 1..8 here).
+inline void calc_laplacian_extended_packed_16to32(
+    const double* extended_packed, double* output_32packed,
+    const double* A_diag_packed, unsigned ni, unsigned nj, unsigned nk,
+    unsigned nb, int FDn, const double* coef_x, const double* coef_y,
+    const double* coef_z, double coef_0) {
+  skgemm_facade::stencil_host(extended_packed, output_32packed, 32,
+                              A_diag_packed, ni, nj, nk, nb, FDn, coef_x,
+                              coef_y, coef_z, coef_0);
+}
+// include/stencil_extended_packed.hpp:199-253: the same operator.
+inline void calc_laplacian_extended_packed_16to32_scalar(
+    const double* extended_packed, double* output_32packed,
+    const double* A_diag_packed, unsigned ni, unsigned nj, unsigned nk,
+    unsigned nb, int FDn, const double* coef_x, const double* coef_y,
+    const double* coef_z, double coef_0) {
+  calc_laplacian_extended_packed_16to32(extended_packed, output_32packed,
+                                        A_diag_packed, ni, nj, nk, nb, FDn,
+                                        coef_x, coef_y, coef_z, coef_0);
+}
+// include/stencil_extended_packed.hpp:276-402.
+inline void calc_laplacian_extended_packed_16to16(
+    const double* extended_packed, double* output_16packed,
+    const double* A_diag_packed, unsigned ni, unsigned nj, unsigned nk,
+    unsigned nb, int FDn, const double* coef_x, const double* coef_y,
+    const double* coef_z, double coef_0) {
+  skgemm_facade::stencil_host(extended_packed, output_16packed, 16,
+                              A_diag_packed, ni, nj, nk, nb, FDn, coef_x,
+                              coef_y, coef_z, coef_0);
+}
+
+// include/halo_packed.hpp:86-123: interior copy, ghost points untouched.
+template <typename T>
+void copy_interior_to_extended_packed(const T* local_packed,
+                                      T* extended_packed, unsigned ni,
+                                      unsigned nj, unsigned nk, unsigned nb,
+                                      int FDn) {
+  static_assert(sizeof(T) == 8, "double only");
+  if (nb == 0) return;
+  const size_t K = (size_t)ni * nj * nk;
+  const size_t Kex = (size_t)(ni + 2 * FDn) * (nj + 2 * FDn) * (nk + 2 * FDn);
+  const size_t t = (nb + 15) / 16;
+  skgemm_facade::convert_host(
+      (const double*)local_packed, t * K * 16, (double*)extended_packed,
+      t * Kex * 16,
+      [&](skgemm_ctx* c, const double* s, double* d) {
+        return skgemm_halo_extend(c, (int)ni, (int)nj, (int)nk, (int)nb, FDn,
+                                  s, d, SKGEMM_HALO_KEEP);
+      },
+      "copy_interior_to_extended_packed");
+}
+
+}  // namespace Packed
+}  // namespace Stencil_method

@@ -86,11 +86,12 @@ __global__ __launch_bounds__(256) void stencil_kernel(
 }
 
 // Every extended point of every tile: the interior copies the local point,
-// the halo the wrapped one (periodic) or zero.  Grid (ceil(ni_ex/16),
-// nj_ex, nk_ex * tiles16), 16 band lanes x 16 points along i.
+// the halo the wrapped one (mode 1), zero (mode 0) or is left alone (mode
+// 2).  Grid (ceil(ni_ex/16), nj_ex, nk_ex * tiles16), 16 band lanes x 16
+// points along i.
 __global__ __launch_bounds__(256) void halo_extend_kernel(
-    const double* __restrict__ local, int ni, int nj, int nk, int F,
-    int periodic, double* __restrict__ ext) {
+    const double* __restrict__ local, int ni, int nj, int nk, int F, int mode,
+    double* __restrict__ ext) {
   const int bl = threadIdx.x & 15;
   const int nie = ni + 2 * F, nje = nj + 2 * F, nke = nk + 2 * F;
   const int ie = blockIdx.x * 16 + (threadIdx.x >> 4);
@@ -103,8 +104,9 @@ __global__ __launch_bounds__(256) void halo_extend_kernel(
   const int i = ie - F, j = je - F, k = ke - F;
   const bool inside =
       i >= 0 && i < ni && j >= 0 && j < nj && k >= 0 && k < nk;
+  if (!inside && mode == 2) return;
   double v = 0.0;
-  if (inside || periodic) {
+  if (inside || mode == 1) {
     const int ii = ((i % ni) + ni) % ni, jj = ((j % nj) + nj) % nj,
               kk = ((k % nk) + nk) % nk;
     const int64_t g = ii + (int64_t)jj * ni + (int64_t)kk * ni * nj;
@@ -138,7 +140,8 @@ extern "C" int skgemm_halo_extend(skgemm_ctx* c, int ni, int nj, int nk,
                                   double* ext16, int boundary) {
   SKG_TRY(check_ctx(c));
   SKG_TRY(check_grid(ni, nj, nk, nb, FDn));
-  if (boundary != SKGEMM_HALO_ZERO && boundary != SKGEMM_HALO_PERIODIC)
+  if (boundary != SKGEMM_HALO_ZERO && boundary != SKGEMM_HALO_PERIODIC &&
+      boundary != SKGEMM_HALO_KEEP)
     return set_error(SKGEMM_ERR_ARG, "unknown halo boundary %d", boundary);
   if (nb == 0) return SKGEMM_OK;
   if (local16 == nullptr || ext16 == nullptr)
@@ -150,8 +153,7 @@ extern "C" int skgemm_halo_extend(skgemm_ctx* c, int ni, int nj, int nk,
   const dim3 grid((nie + 15) / 16, nje, nke * ((nb + 15) / 16));
   c->record_begin();
   hipLaunchKernelGGL(halo_extend_kernel, grid, dim3(256), 0, c->stream,
-                     local16, ni, nj, nk, FDn,
-                     boundary == SKGEMM_HALO_PERIODIC ? 1 : 0, ext16);
+                     local16, ni, nj, nk, FDn, boundary, ext16);
   SKG_HIP(hipGetLastError(), "halo launch");
   c->record_end();
   return SKGEMM_OK;

@@ -308,10 +308,14 @@ class Context:
         check(lib().skgemm_pack(self.h, src_layout, _ptr(src), ld_src,
                                 dst_layout, _ptr(dst), ld_dst, o_ext, r_ext))
 
-    def halo_extend(self, ni, nj, nk, nb, F, local16, ext16, periodic=True):
+    def halo_extend(self, ni, nj, nk, nb, F, local16, ext16, periodic=True,
+                    boundary=None):
+        """boundary: 0 zero, 1 periodic, 2 interior only (default from
+        `periodic`)."""
+        if boundary is None:
+            boundary = 1 if periodic else 0
         check(lib().skgemm_halo_extend(self.h, ni, nj, nk, nb, F,
-                                       _ptr(local16), _ptr(ext16),
-                                       1 if periodic else 0))
+                                       _ptr(local16), _ptr(ext16), boundary))
 
     def stencil_apply(self, ni, nj, nk, nb, F, ext16, A_diag, cx, cy, cz, c0,
                       out, out_tile=32):

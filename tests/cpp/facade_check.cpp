@@ -1,7 +1,8 @@
 // Host program written against the C++ facade exactly as a caller of the
 // reference classes would be (gemm<double>, sme_syrk<double>,
 // sme_gemm_trp_3d<double>, sme_pack<double>, linalg_pack_*,
-// Hamiltonian::nloc_project_vectors_packed): it checks each call against
+// Hamiltonian::nloc_project_vectors_packed, Stencil_method::Packed): it checks
+// each call against
 // closed forms or naive loops computed here and exits non-zero on a
 // mismatch.  Built with g++ against libskgemm.so (no HIP headers needed).
 #include <cmath>
@@ -225,6 +226,61 @@ int main() {
         worst = std::fmax(worst, std::fabs(res32[((size_t)(b / 32) * K + g) * 32 + b % 32] -
                                            (double)want[g + (size_t)b * K]));
     expect(worst < 1e-12, "Hamiltonian::nloc_project_vectors_packed", worst);
+  }
+  // Stencil_method::Packed: interior copy plus the 12th-order stencil of
+  // stencil_extended_test.cpp:151-153 against a naive loop on the extended
+  // grid (ghost points set here).
+  {
+    const unsigned ni = 12, nj = 10, nk = 8, nb = 48;
+    const int F = 6;
+    const unsigned nie = ni + 2 * F, nje = nj + 2 * F, nke = nk + 2 * F;
+    const size_t K = (size_t)ni * nj * nk, Kex = (size_t)nie * nje * nke;
+    std::vector<double> local(3 * K * 16), ext(3 * Kex * 16), A(K),
+        out(2 * K * 32, 0.0);
+    for (size_t e = 0; e < local.size(); ++e) local[e] = std::sin(0.011 * (double)e);
+    for (size_t e = 0; e < ext.size(); ++e) ext[e] = std::cos(0.007 * (double)e);
+    for (size_t g = 0; g < K; ++g) A[g] = 0.1 * (double)(g % 7);
+    Stencil_method::Packed::copy_interior_to_extended_packed(local.data(), ext.data(),
+                                                             ni, nj, nk, nb, F);
+    double bad = 0.0;
+    for (unsigned t = 0; t < 3; ++t)
+      for (unsigned k = 0; k < nke; ++k)
+        for (unsigned j = 0; j < nje; ++j)
+          for (unsigned i = 0; i < nie; ++i) {
+            const size_t ge = i + (size_t)j * nie + (size_t)k * nie * nje;
+            const bool in = i >= (unsigned)F && i < ni + F && j >= (unsigned)F &&
+                            j < nj + F && k >= (unsigned)F && k < nk + F;
+            for (unsigned bl = 0; bl < 16; ++bl) {
+              const double got = ext[(t * Kex + ge) * 16 + bl];
+              const double want =
+                  in ? local[(t * K + (i - F) + (size_t)(j - F) * ni +
+                              (size_t)(k - F) * ni * nj) * 16 + bl]
+                     : std::cos(0.007 * (double)((t * Kex + ge) * 16 + bl));
+              bad = std::fmax(bad, std::fabs(got - want));
+            }
+          }
+    expect(bad == 0.0, "Stencil_method::Packed::copy_interior_to_extended_packed", bad);
+    const double c[7] = {0, -1.0 / 12, 2.0 / 3, -2.5, 8.0 / 3, -1.0 / 4, 1.0 / 30};
+    Stencil_method::Packed::calc_laplacian_extended_packed_16to32(
+        ext.data(), out.data(), A.data(), ni, nj, nk, nb, F, c, c, c, 0.5);
+    double worst = 0.0;
+    for (unsigned b = 0; b < nb; ++b)
+      for (unsigned k = 0; k < nk; ++k)
+        for (unsigned j = 0; j < nj; ++j)
+          for (unsigned i = 0; i < ni; ++i) {
+            const size_t g = i + (size_t)j * ni + (size_t)k * ni * nj;
+            const size_t ge = (i + F) + (size_t)(j + F) * nie + (size_t)(k + F) * nie * nje;
+            auto s = [&](size_t x) { return ext[((size_t)(b / 16) * Kex + x) * 16 + b % 16]; };
+            long double r = (long double)s(ge) * (A[g] + 0.5);
+            for (int q = 1; q <= F; ++q)
+              r += c[q] * ((long double)s(ge + q) + s(ge - q) + s(ge + q * nie) +
+                           s(ge - q * nie) + s(ge + (size_t)q * nie * nje) +
+                           s(ge - (size_t)q * nie * nje));
+            worst = std::fmax(worst, std::fabs(out[((size_t)(b / 32) * K + g) * 32 + b % 32] -
+                                               (double)r));
+          }
+    expect(worst < 1e-12, "Stencil_method::Packed::calc_laplacian_extended_packed_16to32",
+           worst);
   }
   std::printf("%s\n", failures ? "FAILED" : "all facade checks passed");
   return failures ? 1 : 0;

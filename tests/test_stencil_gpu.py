@@ -78,3 +78,23 @@ def test_argument_errors(skg, ctx):
     with pytest.raises(skg.SkgemmError) as e:       # host memory
         ctx.halo_extend(4, 4, 4, 16, 1, np.zeros(1024).ctypes.data, b.ptr)
     assert e.value.code == skg.ERR_ARG
+
+
+def test_interior_copy_leaves_ghosts(skg, ctx, orc, rng):
+    """Boundary 2 (copy_interior_to_extended_packed, halo_packed.hpp:86-123):
+    the interior is copied, every ghost point keeps its previous value."""
+    ni, nj, nk, nb, F = 6, 5, 4, 20, 2
+    local = grid_pack16(rng.uniform(-1, 1, (nb, nk, nj, ni)), nb)
+    K_ex = (ni + 2 * F) * (nj + 2 * F) * (nk + 2 * F)
+    before = np.full(((nb + 15) // 16) * K_ex * 16, 7.0)
+    d_loc = skg.DeviceArray.from_host(ctx, local)
+    d_ext = skg.DeviceArray.from_host(ctx, before)
+    ctx.halo_extend(ni, nj, nk, nb, F, d_loc.ptr, d_ext.ptr, boundary=2)
+    ctx.synchronize()
+    got = d_ext.download()
+    zero = orc.halo_extend(ni, nj, nk, nb, F, local, False)
+    e = np.zeros((nk + 2 * F, nj + 2 * F, ni + 2 * F), bool)
+    e[F:F + nk, F:F + nj, F:F + ni] = True
+    inside = np.repeat(np.tile(e.ravel(), (nb + 15) // 16), 16)
+    assert np.array_equal(got[inside], zero[inside])
+    assert np.all(got[~inside] == 7.0)
