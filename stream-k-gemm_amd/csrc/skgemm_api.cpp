@@ -481,7 +481,10 @@ int64_t tiles_for(int kind, int cfg, int M, int N) {
     const int64_t T = (N + t.bm - 1) / t.bm;
     return T * (T + 1) / 2;
   }
-  return (int64_t)((M + t.bm - 1) / t.bm) * ((N + t.bn - 1) / t.bn);
+  const int tn = strip_cols(cfg, kind, N) ? N / t.bn : (N + t.bn - 1) / t.bn;
+  const int tm =
+      strip_rows(cfg, kind, M, N) ? M / t.bm : (M + t.bm - 1) / t.bm;
+  return (int64_t)tm * tn;
 }
 
 static int64_t iters_per_tile(int cfg, int K) {
@@ -569,8 +572,10 @@ int fill_problem(skgemm_ctx* c, Problem* p, int kind, const Plan& pl, int M,
   p->N = N;
   p->K = K;
   p->syrk = kind == 1;
-  p->tiles_m = (M + t.bm - 1) / t.bm;
-  p->tiles_n = (N + t.bn - 1) / t.bn;
+  p->xcol = strip_cols(pl.cfg, kind, N);
+  p->xrow = strip_rows(pl.cfg, kind, M, N);
+  p->tiles_m = p->xrow ? M / t.bm : (M + t.bm - 1) / t.bm;
+  p->tiles_n = p->xcol ? N / t.bn : (N + t.bn - 1) / t.bn;
   p->iters_per_tile = (int)iters_per_tile(pl.cfg, K);
   // Tile order: keep the smaller operand cached and read the larger once.
   p->row_order = (kind == 0 && (int64_t)M >= (int64_t)N) ? 1 : 0;
@@ -594,6 +599,12 @@ int run_single(skgemm_ctx* c, Problem& p, const Plan& plan_in,
     // register-staged tile of the same shape (same tiles and units).
     pl.cfg = register_sibling(pl.cfg);
     pl.grid = std::max(1, std::min(pl.grid, c->capacity(pl.cfg)));
+    if (p.xcol || p.xrow) {  // no strips there: plain tiling
+      p.xcol = p.xrow = 0;
+      p.tiles_m = (p.M + tile_cfg(pl.cfg).bm - 1) / tile_cfg(pl.cfg).bm;
+      p.tiles_n = (p.N + tile_cfg(pl.cfg).bn - 1) / tile_cfg(pl.cfg).bn;
+      p.num_tiles = (int64_t)p.tiles_m * p.tiles_n;
+    }
   }
   const TileCfg& t = tile_cfg(pl.cfg);
   p.pred = pred;
@@ -605,7 +616,7 @@ int run_single(skgemm_ctx* c, Problem& p, const Plan& plan_in,
   L.num_problems = 1;
   L.total_iters = p.num_tiles * p.kchunks * p.kci;
   L.grid = pl.grid;
-  SKG_TRY(c->ensure_ws((size_t)pl.grid * t.bm * t.bn, pl.grid));
+  SKG_TRY(c->ensure_ws((size_t)pl.grid * slot_doubles(pl.cfg), pl.grid));
   L.ws = c->ws;
   L.flags = c->flags;
   L.status = c->status;

@@ -127,8 +127,10 @@ int describe(const skgemm_problem& q, Problem* p, int* Ml, int* Nl,
 // The 128 x 128 tiles run their LDS-DMA variant (cfg 8) when every operand
 // of the batch has 16-byte aligned pairs (checked by run_plain).
 int batch_cfg(skgemm_ctx* c, const skgemm_problem* q, int n) {
+  // Strip tiles run single problems only: a forced strip configuration
+  // batches with its twin.
   if (c->forced_cfg >= 0 && tile_cfg(c->forced_cfg).bm == tile_cfg(c->forced_cfg).bn)
-    return c->forced_cfg;
+    return strip_twin(c->forced_cfg);
   int big = 0;
   for (int i = 0; i < n; ++i) big = std::max(big, std::min(q[i].M, q[i].N));
   return big >= 256 ? 8 : 1;
@@ -187,8 +189,7 @@ int build_plain_table(skgemm_ctx* c, const skgemm_problem* q, int n,
 int launch_plain_table(skgemm_ctx* c, const PlainTable& tb,
                        const Problem* dtab, const int64_t* dhdr) {
   if (tb.rows.empty()) return SKGEMM_OK;
-  const TileCfg& t = tile_cfg(tb.cfg);
-  SKG_TRY(c->ensure_ws((size_t)tb.grid * t.bm * t.bn, tb.grid));
+  SKG_TRY(c->ensure_ws((size_t)tb.grid * slot_doubles(tb.cfg), tb.grid));
   Launch L{};
   L.problems = dtab;
   L.hdr = dhdr;
@@ -213,7 +214,6 @@ int run_plain(skgemm_ctx* c, const skgemm_problem* q, int n, int use_graph) {
   const int cfg = tb.cfg, grid = tb.grid;
   const int used = (int)tb.rows.size();
   const int64_t iters = tb.iters;
-  const TileCfg& t = tile_cfg(cfg);
   if (used == 0) return SKGEMM_OK;
   SKG_TRY(batch_state(c, n));
   auto* b = c->batch;
@@ -222,7 +222,7 @@ int run_plain(skgemm_ctx* c, const skgemm_problem* q, int n, int use_graph) {
     b->pending = false;
   }
   std::memcpy(b->htab, tb.rows.data(), sizeof(Problem) * used);
-  SKG_TRY(c->ensure_ws((size_t)grid * t.bm * t.bn, grid));
+  SKG_TRY(c->ensure_ws((size_t)grid * slot_doubles(cfg), grid));
   b->hhdr[0] = used;
   b->hhdr[1] = iters;
   SKG_HIP(hipMemcpyAsync(b->dtab, b->htab, sizeof(Problem) * used,

@@ -51,6 +51,10 @@ struct Problem {
   int kci;
   double* part;
   const int* pred;     // single launches: run only when *pred != 0
+  // Columns N - xcol .. N - 1 (xcol <= 16) ride on the last tile column as
+  // an extra strip, or rows M - xrow .. M - 1 on the last tile row
+  // (configurations with xs; at most one of the two is non-zero).
+  int xcol, xrow;
   int64_t tile_begin;  // first flattened tile index of this problem
   int64_t iter_begin;  // first flattened MAC-iteration index
   int64_t num_tiles;
@@ -74,12 +78,22 @@ struct Launch {
 struct TileCfg {
   int bm, bn, bk, wm, wn;
   int ns;  // 0: register-staged LDS double buffer; >= 3: LDS-DMA stages
+  int xs;  // 1: an N (else M) remainder of up to 16 rides on the last tile
+           // column (row) (GEMM, single problems, 16-byte aligned operands)
 };
 int num_tile_cfgs();
 const TileCfg& tile_cfg(int i);
 // The register-staged configuration with the same tile shape (cfg itself
 // when it has no DMA stages).
 int register_sibling(int cfg);
+// The configuration with the same tile and stages but no strip.
+int strip_twin(int cfg);
+// Strip columns (rows) of a problem under `cfg` (0: none; then N (M) is
+// tiled by BN (BM)).  A column strip takes precedence.
+int strip_cols(int cfg, int kind, int N);
+int strip_rows(int cfg, int kind, int M, int N);
+// Doubles of one workgroup's partial-tile slot in the fix-up workspace.
+size_t slot_doubles(int cfg);
 
 // Launch the FP64 Stream-K kernel with tile configuration `cfg`.
 hipError_t launch_dgemm_sk(int cfg, const Launch& L, hipStream_t s);

@@ -45,6 +45,8 @@ static const TileCfg kCfgs[] = {
     {128, 128, 16, 4, 2, 4},  // 8: as 4, 4 DMA stages
     {128, 64, 16, 2, 2, 4},   // 9: as 2, 4 DMA stages
     {64, 64, 16, 2, 2, 4},    // 10: as 1, 4 DMA stages
+    {128, 128, 16, 4, 2, 4, 1},  // 11: as 8, the last tile column (row)
+                                 // carries an N (M) remainder of up to 16
 };
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
@@ -58,6 +60,33 @@ int register_sibling(int cfg) {
         kCfgs[i].bk == t.bk && kCfgs[i].wm == t.wm && kCfgs[i].wn == t.wn)
       return i;
   return cfg;
+}
+int strip_twin(int cfg) {
+  const TileCfg& t = kCfgs[cfg];
+  if (t.xs == 0) return cfg;
+  for (int i = 0; i < kNumCfgs; ++i)
+    if (kCfgs[i].xs == 0 && kCfgs[i].ns == t.ns && kCfgs[i].bm == t.bm &&
+        kCfgs[i].bn == t.bn && kCfgs[i].bk == t.bk && kCfgs[i].wm == t.wm &&
+        kCfgs[i].wn == t.wn)
+      return i;
+  return cfg;
+}
+int strip_cols(int cfg, int kind, int N) {
+  const TileCfg& t = kCfgs[cfg];
+  const int r = N % t.bn;
+  return (t.xs && kind == 0 && N > t.bn && r > 0 && r <= 16) ? r : 0;
+}
+int strip_rows(int cfg, int kind, int M, int N) {
+  const TileCfg& t = kCfgs[cfg];
+  const int r = M % t.bm;
+  return (t.xs && kind == 0 && strip_cols(cfg, kind, N) == 0 && M > t.bm &&
+          r > 0 && r <= 16)
+             ? r
+             : 0;
+}
+size_t slot_doubles(int cfg) {
+  const TileCfg& t = kCfgs[cfg];
+  return (size_t)t.bm * t.bn + (t.xs ? 4 * 64 * t.wm * t.wn : 0);
 }
 
 // LDS row padding (doubles) of an operand image kept as [k][BO + kPad]: the
@@ -414,14 +443,16 @@ struct DmaTile {
   __device__ __forceinline__ static void frag(const double* l, int o, int lq,
                                               double (&f)[4]) {
     if (KC) {
-      const double2 x = *reinterpret_cast<const double2*>(
-          l + o * kPitch + 2 * slot(o, 2 * lq));
-      const double2 y = *reinterpret_cast<const double2*>(
-          l + o * kPitch + 2 * slot(o, 2 * lq + 1));
-      f[0] = x.x;
-      f[1] = x.y;
-      f[2] = y.x;
-      f[3] = y.y;
+      // Scalar loads of adjacent pairs (merged into 16-byte reads).  A
+      // double2 load here makes the compiler drain every LDS-DMA copy in
+      // flight (s_waitcnt vmcnt(0)) before it; the stage it reads is
+      // already ordered by the pipeline's own wait and barrier.
+      const double* x = l + o * kPitch + 2 * slot(o, 2 * lq);
+      const double* y = l + o * kPitch + 2 * slot(o, 2 * lq + 1);
+      f[0] = x[0];
+      f[1] = x[1];
+      f[2] = y[0];
+      f[3] = y[1];
     } else {
 #pragma unroll
       for (int s = 0; s < 4; ++s) f[s] = l[(4 * lq + s) * kPitch + o];
@@ -442,13 +473,25 @@ struct DmaFrag {
   double b[BN / WN / 16][4];
 };
 
-template <int BM, int BN, int BK, int WM, int WN, bool MASKED, int S0, int S1>
+// Strips: a tile of the last tile column may carry 16 more columns
+// (n0 + BN .. n0 + BN + 15; XT = 1) for an N remainder of at most 16, or a
+// tile of the last tile row 16 more rows (m0 + BM ..; XT = 2) for an M
+// remainder of at most 16.  Column strip: wave (wm, wn) computes the
+// strip's 16 x 16 block on its A fragment wn (rows wm TM + 16 wn).  Row
+// strip: on its B fragment wm (columns wn TN + 16 wm).  A strip costs one
+// fragment and one MFMA per wave and sub-step, and no reads of the other
+// operand.  Strip tiles run the unmasked loop.
+constexpr int kStripCol = 1, kStripRow = 2;
+
+template <int BM, int BN, int BK, int WM, int WN, bool MASKED, int XT, int S0,
+          int S1>
 __device__ __forceinline__ void dma_mma(
-    v4d (&acc)[BM / WM / 16][BN / WN / 16],
-    const DmaFrag<BM, BN, BK, WM, WN>& f, uint32_t mask) {
+    v4d (&acc)[BM / WM / 16][BN / WN / 16], v4d& xacc,
+    const DmaFrag<BM, BN, BK, WM, WN>& f, const double (&xf)[4],
+    uint32_t mask, int wm, int wn) {
   constexpr int MB = BM / WM / 16, NB = BN / WN / 16;
 #pragma unroll
-  for (int s = S0; s < S1; ++s)
+  for (int s = S0; s < S1; ++s) {
 #pragma unroll
     for (int i = 0; i < MB; ++i)
 #pragma unroll
@@ -456,6 +499,20 @@ __device__ __forceinline__ void dma_mma(
         if (!MASKED || (mask & (1u << (i * NB + j))))
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(f.a[i][s], f.b[j][s],
                                                            acc[i][j], 0, 0, 0);
+    if constexpr (XT == kStripCol) {
+      static_assert(MB == WN, "column strip: one A fragment per wave column");
+      double ax = f.a[0][s];
+#pragma unroll
+      for (int i = 1; i < MB; ++i) ax = wn == i ? f.a[i][s] : ax;
+      xacc = __builtin_amdgcn_mfma_f64_16x16x4f64(ax, xf[s], xacc, 0, 0, 0);
+    } else if constexpr (XT == kStripRow) {
+      static_assert(NB == WM, "row strip: one B fragment per wave row");
+      double bx = f.b[0][s];
+#pragma unroll
+      for (int j = 1; j < NB; ++j) bx = wm == j ? f.b[j][s] : bx;
+      xacc = __builtin_amdgcn_mfma_f64_16x16x4f64(xf[s], bx, xacc, 0, 0, 0);
+    }
+  }
 }
 
 // The MAC loop of the DMA configurations.  Step kk's fragments are read into
@@ -466,27 +523,33 @@ __device__ __forceinline__ void dma_mma(
 // read), so exactly (NS - 2) steps of copies are younger than the one
 // waited for at each barrier; the last step issues no copy.
 template <int BM, int BN, int BK, int WM, int WN, int AKC, int BKC,
-          bool MASKED, int NS>
+          bool MASKED, int NS, int XT>
 __device__ __forceinline__ void mac_loop_dma(
-    v4d (&acc)[BM / WM / 16][BN / WN / 16], const Operand& A,
+    v4d (&acc)[BM / WM / 16][BN / WN / 16], v4d& xacc, const Operand& A,
     const Operand& B, int K, int m0, int n0, int kbase, int kb, int ke,
     double* lds, int tid, uint32_t mask) {
   using S = TileShape<BM, BN, BK, WM, WN>;
   using TA = DmaTile<BM, BK, AKC>;
   using TB = DmaTile<BN, BK, BKC>;
+  using TX = DmaTile<16, BK, XT == kStripRow ? AKC : BKC>;
   using F = DmaFrag<BM, BN, BK, WM, WN>;
   constexpr int NW = WM * WN;
   constexpr int NDA = TA::kNumDma, NDB = TB::kNumDma;
   static_assert((NDA + NDB) % NW == 0, "DMA instructions vs waves");
+  static_assert(TX::kNumDma <= NW, "strip copies: one per wave at most");
   constexpr int DPW = (NDA + NDB) / NW;
   constexpr int kRegA = dma_region<BM, BK>();
-  constexpr int kStage = kRegA + dma_region<BN, BK>();
+  constexpr int kRegB = dma_region<BN, BK>();
+  constexpr int kStage = kRegA + kRegB + (XT != 0 ? dma_region<16, BK>() : 0);
   constexpr int kAhead = (NS - 2) * DPW;
-  static_assert(kAhead < 64, "vmcnt range");
+  constexpr int kAheadX = (NS - 2) * (DPW + 1);
+  static_assert(kAheadX < 64, "vmcnt range");
   // vmcnt: this step's copies have landed; lgkmcnt 0: this wave's fragment
   // reads are complete, so after the barrier no wave still reads a stage
-  // that the next issue overwrites.
+  // that the next issue overwrites.  In XT tiles waves 0 .. TX::kNumDma - 1
+  // issue one strip copy per step more than the others.
   constexpr int kWait = (kAhead & 15) | ((kAhead >> 4) << 14) | (7 << 4);
+  constexpr int kWaitX = (kAheadX & 15) | ((kAheadX >> 4) << 14) | (7 << 4);
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   int wm, wn;
@@ -517,6 +580,22 @@ __device__ __forceinline__ void mac_loop_dma(
     }
     ko[d] = k;
   }
+  // Strip copy of waves 0 .. TX::kNumDma - 1: B columns n0 + BN + o
+  // (column strip) or A rows m0 + BM + o (row strip).
+  int xrel = 0, xk = 0;
+  bool xin = false;
+  const bool xwave = XT != 0 && wave < TX::kNumDma;
+  if (xwave) {
+    const Operand& X = XT == kStripRow ? A : B;
+    int o, k;
+    TX::coords(wave, lane, o, k);
+    o += XT == kStripRow ? BM : BN;
+    const int pm = (1 << X.pshift) - 1;
+    xrel = (int)(((int64_t)(o >> X.pshift) * X.pstride +
+                  (int64_t)(o & pm) * X.os + (int64_t)k * X.ks) * 8);
+    xin = o < (XT == kStripRow ? olimA : olimB);
+    xk = k;
+  }
   const double* baseA = A.ptr + op_addr(A, m0, 0);
   const double* baseB = B.ptr + op_addr(B, n0, 0);
   const int64_t stepA = (int64_t)BK * A.ks, stepB = (int64_t)BK * B.ks;
@@ -538,6 +617,10 @@ __device__ __forceinline__ void mac_loop_dma(
       else
         dma16(rb, base + kRegA + (q - NDA) * TB::kRpd * TB::kPitch, off);
     }
+    if (xwave)
+      dma16(XT == kStripRow ? ra : rb,
+            base + kRegA + kRegB + wave * TX::kRpd * TX::kPitch,
+            (xin && xk < klim) ? xrel : kOOB);
   };
   auto read = [&](F& f, int stage) {
     const double* la = lds + stage * kStage;
@@ -549,9 +632,20 @@ __device__ __forceinline__ void mac_loop_dma(
     for (int j = 0; j < S::NB; ++j)
       TB::frag(lb, wn * S::TN + j * 16 + lr, lq, f.b[j]);
   };
+  // The strip fragment is single-buffered: step kk + 1's is read once its
+  // stage is published, after step kk's last MFMA sub-step has used the
+  // registers (the stage stays intact until the next barrier).
+  double xf[4] = {0.0, 0.0, 0.0, 0.0};
+  auto readx = [&](int stage) {
+    if constexpr (XT != 0)
+      TX::frag(lds + stage * kStage + kRegA + kRegB, lr, lq, xf);
+  };
   auto publish = [&]() {
     __builtin_amdgcn_sched_barrier(0);
-    __builtin_amdgcn_s_waitcnt(kWait);
+    if (xwave)
+      __builtin_amdgcn_s_waitcnt(kWaitX);
+    else
+      __builtin_amdgcn_s_waitcnt(kWait);
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
@@ -562,19 +656,24 @@ __device__ __forceinline__ void mac_loop_dma(
   publish();
   F f0, f1;
   read(f0, 0);
+  readx(0);
   int st = 0;  // stage of step kk
 #pragma unroll 1
   for (; kk + 1 < kend; ++kk) {
     issue(st == 0 ? NS - 1 : st - 1, kk + NS - 1);
-    dma_mma<BM, BN, BK, WM, WN, MASKED, 0, 3>(acc, f0, mask);
+    dma_mma<BM, BN, BK, WM, WN, MASKED, XT, 0, 3>(acc, xacc, f0, xf, mask,
+                                                   wm, wn);
     publish();
     st = st + 1 == NS ? 0 : st + 1;
     read(f1, st);
     __builtin_amdgcn_sched_barrier(0);  // reads issue before the MFMAs
-    dma_mma<BM, BN, BK, WM, WN, MASKED, 3, 4>(acc, f0, mask);
+    dma_mma<BM, BN, BK, WM, WN, MASKED, XT, 3, 4>(acc, xacc, f0, xf, mask,
+                                                   wm, wn);
     f0 = f1;
+    readx(st);
   }
-  dma_mma<BM, BN, BK, WM, WN, MASKED, 0, 4>(acc, f0, mask);
+  dma_mma<BM, BN, BK, WM, WN, MASKED, XT, 0, 4>(acc, xacc, f0, xf, mask, wm,
+                                                 wn);
   __builtin_amdgcn_s_waitcnt(0);  // clamped tail copies
   __syncthreads();
 }
@@ -593,10 +692,18 @@ __device__ __forceinline__ T load_uniform(const T* p) {
   return out;
 }
 
-template <int BM, int BN, int BK, int WM, int WN, int NS, bool BATCHED>
+// XS = kStripCol (kStripRow): tiles of the last tile column (row) may carry
+// the column (row) strip (Problem::xcol / xrow > 0; single-problem DMA
+// launches only).  One kernel per strip kind keeps each within registers.
+template <int BM, int BN, int BK, int WM, int WN, int NS, bool BATCHED,
+          int XS = 0>
 __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
   using S = TileShape<BM, BN, BK, WM, WN>;
   constexpr int NT = S::NT, TM = S::TM, TN = S::TN, MB = S::MB, NB = S::NB;
+  static_assert(!XS || (NS >= 3 && !BATCHED), "strip: single DMA launches");
+  // Partial-tile slot of one workgroup (the strip block of each wave after
+  // the tile's blocks).
+  constexpr int64_t kSlot = (int64_t)BM * BN + (XS ? 4 * NT : 0);
 
   extern __shared__ __align__(16) double lds[];
 
@@ -681,12 +788,20 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
       tj = (int)(tile / tiles_m);
     }
     const int m0 = ti * BM, n0 = tj * BN;
+    // Strip tiles: the last tile column (row) of a problem whose N (M)
+    // remainder of at most 16 rides on it.
+    const bool xt = XS == kStripCol && L.single.xcol > 0 && tj == tiles_n - 1;
+    const bool yt = XS == kStripRow && L.single.xrow > 0 && ti == tiles_m - 1;
 
     v4d acc[MB][NB];
 #pragma unroll
     for (int i = 0; i < MB; ++i)
 #pragma unroll
       for (int j = 0; j < NB; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+    v4d xacc = v4d{0.0, 0.0, 0.0, 0.0};
+    // This lane's strip outputs: rows srow + 4 v, column scol.
+    const int srow = (yt ? m0 + BM : m0 + wm * TM + wn * 16) + lq;
+    const int scol = (yt ? n0 + wn * TN + wm * 16 : n0 + BN) + lr;
 
     // ---- MAC loop, one branch-free variant per operand layout -----------
     const int Mx = BATCHED ? load_uniform(&tab[pi].M) : L.single.M;
@@ -714,11 +829,19 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
     // unaligned operand reaching one is reported through the status word.
 #define SKG_MAC(V)                                                           \
   case V:                                                                    \
-    if constexpr (NS >= 3 && (V & 1))                                        \
-      mac_loop_dma<BM, BN, BK, WM, WN, (V >> 2) & 1, (V >> 1) & 1,           \
-                   ((V >> 3) & 1) != 0, NS>(acc, A, B, K, m0, n0, c * kci,   \
-                                            kb, ke, lds, tid, mask);         \
-    else if constexpr (NS >= 3)                                              \
+    if constexpr (NS >= 3 && (V & 1)) {                                      \
+      /* strip tiles unmasked: edge blocks are computed on zero-filled     \
+         operands and not stored */                                          \
+      if (XS != 0 && (xt || yt))                                             \
+        mac_loop_dma<BM, BN, BK, WM, WN, (V >> 2) & 1, (V >> 1) & 1, false,  \
+                     NS, XS>(acc, xacc, A, B, K, m0, n0, c * kci, kb, ke,    \
+                             lds, tid, mask);                                \
+      else                                                                   \
+        mac_loop_dma<BM, BN, BK, WM, WN, (V >> 2) & 1, (V >> 1) & 1,         \
+                     ((V >> 3) & 1) != 0, NS, 0>(acc, xacc, A, B, K, m0, n0, \
+                                                 c * kci, kb, ke, lds, tid,  \
+                                                 mask);                      \
+    } else if constexpr (NS >= 3)                                            \
       if (tid == 0) atomicOr(L.status, 2);                                   \
     if constexpr (NS == 0)                                                   \
       mac_loop<BM, BN, BK, WM, WN, (V >> 2) & 1, (V >> 1) & 1, V & 1,        \
@@ -744,7 +867,7 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
     bool write_out = true;
     if (kb != 0) {
       // Not the owner: spill the partial to this workgroup's slot.
-      gdouble* slot = (gdouble*)(L.ws) + (int64_t)w * (BM * BN);
+      gdouble* slot = (gdouble*)(L.ws) + (int64_t)w * kSlot;
 #pragma unroll
       for (int i = 0; i < MB; ++i)
 #pragma unroll
@@ -754,6 +877,12 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
             __hip_atomic_store(slot + ((i * NB + j) * 4 + v) * NT + tid,
                                acc[i][j][v], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
+      if (XS != 0 && (xt || yt)) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          __hip_atomic_store(slot + (MB * NB * 4 + v) * NT + tid, xacc[v],
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
       if (tid == 0)
@@ -779,7 +908,7 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
                              __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
-        gdouble* slot = (gdouble*)(L.ws) + (int64_t)w2 * (BM * BN);
+        gdouble* slot = (gdouble*)(L.ws) + (int64_t)w2 * kSlot;
 #pragma unroll
         for (int i = 0; i < MB; ++i)
 #pragma unroll
@@ -789,6 +918,13 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
               acc[i][j][v] += __hip_atomic_load(
                   slot + ((i * NB + j) * 4 + v) * NT + tid, __ATOMIC_RELAXED,
                   __HIP_MEMORY_SCOPE_AGENT);
+        if (XS != 0 && (xt || yt)) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            xacc[v] += __hip_atomic_load(slot + (MB * NB * 4 + v) * NT + tid,
+                                         __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
     }
 
@@ -822,6 +958,13 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
             }
           }
         }
+        if (XS != 0 && (xt || yt)) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = srow + 4 * v;
+            if (row < M && scol < N) pb[(int64_t)row * N + scol] = xacc[v];
+          }
+        }
         it = t0 + ke;
         continue;
       }
@@ -845,26 +988,40 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
           }
         }
       }
+      if (XS != 0 && (xt || yt)) {
+        gdouble* dst = dbase + out_addr(D, srow, scol);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = srow + 4 * v;
+          if (row < M && scol < N) {
+            double val = alpha * xacc[v];
+            if (beta != 0.0) val = fma(beta, dst[v * rstep], val);
+            dst[v * rstep] = val + gamma;
+          }
+        }
+      }
     }
     it = t0 + ke;
   }
 }
 
-template <int BM, int BN, int BK, int WM, int WN, int NS>
+template <int BM, int BN, int BK, int WM, int WN, int NS, bool XS = false>
 static size_t lds_bytes_t() {
   size_t d = 2ull * TileShape<BM, BN, BK, WM, WN>::kBufDoubles;
   if constexpr (NS >= 3) {
     const size_t dma =
-        (size_t)NS * (dma_region<BM, BK>() + dma_region<BN, BK>());
+        (size_t)NS * (dma_region<BM, BK>() + dma_region<BN, BK>() +
+                      (XS ? dma_region<16, BK>() : 0));
     if (dma > d) d = dma;
   }
   return d * sizeof(double);
 }
 
-template <int BM, int BN, int BK, int WM, int WN, int NS, bool BATCHED>
+template <int BM, int BN, int BK, int WM, int WN, int NS, bool BATCHED,
+          int XS = 0>
 static hipError_t launch_v(const Launch& L, hipStream_t s) {
-  auto kfn = sk_dgemm_kernel<BM, BN, BK, WM, WN, NS, BATCHED>;
-  const size_t lds = lds_bytes_t<BM, BN, BK, WM, WN, NS>();
+  auto kfn = sk_dgemm_kernel<BM, BN, BK, WM, WN, NS, BATCHED, XS>;
+  const size_t lds = lds_bytes_t<BM, BN, BK, WM, WN, NS, XS != 0>();
   static bool attr_set = false;
   if (!attr_set) {
     hipError_t e = hipFuncSetAttribute(
@@ -877,22 +1034,39 @@ static hipError_t launch_v(const Launch& L, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int BM, int BN, int BK, int WM, int WN, int NS>
+template <int BM, int BN, int BK, int WM, int WN, int NS, bool XS = false>
 static hipError_t launch_t(const Launch& L, hipStream_t s) {
-  if (L.problems != nullptr) return launch_v<BM, BN, BK, WM, WN, NS, true>(L, s);
-  return launch_v<BM, BN, BK, WM, WN, NS, false>(L, s);
+  if constexpr (XS) {
+    // Strip tiles run single problems only (batched tables use the
+    // configuration without the strip).
+    if (L.problems != nullptr) return hipErrorInvalidValue;
+    if (L.single.xrow > 0)
+      return launch_v<BM, BN, BK, WM, WN, NS, false, kStripRow>(L, s);
+    return launch_v<BM, BN, BK, WM, WN, NS, false, kStripCol>(L, s);
+  } else {
+    if (L.problems != nullptr)
+      return launch_v<BM, BN, BK, WM, WN, NS, true>(L, s);
+    return launch_v<BM, BN, BK, WM, WN, NS, false>(L, s);
+  }
 }
 
-template <int BM, int BN, int BK, int WM, int WN, int NS>
+template <int BM, int BN, int BK, int WM, int WN, int NS, bool XS = false>
 static int occupancy_t() {
   // Both variants (single problem / batched table) must fit: the persistent
   // grid is sized from the smaller of the two.
-  const void* fns[2] = {
-      reinterpret_cast<const void*>(
-          sk_dgemm_kernel<BM, BN, BK, WM, WN, NS, true>),
-      reinterpret_cast<const void*>(
-          sk_dgemm_kernel<BM, BN, BK, WM, WN, NS, false>)};
-  const size_t lds = lds_bytes_t<BM, BN, BK, WM, WN, NS>();
+  const void* fns[2];
+  if constexpr (XS) {
+    fns[0] = reinterpret_cast<const void*>(
+        sk_dgemm_kernel<BM, BN, BK, WM, WN, NS, false, kStripRow>);
+    fns[1] = reinterpret_cast<const void*>(
+        sk_dgemm_kernel<BM, BN, BK, WM, WN, NS, false, kStripCol>);
+  } else {
+    fns[0] = reinterpret_cast<const void*>(
+        sk_dgemm_kernel<BM, BN, BK, WM, WN, NS, true>);
+    fns[1] = reinterpret_cast<const void*>(
+        sk_dgemm_kernel<BM, BN, BK, WM, WN, NS, false>);
+  }
+  const size_t lds = lds_bytes_t<BM, BN, BK, WM, WN, NS, XS>();
   int best = 1 << 30;
   for (const void* f : fns) {
     hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -919,6 +1093,7 @@ static int occupancy_t() {
     case 8: return FN<128, 128, 16, 4, 2, 4>(__VA_ARGS__);  \
     case 9: return FN<128, 64, 16, 2, 2, 4>(__VA_ARGS__);   \
     case 10: return FN<64, 64, 16, 2, 2, 4>(__VA_ARGS__);   \
+    case 11: return FN<128, 128, 16, 4, 2, 4, true>(__VA_ARGS__); \
     default: break;                                        \
   }
 

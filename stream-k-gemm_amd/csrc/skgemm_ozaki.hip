@@ -1288,8 +1288,7 @@ int ozaki_reserve(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
     if (rc != SKGEMM_OK) return rc;
   }
   Plan pl = plan(c, kind, la, lb, M, N, K);
-  const TileCfg& t = tile_cfg(pl.cfg);
-  return c->ensure_ws((size_t)pl.grid * t.bm * t.bn, pl.grid);
+  return c->ensure_ws((size_t)pl.grid * slot_doubles(pl.cfg), pl.grid);
 }
 
 // Shared driver.  kind 0: C = A B; kind 1: C = Psi^T Psi (B == A).

@@ -102,6 +102,11 @@ extern "C" int skgemm_autotune(skgemm_ctx* c, int kind, int la, int lb, int M,
   for (int cfg = 0; cfg < num_tile_cfgs(); ++cfg) {
     const TileCfg& t = tile_cfg(cfg);
     if (kind == 1 && t.bm != t.bn) continue;
+    // A strip configuration differs from its twin only when the strip
+    // applies to this shape.
+    if (t.xs && strip_cols(cfg, kind, kN) == 0 &&
+        strip_rows(cfg, kind, kM, kN) == 0)
+      continue;
     const int cap = c->capacity(cfg);
     std::set<int> grids = {cap, std::min(cap, 2 * c->num_cus), std::min(cap, c->num_cus)};
     const int64_t ipt = std::max<int64_t>(1, (K + t.bk - 1) / t.bk);

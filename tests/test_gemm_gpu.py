@@ -369,3 +369,77 @@ def test_every_tile_every_layout(skg, ctx, orc, rng, pad):
                                D, N, 0.0)
                 assert rel_fro(D, ref) < TOL, (tile, la, lb, pad)
     ctx.set_tuning(-1, 0)
+
+
+STRIP_TILE = 11   # 128 x 128 LDS-DMA tile whose last tile column carries
+                  # an N remainder of up to 16 columns
+
+
+@pytest.mark.parametrize("M,N,K", [(256, 136, 64), (300, 144, 70),
+                                   (128, 130, 520), (384, 520, 37),
+                                   (256, 129, 33)])
+def test_strip_tiles(skg, ctx, rng, M, N, K):
+    """The strip configuration on every A / B layout, row-major and
+    band-packed D, with alpha, beta and gamma, under the default grid,
+    split tiles (Stream-K fix-up of the strip) and K chunks.  Odd N or K
+    takes the register-staged sibling with plain tiling."""
+    A = rng.uniform(-1, 1, (M, K))
+    B = rng.uniform(-1, 1, (K, N))
+    D0 = rng.uniform(-1, 1, (M, N))
+    alpha, beta, gamma = 0.75, 0.5, 0.25
+    ref = alpha * (A @ B) + beta * D0 + gamma
+    info = skg.tile_info(STRIP_TILE)
+    assert (info["bm"], info["bn"], info["waves"]) == (128, 128, 8)
+    try:
+        for grid, kchunks in ((0, 1), (5, 1), (0, 3)):
+            ctx.set_tuning(STRIP_TILE, grid)
+            ctx.set_kchunks(kchunks)
+            for la in range(3):
+                Ab, lda = encode_a(A, la, 2)
+                for lb in range(4):
+                    Bb, ldb = encode_b(B, lb, 2)
+                    for ld_ in (0, 2):
+                        Db, ldd = encode_d(D0, ld_)
+                        ctx.dgemm_host(la, lb, ld_, M, N, K, alpha, Ab, lda,
+                                       Bb, ldb, beta, Db, ldd, gamma)
+                        got = decode_d(Db, ld_, M, N, ldd)
+                        err = rel_fro(got, ref)
+                        assert err < TOL, (grid, kchunks, la, lb, ld_, err)
+    finally:
+        ctx.set_tuning(-1, 0)
+        ctx.set_kchunks(0)
+
+
+def test_strip_plan_and_c2_rotation(skg, ctx, rng):
+    """N = 520 under the strip tile: 4 tile columns (the last one 136 wide);
+    the c2 rotation (grid-packed Psi 32768 x 520, row-major Q, band-packed
+    out) agrees with numpy and bit for bit with the plain 128 x 128 tile
+    (same MFMA sequence per 16 x 16 block, no split units)."""
+    ctx.set_tuning(STRIP_TILE, 0)
+    try:
+        plan = ctx.query_plan(0, 32768, 520, 520, skg.ROW_MAJOR, skg.ROW_MAJOR)
+        assert plan["tile"] == STRIP_TILE and plan["tiles"] == 256 * 4
+        plan = ctx.query_plan(0, 32768, 600, 520, skg.ROW_MAJOR, skg.ROW_MAJOR)
+        assert plan["tiles"] == 256 * 5            # remainder 88: no strip
+    finally:
+        ctx.set_tuning(-1, 0)
+    Nd, nb = 32768, 520
+    psi = rng.standard_normal((Nd, nb)) / np.sqrt(Nd)
+    Q = rng.uniform(-1, 1, (nb, nb))
+    ref = psi @ Q
+    Ab, lda = encode_a(psi, 2, 0)
+    out = {}
+    for tile in (STRIP_TILE, 8):
+        # 256 workgroups: 1024 (1280) tiles of 33 K steps split into whole
+        # tiles, 4 (5) per workgroup, so no unit is split.
+        ctx.set_tuning(tile, 256)
+        try:
+            Db = np.zeros(((nb + 15) // 16) * Nd * 16)
+            ctx.dgemm_host(skg.PACK16, skg.ROW_MAJOR, skg.PACK16, Nd, nb, nb,
+                           1.0, Ab, lda, np.ascontiguousarray(Q.ravel()), nb,
+                           0.0, Db, Nd, 0.0)
+        finally:
+            ctx.set_tuning(-1, 0)
+        out[tile] = decode_d(Db, 2, Nd, nb, Nd)
+        assert rel_fro(out[tile], ref) < TOL
+    assert np.array_equal(out[STRIP_TILE], out[8])
