@@ -377,12 +377,16 @@ STRIP_TILE = 11   # 128 x 128 LDS-DMA tile whose last tile column carries
 
 @pytest.mark.parametrize("M,N,K", [(256, 136, 64), (300, 144, 70),
                                    (128, 130, 520), (384, 520, 37),
-                                   (256, 129, 33)])
+                                   (256, 129, 33), (136, 256, 64),
+                                   (520, 300, 37), (144, 128, 70)])
 def test_strip_tiles(skg, ctx, rng, M, N, K):
-    """The strip configuration on every A / B layout, row-major and
-    band-packed D, with alpha, beta and gamma, under the default grid,
-    split tiles (Stream-K fix-up of the strip) and K chunks.  Odd N or K
-    takes the register-staged sibling with plain tiling."""
+    """The strip configuration on every A / B layout, row-major,
+    column-major (launched transposed, so a column strip becomes a row
+    strip and back) and band-packed D, with alpha, beta and gamma, under
+    the default grid, split tiles (Stream-K fix-up of the strip) and K
+    chunks.  Column strips (N remainder), row strips (M remainder, N a
+    multiple of 128 or its remainder above 16); odd N or K takes the
+    register-staged sibling with plain tiling."""
     A = rng.uniform(-1, 1, (M, K))
     B = rng.uniform(-1, 1, (K, N))
     D0 = rng.uniform(-1, 1, (M, N))
@@ -398,7 +402,7 @@ def test_strip_tiles(skg, ctx, rng, M, N, K):
                 Ab, lda = encode_a(A, la, 2)
                 for lb in range(4):
                     Bb, ldb = encode_b(B, lb, 2)
-                    for ld_ in (0, 2):
+                    for ld_ in (0, 1, 2):
                         Db, ldd = encode_d(D0, ld_)
                         ctx.dgemm_host(la, lb, ld_, M, N, K, alpha, Ab, lda,
                                        Bb, ldb, beta, Db, ldd, gamma)
@@ -443,3 +447,26 @@ def test_strip_plan_and_c2_rotation(skg, ctx, rng):
         out[tile] = decode_d(Db, 2, Nd, nb, Nd)
         assert rel_fro(out[tile], ref) < TOL
     assert np.array_equal(out[STRIP_TILE], out[8])
+
+
+def test_strip_c2_rotation_column_major(skg, ctx, rng):
+    """The c2 rotation as the step benchmark runs it: Psi 32768 x 520
+    column-major, Q row-major, D column-major, launched transposed (Q^T
+    Psi^T, a row strip of 8 on the last tile row) under the autotuner's
+    choice of the strip tile."""
+    Nd, nb = 32768, 520
+    psi = rng.standard_normal((Nd, nb)) / np.sqrt(Nd)
+    Q = rng.uniform(-1, 1, (nb, nb))
+    ref = psi @ Q
+    Ab, lda = encode_a(psi, 1, 0)
+    for grid in (0, 100):
+        ctx.set_tuning(STRIP_TILE, grid)
+        try:
+            D = np.zeros(Nd * nb)
+            ctx.dgemm_host(skg.COL_MAJOR, skg.ROW_MAJOR, skg.COL_MAJOR, Nd,
+                           nb, nb, 1.0, Ab, lda,
+                           np.ascontiguousarray(Q.ravel()), nb, 0.0, D, Nd,
+                           0.0)
+        finally:
+            ctx.set_tuning(-1, 0)
+        assert rel_fro(decode_d(D, 1, Nd, nb, Nd), ref) < TOL, grid
