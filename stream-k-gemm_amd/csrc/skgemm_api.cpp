@@ -478,7 +478,8 @@ namespace skg {
 int64_t tiles_for(int kind, int cfg, int M, int N) {
   const TileCfg& t = tile_cfg(cfg);
   if (kind == 1) {
-    const int64_t T = (N + t.bm - 1) / t.bm;
+    const int64_t T =
+        strip_rows(cfg, kind, N, N) ? N / t.bm : (N + t.bm - 1) / t.bm;
     return T * (T + 1) / 2;
   }
   const int tn = strip_cols(cfg, kind, N) ? N / t.bn : (N + t.bn - 1) / t.bn;
@@ -576,6 +577,7 @@ int fill_problem(skgemm_ctx* c, Problem* p, int kind, const Plan& pl, int M,
   p->xrow = strip_rows(pl.cfg, kind, M, N);
   p->tiles_m = p->xrow ? M / t.bm : (M + t.bm - 1) / t.bm;
   p->tiles_n = p->xcol ? N / t.bn : (N + t.bn - 1) / t.bn;
+  if (kind == 1) p->tiles_n = p->tiles_m;
   p->iters_per_tile = (int)iters_per_tile(pl.cfg, K);
   // Tile order: keep the smaller operand cached and read the larger once.
   p->row_order = (kind == 0 && (int64_t)M >= (int64_t)N) ? 1 : 0;
@@ -603,7 +605,8 @@ int run_single(skgemm_ctx* c, Problem& p, const Plan& plan_in,
       p.xcol = p.xrow = 0;
       p.tiles_m = (p.M + tile_cfg(pl.cfg).bm - 1) / tile_cfg(pl.cfg).bm;
       p.tiles_n = (p.N + tile_cfg(pl.cfg).bn - 1) / tile_cfg(pl.cfg).bn;
-      p.num_tiles = (int64_t)p.tiles_m * p.tiles_n;
+      p.num_tiles = p.syrk ? (int64_t)p.tiles_m * (p.tiles_m + 1) / 2
+                           : (int64_t)p.tiles_m * p.tiles_n;
     }
   }
   const TileCfg& t = tile_cfg(pl.cfg);

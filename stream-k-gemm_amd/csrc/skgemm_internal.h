@@ -79,7 +79,8 @@ struct TileCfg {
   int bm, bn, bk, wm, wn;
   int ns;  // 0: register-staged LDS double buffer; >= 3: LDS-DMA stages
   int xs;  // 1: an N (else M) remainder of up to 16 rides on the last tile
-           // column (row) (GEMM, single problems, 16-byte aligned operands)
+           // column (row) (single problems, 16-byte aligned operands; a
+           // SYRK takes the row strip)
 };
 int num_tile_cfgs();
 const TileCfg& tile_cfg(int i);

@@ -47,6 +47,7 @@ static const TileCfg kCfgs[] = {
     {64, 64, 16, 2, 2, 4},    // 10: as 1, 4 DMA stages
     {128, 128, 16, 4, 2, 4, 1},  // 11: as 8, the last tile column (row)
                                  // carries an N (M) remainder of up to 16
+    {64, 64, 16, 2, 2, 4, 1},    // 12: as 10, with strips (also SYRK rows)
 };
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
@@ -79,14 +80,14 @@ int strip_cols(int cfg, int kind, int N) {
 int strip_rows(int cfg, int kind, int M, int N) {
   const TileCfg& t = kCfgs[cfg];
   const int r = M % t.bm;
-  return (t.xs && kind == 0 && strip_cols(cfg, kind, N) == 0 && M > t.bm &&
-          r > 0 && r <= 16)
+  return (t.xs && strip_cols(cfg, kind, N) == 0 && M > t.bm && r > 0 &&
+          r <= 16)
              ? r
              : 0;
 }
 size_t slot_doubles(int cfg) {
   const TileCfg& t = kCfgs[cfg];
-  return (size_t)t.bm * t.bn + (t.xs ? 4 * 64 * t.wm * t.wn : 0);
+  return (size_t)t.bm * t.bn + (t.xs ? 8 * 64 * t.wm * t.wn : 0);
 }
 
 // LDS row padding (doubles) of an operand image kept as [k][BO + kPad]: the
@@ -480,13 +481,16 @@ struct DmaFrag {
 // strip's 16 x 16 block on its A fragment wn (rows wm TM + 16 wn).  Row
 // strip: on its B fragment wm (columns wn TN + 16 wm).  A strip costs one
 // fragment and one MFMA per wave and sub-step, and no reads of the other
-// operand.  Strip tiles run the unmasked loop.
-constexpr int kStripCol = 1, kStripRow = 2;
+// operand.  Strip tiles run the unmasked loop.  In a SYRK (B = A) the
+// row strip of the last diagonal tile also yields the corner block (rows
+// and columns past the last full tile) from the strip fragment alone:
+// wave 0 multiplies it by itself (XT = kStripCorner).
+constexpr int kStripCol = 1, kStripRow = 2, kStripCorner = 6;
 
 template <int BM, int BN, int BK, int WM, int WN, bool MASKED, int XT, int S0,
           int S1>
 __device__ __forceinline__ void dma_mma(
-    v4d (&acc)[BM / WM / 16][BN / WN / 16], v4d& xacc,
+    v4d (&acc)[BM / WM / 16][BN / WN / 16], v4d& xacc, v4d& cacc,
     const DmaFrag<BM, BN, BK, WM, WN>& f, const double (&xf)[4],
     uint32_t mask, int wm, int wn) {
   constexpr int MB = BM / WM / 16, NB = BN / WN / 16;
@@ -505,12 +509,15 @@ __device__ __forceinline__ void dma_mma(
 #pragma unroll
       for (int i = 1; i < MB; ++i) ax = wn == i ? f.a[i][s] : ax;
       xacc = __builtin_amdgcn_mfma_f64_16x16x4f64(ax, xf[s], xacc, 0, 0, 0);
-    } else if constexpr (XT == kStripRow) {
+    } else if constexpr ((XT & kStripRow) != 0) {
       static_assert(NB == WM, "row strip: one B fragment per wave row");
       double bx = f.b[0][s];
 #pragma unroll
       for (int j = 1; j < NB; ++j) bx = wm == j ? f.b[j][s] : bx;
       xacc = __builtin_amdgcn_mfma_f64_16x16x4f64(xf[s], bx, xacc, 0, 0, 0);
+      if constexpr (XT == kStripCorner)
+        cacc = __builtin_amdgcn_mfma_f64_16x16x4f64(xf[s], xf[s], cacc, 0, 0,
+                                                    0);
     }
   }
 }
@@ -525,13 +532,14 @@ __device__ __forceinline__ void dma_mma(
 template <int BM, int BN, int BK, int WM, int WN, int AKC, int BKC,
           bool MASKED, int NS, int XT>
 __device__ __forceinline__ void mac_loop_dma(
-    v4d (&acc)[BM / WM / 16][BN / WN / 16], v4d& xacc, const Operand& A,
+    v4d (&acc)[BM / WM / 16][BN / WN / 16], v4d& xacc, v4d& cacc,
+    const Operand& A,
     const Operand& B, int K, int m0, int n0, int kbase, int kb, int ke,
     double* lds, int tid, uint32_t mask) {
   using S = TileShape<BM, BN, BK, WM, WN>;
   using TA = DmaTile<BM, BK, AKC>;
   using TB = DmaTile<BN, BK, BKC>;
-  using TX = DmaTile<16, BK, XT == kStripRow ? AKC : BKC>;
+  using TX = DmaTile<16, BK, (XT & kStripRow) != 0 ? AKC : BKC>;
   using F = DmaFrag<BM, BN, BK, WM, WN>;
   constexpr int NW = WM * WN;
   constexpr int NDA = TA::kNumDma, NDB = TB::kNumDma;
@@ -586,14 +594,15 @@ __device__ __forceinline__ void mac_loop_dma(
   bool xin = false;
   const bool xwave = XT != 0 && wave < TX::kNumDma;
   if (xwave) {
-    const Operand& X = XT == kStripRow ? A : B;
+    constexpr bool kRow = (XT & kStripRow) != 0;
+    const Operand& X = kRow ? A : B;
     int o, k;
     TX::coords(wave, lane, o, k);
-    o += XT == kStripRow ? BM : BN;
+    o += kRow ? BM : BN;
     const int pm = (1 << X.pshift) - 1;
     xrel = (int)(((int64_t)(o >> X.pshift) * X.pstride +
                   (int64_t)(o & pm) * X.os + (int64_t)k * X.ks) * 8);
-    xin = o < (XT == kStripRow ? olimA : olimB);
+    xin = o < (kRow ? olimA : olimB);
     xk = k;
   }
   const double* baseA = A.ptr + op_addr(A, m0, 0);
@@ -618,7 +627,7 @@ __device__ __forceinline__ void mac_loop_dma(
         dma16(rb, base + kRegA + (q - NDA) * TB::kRpd * TB::kPitch, off);
     }
     if (xwave)
-      dma16(XT == kStripRow ? ra : rb,
+      dma16((XT & kStripRow) != 0 ? ra : rb,
             base + kRegA + kRegB + wave * TX::kRpd * TX::kPitch,
             (xin && xk < klim) ? xrel : kOOB);
   };
@@ -661,19 +670,19 @@ __device__ __forceinline__ void mac_loop_dma(
 #pragma unroll 1
   for (; kk + 1 < kend; ++kk) {
     issue(st == 0 ? NS - 1 : st - 1, kk + NS - 1);
-    dma_mma<BM, BN, BK, WM, WN, MASKED, XT, 0, 3>(acc, xacc, f0, xf, mask,
-                                                   wm, wn);
+    dma_mma<BM, BN, BK, WM, WN, MASKED, XT, 0, 3>(acc, xacc, cacc, f0, xf,
+                                                   mask, wm, wn);
     publish();
     st = st + 1 == NS ? 0 : st + 1;
     read(f1, st);
     __builtin_amdgcn_sched_barrier(0);  // reads issue before the MFMAs
-    dma_mma<BM, BN, BK, WM, WN, MASKED, XT, 3, 4>(acc, xacc, f0, xf, mask,
-                                                   wm, wn);
+    dma_mma<BM, BN, BK, WM, WN, MASKED, XT, 3, 4>(acc, xacc, cacc, f0, xf,
+                                                   mask, wm, wn);
     f0 = f1;
     readx(st);
   }
-  dma_mma<BM, BN, BK, WM, WN, MASKED, XT, 0, 4>(acc, xacc, f0, xf, mask, wm,
-                                                 wn);
+  dma_mma<BM, BN, BK, WM, WN, MASKED, XT, 0, 4>(acc, xacc, cacc, f0, xf, mask,
+                                                 wm, wn);
   __builtin_amdgcn_s_waitcnt(0);  // clamped tail copies
   __syncthreads();
 }
@@ -701,9 +710,9 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
   using S = TileShape<BM, BN, BK, WM, WN>;
   constexpr int NT = S::NT, TM = S::TM, TN = S::TN, MB = S::MB, NB = S::NB;
   static_assert(!XS || (NS >= 3 && !BATCHED), "strip: single DMA launches");
-  // Partial-tile slot of one workgroup (the strip block of each wave after
-  // the tile's blocks).
-  constexpr int64_t kSlot = (int64_t)BM * BN + (XS ? 4 * NT : 0);
+  // Partial-tile slot of one workgroup (the strip block and the corner
+  // block of each wave after the tile's blocks).
+  constexpr int64_t kSlot = (int64_t)BM * BN + (XS ? 8 * NT : 0);
 
   extern __shared__ __align__(16) double lds[];
 
@@ -792,6 +801,8 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
     // remainder of at most 16 rides on it.
     const bool xt = XS == kStripCol && L.single.xcol > 0 && tj == tiles_n - 1;
     const bool yt = XS == kStripRow && L.single.xrow > 0 && ti == tiles_m - 1;
+    // SYRK: wave 0 of the last diagonal tile also forms the corner block.
+    const bool corner = yt && syrk && tj == ti && wave == 0;
 
     v4d acc[MB][NB];
 #pragma unroll
@@ -799,6 +810,7 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
 #pragma unroll
       for (int j = 0; j < NB; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
     v4d xacc = v4d{0.0, 0.0, 0.0, 0.0};
+    v4d cacc = v4d{0.0, 0.0, 0.0, 0.0};
     // This lane's strip outputs: rows srow + 4 v, column scol.
     const int srow = (yt ? m0 + BM : m0 + wm * TM + wn * 16) + lq;
     const int scol = (yt ? n0 + wn * TN + wm * 16 : n0 + BN) + lr;
@@ -832,15 +844,20 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
     if constexpr (NS >= 3 && (V & 1)) {                                      \
       /* strip tiles unmasked: edge blocks are computed on zero-filled     \
          operands and not stored */                                          \
-      if (XS != 0 && (xt || yt))                                             \
+      if (XS == kStripRow && corner)                                         \
         mac_loop_dma<BM, BN, BK, WM, WN, (V >> 2) & 1, (V >> 1) & 1, false,  \
-                     NS, XS>(acc, xacc, A, B, K, m0, n0, c * kci, kb, ke,    \
-                             lds, tid, mask);                                \
+                     NS, XS == kStripRow ? kStripCorner : 0>(                \
+            acc, xacc, cacc, A, B, K, m0, n0, c * kci, kb, ke, lds, tid,     \
+            mask);                                                           \
+      else if (XS != 0 && (xt || yt))                                        \
+        mac_loop_dma<BM, BN, BK, WM, WN, (V >> 2) & 1, (V >> 1) & 1, false,  \
+                     NS, XS>(acc, xacc, cacc, A, B, K, m0, n0, c * kci, kb,  \
+                             ke, lds, tid, mask);                            \
       else                                                                   \
         mac_loop_dma<BM, BN, BK, WM, WN, (V >> 2) & 1, (V >> 1) & 1,         \
-                     ((V >> 3) & 1) != 0, NS, 0>(acc, xacc, A, B, K, m0, n0, \
-                                                 c * kci, kb, ke, lds, tid,  \
-                                                 mask);                      \
+                     ((V >> 3) & 1) != 0, NS, 0>(acc, xacc, cacc, A, B, K,   \
+                                                 m0, n0, c * kci, kb, ke,    \
+                                                 lds, tid, mask);            \
     } else if constexpr (NS >= 3)                                            \
       if (tid == 0) atomicOr(L.status, 2);                                   \
     if constexpr (NS == 0)                                                   \
@@ -879,9 +896,12 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
                                __HIP_MEMORY_SCOPE_AGENT);
       if (XS != 0 && (xt || yt)) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
+        for (int v = 0; v < 4; ++v) {
           __hip_atomic_store(slot + (MB * NB * 4 + v) * NT + tid, xacc[v],
                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(slot + (MB * NB * 4 + 4 + v) * NT + tid, cacc[v],
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
@@ -920,10 +940,14 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
                   __HIP_MEMORY_SCOPE_AGENT);
         if (XS != 0 && (xt || yt)) {
 #pragma unroll
-          for (int v = 0; v < 4; ++v)
+          for (int v = 0; v < 4; ++v) {
             xacc[v] += __hip_atomic_load(slot + (MB * NB * 4 + v) * NT + tid,
                                          __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT);
+            cacc[v] += __hip_atomic_load(
+                slot + (MB * NB * 4 + 4 + v) * NT + tid, __ATOMIC_RELAXED,
+                __HIP_MEMORY_SCOPE_AGENT);
+          }
         }
       }
     }
@@ -965,6 +989,15 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
             if (row < M && scol < N) pb[(int64_t)row * N + scol] = xacc[v];
           }
         }
+        if (XS == kStripRow && corner) {
+          const int col = n0 + BN + lr;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = m0 + BM + lq + 4 * v;
+            if (row < M && col < N && row >= col)
+              pb[(int64_t)row * N + col] = cacc[v];
+          }
+        }
         it = t0 + ke;
         continue;
       }
@@ -995,6 +1028,19 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
           const int row = srow + 4 * v;
           if (row < M && scol < N) {
             double val = alpha * xacc[v];
+            if (beta != 0.0) val = fma(beta, dst[v * rstep], val);
+            dst[v * rstep] = val + gamma;
+          }
+        }
+      }
+      if (XS == kStripRow && corner) {
+        const int col = n0 + BN + lr;
+        gdouble* dst = dbase + out_addr(D, m0 + BM + lq, col);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = m0 + BM + lq + 4 * v;
+          if (row < M && col < N && row >= col) {
+            double val = alpha * cacc[v];
             if (beta != 0.0) val = fma(beta, dst[v * rstep], val);
             dst[v * rstep] = val + gamma;
           }
@@ -1094,6 +1140,7 @@ static int occupancy_t() {
     case 9: return FN<128, 64, 16, 2, 2, 4>(__VA_ARGS__);   \
     case 10: return FN<64, 64, 16, 2, 2, 4>(__VA_ARGS__);   \
     case 11: return FN<128, 128, 16, 4, 2, 4, true>(__VA_ARGS__); \
+    case 12: return FN<64, 64, 16, 2, 2, 4, true>(__VA_ARGS__);   \
     default: break;                                        \
   }
 

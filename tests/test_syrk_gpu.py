@@ -167,3 +167,54 @@ def test_every_square_tile_column_major(skg, ctx, orc, rng):
             assert err < TOL, (tile, grid, kchunks, err)
     ctx.set_tuning(-1, 0)
     ctx.set_kchunks(0)
+
+
+@pytest.mark.parametrize("tile,N,K", [(12, 72, 100), (12, 80, 33),
+                                      (12, 130, 520), (12, 200, 64),
+                                      (11, 136, 70), (11, 144, 37)])
+def test_strip_tiles(skg, ctx, rng, tile, N, K):
+    """Strip tiles on a SYRK: N = T x BM + r (r <= 16) tiles a T x T
+    triangle whose last tile row carries the r remainder rows; wave 0 of the
+    last diagonal tile forms the r x r corner from the strip alone.  Every
+    layout, alpha and beta, default grid, split tiles and K chunks; the
+    upper triangle is never written.  (N = 200 has remainder 8 on 64-wide
+    tiles; tile 12 is 64 x 64, tile 11 128 x 128.)"""
+    A = rng.uniform(-1, 1, (K, N))
+    C0 = rng.uniform(-1, 1, (N, N))
+    ref = 0.5 * (A.T @ A) - 1.25 * C0
+    il, iu = np.tril_indices(N), np.triu_indices(N, 1)
+    try:
+        for grid, kchunks in ((0, 1), (3, 1), (0, 3)):
+            ctx.set_tuning(tile, grid)
+            ctx.set_kchunks(kchunks)
+            for layout in range(4):
+                Ab, lda = encode_b(A, layout, 2)
+                C = C0.copy().ravel()
+                ctx.dsyrk_host(layout, N, K, 0.5, Ab, lda, -1.25, C, N)
+                Cm = C.reshape(N, N)
+                err = rel_fro(Cm[il], ref[il])
+                assert err < TOL, (grid, kchunks, layout, err)
+                assert np.array_equal(Cm[iu], C0[iu])
+    finally:
+        ctx.set_tuning(-1, 0)
+        ctx.set_kchunks(0)
+
+
+def test_strip_c2_syrk(skg, ctx, rng):
+    """The c2 SYRK (Psi 32768 x 520 column-major) on the 64 x 64 strip tile:
+    8 x 9 / 2 = 36 tiles (the plan reports them), the 8 remainder rows on
+    the last tile row and the 8 x 8 corner."""
+    Nd, N = 32768, 520
+    ctx.set_tuning(12, 0)
+    try:
+        plan = ctx.query_plan(1, N, N, Nd, skg.COL_MAJOR)
+        assert plan["tile"] == 12 and plan["tiles"] == 36
+        psi = rng.standard_normal((Nd, N)) / np.sqrt(Nd)
+        Ab, lda = encode_b(psi, 1, 0)
+        C = np.zeros(N * N)
+        ctx.dsyrk_host(skg.COL_MAJOR, N, Nd, 1.0, Ab, lda, 0.0, C, N)
+    finally:
+        ctx.set_tuning(-1, 0)
+    ref = psi.T @ psi
+    il = np.tril_indices(N)
+    assert rel_fro(C.reshape(N, N)[il], ref[il]) < TOL
