@@ -440,6 +440,8 @@ void skgemm_ctx::free_all() {
   status = nullptr;
   free_extra();
   free_batch();
+  for (auto& kv : split_tables) hipFree(kv.second);
+  split_tables.clear();
 }
 
 void skgemm_ctx::record_begin() {
@@ -592,6 +594,86 @@ int fill_problem(skgemm_ctx* c, Problem* p, int kind, const Plan& pl, int M,
   return SKGEMM_OK;
 }
 
+// Cost-weighted Stream-K split for problems with strip tiles.  A strip
+// adds one MFMA block per wave and sub-step to its tile (two for wave 0 of
+// a SYRK's corner tile), so its iterations take (wb + 1) / wb of a plain
+// tile's (wb = blocks per wave); an even split of iterations would leave
+// the workgroups that hold strip tiles on the critical path.  Workgroup w
+// starts at the first iteration whose weighted prefix reaches w / G of the
+// total.  The table is computed once per shape and plan and kept on the
+// device; while a stream capture is active and no table exists yet, the
+// even split is used (no synchronous copy inside a capture).
+static const int64_t* split_table(skgemm_ctx* c, const Problem& p, int cfg,
+                                  int G) {
+  if (p.xrow == 0 && p.xcol == 0) return nullptr;
+  const std::vector<int64_t> key = {p.syrk,     p.M,        p.N,
+                                    p.K,        cfg,        G,
+                                    p.kchunks,  p.kci,      p.row_order,
+                                    p.xrow,     p.xcol,     p.tiles_m,
+                                    p.tiles_n};
+  auto found = c->split_tables.find(key);
+  if (found != c->split_tables.end()) return found->second;
+  if (c->capturing || c->split_tables.size() >= 64) return nullptr;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess ||
+      cs != hipStreamCaptureStatusNone)
+    return nullptr;
+  const TileCfg& t = tile_cfg(cfg);
+  const int wb = (t.bm / t.wm / 16) * (t.bn / t.wn / 16);
+  std::vector<int> wt((size_t)p.num_tiles);
+  for (int64_t tile = 0; tile < p.num_tiles; ++tile) {
+    int64_t ti, tj;
+    if (p.syrk) {
+      ti = (int64_t)((std::sqrt(8.0 * (double)tile + 1.0) - 1.0) * 0.5);
+      while (ti * (ti + 1) / 2 > tile) --ti;
+      while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
+      tj = tile - ti * (ti + 1) / 2;
+    } else if (p.row_order) {
+      ti = tile / p.tiles_n;
+      tj = tile % p.tiles_n;
+    } else {
+      ti = tile % p.tiles_m;
+      tj = tile / p.tiles_m;
+    }
+    const bool strip = (p.xrow && ti == p.tiles_m - 1) ||
+                       (p.xcol && tj == p.tiles_n - 1);
+    wt[tile] = wb + (strip ? 1 : 0) + (strip && p.syrk && ti == tj ? 1 : 0);
+  }
+  const int64_t units = p.num_tiles * p.kchunks;
+  const int64_t total = units * p.kci;
+  double per_chunk = 0.0;
+  for (int w : wt) per_chunk += w;
+  const double W = per_chunk * p.kchunks * p.kci;
+  std::vector<int64_t> st((size_t)G + 1);
+  st[0] = 0;
+  st[G] = total;
+  int64_t u = 0;
+  double cum = 0.0;  // weighted prefix at the start of unit u
+  for (int w = 1; w < G; ++w) {
+    const double target = W * (double)w / (double)G;
+    while (u < units &&
+           cum + (double)wt[u % p.num_tiles] * p.kci <= target) {
+      cum += (double)wt[u % p.num_tiles] * p.kci;
+      ++u;
+    }
+    int64_t i = u * p.kci;
+    if (u < units) {
+      const double wu = wt[u % p.num_tiles];
+      i += std::min<int64_t>(p.kci, (int64_t)std::ceil((target - cum) / wu));
+    }
+    st[w] = std::max(st[w - 1], std::min(i, total));
+  }
+  int64_t* d = nullptr;
+  if (hipMalloc(&d, sizeof(int64_t) * st.size()) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, st.data(), sizeof(int64_t) * st.size(),
+                hipMemcpyHostToDevice) != hipSuccess) {
+    hipFree(d);
+    return nullptr;
+  }
+  c->split_tables[key] = d;
+  return d;
+}
+
 // Launch one problem (already described) with the given plan.
 int run_single(skgemm_ctx* c, Problem& p, const Plan& plan_in,
                const int* pred, bool timed) {
@@ -619,6 +701,7 @@ int run_single(skgemm_ctx* c, Problem& p, const Plan& plan_in,
   L.num_problems = 1;
   L.total_iters = p.num_tiles * p.kchunks * p.kci;
   L.grid = pl.grid;
+  L.starts = split_table(c, p, pl.cfg, pl.grid);
   SKG_TRY(c->ensure_ws((size_t)pl.grid * slot_doubles(pl.cfg), pl.grid));
   L.ws = c->ws;
   L.flags = c->flags;

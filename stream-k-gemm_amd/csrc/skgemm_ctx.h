@@ -59,6 +59,10 @@ struct skgemm_ctx {
   int forced_kchunks = 0;
   std::map<std::tuple<int, int, int, int, int, int>, skg::Plan> tuned;
   std::map<int, int> occ_cache;
+  // Cost-weighted Stream-K splits (device tables of grid + 1 first
+  // iterations) per problem shape and plan; kept until the context ends, so
+  // a captured graph may hold one.
+  std::map<std::vector<int64_t>, int64_t*> split_tables;
 
   // Split-form workspace (skgemm_ozaki.hip) and batched-launch state
   // (skgemm_batch.cpp).  While a launch sequence is being captured into a

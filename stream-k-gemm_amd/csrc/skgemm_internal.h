@@ -68,6 +68,9 @@ struct Launch {
   int num_problems;         // single launches (batched: from hdr)
   int64_t total_iters;
   int grid;             // persistent workgroups
+  // Single launches: first flattened iteration of each workgroup (grid + 1
+  // entries, a cost-weighted split), or nullptr for the even split.
+  const int64_t* starts;
   double* ws;           // grid partial tiles of BM*BN doubles
   int* flags;           // grid flags: 0 empty, 1 partial ready
   int* status;          // bit 0: a fix-up wait hit its spin limit

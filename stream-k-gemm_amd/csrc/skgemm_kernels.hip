@@ -739,8 +739,14 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
   const int64_t total =
       BATCHED ? load_uniform(&L.hdr[1]) : L.total_iters;
   const int nprob = BATCHED ? (int)load_uniform(&L.hdr[0]) : 1;
-  int64_t it = wg_start(w, total, G);
-  const int64_t end = wg_start(w + 1, total, G);
+  // Iteration range of workgroup x: the even split, or the cost-weighted
+  // split the host tabulated (strip tiles cost more per iteration).
+  auto wstart = [&](int x) -> int64_t {
+    return (!BATCHED && L.starts != nullptr) ? load_uniform(&L.starts[x])
+                                             : wg_start(x, total, G);
+  };
+  int64_t it = wstart(w);
+  const int64_t end = wstart(w + 1);
   int pi = 0;
   const Problem* tab = L.problems;
   if (BATCHED && it < end) {
@@ -912,8 +918,8 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
     } else if (ke < kci) {
       // Owner of a split unit: add the peers' partials in peer order.
       const int64_t tile_end = t0 + kci;
-      for (int w2 = w + 1; w2 < G && wg_start(w2, total, G) < tile_end;
-           ++w2) {
+      for (int w2 = w + 1; w2 < G && wstart(w2) < tile_end; ++w2) {
+        if (wstart(w2 + 1) == wstart(w2)) continue;  // empty range: no part
         if (tid == 0) {
           int64_t n = 0;
           while (__hip_atomic_load(&L.flags[w2], __ATOMIC_RELAXED,
