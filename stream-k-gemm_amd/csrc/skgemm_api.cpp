@@ -223,8 +223,8 @@ extern "C" int skgemm_create(int device, skgemm_ctx** out) {
   if ((e = hipEventCreate(&c->ev0)) != hipSuccess ||
       (e = hipEventCreate(&c->ev1)) != hipSuccess ||
       (e = hipMalloc(&c->status, sizeof(int))) != hipSuccess ||
-      (e = hipMemset(c->status, 0, sizeof(int))) != hipSuccess ||
-      (e = hipDeviceSynchronize()) != hipSuccess) {
+      (e = launch_zero_ints(c->status, 1, c->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(c->stream)) != hipSuccess) {
     skgemm_destroy(c);
     return hip_error(e, "context setup");
   }
@@ -333,20 +333,26 @@ extern "C" int skgemm_memcpy_d2h(skgemm_ctx* c, void* dst, const void* src,
 
 // ---- skgemm_ctx members -------------------------------------------------------
 
+// Status readback on the context's own stream (called after it has been
+// synchronized, never inside a capture): operations on the legacy stream
+// are refused while another thread captures a graph.
 int skgemm_ctx::check_status() {
   int h = 0;
-  SKG_HIP(hipMemcpy(&h, status, sizeof(int), hipMemcpyDeviceToHost),
+  SKG_HIP(hipMemcpyAsync(&h, status, sizeof(int), hipMemcpyDeviceToHost,
+                         stream),
           "status readback");
+  SKG_HIP(hipStreamSynchronize(stream), "status readback");
   if (h & 2) {
-    hipMemset(status, 0, sizeof(int));
+    launch_zero_ints(status, 1, stream);
+    hipStreamSynchronize(stream);
     return set_error(SKGEMM_ERR_NOTSUP,
                      "internal: an LDS-DMA tile configuration was given an "
                      "operand without 16-byte aligned pairs");
   }
   if (h != 0) {
-    hipMemset(status, 0, sizeof(int));
-    if (flags) hipMemset(flags, 0, sizeof(int) * flags_n);
-    hipDeviceSynchronize();
+    launch_zero_ints(status, 1, stream);
+    if (flags) launch_zero_ints(flags, flags_n, stream);
+    hipStreamSynchronize(stream);
     return set_error(SKGEMM_ERR_TIMEOUT,
                      "a Stream-K fix-up wait exceeded its spin limit (%lld)",
                      (long long)spin_limit);
@@ -382,10 +388,16 @@ int skgemm_ctx::ensure_ws(size_t ws_doubles, int nflags) {
       flags_n = 0;
       return set_error(SKGEMM_ERR_NOMEM, "flags: %s", hipGetErrorString(e));
     }
-    // Flags start at zero; every consumer resets the flag it consumed, so
-    // they are zero again at the end of every launch.
-    SKG_HIP(hipMemset(flags, 0, sizeof(int) * nflags), "hipMemset flags");
-    SKG_HIP(hipDeviceSynchronize(), "flags init");
+    // Flags start at zero: a fill kernel on this stream, completed before
+    // returning (a later call may run on another stream); inside a capture
+    // it becomes a graph node ahead of the launch, harmless on replays since
+    // the flags are zero between launches.  Every consumer resets the flag
+    // it consumed, so they are zero again at the end of every launch.
+    SKG_HIP(launch_zero_ints(flags, nflags, stream), "flags init");
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (!capturing && hipStreamIsCapturing(stream, &cs) == hipSuccess &&
+        cs == hipStreamCaptureStatusNone)
+      SKG_HIP(hipStreamSynchronize(stream), "flags init");
     flags_n = nflags;
   }
   return SKGEMM_OK;
@@ -665,8 +677,9 @@ static const int64_t* split_table(skgemm_ctx* c, const Problem& p, int cfg,
   }
   int64_t* d = nullptr;
   if (hipMalloc(&d, sizeof(int64_t) * st.size()) != hipSuccess) return nullptr;
-  if (hipMemcpy(d, st.data(), sizeof(int64_t) * st.size(),
-                hipMemcpyHostToDevice) != hipSuccess) {
+  if (hipMemcpyAsync(d, st.data(), sizeof(int64_t) * st.size(),
+                     hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
     hipFree(d);
     return nullptr;
   }

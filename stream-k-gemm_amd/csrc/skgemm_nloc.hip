@@ -129,12 +129,15 @@ static void nloc_free(skgemm_nloc* h) {
 }
 
 template <typename T>
-static int upload(T** dst, const T* src, size_t n, const char* what) {
+static int upload(T** dst, const T* src, size_t n, const char* what,
+                  hipStream_t stream) {
   if (n == 0) return SKGEMM_OK;
   if (hipMalloc(dst, n * sizeof(T)) != hipSuccess)
     return set_error(SKGEMM_ERR_NOMEM, "nloc %s (%zu bytes)", what,
                      n * sizeof(T));
-  if (hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+  if (hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice,
+                     stream) != hipSuccess ||
+      hipStreamSynchronize(stream) != hipSuccess)
     return set_error(SKGEMM_ERR_HIP, "nloc %s upload", what);
   return SKGEMM_OK;
 }
@@ -229,16 +232,16 @@ extern "C" int skgemm_nloc_create(skgemm_ctx* c, int Nd, int num_proj,
                ? SKGEMM_OK
                : set_error(SKGEMM_ERR_HIP, "hipSetDevice");
   if (rc == SKGEMM_OK)
-    rc = upload(&h->index, index_all, (size_t)h->rows, "index");
+    rc = upload(&h->index, index_all, (size_t)h->rows, "index", c->stream);
   if (rc == SKGEMM_OK)
-    rc = upload(&h->chi, chi_all, (size_t)h->choff[num_proj], "chi");
+    rc = upload(&h->chi, chi_all, (size_t)h->choff[num_proj], "chi", c->stream);
   if (rc == SKGEMM_OK)
-    rc = upload(&h->factor, factor.data(), factor.size(), "gamma");
-  if (rc == SKGEMM_OK) rc = upload(&h->point, point.data(), point.size(), "points");
+    rc = upload(&h->factor, factor.data(), factor.size(), "gamma", c->stream);
+  if (rc == SKGEMM_OK) rc = upload(&h->point, point.data(), point.size(), "points", c->stream);
   if (rc == SKGEMM_OK && h->npoints > 0)
-    rc = upload(&h->ptr, ptr.data(), ptr.size(), "point table");
+    rc = upload(&h->ptr, ptr.data(), ptr.size(), "point table", c->stream);
   if (rc == SKGEMM_OK)
-    rc = upload(&h->rows_of, rows_of.data(), rows_of.size(), "row table");
+    rc = upload(&h->rows_of, rows_of.data(), rows_of.size(), "row table", c->stream);
   if (rc != SKGEMM_OK) {
     nloc_free(h);
     delete h;
@@ -308,14 +311,19 @@ static int nloc_tables(skgemm_nloc* h, int nb, double dv) {
     return set_error(SKGEMM_ERR_NOMEM, "nloc problem tables");
   const int64_t hdr[4] = {(int64_t)h->fw.rows.size(), h->fw.iters,
                           (int64_t)h->bw.rows.size(), h->bw.iters};
-  SKG_HIP(hipMemcpy(h->d_fw, h->fw.rows.data(),
-                    h->fw.rows.size() * sizeof(Problem), hipMemcpyHostToDevice),
+  // On the context's stream, completed before returning (hdr is local).
+  SKG_HIP(hipMemcpyAsync(h->d_fw, h->fw.rows.data(),
+                         h->fw.rows.size() * sizeof(Problem),
+                         hipMemcpyHostToDevice, c->stream),
           "nloc table upload");
-  SKG_HIP(hipMemcpy(h->d_bw, h->bw.rows.data(),
-                    h->bw.rows.size() * sizeof(Problem), hipMemcpyHostToDevice),
+  SKG_HIP(hipMemcpyAsync(h->d_bw, h->bw.rows.data(),
+                         h->bw.rows.size() * sizeof(Problem),
+                         hipMemcpyHostToDevice, c->stream),
           "nloc table upload");
-  SKG_HIP(hipMemcpy(h->d_hdr, hdr, sizeof(hdr), hipMemcpyHostToDevice),
+  SKG_HIP(hipMemcpyAsync(h->d_hdr, hdr, sizeof(hdr), hipMemcpyHostToDevice,
+                         c->stream),
           "nloc table upload");
+  SKG_HIP(hipStreamSynchronize(c->stream), "nloc table upload");
   h->table_nb = nb;
   h->fw_dv = dv;
   h->table_cfg = c->forced_cfg;
