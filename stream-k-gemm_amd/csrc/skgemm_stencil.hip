@@ -5,13 +5,14 @@
 // Layouts: local grid [ceil(nb/16)][K][16], K = ni nj nk, g = i + j ni +
 // k ni nj; extended grid [ceil(nb/16)][K_ex][16] with the interior at offset
 // (F, F, F), K_ex = (ni+2F)(nj+2F)(nk+2F); output [ceil(nb/T)][K][T], T = 32
-// or 16.  A grid point's 16 bands are 128 contiguous bytes, so one wave (16
+// or 16.  A grid point's 16 bands are 128 contiguous bytes, so a wave (16
 // band lanes x 4 consecutive grid points along i) reads and writes 512-byte
-// runs.  Both kernels are HBM-bound: the stencil's 6F + 1 reads per output
-// come from L2 / L1 (neighbours along i are in the same or adjacent lines,
-// along j and k in lines that the neighbouring workgroups of the same plane
-// and of the next planes read too), so its algorithmic traffic is the
-// extended input once, the diagonal once and the output once.
+// runs.  Both kernels are HBM-bound.  The stencil takes its x / y
+// neighbours from an LDS copy of each plane (the block's points and their
+// cross-shaped halo) and its z neighbours from registers, so the extended
+// input is read about once from HBM (the halo re-reads hit L2); its
+// algorithmic traffic is the extended input once, the diagonal once and the
+// output once.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -29,57 +30,135 @@ struct StencilCoef {
   double c0;
 };
 
-// One workgroup: 16 band lanes x 16 consecutive i of one j row of one
-// 16-band tile, marching through kStencilPlanes planes of k; grid
-// (ceil(ni/16), nj, tiles16 * ceil(nk / kStencilPlanes)).  Each lane keeps
-// its own column's 2F + 1 values along k in registers, so a plane step loads
-// one new z value and the 4F x / y neighbours (lines that the lanes and the
-// neighbouring rows' workgroups of the same plane share in L1 / L2), not 6F.
+// One workgroup: 16 band lanes x an 8 x 8 block of (i, j) points of one
+// 16-band tile (1024 threads), marching through kStencilPlanes planes of k;
+// grid (ceil(ni/8) ceil(nj/8), ceil(nk/kStencilPlanes), tiles16).  A thread
+// keeps its column's 2F + 1 values along k in registers (one new z load per
+// plane).  Per plane the block's centres and the cross-shaped x / y halo
+// (F points either side of each block row and column, 32 F points) go into
+// LDS, from which the 4F x / y neighbours are read: about 1 + 32F/64 global
+// loads per output instead of 4F + 1.  The next plane's halo is loaded into
+// registers while the current plane is computed; the barriers wait on LDS
+// only, so those loads stay in flight across them.
 constexpr int kStencilPlanes = 8;
+constexpr int kSB = 8;  // block edge (points along i and along j)
 
+// Two workgroups per CU (8 waves per SIMD, at most 64 VGPRs) up to FDn 6:
+// 157 us against 170 us with one on the c2 grid (32^3, 520 bands, FDn 6);
+// FDn 7 and 8 would spill there and keep one.
 template <int F>
-__global__ __launch_bounds__(256) void stencil_kernel(
+__global__ __launch_bounds__(1024)
+__attribute__((amdgpu_waves_per_eu(F <= 6 ? 8 : 4, 8))) void stencil_kernel(
     const double* __restrict__ ext, const double* __restrict__ A_diag, int ni,
     int nj, int nk, int nb, StencilCoef c, double* __restrict__ out,
     int tshift) {
-  const int bl = threadIdx.x & 15;
-  const int i = blockIdx.x * 16 + (threadIdx.x >> 4);
-  const int j = blockIdx.y;
-  const int kchunks = (nk + kStencilPlanes - 1) / kStencilPlanes;
-  const int t16 = blockIdx.z / kchunks;
-  const int k0 = (blockIdx.z % kchunks) * kStencilPlanes;
+  constexpr int E = kSB + 2 * F;             // LDS plane edge
+  constexpr int kHalo = 4 * kSB * F;         // halo points per plane
+  constexpr int kHs = (kHalo + 63) / 64;     // halo points per thread
+  __shared__ double sl[E * E * 16];          // [j + F][i + F][band]
+  const int tid = threadIdx.x;
+  const int bl = tid & 15;
+  const int p = tid >> 4;                    // 0..63
+  const int nbi = (ni + kSB - 1) / kSB;
+  const int i0 = (blockIdx.x % nbi) * kSB, j0 = (blockIdx.x / nbi) * kSB;
+  const int pi = p % kSB, pj = p / kSB;
+  const int i = i0 + pi, j = j0 + pj;
+  const int k0 = blockIdx.y * kStencilPlanes;
   const int k1 = k0 + kStencilPlanes < nk ? k0 + kStencilPlanes : nk;
+  const int t16 = blockIdx.z;
   const int b = t16 * 16 + bl;
-  if (i >= ni || b >= nb) return;
+  const bool band_ok = b < nb;
+  const bool own = band_ok && i < ni && j < nj;
+  // Block points past the domain edge but inside the extended grid hold
+  // ghost values that the edge points read as x / y neighbours.
+  const bool inext = band_ok && i < ni + F && j < nj + F;
   const int64_t K = (int64_t)ni * nj * nk;
-  const int64_t sj = ni + 2 * F;
+  const int nie = ni + 2 * F;
+  const int64_t sj = nie;
   const int64_t sk = sj * (nj + 2 * F);
   const int64_t Kex = sk * (nk + 2 * F);
   const double* s = ext + (int64_t)t16 * Kex * 16 + bl;
   const int T = 1 << tshift;
   double* o = out + ((int64_t)(b >> tshift) * K) * T + (b & (T - 1));
-  // Column (i, j): extended index of plane k is col + (k + F) sk.
-  const int64_t col = (i + F) + (j + F) * sj;
+
+  // Halo points of this thread: index h = p + 64 m.  h < 2 kSB F: x halo of
+  // block row h / (2F), offset h % (2F) -> i = -F .. -1, kSB .. kSB + F - 1;
+  // else the y halo of block column (h') / (2F) likewise along j.
+  int hoff[kHs];     // LDS offset (points) of halo point m, -1: none
+  int64_t hcol[kHs]; // extended column index (i + F) + (j + F) sj
+#pragma unroll
+  for (int m = 0; m < kHs; ++m) {
+    const int h = p + 64 * m;
+    hoff[m] = -1;
+    hcol[m] = 0;
+    if (h < kHalo && band_ok) {
+      int hi, hj;
+      const int x = h < 2 * kSB * F ? h : h - 2 * kSB * F;
+      const int line = x / (2 * F), d = x % (2 * F);
+      const int a = d < F ? d - F : kSB + d - F;
+      if (h < 2 * kSB * F) {
+        hi = a;
+        hj = line;
+      } else {
+        hi = line;
+        hj = a;
+      }
+      // Used only by block points inside the grid; those halo points lie
+      // inside the extended grid.
+      const int gi = i0 + hi, gj = j0 + hj;
+      const bool need = (h < 2 * kSB * F) ? (j0 + line < nj && gi < ni + F)
+                                           : (i0 + line < ni && gj < nj + F);
+      if (need) {
+        hoff[m] = (hj + F) * E + (hi + F);
+        hcol[m] = (int64_t)(gi + F) + (int64_t)(gj + F) * sj;
+      }
+    }
+  }
+  // Column of this thread's own point.
+  const int64_t col = (int64_t)(i + F) + (int64_t)(j + F) * sj;
   double q[2 * F + 1];  // q[F + d] = psi at plane k + d
+  double hv[kHs];
 #pragma unroll
-  for (int d = 0; d < 2 * F; ++d) q[d] = s[(col + (k0 + d) * sk) * 16];
+  for (int d = 0; d < 2 * F; ++d)
+    q[d] = inext ? s[(col + (int64_t)(k0 + d) * sk) * 16] : 0.0;
+  auto load_halo = [&](int k) {
+#pragma unroll
+    for (int m = 0; m < kHs; ++m)
+      hv[m] = hoff[m] >= 0 ? s[(hcol[m] + (int64_t)(k + F) * sk) * 16] : 0.0;
+  };
+  load_halo(k0);
+  const int me = ((pj + F) * E + (pi + F)) * 16 + bl;
   for (int k = k0; k < k1; ++k) {
-    q[2 * F] = s[(col + (k + 2 * F) * sk) * 16];
-    const int64_t gex = col + (k + F) * sk;
-    const int64_t g = i + (int64_t)j * ni + (int64_t)k * ni * nj;
-    // Same order as the reference's scalar version: centre, x terms, y
-    // terms, z terms, r ascending in each (each term one fused
-    // multiply-add).
-    double res = q[F] * (A_diag[g] + c.c0);
+    q[2 * F] = inext ? s[(col + (int64_t)(k + 2 * F) * sk) * 16] : 0.0;
+    sl[me] = q[F];
 #pragma unroll
-    for (int r = 1; r <= F; ++r)
-      res = fma(c.x[r], s[(gex + r) * 16] + s[(gex - r) * 16], res);
+    for (int m = 0; m < kHs; ++m)
+      if (hoff[m] >= 0) sl[hoff[m] * 16 + bl] = hv[m];
+    if (k + 1 < k1) load_halo(k + 1);
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): LDS stores done
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (own) {
+      const int64_t g = i + (int64_t)j * ni + (int64_t)k * ni * nj;
+      // Same order as the reference's scalar version: centre, x terms, y
+      // terms, z terms, r ascending in each (each term one fused
+      // multiply-add).
+      double res = q[F] * (A_diag[g] + c.c0);
 #pragma unroll
-    for (int r = 1; r <= F; ++r)
-      res = fma(c.y[r], s[(gex + r * sj) * 16] + s[(gex - r * sj) * 16], res);
+      for (int r = 1; r <= F; ++r)
+        res = fma(c.x[r], sl[me + r * 16] + sl[me - r * 16], res);
 #pragma unroll
-    for (int r = 1; r <= F; ++r) res = fma(c.z[r], q[F + r] + q[F - r], res);
-    o[g * T] = res;
+      for (int r = 1; r <= F; ++r)
+        res = fma(c.y[r], sl[me + r * E * 16] + sl[me - r * E * 16], res);
+#pragma unroll
+      for (int r = 1; r <= F; ++r) res = fma(c.z[r], q[F + r] + q[F - r], res);
+      o[g * T] = res;
+    }
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // this thread's LDS reads done
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
 #pragma unroll
     for (int d = 0; d < 2 * F; ++d) q[d] = q[d + 1];
   }
@@ -187,12 +266,13 @@ extern "C" int skgemm_stencil_apply(skgemm_ctx* c, int ni, int nj, int nk,
   cf.c0 = coef_0;
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
   const int kchunks = (nk + kStencilPlanes - 1) / kStencilPlanes;
-  const dim3 grid((ni + 15) / 16, nj, kchunks * ((nb + 15) / 16));
+  const dim3 grid(((ni + kSB - 1) / kSB) * ((nj + kSB - 1) / kSB), kchunks,
+                  (nb + 15) / 16);
   const int ts = out_tile == 32 ? 5 : 4;
   c->record_begin();
 #define SKG_STENCIL(F)                                                     \
   case F:                                                                  \
-    hipLaunchKernelGGL(stencil_kernel<F>, grid, dim3(256), 0, c->stream,   \
+    hipLaunchKernelGGL(stencil_kernel<F>, grid, dim3(1024), 0, c->stream,  \
                        ext16, A_diag, ni, nj, nk, nb, cf, out, ts);        \
     break;
   switch (FDn) {
