@@ -166,33 +166,45 @@ __attribute__((amdgpu_waves_per_eu(F <= 6 ? 8 : 4, 8))) void stencil_kernel(
 
 // Every extended point of every tile: the interior copies the local point,
 // the halo the wrapped one (mode 1), zero (mode 0) or is left alone (mode
-// 2).  Grid (ceil(ni_ex/16), nj_ex, nk_ex * tiles16), 16 band lanes x 16
-// points along i.
+// 2).  A workgroup (16 band lanes x 16 points along i) fills whole rows of
+// kHaloPlanes planes of one j; grid (nj_ex, ceil(nk_ex / kHaloPlanes) x
+// tiles16).  The j and k wraps are formed once per row.
+constexpr int kHaloPlanes = 4;
+
+__device__ __forceinline__ int wrap(int x, int n) { return ((x % n) + n) % n; }
+
 __global__ __launch_bounds__(256) void halo_extend_kernel(
     const double* __restrict__ local, int ni, int nj, int nk, int F, int mode,
     double* __restrict__ ext) {
-  const int bl = threadIdx.x & 15;
+  const int bl = threadIdx.x & 15, li = threadIdx.x >> 4;
   const int nie = ni + 2 * F, nje = nj + 2 * F, nke = nk + 2 * F;
-  const int ie = blockIdx.x * 16 + (threadIdx.x >> 4);
-  const int je = blockIdx.y;
-  const int ke = blockIdx.z % nke;
-  const int t16 = blockIdx.z / nke;
-  if (ie >= nie) return;
+  const int kchunks = (nke + kHaloPlanes - 1) / kHaloPlanes;
+  const int je = blockIdx.x;
+  const int t16 = blockIdx.y / kchunks;
+  const int ke0 = (blockIdx.y % kchunks) * kHaloPlanes;
+  const int ke1 = ke0 + kHaloPlanes < nke ? ke0 + kHaloPlanes : nke;
   const int64_t K = (int64_t)ni * nj * nk;
   const int64_t Kex = (int64_t)nie * nje * nke;
-  const int i = ie - F, j = je - F, k = ke - F;
-  const bool inside =
-      i >= 0 && i < ni && j >= 0 && j < nj && k >= 0 && k < nk;
-  if (!inside && mode == 2) return;
-  double v = 0.0;
-  if (inside || mode == 1) {
-    const int ii = ((i % ni) + ni) % ni, jj = ((j % nj) + nj) % nj,
-              kk = ((k % nk) + nk) % nk;
-    const int64_t g = ii + (int64_t)jj * ni + (int64_t)kk * ni * nj;
-    v = local[((int64_t)t16 * K + g) * 16 + bl];
+  const double* src = local + (int64_t)t16 * K * 16 + bl;
+  double* dst = ext + (int64_t)t16 * Kex * 16 + bl;
+  const int j = je - F;
+  const bool jin = j >= 0 && j < nj;
+  const int jj = wrap(j, nj);
+  for (int ke = ke0; ke < ke1; ++ke) {
+    const int k = ke - F;
+    const bool row_in = jin && k >= 0 && k < nk;
+    if (!row_in && mode == 2) continue;  // uniform: no interior point here
+    const int64_t srow = ((int64_t)wrap(k, nk) * nj + jj) * ni;
+    const int64_t drow = ((int64_t)ke * nje + je) * nie;
+    for (int ie = li; ie < nie; ie += 16) {
+      const int i = ie - F;
+      const bool inside = row_in && i >= 0 && i < ni;
+      if (!inside && mode == 2) continue;
+      double v = 0.0;
+      if (inside || mode == 1) v = src[(srow + wrap(i, ni)) * 16];
+      dst[(drow + ie) * 16] = v;
+    }
   }
-  const int64_t gex = ie + (int64_t)je * nie + (int64_t)ke * nie * nje;
-  ext[((int64_t)t16 * Kex + gex) * 16 + bl] = v;
 }
 
 int check_grid(int ni, int nj, int nk, int nb, int F) {
@@ -202,9 +214,14 @@ int check_grid(int ni, int nj, int nk, int nb, int F) {
   if (F < 1 || F > kMaxFDn)
     return set_error(SKGEMM_ERR_ARG, "FDn %d outside 1..%d", F, kMaxFDn);
   const int64_t ex = (int64_t)(ni + 2 * F) * (nj + 2 * F) * (nk + 2 * F);
-  // Launch grids: y = nj + 2F rows, z = (nk + 2F) planes x 16-band tiles.
-  if (ex * 16 >= ((int64_t)1 << 40) || nj + 2 * F > 65535 ||
-      (int64_t)(nk + 2 * F) * ((nb + 15) / 16) > 65535)
+  const int64_t tiles = (nb + 15) / 16;
+  // Launch grids: halo (nj + 2F, ceil((nk + 2F) / kHaloPlanes) x tiles);
+  // stencil (blocks of the i-j plane, k chunks, tiles).
+  const int64_t halo_y = (nk + 2 * F + kHaloPlanes - 1) / kHaloPlanes * tiles;
+  const int64_t st_x = (int64_t)((ni + kSB - 1) / kSB) * ((nj + kSB - 1) / kSB);
+  if (ex * 16 >= ((int64_t)1 << 40) || halo_y > 65535 || tiles > 65535 ||
+      st_x > 0x7fffffff ||
+      (nk + kStencilPlanes - 1) / kStencilPlanes > 65535)
     return set_error(SKGEMM_ERR_NOTSUP, "grid too large for one launch");
   return SKGEMM_OK;
 }
@@ -228,8 +245,8 @@ extern "C" int skgemm_halo_extend(skgemm_ctx* c, int ni, int nj, int nk,
   SKG_TRY(check_dev(local16, "local16"));
   SKG_TRY(check_dev(ext16, "ext16"));
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
-  const int nie = ni + 2 * FDn, nje = nj + 2 * FDn, nke = nk + 2 * FDn;
-  const dim3 grid((nie + 15) / 16, nje, nke * ((nb + 15) / 16));
+  const int nje = nj + 2 * FDn, nke = nk + 2 * FDn;
+  const dim3 grid(nje, ((nke + kHaloPlanes - 1) / kHaloPlanes) * ((nb + 15) / 16));
   c->record_begin();
   hipLaunchKernelGGL(halo_extend_kernel, grid, dim3(256), 0, c->stream,
                      local16, ni, nj, nk, FDn, boundary, ext16);
