@@ -6,21 +6,21 @@
 // ncol_p; gamma_p):  X_p = dv chi_p^T psi[idx_p, :],  X_p *= gamma_p (first
 // projector of each atom only),  result[idx_p, :] += chi_p X_p.
 // Device sequence, all on the context's stream:
-//   1. gather: filtered[R, b] = psi16[idx(R), b] for every projector row R
-//      (rows of all projectors stacked, row-major, ld = nb);
-//   2. forward products: one persistent batched Stream-K launch, one
-//      problem per projector (X_p = dv chi_p^T filtered_p);
-//   3. scale: X rows times gamma where the atom rule applies (times 1.0,
-//      exact, elsewhere);
-//   4. backward products: one batched launch (temp_p = chi_p X_p);
-//   The two problem tables are described once per band count and kept in
-//   device memory, so an application is five launches and no host work.
-//   5. scatter: result[g, b] += temp[R, b] for every row R with grid index
+//   1. forward products fused with the gather: X_p = dv chi_p^T psi[idx_p,
+//      :] read straight from psi16 at the projector's grid points, times
+//      gamma where the atom rule applies (times 1.0, exact, elsewhere); one
+//      workgroup per (projector, 64 bands);
+//   2. backward products: one persistent batched Stream-K launch, one
+//      problem per projector (temp_p = chi_p X_p); its problem table is
+//      described once per band count and kept in device memory, so an
+//      application is three launches and no host work;
+//   3. scatter: result[g, b] += temp[R, b] for every row R with grid index
 //      g, each grid point by one thread in projector order (the rows that
 //      touch g are listed in a CSR table built at create time), so shared
 //      grid points need no atomics and the sums keep the reference's order.
-// HBM-bound: the gather reads and the scatter reads and writes nrow_total x
-// nb doubles; the products are small (ncol <= a few tens).
+// HBM-bound: the forward pass reads nrow_total x nb doubles of psi (repeats
+// from overlapping projectors mostly hit L2), the backward products write
+// and the scatter reads them; the products are small (ncol <= a few tens).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,31 +34,79 @@
 namespace skg {
 namespace {
 
-// One lane per band, 64 consecutive bands per wave: a 16-band panel row is
-// 128 contiguous bytes of psi16.
-__global__ void nloc_gather_kernel(const double* __restrict__ psi16, int Nd,
-                                   int nb, const int* __restrict__ index,
-                                   int64_t rows, double* __restrict__ filtered) {
-  const int b = blockIdx.x * 64 + (threadIdx.x & 63);
-  if (b >= nb) return;
-  for (int64_t R = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6); R < rows;
-       R += (int64_t)gridDim.y * 4) {
-    const int g = index[R];
-    filtered[R * nb + b] = psi16[((int64_t)(b >> 4) * Nd + g) * 16 + (b & 15)];
+// Forward products fused with the gather:
+//   X[coff_p + c][b] = (dv sum_r chi_p[r][c] psi(index[roff_p + r], b)) factor
+// One workgroup per (projector, 64 bands) (grid x: projectors, y: groups
+// of 64 bands), 8 waves.  Rows go in chunks of 64: the chunk's chi values
+// are staged in LDS (row-major, read as broadcasts) and wave w takes rows w,
+// w + 8, ... of the chunk (8 gathered loads in flight), accumulating 16
+// columns per lane; the 8 partial sums are added in wave order through LDS.
+// Columns past 16 take further passes.  chi_p is column-major nrow x ncol.
+struct NlocProj {
+  int64_t roff, choff, coff;
+  int nrow, ncol;
+};
+constexpr int kNlocCols = 16, kNlocWaves = 8, kNlocChunk = 64;
+
+__global__ __launch_bounds__(kNlocWaves * 64) void nloc_project_kernel(
+    const double* __restrict__ psi16, int Nd, int nb,
+    const int* __restrict__ index, const double* __restrict__ chi,
+    const NlocProj* __restrict__ proj, const double* __restrict__ factor,
+    double dv, double* __restrict__ X) {
+  __shared__ double part[kNlocWaves][kNlocCols][64];
+  __shared__ double chs[kNlocChunk][kNlocCols];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int b = blockIdx.y * 64 + lane;
+  const NlocProj P = proj[blockIdx.x];
+  const bool bok = b < nb;
+  const double* src = psi16 + (int64_t)(b >> 4) * Nd * 16 + (b & 15);
+  const int* idx = index + P.roff;
+  constexpr int kPer = kNlocChunk / kNlocWaves;  // rows per wave and chunk
+  for (int c0 = 0; c0 < P.ncol; c0 += kNlocCols) {
+    const int nc = P.ncol - c0 < kNlocCols ? P.ncol - c0 : kNlocCols;
+    const double* ch = chi + P.choff + (int64_t)c0 * P.nrow;
+    double acc[kNlocCols];
+#pragma unroll
+    for (int c = 0; c < kNlocCols; ++c) acc[c] = 0.0;
+    for (int r0 = 0; r0 < P.nrow; r0 += kNlocChunk) {
+      __syncthreads();  // the previous chunk's chi reads are done
+      for (int e = tid; e < kNlocChunk * kNlocCols; e += kNlocWaves * 64) {
+        const int r = e / kNlocCols, c = e % kNlocCols;
+        chs[r][c] =
+            (c < nc && r0 + r < P.nrow) ? ch[(int64_t)c * P.nrow + r0 + r] : 0.0;
+      }
+      double v[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int r = r0 + w + kNlocWaves * u;
+        v[u] = (bok && r < P.nrow) ? src[(int64_t)idx[r] * 16] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < kPer; ++u)
+#pragma unroll
+        for (int c = 0; c < kNlocCols; ++c)
+          acc[c] = fma(chs[w + kNlocWaves * u][c], v[u], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < kNlocCols; ++c) part[w][c][lane] = acc[c];
+    __syncthreads();
+    for (int c = w; c < nc; c += kNlocWaves) {
+      double sum = part[0][c][lane];
+#pragma unroll
+      for (int q = 1; q < kNlocWaves; ++q) sum += part[q][c][lane];
+      if (bok) {
+        const int64_t col = P.coff + c0 + c;
+        X[col * nb + b] = (dv * sum) * factor[col];
+      }
+    }
+    __syncthreads();
   }
 }
 
-__global__ void nloc_scale_kernel(double* __restrict__ X, int nb,
-                                  const double* __restrict__ factor,
-                                  int64_t rows) {
-  const int b = blockIdx.x * 64 + (threadIdx.x & 63);
-  if (b >= nb) return;
-  for (int64_t R = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6); R < rows;
-       R += (int64_t)gridDim.y * 4)
-    X[R * nb + b] *= factor[R];
-}
-
-// Grid point u (sorted list `point`) receives the rows ptr[u] .. ptr[u+1]
+// One lane per band, 64 consecutive bands per wave (a 16-band panel row is
+// 128 contiguous bytes).  Grid point u (sorted list `point`) receives the rows ptr[u] .. ptr[u+1]
 // of temp in projector order.
 __global__ void nloc_scatter_kernel(const double* __restrict__ temp, int nb,
                                     const int* __restrict__ point,
@@ -96,35 +144,33 @@ struct skgemm_nloc {
   std::vector<int> nrow, ncol;
   std::vector<int64_t> roff, coff, choff;  // row, column, chi offsets
   int64_t rows = 0, cols = 0;
-  bool any_scale = false;
   // device copies
   int* index = nullptr;       // rows
   double* chi = nullptr;      // sum nrow * ncol
   double* factor = nullptr;   // cols
+  NlocProj* proj = nullptr;   // num_proj (rows, chi and column offsets)
   int* point = nullptr;       // npoints
   int* ptr = nullptr;         // npoints + 1
   int* rows_of = nullptr;     // rows
   int64_t npoints = 0;
   // workspace, grown with nb
-  double* filtered = nullptr;
   double* X = nullptr;
   double* temp = nullptr;
   int ws_nb = 0;
   // The two grouped products, described for `table_nb` bands and kept in
   // device memory (their rows depend on nb and the workspace only).
-  PlainTable fw, bw;
-  Problem* d_fw = nullptr;
+  PlainTable bw;
   Problem* d_bw = nullptr;
-  int64_t* d_hdr = nullptr;  // {rows, iterations} of fw, then of bw
+  int64_t* d_hdr = nullptr;  // {rows, iterations} of bw
   int table_nb = -1, table_cfg = -2, table_grid = -2;
-  double fw_dv = 0.0;
 };
 
 static void nloc_free(skgemm_nloc* h) {
   for (void* p : {(void*)h->index, (void*)h->chi, (void*)h->factor,
+                  (void*)h->proj,
                   (void*)h->point, (void*)h->ptr, (void*)h->rows_of,
-                  (void*)h->filtered, (void*)h->X, (void*)h->temp,
-                  (void*)h->d_fw, (void*)h->d_bw, (void*)h->d_hdr})
+                  (void*)h->X, (void*)h->temp, (void*)h->d_bw,
+                  (void*)h->d_hdr})
     if (p) hipFree(p);
 }
 
@@ -202,7 +248,6 @@ extern "C" int skgemm_nloc_create(skgemm_ctx* c, int Nd, int num_proj,
       atom = atom_index[p];
       for (int j = 0; j < ncol[p]; ++j)
         factor[h->coff[p] + j] = gamma_all[h->coff[p] + j];
-      if (ncol[p] > 0) h->any_scale = true;
     }
   }
   // Scatter table: grid points in ascending order, each with its rows in
@@ -237,6 +282,12 @@ extern "C" int skgemm_nloc_create(skgemm_ctx* c, int Nd, int num_proj,
     rc = upload(&h->chi, chi_all, (size_t)h->choff[num_proj], "chi", c->stream);
   if (rc == SKGEMM_OK)
     rc = upload(&h->factor, factor.data(), factor.size(), "gamma", c->stream);
+  std::vector<NlocProj> meta((size_t)num_proj);
+  for (int p = 0; p < num_proj; ++p)
+    meta[p] = NlocProj{h->roff[p], h->choff[p], h->coff[p], nrow[p], ncol[p]};
+  if (rc == SKGEMM_OK)
+    rc = upload(&h->proj, meta.data(), meta.size(), "projector table",
+                c->stream);
   if (rc == SKGEMM_OK) rc = upload(&h->point, point.data(), point.size(), "points", c->stream);
   if (rc == SKGEMM_OK && h->npoints > 0)
     rc = upload(&h->ptr, ptr.data(), ptr.size(), "point table", c->stream);
@@ -251,32 +302,15 @@ extern "C" int skgemm_nloc_create(skgemm_ctx* c, int Nd, int num_proj,
   return SKGEMM_OK;
 }
 
-// Describe the forward products X_p (ncol x nb, row-major) = dv chi_p^T
-// filtered_p, chi_p^T being chi_p (column-major nrow x ncol) read row-major,
-// and the backward products temp_p = chi_p X_p; upload both tables.
-static int nloc_tables(skgemm_nloc* h, int nb, double dv) {
+// Describe the backward products temp_p (nrow x nb, row-major) = chi_p
+// X_p (chi_p column-major nrow x ncol, X_p row-major ncol x nb) and upload
+// their table.
+static int nloc_tables(skgemm_nloc* h, int nb) {
   skgemm_ctx* c = h->ctx;
-  std::vector<skgemm_problem> fw, bw;
+  std::vector<skgemm_problem> bw;
   for (int p = 0; p < h->num_proj; ++p) {
     const int nr = h->nrow[p], nc = h->ncol[p];
-    if (nc == 0) continue;
-    skgemm_problem f{};
-    f.kind = 0;
-    f.layout_a = SKGEMM_ROW_MAJOR;
-    f.layout_b = SKGEMM_ROW_MAJOR;
-    f.layout_d = SKGEMM_ROW_MAJOR;
-    f.M = nc;
-    f.N = nb;
-    f.K = nr;
-    f.alpha = dv;
-    f.A = h->chi + h->choff[p];
-    f.lda = std::max(nr, 1);
-    f.B = h->filtered + h->roff[p] * nb;
-    f.ldb = nb;
-    f.D = h->X + h->coff[p] * nb;
-    f.ldd = nb;
-    fw.push_back(f);
-    if (nr == 0) continue;
+    if (nc == 0 || nr == 0) continue;
     skgemm_problem g{};
     g.kind = 0;
     g.layout_a = SKGEMM_COL_MAJOR;
@@ -295,27 +329,17 @@ static int nloc_tables(skgemm_nloc* h, int nb, double dv) {
     bw.push_back(g);
   }
   h->table_nb = -1;
-  SKG_TRY(build_plain_table(c, fw.data(), (int)fw.size(), &h->fw));
   SKG_TRY(build_plain_table(c, bw.data(), (int)bw.size(), &h->bw));
   SKG_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-  for (Problem** d : {&h->d_fw, &h->d_bw}) {
-    if (*d) hipFree(*d);
-    *d = nullptr;
-  }
-  const size_t nf = std::max<size_t>(h->fw.rows.size(), 1);
+  if (h->d_bw) hipFree(h->d_bw);
+  h->d_bw = nullptr;
   const size_t nbk = std::max<size_t>(h->bw.rows.size(), 1);
-  if (hipMalloc(&h->d_fw, nf * sizeof(Problem)) != hipSuccess ||
-      hipMalloc(&h->d_bw, nbk * sizeof(Problem)) != hipSuccess ||
+  if (hipMalloc(&h->d_bw, nbk * sizeof(Problem)) != hipSuccess ||
       (h->d_hdr == nullptr &&
-       hipMalloc(&h->d_hdr, 4 * sizeof(int64_t)) != hipSuccess))
-    return set_error(SKGEMM_ERR_NOMEM, "nloc problem tables");
-  const int64_t hdr[4] = {(int64_t)h->fw.rows.size(), h->fw.iters,
-                          (int64_t)h->bw.rows.size(), h->bw.iters};
+       hipMalloc(&h->d_hdr, 2 * sizeof(int64_t)) != hipSuccess))
+    return set_error(SKGEMM_ERR_NOMEM, "nloc problem table");
+  const int64_t hdr[2] = {(int64_t)h->bw.rows.size(), h->bw.iters};
   // On the context's stream, completed before returning (hdr is local).
-  SKG_HIP(hipMemcpyAsync(h->d_fw, h->fw.rows.data(),
-                         h->fw.rows.size() * sizeof(Problem),
-                         hipMemcpyHostToDevice, c->stream),
-          "nloc table upload");
   SKG_HIP(hipMemcpyAsync(h->d_bw, h->bw.rows.data(),
                          h->bw.rows.size() * sizeof(Problem),
                          hipMemcpyHostToDevice, c->stream),
@@ -325,7 +349,6 @@ static int nloc_tables(skgemm_nloc* h, int nb, double dv) {
           "nloc table upload");
   SKG_HIP(hipStreamSynchronize(c->stream), "nloc table upload");
   h->table_nb = nb;
-  h->fw_dv = dv;
   h->table_cfg = c->forced_cfg;
   h->table_grid = c->forced_grid;
   return SKGEMM_OK;
@@ -349,34 +372,28 @@ extern "C" int skgemm_nloc_apply(skgemm_nloc* h, int nb, double dv,
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
   if (nb > h->ws_nb) {
     SKG_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-    for (double** p : {&h->filtered, &h->X, &h->temp}) {
+    for (double** p : {&h->X, &h->temp}) {
       if (*p) hipFree(*p);
       *p = nullptr;
     }
     h->ws_nb = 0;
-    if (hipMalloc(&h->filtered, (size_t)h->rows * nb * sizeof(double)) != hipSuccess ||
-        hipMalloc(&h->X, (size_t)h->cols * nb * sizeof(double)) != hipSuccess ||
+    if (hipMalloc(&h->X, (size_t)h->cols * nb * sizeof(double)) != hipSuccess ||
         hipMalloc(&h->temp, (size_t)h->rows * nb * sizeof(double)) != hipSuccess)
       return set_error(SKGEMM_ERR_NOMEM, "nloc workspace for %d bands", nb);
     h->ws_nb = nb;
     ++c->mem_gen;
   }
-  hipLaunchKernelGGL(nloc_gather_kernel, rows_grid(nb, h->rows), dim3(256), 0,
-                     c->stream, psi16, h->Nd, nb, h->index, h->rows,
-                     h->filtered);
-  SKG_HIP(hipGetLastError(), "nloc gather launch");
-
   if (nb != h->table_nb || h->ws_nb != nb || c->forced_cfg != h->table_cfg ||
-      c->forced_grid != h->table_grid || dv != h->fw_dv) {
-    SKG_TRY(nloc_tables(h, nb, dv));
+      c->forced_grid != h->table_grid) {
+    SKG_TRY(nloc_tables(h, nb));
   }
-  SKG_TRY(launch_plain_table(c, h->fw, h->d_fw, h->d_hdr));
-  if (h->any_scale) {
-    hipLaunchKernelGGL(nloc_scale_kernel, rows_grid(nb, h->cols), dim3(256), 0,
-                       c->stream, h->X, nb, h->factor, h->cols);
-    SKG_HIP(hipGetLastError(), "nloc scale launch");
-  }
-  SKG_TRY(launch_plain_table(c, h->bw, h->d_bw, h->d_hdr + 2));
+  hipLaunchKernelGGL(nloc_project_kernel,
+                     dim3((unsigned)h->num_proj, (nb + 63) / 64),
+                     dim3(kNlocWaves * 64), 0,
+                     c->stream, psi16, h->Nd, nb, h->index, h->chi, h->proj,
+                     h->factor, dv, h->X);
+  SKG_HIP(hipGetLastError(), "nloc forward launch");
+  SKG_TRY(launch_plain_table(c, h->bw, h->d_bw, h->d_hdr));
   hipLaunchKernelGGL(nloc_scatter_kernel, rows_grid(nb, h->npoints), dim3(256),
                      0, c->stream, h->temp, nb, h->point, h->ptr, h->rows_of,
                      h->npoints, h->Nd, result_tile == 32 ? 5 : 4, result);
