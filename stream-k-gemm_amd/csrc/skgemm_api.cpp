@@ -220,14 +220,21 @@ extern "C" int skgemm_create(int device, skgemm_ctx** out) {
     return hip_error(e, "hipStreamCreate");
   }
   c->stream = c->own_stream;
-  if ((e = hipEventCreate(&c->ev0)) != hipSuccess ||
-      (e = hipEventCreate(&c->ev1)) != hipSuccess ||
+  // Timing-only events (no system-scope fence on record: a fenced record
+  // also writes back and invalidates the caches between the launches it
+  // brackets, which slowed the next launch measurably).
+  if ((e = hipEventCreateWithFlags(&c->ev0, hipEventDisableSystemFence)) !=
+          hipSuccess ||
+      (e = hipEventCreateWithFlags(&c->ev1, hipEventDisableSystemFence)) !=
+          hipSuccess ||
       (e = hipMalloc(&c->status, sizeof(int))) != hipSuccess ||
       (e = launch_zero_ints(c->status, 1, c->stream)) != hipSuccess ||
       (e = hipStreamSynchronize(c->stream)) != hipSuccess) {
     skgemm_destroy(c);
     return hip_error(e, "context setup");
   }
+  c->last0 = c->ev0;
+  c->last1 = c->ev1;
   c->spin_limit = 1ll << 26;
   *out = c;
   return SKGEMM_OK;
@@ -456,21 +463,26 @@ void skgemm_ctx::free_all() {
   split_tables.clear();
 }
 
+// One event pair per launch: the timing ring's next pair while the ring has
+// room, else ev0 / ev1 (a second pair at the same boundaries would only add
+// device time between launches).  last0 / last1 name the pair of the most
+// recent launch.
 void skgemm_ctx::record_begin() {
   if (capturing) return;
-  hipEventRecord(ev0, stream);
-  if (tring_used < (int)tring.size())
-    hipEventRecord(tring[tring_used].first, stream);
+  last0 = tring_used < (int)tring.size() ? tring[tring_used].first : ev0;
+  hipEventRecord(last0, stream);
 }
 
 void skgemm_ctx::record_end() {
   if (capturing) return;
-  hipEventRecord(ev1, stream);
-  timed = true;
   if (tring_used < (int)tring.size()) {
-    hipEventRecord(tring[tring_used].second, stream);
+    last1 = tring[tring_used].second;
     ++tring_used;
+  } else {
+    last1 = ev1;
   }
+  hipEventRecord(last1, stream);
+  timed = true;
 }
 
 int skgemm_ctx::capacity(int cfg) {
@@ -1115,10 +1127,15 @@ extern "C" int skgemm_timing_enable(skgemm_ctx* c, int capacity) {
   }
   c->tring.clear();
   c->tring_used = 0;
+  c->last0 = c->ev0;
+  c->last1 = c->ev1;
+  c->timed = false;
+  // Timing-only events, no system-scope fence on record (as ev0 / ev1).
   for (int i = 0; i < capacity; ++i) {
     hipEvent_t a, b;
-    SKG_HIP(hipEventCreate(&a), "hipEventCreate");
-    hipError_t e = hipEventCreate(&b);
+    SKG_HIP(hipEventCreateWithFlags(&a, hipEventDisableSystemFence),
+            "hipEventCreate");
+    hipError_t e = hipEventCreateWithFlags(&b, hipEventDisableSystemFence);
     if (e != hipSuccess) {
       hipEventDestroy(a);
       return hip_error(e, "hipEventCreate");
@@ -1150,9 +1167,9 @@ extern "C" int skgemm_timing_read(skgemm_ctx* c, double* ms, int max,
 extern "C" int skgemm_last_kernel_ms(skgemm_ctx* c, double* ms) {
   SKG_TRY(check_ctx(c));
   if (!c->timed) return set_error(SKGEMM_ERR_STATE, "no timed launch yet");
-  SKG_HIP(hipEventSynchronize(c->ev1), "hipEventSynchronize");
+  SKG_HIP(hipEventSynchronize(c->last1), "hipEventSynchronize");
   float f = 0.f;
-  SKG_HIP(hipEventElapsedTime(&f, c->ev0, c->ev1), "hipEventElapsedTime");
+  SKG_HIP(hipEventElapsedTime(&f, c->last0, c->last1), "hipEventElapsedTime");
   *ms = f;
   return SKGEMM_OK;
 }

@@ -30,6 +30,7 @@ struct skgemm_ctx {
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t last0 = nullptr, last1 = nullptr;  // pair of the last launch
   bool timed = false;
   // Optional ring of event pairs, one per launch (skgemm_timing_*).
   std::vector<std::pair<hipEvent_t, hipEvent_t>> tring;

@@ -95,8 +95,9 @@ extern "C" int skgemm_autotune(skgemm_ctx* c, int kind, int la, int lb, int M,
                      : skgemm_dsyrk(c, la, N, K, 1.0, A, lda, 0.0, D, N);
   };
   hipEvent_t e0, e1;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
+  // Timing-only events without the system-scope fence, like the context's.
+  hipEventCreateWithFlags(&e0, hipEventDisableSystemFence);
+  hipEventCreateWithFlags(&e1, hipEventDisableSystemFence);
   double best = 1e30;
   Plan bp{-1, 0, 1};
   for (int cfg = 0; cfg < num_tile_cfgs(); ++cfg) {
