@@ -716,7 +716,6 @@ int run_single(skgemm_ctx* c, Problem& p, const Plan& plan_in,
                            : (int64_t)p.tiles_m * p.tiles_n;
     }
   }
-  const TileCfg& t = tile_cfg(pl.cfg);
   p.pred = pred;
   p.tile_begin = 0;
   p.iter_begin = 0;

@@ -180,12 +180,10 @@ extern "C" int skgemm_load_tuning(skgemm_ctx* c, const char* cache_path) {
   if (f == nullptr) return set_error(SKGEMM_ERR_ARG, "cannot open %s", cache_path);
   int kind, la, lb, M, N, K, cfg, grid, kc;
   double us;
-  int n = 0;
   while (std::fscanf(f, "%d %d %d %d %d %d %d %d %d %lf", &kind, &la, &lb, &M,
                      &N, &K, &cfg, &grid, &kc, &us) == 10) {
     if (cfg < 0 || cfg >= num_tile_cfgs() || grid <= 0 || kc <= 0) continue;
     c->tuned[std::make_tuple(kind, la, lb, M, N, K)] = Plan{cfg, grid, kc};
-    ++n;
   }
   std::fclose(f);
   return SKGEMM_OK;
