@@ -38,7 +38,8 @@ def _compile(src):
     if src.endswith(".hip"):
         cmd += ["-x", "hip"] + DEVICE
     else:
-        cmd += ["-D__HIP_PLATFORM_AMD__", "-pthread"]
+        # Host translation units: any device pass hipcc runs targets gfx950.
+        cmd += ["--offload-arch=gfx950", "-D__HIP_PLATFORM_AMD__", "-pthread"]
     cmd += ["-I", CSRC, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     return src, obj, r
