@@ -371,7 +371,10 @@ int skgemm_last_kernel_ms(skgemm_ctx* ctx, double* ms);
  * bracketed by a pair of HIP events recorded on the launch stream (up to
  * `capacity` launches; 0 disables).  skgemm_timing_read waits for the last
  * recorded launch, writes up to `max` durations in launch order (ms) and
- * starts a new recording. */
+ * starts a new recording.  Each launch records one event pair (the ring's
+ * while it has room, else the context's own), created without the
+ * system-scope fence, so timing adds no cache flush between launches;
+ * skgemm_timing_enable clears the last-launch time. */
 int skgemm_timing_enable(skgemm_ctx* ctx, int capacity);
 int skgemm_timing_read(skgemm_ctx* ctx, double* ms, int max, int* count);
 
