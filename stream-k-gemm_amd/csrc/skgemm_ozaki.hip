@@ -231,16 +231,16 @@ __device__ __forceinline__ uint32_t pack4(int r0, int r1, int r2, int r3) {
 // planes[l][o][k] (row pitch kpad bytes) for o < opad, k < kpad; zeros past
 // the matrix.  Operand contiguous along k: workgroup (o, k segment of
 // 4096), 4 consecutive k per thread and step, segments taken from high k
-// to low k.
+// to low k; a launch covers segments seg0 .. seg0 + nsegc - 1 (all of them,
+// or one K chunk of the pipelined SYRK).
 constexpr int kSplitSeg = 4096;
 
 template <bool WIDE>
 __global__ __launch_bounds__(256) void split_k_kernel(
     Operand op, int K, const int* sigma, Tables T, int8_t* planes,
-    int64_t plane, int kpad) {
-  const int nseg = (kpad + kSplitSeg - 1) / kSplitSeg;
-  const int o = blockIdx.x / nseg;
-  const int seg = nseg - 1 - (blockIdx.x - o * nseg);
+    int64_t plane, int kpad, int seg0, int nsegc) {
+  const int o = blockIdx.x / nsegc;
+  const int seg = seg0 + nsegc - 1 - (blockIdx.x - o * nsegc);
   const bool in_o = o < op.extent;
   const int s = in_o ? sigma[o] : 0;
   const double* row = in_o ? op.ptr + op_addr(op, o, 0) : op.ptr;
@@ -339,6 +339,7 @@ struct GemmLaunch {
   int grid;
   int dp;          // 1: each workgroup takes whole units (no fix-up)
   int kc, kcs;     // data-parallel: K chunks per tile, MAC steps per chunk
+  int c0, nck;     // data-parallel: this launch's chunks c0 .. c0 + nck - 1
   int64_t units;   // L * kc * ntiles (data-parallel), L * ntiles (Stream-K)
   int64_t total;   // units * ipt (Stream-K mode)
   int* ws;
@@ -701,12 +702,13 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
     // tiles of one plane and K chunk that share panels in that XCD's L2 and
     // advance through K together.  Chunk c writes its own residue plane
     // (c L + l); the CRT pass adds the chunks modulo m_l.
-    const int64_t per_plane = (int64_t)G.kc * G.ntiles;
+    const int64_t per_plane = (int64_t)G.nck * G.ntiles;
     for (int64_t u = w; u < G.units; u += Gn) {
       const int l = (int)(u / per_plane);
       const int64_t r = u - (int64_t)l * per_plane;
-      const int c = (int)(r / G.ntiles);
-      const int64_t tile = r - (int64_t)c * G.ntiles;
+      const int cr = (int)(r / G.ntiles);  // chunk within this launch
+      const int64_t tile = r - (int64_t)cr * G.ntiles;
+      const int c = G.c0 + cr;
       const int kb = c * G.kcs;
       const int ke = min(G.ipt, kb + G.kcs);
       int m0, n0;
@@ -1059,6 +1061,24 @@ struct skgemm_ctx::Extra {
   void* buf[kBufs] = {nullptr};
   size_t n[kBufs] = {0};
   uint64_t* gen = nullptr;  // the context's mem_gen
+  // Pipelined split SYRK: a second stream for the INT8 products and the
+  // events that order them after each K chunk's residue pass.
+  static constexpr int kMaxPipe = 8;
+  hipStream_t aux = nullptr;
+  hipEvent_t ev[kMaxPipe + 1] = {nullptr};
+  bool pipe_ready() {
+    if (aux) return true;
+    if (hipStreamCreateWithFlags(&aux, hipStreamNonBlocking) != hipSuccess) {
+      aux = nullptr;
+      return false;
+    }
+    for (auto& e : ev)
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        e = nullptr;
+        return false;
+      }
+    return true;
+  }
   void* get(int i, size_t bytes) {
     if (bytes > n[i]) {
       if (gen) ++*gen;
@@ -1077,6 +1097,10 @@ struct skgemm_ctx::Extra {
 
 void skgemm_ctx::free_extra() {
   if (extra == nullptr) return;
+  if (extra->aux) hipStreamSynchronize(extra->aux);
+  for (auto& e : extra->ev)
+    if (e) hipEventDestroy(e);
+  if (extra->aux) hipStreamDestroy(extra->aux);
   for (int i = 0; i < Extra::kBufs; ++i)
     if (extra->buf[i]) hipFree(extra->buf[i]);
   delete extra;
@@ -1117,10 +1141,10 @@ int prepare_operand(skgemm_ctx* c, const Operand& op, int K, const Tables& T,
     const dim3 grid((unsigned)((int64_t)opad * nseg));
     if (wide)
       hipLaunchKernelGGL(split_k_kernel<true>, grid, dim3(256), 0, c->stream,
-                         op, K, sigma, T, planes, plane, kpad);
+                         op, K, sigma, T, planes, plane, kpad, 0, nseg);
     else
       hipLaunchKernelGGL(split_k_kernel<false>, grid, dim3(256), 0, c->stream,
-                         op, K, sigma, T, planes, plane, kpad);
+                         op, K, sigma, T, planes, plane, kpad, 0, nseg);
   } else {
     if (O > 0) {
       const int nseg = norm_segments(O, K);
@@ -1320,7 +1344,33 @@ int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
   int8_t* planesA = (int8_t*)X->get(kPlanesA, (size_t)L * pa);
   int8_t* planesB = kind == 1 ? planesA : (int8_t*)X->get(kPlanesB, (size_t)L * pb);
   const int mp = (int)round_up(M, 16);
-  uint8_t* R = (uint8_t*)X->get(kRes, (size_t)op.kc * L * N * mp);
+  // Pipelined SYRK, opt-in with SKGEMM_OZ_PIPE=1 (k-contiguous operand,
+  // data-parallel plan, not inside a capture, not on a serving slot): K is
+  // cut into `pipe` chunks of
+  // `chunk_k` (a multiple of the residue pass's 4096-wide segments); the
+  // residue pass of chunk c + 1 runs on the context's stream while the
+  // INT8 products of chunk c run on a second stream, and the CRT pass adds
+  // the chunks' residues modulo m.  Integer results are identical to every
+  // other plan's.  Off by default: on a 65536 x 1024 operand the overlap
+  // costs more than it hides (eight narrower INT8 launches sharing the
+  // chip with an HBM-bound pass, eight residue planes per modulus to write
+  // and sum), 1.06-1.08 ms against 1.00-1.02 ms for one launch.
+  int pipe = 1, chunk_k = 0;
+  const char* pipe_env = std::getenv("SKGEMM_OZ_PIPE");
+  if (kind == 1 && A.kcontig && op.dp && !c->capturing && !c->oz_no_wait &&
+      pipe_env != nullptr && pipe_env[0] == '1') {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(c->stream, &cs) == hipSuccess &&
+        cs == hipStreamCaptureStatusNone) {
+      chunk_k = (int)round_up((kpad + skgemm_ctx::Extra::kMaxPipe - 1) /
+                                  skgemm_ctx::Extra::kMaxPipe,
+                              kSplitSeg);
+      pipe = (kpad + chunk_k - 1) / chunk_k;
+      if (pipe < 2 || !X->pipe_ready()) pipe = 1;
+    }
+  }
+  uint8_t* R = (uint8_t*)X->get(
+      kRes, (size_t)std::max(op.kc, pipe) * L * N * mp);
   double* n2a = (double*)X->get(kNormA, sizeof(double) * opa);
   double* n2b = kind == 1 ? n2a : (double*)X->get(kNormB, sizeof(double) * opb);
   int* sga = (int*)X->get(kSigA, sizeof(int) * opa);
@@ -1341,8 +1391,15 @@ int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
   // inside a captured graph.
   hipError_t e = launch_zero_ints(fallback, 1, c->stream);
   if (e != hipSuccess) return hip_error(e, "clear split-form flag");
-  int rc = prepare_operand(c, A, K, T, planesA, pa, kpad, opa, n2a, sga,
-                           fallback, kPartA);
+  int rc = SKGEMM_OK;
+  if (pipe > 1) {
+    // Norms of the whole operand first (the scale of a row needs all of K).
+    hipLaunchKernelGGL(norm_k_kernel, dim3(A.extent), dim3(256), 0, c->stream,
+                       A, K, T.h, n2a, sga, fallback);
+  } else {
+    rc = prepare_operand(c, A, K, T, planesA, pa, kpad, opa, n2a, sga,
+                         fallback, kPartA);
+  }
   if (rc != SKGEMM_OK) return rc;
   if (kind == 0) {
     rc = prepare_operand(c, B, K, T, planesB, pb, kpad, opb, n2b, sgb,
@@ -1376,6 +1433,8 @@ int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
   // No empty chunk: kcs = ceil(ipt / kc), then kc = ceil(ipt / kcs) <= kc.
   G.kcs = (G.ipt + op.kc - 1) / op.kc;
   G.kc = (G.ipt + G.kcs - 1) / G.kcs;
+  G.c0 = 0;
+  G.nck = G.kc;
   G.units = G.ntiles * L * (G.dp ? G.kc : 1);
   G.total = G.ntiles * L * G.ipt;
   G.grid = (int)std::max<int64_t>(1, std::min<int64_t>(op.grid, G.dp ? G.units
@@ -1388,13 +1447,47 @@ int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
   G.flags = c->flags;
   G.status = c->status;
   G.spin_limit = c->spin_limit;
-  e = launch_gemm(op.cfg, G, c->stream);
-  if (e != hipSuccess) return hip_error(e, "ozaki gemm launch");
+  if (pipe > 1) {
+    G.kcs = chunk_k / gc.bk;
+    G.kc = pipe;
+    G.nck = 1;
+    G.units = G.ntiles * L;
+    G.grid = (int)std::max<int64_t>(1, std::min<int64_t>(op.grid, G.units));
+    const int nseg = (kpad + kSplitSeg - 1) / kSplitSeg;
+    const int spc = chunk_k / kSplitSeg;  // segments per chunk
+    const bool wide = T.h > 54;
+    for (int ch = 0; ch < pipe; ++ch) {
+      const int seg0 = ch * spc, nsegc = std::min(spc, nseg - seg0);
+      const dim3 grid((unsigned)((int64_t)opa * nsegc));
+      if (wide)
+        hipLaunchKernelGGL(split_k_kernel<true>, grid, dim3(256), 0,
+                           c->stream, A, K, sga, T, planesA, pa, kpad, seg0,
+                           nsegc);
+      else
+        hipLaunchKernelGGL(split_k_kernel<false>, grid, dim3(256), 0,
+                           c->stream, A, K, sga, T, planesA, pa, kpad, seg0,
+                           nsegc);
+      if ((e = hipGetLastError()) != hipSuccess)
+        return hip_error(e, "ozaki split");
+      if ((e = hipEventRecord(X->ev[ch], c->stream)) != hipSuccess ||
+          (e = hipStreamWaitEvent(X->aux, X->ev[ch], 0)) != hipSuccess)
+        return hip_error(e, "ozaki pipeline");
+      G.c0 = ch;
+      e = launch_gemm(op.cfg, G, X->aux);
+      if (e != hipSuccess) return hip_error(e, "ozaki gemm launch");
+    }
+    if ((e = hipEventRecord(X->ev[pipe], X->aux)) != hipSuccess ||
+        (e = hipStreamWaitEvent(c->stream, X->ev[pipe], 0)) != hipSuccess)
+      return hip_error(e, "ozaki pipeline");
+  } else {
+    e = launch_gemm(op.cfg, G, c->stream);
+    if (e != hipSuccess) return hip_error(e, "ozaki gemm launch");
+  }
 
   CrtLaunch P{};
   P.R = R;
   P.mp = mp;
-  P.kc = G.dp ? G.kc : 1;
+  P.kc = pipe > 1 ? pipe : (G.dp ? G.kc : 1);
   P.r_plane = (int64_t)N * mp;
   P.T = T;
   P.sig_a = sga;

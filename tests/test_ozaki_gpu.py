@@ -252,3 +252,39 @@ def test_fallback_to_plain_fp64(skg, ctx, rng, kind):
     ctx.dgemm_host(0, 0, 0, M, N, K, 1.5, Ab, K, Bb, N, 0.5, D2, N, 0.25)
     assert b == -1.0
     assert np.array_equal(D1, D2, equal_nan=True)
+
+
+@pytest.mark.parametrize("K,N", [(20000, 300), (40000, 100)])
+def test_pipelined_split_syrk_identical(skg, ctx, rng, K, N):
+    """With SKGEMM_OZ_PIPE=1, a k-contiguous operand under a data-parallel
+    plan takes the pipelined sequence: K is cut into chunks of whole 4096-wide residue segments
+    (K = 20000: five chunks of one segment; K = 40000: five chunks of two),
+    the residue pass of chunk c + 1 runs beside the INT8 products of chunk c
+    on a second stream, and every chunk after the first writes its own
+    residue planes.  Its integers, hence its result, equal those of the
+    single-launch data-parallel plan (no SKGEMM_OZ_PIPE) and of Stream-K,
+    and it stays within 1e-13 of the plain FP64 SYRK."""
+    X = c3_like(rng, K, N)
+    Xc = np.ascontiguousarray(X.T.ravel())   # column-major K x N, ld = K
+    outs = []
+    try:
+        for plan, pipe in (("6,1", "1"), ("6,1,0,3", "1"), ("6,1", None),
+                           ("1,0", "1"), (None, None)):
+            for key, val in (("SKGEMM_OZ_PLAN", plan), ("SKGEMM_OZ_PIPE", pipe)):
+                if val is None:
+                    os.environ.pop(key, None)
+                else:
+                    os.environ[key] = val
+            print("plan", plan, "pipe", pipe, "K", K, "N", N, flush=True)
+            C = np.zeros(N * N)
+            ctx.dsyrk_ozaki_host(skg.COL_MAJOR, N, K, 1.0, Xc, K, 0.0, C, N)
+            outs.append(C)
+    finally:
+        os.environ.pop("SKGEMM_OZ_PLAN", None)
+        os.environ.pop("SKGEMM_OZ_PIPE", None)
+    for C in outs[1:]:
+        assert np.array_equal(C, outs[0])
+    ref = np.zeros(N * N)
+    ctx.dsyrk_host(skg.COL_MAJOR, N, K, 1.0, Xc, K, 0.0, ref, N)
+    il = np.tril_indices(N)
+    assert rel_fro(outs[0].reshape(N, N)[il], ref.reshape(N, N)[il]) < 1e-13
