@@ -545,10 +545,13 @@ static int default_cfg(skgemm_ctx* c, int kind, int M, int N, int K) {
   return 1;
 }
 
-static int clamp_grid(skgemm_ctx* c, int cfg, int g, int64_t total) {
+// Whole XCD rounds, except for a grid of exactly one (K chunk, tile) unit
+// per workgroup, which is kept (data-parallel: no fix-up).
+static int clamp_grid(skgemm_ctx* c, int cfg, int g, int64_t total,
+                      int64_t units) {
   const int cap = c->capacity(cfg);
   g = std::min(g, cap);
-  if (g >= 16) g &= ~7;  // whole XCD rounds
+  if (g >= 16 && g != units) g &= ~7;
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, total));
 }
 
@@ -586,8 +589,8 @@ Plan plan(skgemm_ctx* c, int kind, int la, int lb, int M, int N, int K) {
   const int64_t ipt = iters_per_tile(pl.cfg, K);
   pl.kchunks = (int)std::max<int64_t>(1, std::min<int64_t>(pl.kchunks, ipt));
   const int64_t kci = (ipt + pl.kchunks - 1) / pl.kchunks;
-  const int64_t total = tiles_for(kind, pl.cfg, M, N) * pl.kchunks * kci;
-  pl.grid = clamp_grid(c, pl.cfg, pl.grid, total);
+  const int64_t units = tiles_for(kind, pl.cfg, M, N) * pl.kchunks;
+  pl.grid = clamp_grid(c, pl.cfg, pl.grid, units * kci, units);
   return pl;
 }
 
@@ -630,6 +633,10 @@ int fill_problem(skgemm_ctx* c, Problem* p, int kind, const Plan& pl, int M,
 static const int64_t* split_table(skgemm_ctx* c, const Problem& p, int cfg,
                                   int G) {
   if (p.xrow == 0 && p.xcol == 0) return nullptr;
+  // One whole unit per workgroup: the even split is already aligned to the
+  // units (no fix-up, and the workgroups of one XCD advance through K
+  // together, sharing panels in its L2).
+  if (p.num_tiles * p.kchunks == G) return nullptr;
   const std::vector<int64_t> key = {p.syrk,     p.M,        p.N,
                                     p.K,        cfg,        G,
                                     p.kchunks,  p.kci,      p.row_order,
@@ -703,7 +710,7 @@ static const int64_t* split_table(skgemm_ctx* c, const Problem& p, int cfg,
 int run_single(skgemm_ctx* c, Problem& p, const Plan& plan_in,
                const int* pred, bool timed) {
   Plan pl = plan_in;
-  if (tile_cfg(pl.cfg).ns >= 3 && !(p.a.vec2 && p.b.vec2)) {
+  if (tile_cfg(pl.cfg).ns >= 2 && !(p.a.vec2 && p.b.vec2)) {
     // LDS-DMA tiles move 16-byte pairs: unaligned operands take the
     // register-staged tile of the same shape (same tiles and units).
     pl.cfg = register_sibling(pl.cfg);

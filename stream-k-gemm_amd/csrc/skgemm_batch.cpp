@@ -161,7 +161,7 @@ int build_plain_table(skgemm_ctx* c, const skgemm_problem* q, int n,
     rows.push_back(r);
   }
   // DMA tiles move 16-byte pairs; same tiles, register-staged loads else.
-  if (tile_cfg(cfg).ns >= 3 && !aligned) cfg = register_sibling(cfg);
+  if (tile_cfg(cfg).ns >= 2 && !aligned) cfg = register_sibling(cfg);
   int grid = c->capacity(cfg);
   if (c->forced_grid > 0) grid = std::min(grid, c->forced_grid);
   grid = std::max(8, grid - grid % 8);

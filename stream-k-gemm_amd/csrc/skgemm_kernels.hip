@@ -48,6 +48,13 @@ static const TileCfg kCfgs[] = {
     {128, 128, 16, 4, 2, 4, 1},  // 11: as 8, the last tile column (row)
                                  // carries an N (M) remainder of up to 16
     {64, 64, 16, 2, 2, 4, 1},    // 12: as 10, with strips (also SYRK rows)
+    // Two DMA stages (69.6 / 36.9 KB of LDS): two (four) workgroups per CU,
+    // so one workgroup's copy waits and output stores overlap another's
+    // MFMAs.
+    {128, 128, 16, 4, 2, 2, 1},  // 13: as 11 with 2 stages
+    {64, 64, 16, 2, 2, 2, 1},    // 14: as 12 with 2 stages
+    {128, 128, 16, 4, 2, 2},     // 15: strip twin of 13 (batched tables)
+    {64, 64, 16, 2, 2, 2},       // 16: strip twin of 14
 };
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
@@ -390,7 +397,7 @@ __device__ __forceinline__ void mac_loop(
 }
 
 // ---------------------------------------------------------------------------
-// LDS-DMA pipeline (tile configurations with ns >= 3).  Operand tiles go
+// LDS-DMA pipeline (tile configurations with ns >= 2).  Operand tiles go
 // from memory straight into LDS with 16-byte buffer loads to LDS (lane l of
 // a wave instruction writes LDS base + 16 l), ns stages deep, so no
 // registers or LDS-store instructions are spent on staging and ns - 1 steps
@@ -709,7 +716,7 @@ template <int BM, int BN, int BK, int WM, int WN, int NS, bool BATCHED,
 __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
   using S = TileShape<BM, BN, BK, WM, WN>;
   constexpr int NT = S::NT, TM = S::TM, TN = S::TN, MB = S::MB, NB = S::NB;
-  static_assert(!XS || (NS >= 3 && !BATCHED), "strip: single DMA launches");
+  static_assert(!XS || (NS >= 2 && !BATCHED), "strip: single DMA launches");
   // Partial-tile slot of one workgroup (the strip block and the corner
   // block of each wave after the tile's blocks).
   constexpr int64_t kSlot = (int64_t)BM * BN + (XS ? 8 * NT : 0);
@@ -847,7 +854,7 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
     // unaligned operand reaching one is reported through the status word.
 #define SKG_MAC(V)                                                           \
   case V:                                                                    \
-    if constexpr (NS >= 3 && (V & 1)) {                                      \
+    if constexpr (NS >= 2 && (V & 1)) {                                      \
       /* strip tiles unmasked: edge blocks are computed on zero-filled     \
          operands and not stored */                                          \
       if (XS == kStripRow && corner)                                         \
@@ -864,7 +871,7 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
                      ((V >> 3) & 1) != 0, NS, 0>(acc, xacc, cacc, A, B, K,   \
                                                  m0, n0, c * kci, kb, ke,    \
                                                  lds, tid, mask);            \
-    } else if constexpr (NS >= 3)                                            \
+    } else if constexpr (NS >= 2)                                            \
       if (tid == 0) atomicOr(L.status, 2);                                   \
     if constexpr (NS == 0)                                                   \
       mac_loop<BM, BN, BK, WM, WN, (V >> 2) & 1, (V >> 1) & 1, V & 1,        \
@@ -1060,7 +1067,7 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
 template <int BM, int BN, int BK, int WM, int WN, int NS, bool XS = false>
 static size_t lds_bytes_t() {
   size_t d = 2ull * TileShape<BM, BN, BK, WM, WN>::kBufDoubles;
-  if constexpr (NS >= 3) {
+  if constexpr (NS >= 2) {
     const size_t dma =
         (size_t)NS * (dma_region<BM, BK>() + dma_region<BN, BK>() +
                       (XS ? dma_region<16, BK>() : 0));
@@ -1147,6 +1154,10 @@ static int occupancy_t() {
     case 10: return FN<64, 64, 16, 2, 2, 4>(__VA_ARGS__);   \
     case 11: return FN<128, 128, 16, 4, 2, 4, true>(__VA_ARGS__); \
     case 12: return FN<64, 64, 16, 2, 2, 4, true>(__VA_ARGS__);   \
+    case 13: return FN<128, 128, 16, 4, 2, 2, true>(__VA_ARGS__); \
+    case 14: return FN<64, 64, 16, 2, 2, 2, true>(__VA_ARGS__);   \
+    case 15: return FN<128, 128, 16, 4, 2, 2>(__VA_ARGS__);  \
+    case 16: return FN<64, 64, 16, 2, 2, 2>(__VA_ARGS__);    \
     default: break;                                        \
   }
 

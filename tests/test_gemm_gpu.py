@@ -373,13 +373,15 @@ def test_every_tile_every_layout(skg, ctx, orc, rng, pad):
 
 STRIP_TILE = 11   # 128 x 128 LDS-DMA tile whose last tile column carries
                   # an N remainder of up to 16 columns
+STRIP_TILE_2 = 13  # the same with two DMA stages
 
 
+@pytest.mark.parametrize("tile", [STRIP_TILE, STRIP_TILE_2])
 @pytest.mark.parametrize("M,N,K", [(256, 136, 64), (300, 144, 70),
                                    (128, 130, 520), (384, 520, 37),
                                    (256, 129, 33), (136, 256, 64),
                                    (520, 300, 37), (144, 128, 70)])
-def test_strip_tiles(skg, ctx, rng, M, N, K):
+def test_strip_tiles(skg, ctx, rng, M, N, K, tile):
     """The strip configuration on every A / B layout, row-major,
     column-major (launched transposed, so a column strip becomes a row
     strip and back) and band-packed D, with alpha, beta and gamma, under
@@ -392,11 +394,11 @@ def test_strip_tiles(skg, ctx, rng, M, N, K):
     D0 = rng.uniform(-1, 1, (M, N))
     alpha, beta, gamma = 0.75, 0.5, 0.25
     ref = alpha * (A @ B) + beta * D0 + gamma
-    info = skg.tile_info(STRIP_TILE)
+    info = skg.tile_info(tile)
     assert (info["bm"], info["bn"], info["waves"]) == (128, 128, 8)
     try:
         for grid, kchunks in ((0, 1), (5, 1), (0, 3)):
-            ctx.set_tuning(STRIP_TILE, grid)
+            ctx.set_tuning(tile, grid)
             ctx.set_kchunks(kchunks)
             for la in range(3):
                 Ab, lda = encode_a(A, la, 2)

@@ -171,14 +171,17 @@ def test_every_square_tile_column_major(skg, ctx, orc, rng):
 
 @pytest.mark.parametrize("tile,N,K", [(12, 72, 100), (12, 80, 33),
                                       (12, 130, 520), (12, 200, 64),
-                                      (11, 136, 70), (11, 144, 37)])
+                                      (11, 136, 70), (11, 144, 37),
+                                      (14, 80, 33), (14, 200, 64),
+                                      (13, 136, 70)])
 def test_strip_tiles(skg, ctx, rng, tile, N, K):
     """Strip tiles on a SYRK: N = T x BM + r (r <= 16) tiles a T x T
     triangle whose last tile row carries the r remainder rows; wave 0 of the
     last diagonal tile forms the r x r corner from the strip alone.  Every
     layout, alpha and beta, default grid, split tiles and K chunks; the
     upper triangle is never written.  (N = 200 has remainder 8 on 64-wide
-    tiles; tile 12 is 64 x 64, tile 11 128 x 128.)"""
+    tiles; tile 12 is 64 x 64, tile 11 128 x 128, tiles 14 and 13 the same
+    with two DMA stages.)"""
     A = rng.uniform(-1, 1, (K, N))
     C0 = rng.uniform(-1, 1, (N, N))
     ref = 0.5 * (A.T @ A) - 1.25 * C0
