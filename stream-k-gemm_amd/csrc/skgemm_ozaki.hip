@@ -235,6 +235,29 @@ __device__ __forceinline__ uint32_t pack4(int r0, int r1, int r2, int r3) {
 // or one K chunk of the pipelined SYRK).
 constexpr int kSplitSeg = 4096;
 
+// Residues of row elements k0 .. k0 + 3 (zero past K or outside the
+// operand), one dword per plane.
+template <bool WIDE>
+__device__ __forceinline__ void split4(const double* row, bool in_o, int K,
+                                       int s, const Tables& T, int8_t* dst,
+                                       int64_t plane, int k0) {
+  Split p[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int k = k0 + j;
+    const double a = (in_o && k < K) ? row[k] : 0.0;
+    p[j] = split26<WIDE>(rint(ldexp(a, s)));
+  }
+  for (int l = 0; l < T.L; ++l) {
+    const double m = T.md[l], inv = T.inv_m[l], c = T.c26[l], c2 = T.c52[l];
+    const uint32_t v = pack4(residue<WIDE>(p[0], m, inv, c, c2),
+                             residue<WIDE>(p[1], m, inv, c, c2),
+                             residue<WIDE>(p[2], m, inv, c, c2),
+                             residue<WIDE>(p[3], m, inv, c, c2));
+    *reinterpret_cast<uint32_t*>(dst + l * plane + k0) = v;
+  }
+}
+
 template <bool WIDE>
 __global__ __launch_bounds__(256) void split_k_kernel(
     Operand op, int K, const int* sigma, Tables T, int8_t* planes,
@@ -248,21 +271,108 @@ __global__ __launch_bounds__(256) void split_k_kernel(
   for (int step = 3; step >= 0; --step) {
     const int k0 = seg * kSplitSeg + step * 1024 + threadIdx.x * 4;
     if (k0 >= kpad) continue;
-    Split p[4];
+    split4<WIDE>(row, in_o, K, s, T, dst, plane, k0);
+  }
+}
+
+// Operand contiguous along k, norm and residues in one kernel: one
+// 1024-thread workgroup per free index o (padding rows included).  The
+// norm uses norm_k_kernel's grouping and order exactly (thread t + 256 q
+// runs that kernel's partial sum q of thread t; the four are added as
+// (s0 + s1) + (s2 + s3), then the same tree), so norms, scales and
+// residues are bit-identical to norm_k_kernel followed by split_k_kernel.
+// The row is then split from high k to low k while it is recent in the
+// caches.
+template <bool WIDE>
+__global__ __launch_bounds__(1024) void norm_split_k_kernel(
+    Operand op, int K, int h, Tables T, int8_t* planes, int64_t plane,
+    int kpad, double* norm2, int* sigma, int* fallback) {
+  __shared__ double red[2][1024];
+  __shared__ int s_sig;
+  const int o = blockIdx.x;
+  const int tid = threadIdx.x;
+  const bool in_o = o < op.extent;  // uniform over the workgroup
+  const double* row = in_o ? op.ptr + op_addr(op, o, 0) : op.ptr;
+  int s = 0;
+  if (in_o) {
+    const int t = tid & 255, q = tid >> 8;
+    double acc = 0.0, mx = 0.0;
+    int k = t;
+    // Four loads in flight per thread; the sum stays one ordered chain.
+    for (; k + 3 * 1024 + 768 < K; k += 4 * 1024) {
+      const double x0 = row[k + 256 * q], x1 = row[k + 1024 + 256 * q],
+                   x2 = row[k + 2048 + 256 * q], x3 = row[k + 3072 + 256 * q];
+      acc = fma(x0, x0, acc);
+      acc = fma(x1, x1, acc);
+      acc = fma(x2, x2, acc);
+      acc = fma(x3, x3, acc);
+      mx = fmax(mx, fmax(fmax(fabs(x0), fabs(x1)), fmax(fabs(x2), fabs(x3))));
+    }
+    for (; k + 768 < K; k += 1024) {
+      const double x = row[k + 256 * q];
+      acc = fma(x, x, acc);
+      mx = fmax(mx, fabs(x));
+    }
+    if (q == 0)
+      for (; k < K; k += 256) {
+        const double x = row[k];
+        acc = fma(x, x, acc);
+        mx = fmax(mx, fabs(x));
+      }
+    red[0][tid] = acc;
+    red[1][tid] = mx;
+    __syncthreads();
+    if (tid < 256) {
+      red[0][tid] = (red[0][tid] + red[0][tid + 256]) +
+                    (red[0][tid + 512] + red[0][tid + 768]);
+      red[1][tid] = fmax(fmax(red[1][tid], red[1][tid + 256]),
+                         fmax(red[1][tid + 512], red[1][tid + 768]));
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (tid < w) {
+        red[0][tid] += red[0][tid + w];
+        red[1][tid] = fmax(red[1][tid], red[1][tid + w]);
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      finish_norm(red[0][0], red[1][0], o, h, norm2, sigma, fallback);
+      s_sig = sigma_of(red[0][0], h);
+    }
+    __syncthreads();
+    s = s_sig;
+  }
+  int8_t* dst = planes + (int64_t)o * kpad;
+  const int nseg = (kpad + kSplitSeg - 1) / kSplitSeg;
+  // The next segment's four values are loaded before this one is split.
+  auto load4 = [&](int k0, double (&x)[4]) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int k = k0 + j;
-      const double a = (in_o && k < K) ? row[k] : 0.0;
-      p[j] = split26<WIDE>(rint(ldexp(a, s)));
+    for (int j = 0; j < 4; ++j)
+      x[j] = (in_o && k0 + j < K) ? row[k0 + j] : 0.0;
+  };
+  double cur[4], nxt[4] = {0.0, 0.0, 0.0, 0.0};
+  int k0 = (nseg - 1) * kSplitSeg + tid * 4;
+  load4(k0, cur);
+  for (int seg = nseg - 1; seg >= 0; --seg) {
+    if (seg > 0) load4(k0 - kSplitSeg, nxt);
+    if (k0 < kpad) {
+      Split p[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) p[j] = split26<WIDE>(rint(ldexp(cur[j], s)));
+      for (int l = 0; l < T.L; ++l) {
+        const double m = T.md[l], inv = T.inv_m[l], c = T.c26[l],
+                     c2 = T.c52[l];
+        const uint32_t v = pack4(residue<WIDE>(p[0], m, inv, c, c2),
+                                 residue<WIDE>(p[1], m, inv, c, c2),
+                                 residue<WIDE>(p[2], m, inv, c, c2),
+                                 residue<WIDE>(p[3], m, inv, c, c2));
+        *reinterpret_cast<uint32_t*>(dst + l * plane + k0) = v;
+      }
     }
-    for (int l = 0; l < T.L; ++l) {
-      const double m = T.md[l], inv = T.inv_m[l], c = T.c26[l], c2 = T.c52[l];
-      const uint32_t v = pack4(residue<WIDE>(p[0], m, inv, c, c2),
-                               residue<WIDE>(p[1], m, inv, c, c2),
-                               residue<WIDE>(p[2], m, inv, c, c2),
-                               residue<WIDE>(p[3], m, inv, c, c2));
-      *reinterpret_cast<uint32_t*>(dst + l * plane + k0) = v;
-    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) cur[j] = nxt[j];
+    k0 -= kSplitSeg;
   }
 }
 
@@ -1133,7 +1243,18 @@ int prepare_operand(skgemm_ctx* c, const Operand& op, int K, const Tables& T,
                     double* norm2, int* sigma, int* fallback, int part_buf) {
   const int O = op.extent;
   const bool wide = T.h > 54;
-  if (op.kcontig) {
+  const char* fuse = std::getenv("SKGEMM_OZ_FUSE");
+  if (op.kcontig && !(fuse != nullptr && fuse[0] == '0')) {
+    // One pass: norm, scale and residues per row.
+    if (wide)
+      hipLaunchKernelGGL(norm_split_k_kernel<true>, dim3(opad), dim3(1024), 0,
+                         c->stream, op, K, T.h, T, planes, plane, kpad, norm2,
+                         sigma, fallback);
+    else
+      hipLaunchKernelGGL(norm_split_k_kernel<false>, dim3(opad), dim3(1024),
+                         0, c->stream, op, K, T.h, T, planes, plane, kpad,
+                         norm2, sigma, fallback);
+  } else if (op.kcontig) {
     if (O > 0)
       hipLaunchKernelGGL(norm_k_kernel, dim3(O), dim3(256), 0, c->stream, op,
                          K, T.h, norm2, sigma, fallback);

@@ -288,3 +288,40 @@ def test_pipelined_split_syrk_identical(skg, ctx, rng, K, N):
     ctx.dsyrk_host(skg.COL_MAJOR, N, K, 1.0, Xc, K, 0.0, ref, N)
     il = np.tril_indices(N)
     assert rel_fro(outs[0].reshape(N, N)[il], ref.reshape(N, N)[il]) < 1e-13
+
+
+@pytest.mark.parametrize("K,N", [(5000, 70), (700, 33), (4096, 300),
+                                 (9000, 257)])
+def test_fused_norm_split_identical(skg, ctx, rng, K, N):
+    """A k-contiguous operand takes one kernel for norms, scales and
+    residues; it reproduces the two-kernel pass (SKGEMM_OZ_FUSE=0) bit for
+    bit: K with a remainder past the last 1024 block, K below 768, row
+    counts that leave padding rows, SYRK and GEMM.  (The restatement
+    checks above run through the fused pass for their k-contiguous
+    layouts.)"""
+    X = c3_like(rng, K, N)
+    Xc = np.ascontiguousarray(X.T.ravel())   # column-major K x N, ld = K
+    A = rng.standard_normal((N, K)) * 10.0 ** rng.uniform(0, 4, N)[:, None]
+    B = rng.standard_normal((K, 40))
+    outs = []
+    try:
+        for fuse in (None, "0"):
+            if fuse is None:
+                os.environ.pop("SKGEMM_OZ_FUSE", None)
+            else:
+                os.environ["SKGEMM_OZ_FUSE"] = fuse
+            print("fuse", fuse, "K", K, "N", N, flush=True)
+            C = np.zeros(N * N)
+            b1 = ctx.dsyrk_ozaki_host(skg.COL_MAJOR, N, K, 1.0, Xc, K, 0.0, C,
+                                      N)
+            D = np.zeros(N * 40)
+            b2 = ctx.dgemm_ozaki_host(0, 0, 0, N, 40, K, 1.0,
+                                      np.ascontiguousarray(A.ravel()), K,
+                                      np.ascontiguousarray(B.ravel()), 40,
+                                      0.0, D, 40, 0.0)
+            outs.append((C, D, b1, b2))
+    finally:
+        os.environ.pop("SKGEMM_OZ_FUSE", None)
+    (C0, D0, b10, b20), (C1, D1, b11, b21) = outs
+    assert np.array_equal(C0, C1) and np.array_equal(D0, D1)
+    assert b10 == b11 and b20 == b21
