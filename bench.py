@@ -193,8 +193,9 @@ def bench_c1(skg, ctx, reps):
 def bench_split(skg, ctx, reps):
     """Split-form SYRK on Psi 65536 x 1024 (columns scaled by 10^u, u ~
     U(0, 6), seed 3) against the autotuned plain FP64 SYRK on the same
-    input; error of the split result against the plain FP64 result and the
-    bound the library returns."""
+    input, and the split-form rotation Psi Q against the plain FP64
+    rotation; error of each split result against the plain FP64 result and
+    the bound the library returns."""
     Nd, Ns = 65536, 1024
     psi = skg.DeviceArray(ctx, Nd * Ns)
     ctx.fill_normal(psi.ptr, Nd * Ns, 3, 1.0 / np.sqrt(Nd))
@@ -206,7 +207,8 @@ def bench_split(skg, ctx, reps):
         skg.COL_MAJOR, Ns, Nd, 1.0, psi.ptr, Nd, 0.0, plain.ptr, Ns), reps)
     out = {"workload": "SYRK Ms = Psi^T Psi (lower), Psi 65536x1024 FP64 "
                        "col-major, columns scaled by 10^U(0,6); split form "
-                       "vs plain FP64, and the plain FP64 rotation Psi*Q",
+                       "vs plain FP64; rotation Psi*Q, plain FP64 and "
+                       "split form",
            "plain_fp64": {"ms": t_plain, "plan": plan}}
     f = float(Ns) * (Ns + 1) * Nd
     out["plain_fp64"]["tflops"] = f / (t_plain * 1e-3) / 1e12
@@ -235,9 +237,27 @@ def bench_split(skg, ctx, reps):
         "ms": t_rot, "tflops": fr / (t_rot * 1e-3) / 1e12,
         "frac_fp64_peak": fr / (t_rot * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
         "plan": rplan}
+    # Split-form rotation on the same operands (14 moduli), its error
+    # against the plain FP64 rotation and the bound the library returns.
+    rot2 = skg.DeviceArray(ctx, Nd * Ns)
+
+    def split_rot():
+        return ctx.dgemm_ozaki(skg.COL_MAJOR, skg.ROW_MAJOR, skg.COL_MAJOR,
+                               Nd, Ns, Ns, 1.0, psi.ptr, Nd, q.ptr, Ns, 0.0,
+                               rot2.ptr, Nd, 0.0, slices=14)
+    rbound = split_rot()
+    t_srot = median_ms(ctx, split_rot, reps)
+    got, want = rot2.download(), rot.download()
+    out["split_rotation_L14"] = {
+        "ms": t_srot, "tflops_fp64_equivalent": fr / (t_srot * 1e-3) / 1e12,
+        "speedup_vs_plain": t_rot / t_srot,
+        "rel_fro_vs_plain_fp64": float(np.linalg.norm(got - want) /
+                                       np.linalg.norm(want)),
+        "stated_bound": rbound, "moduli": 14}
+    del got, want
     out["plain_fp64"]["frac_fp64_peak"] = (out["plain_fp64"]["tflops"] /
                                            PEAK_FP64_TFLOPS)
-    for x in (psi, q, rot):
+    for x in (psi, q, rot, rot2):
         x.free()
     return out
 
