@@ -457,11 +457,12 @@ def bench_batched(skg, ctx, reps):
     return out
 
 
-def bench_serving(skg, ctx, c4_tflops, seconds):
+def bench_serving(skg, ctx, c4_tflops, seconds, precision):
     """200 resident systems (Ns ~ U[128, 2048], Nd ~ U[8192, 65536], seed 6,
     ~64 GB of Psi); overlap and rotation requests arrive as a Poisson
-    process at ~70% of the batched throughput; latency = submit ->
-    completion observed by the server."""
+    process at ~70% of the plain FP64 batched throughput; latency = submit
+    -> completion observed by the server.  `precision`: plain FP64 or the
+    split form (14 moduli), same arrivals."""
     import threading
     rng = np.random.default_rng(6)
     systems = []
@@ -488,7 +489,7 @@ def bench_serving(skg, ctx, c4_tflops, seconds):
     max_out = max(s[2] * s[3] for s in systems)
     outs = [skg.DeviceArray(ctx, max_out) for _ in range(ring)]
     srv = skg.Server(0, hbm_budget=int(pool_gb * 1e9) + (16 << 30),
-                     max_batch=16, max_inflight=2, precision=skg.FP64)
+                     max_batch=16, max_inflight=2, precision=precision)
     lat = []
     try:
         ids = [srv.add_system(s[0].ptr, s[2], s[3]) for s in systems]
@@ -532,7 +533,9 @@ def bench_serving(skg, ctx, c4_tflops, seconds):
     lat = np.array(lat)
     out = {"workload": "200 resident systems (Ns ~ U[128,2048], Nd ~ "
                        "U[8192,65536], seed 6), Poisson arrivals, overlap or "
-                       "rotation per request, plain FP64",
+                       "rotation per request, " +
+                       ("plain FP64" if precision == skg.FP64 else
+                        "split form (14 moduli)"),
            "resident_psi_gb": pool_gb, "arrival_rate_per_s": lam,
            "requests": int(n_req), "wall_s": wall,
            "latency_ms_p50": float(np.percentile(lat, 50)) / 1e3,
@@ -684,9 +687,12 @@ def main():
         extra["batched"] = bench_batched(skg, ctx, 5)
     if "serving" in sections:
         log("serving (200 systems, Poisson) ...")
-        extra["serving"] = bench_serving(
-            skg, ctx, extra["batched"]["plain_fp64_graph"]["tflops"],
-            args.serve_seconds)
+        rate_tflops = extra["batched"]["plain_fp64_graph"]["tflops"]
+        extra["serving"] = bench_serving(skg, ctx, rate_tflops,
+                                         args.serve_seconds, skg.FP64)
+        log("serving, split form ...")
+        extra["serving"]["split_form"] = bench_serving(
+            skg, ctx, rate_tflops, args.serve_seconds, skg.OZAKI_INT8)
 
     cpu = None
     if not args.no_cpu:
