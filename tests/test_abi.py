@@ -136,6 +136,38 @@ def test_streamk_split_covers_once_and_owners_wait_forward(num_tiles, ipt, G):
     assert set(owners) == set(peers)
 
 
+@pytest.mark.parametrize("tiles,kchunks,kci", [(36, 14, 146), (36, 7, 293),
+                                                (45, 11, 187), (10, 25, 82)])
+def test_data_parallel_grid_is_unit_aligned(tiles, kchunks, kci):
+    """A grid of exactly one (K chunk, tile) unit per workgroup (kept as is
+    by the planner, not rounded to whole XCD rounds, and run on the even
+    split): every workgroup takes exactly one whole unit, so no tile needs
+    a fix-up."""
+    G = tiles * kchunks
+    owners, peers, cover, total = simulate(G, kci, G)
+    assert np.all(cover == 1)
+    assert not owners and not peers
+    assert all(wg_start(w, total, G) == w * kci for w in range(G + 1))
+
+
+def test_committed_traffic_figures_follow_their_counters():
+    """profiles/traffic.json: bytes per launch = (FETCH_SIZE x fetch factor
+    + WRITE_SIZE x write factor) KiB, each factor the one that makes the
+    copy kernel read and write exactly its 1 GiB."""
+    import json
+    d = json.load(open(os.path.join(ROOT, "profiles", "traffic.json")))
+    cal = d["calibration"]
+    ff = cal["copy_kernel_bytes_each_way"] / 1024 / cal["fetch_kib_raw"]
+    wf = cal["copy_kernel_bytes_each_way"] / 1024 / cal["write_kib_raw"]
+    assert abs(ff - cal["fetch_factor"]) < 1e-3
+    assert abs(wf - cal["write_factor"]) < 1e-3
+    for key in ("rotation", "syrk"):
+        e = d[key]
+        want = (e["fetch_kib_raw"] * ff + e["write_kib_raw"] * wf) * 1024
+        assert abs(e["bytes_per_launch"] - want) <= 1e-4 * want, key
+        assert e["bytes_per_launch"] > e["algorithmic_bytes"]
+
+
 @pytest.mark.parametrize("G", [1, 7, 8, 9, 37, 256, 511, 512, 1024])
 def test_xcd_remap_is_a_bijection(G):
     ids = sorted(logical_id(g, G) for g in range(G))
