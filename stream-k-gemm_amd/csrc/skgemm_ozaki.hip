@@ -56,11 +56,14 @@ struct Tables {
   double c52[kMaxMod];    // 2^52 mod m_l, symmetric
   // CRT: x = sum_l r_l W_l - q P (mod 2^128),
   // W_l = (P / m_l) ((P / m_l)^{-1} mod m_l), q = floor(sum_l r_l w_l + 1/2)
-  // with w_l = W_l / P rounded to FP64.
-  double W[kMaxMod][4];  // W_l in 32-bit limbs (exact), least significant first
+  // with w_l = W_l / P rounded to FP64.  W_l and P in limbs (exact, least
+  // significant first): three of 37 bits for L <= kLimb37Max, else four of
+  // 32 bits.
+  double W[kMaxMod][4];
   double wfrac[kMaxMod];
   double P[4];
 };
+constexpr int kLimb37Max = 14;  // product of the first 14 moduli < 2^111
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
@@ -913,43 +916,50 @@ struct CrtLaunch {
   double* fro2;  // per workgroup: its part of ||C||_F^2 (SYRK: both halves)
 };
 
-// x = S - q P with S = sum_l r_l W_l, in 32-bit limbs held in FP64
-// accumulators: every product r_l W_lj < 2^40 and every limb sum < 2^45, so
-// all of it is exact FMA arithmetic (full-rate FP64 instead of quarter-rate
-// 64-bit integer multiplies).  q = floor(S / P + 1/2) comes from the FP64
-// estimate f, within 2^-36 of the true value, whose fractional part lies in
-// [1/4, 3/4] because |x| <= P / 4.  Carries are propagated exactly with
-// floor(b 2^-32).  r[l] holds the residue in byte e of word l.  Returns
-// x = hi 2^64 + lo with lo unsigned.
+// x = S - q P with S = sum_l r_l W_l, in NL limbs of B bits held in FP64
+// accumulators: up to 14 moduli (P < 2^111) three 37-bit limbs, above that
+// four 32-bit limbs.  Every product r_l W_lj < 2^45 (2^40) and every limb
+// sum < 2^49 (2^45), so all of it is exact FMA arithmetic (full-rate FP64
+// instead of quarter-rate 64-bit integer multiplies).  q = floor(S / P +
+// 1/2) comes from the FP64 estimate f, within 2^-36 of the true value,
+// whose fractional part lies in [1/4, 3/4] because |x| <= P / 4.  Carries
+// are propagated exactly with floor(b 2^-B).  r[l] holds the residue in
+// byte e of word l.  Returns x = hi 2^64 + lo with lo unsigned.
+template <int NL, int B>
 __device__ __forceinline__ void crt(const Tables& T, const uint32_t* rw, int e,
                                     int64_t& hi, uint64_t& lo) {
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, f = 0.5;
+  double a[NL], f = 0.5;
+#pragma unroll
+  for (int j = 0; j < NL; ++j) a[j] = 0.0;
 #pragma unroll
   for (int l = 0; l < kMaxMod; ++l) {
     if (l < T.L) {
       const double r = (double)((rw[l] >> (8 * e)) & 255u);
-      a0 = fma(r, T.W[l][0], a0);
-      a1 = fma(r, T.W[l][1], a1);
-      a2 = fma(r, T.W[l][2], a2);
-      a3 = fma(r, T.W[l][3], a3);
+#pragma unroll
+      for (int j = 0; j < NL; ++j) a[j] = fma(r, T.W[l][j], a[j]);
       f = fma(r, T.wfrac[l], f);
     }
   }
   const double q = floor(f);
-  const double two32 = 4294967296.0, inv32 = 2.3283064365386963e-10;
-  double b0 = fma(-q, T.P[0], a0), b1 = fma(-q, T.P[1], a1),
-         b2 = fma(-q, T.P[2], a2), b3 = fma(-q, T.P[3], a3);
-  double c = floor(b0 * inv32);
-  b0 = fma(-c, two32, b0);
-  b1 += c;
-  c = floor(b1 * inv32);
-  b1 = fma(-c, two32, b1);
-  b2 += c;
-  c = floor(b2 * inv32);
-  b2 = fma(-c, two32, b2);
-  b3 += c;
-  lo = (uint64_t)(uint32_t)b0 | ((uint64_t)(uint32_t)b1 << 32);
-  hi = (int64_t)(((uint64_t)(int64_t)b3 << 32) | (uint64_t)(uint32_t)b2);
+  const double two = (double)(1ull << B), inv = 1.0 / (double)(1ull << B);
+  double b[NL];
+#pragma unroll
+  for (int j = 0; j < NL; ++j) b[j] = fma(-q, T.P[j], a[j]);
+#pragma unroll
+  for (int j = 0; j + 1 < NL; ++j) {
+    const double c = floor(b[j] * inv);
+    b[j] = fma(-c, two, b[j]);
+    b[j + 1] += c;
+  }
+  if constexpr (NL == 4) {
+    lo = (uint64_t)(uint32_t)b[0] | ((uint64_t)(uint32_t)b[1] << 32);
+    hi = (int64_t)(((uint64_t)(int64_t)b[3] << 32) | (uint64_t)(uint32_t)b[2]);
+  } else {
+    // x = b2 2^74 + b1 2^37 + b0 with b0, b1 in [0, 2^37).
+    const uint64_t l0 = (uint64_t)b[0], l1 = (uint64_t)b[1];
+    lo = l0 | (l1 << 37);
+    hi = (int64_t)b[2] * 1024 + (int64_t)(l1 >> 27);
+  }
 }
 
 // Blocks of 256 rows x 4 columns; a thread takes 4 consecutive rows of one
@@ -1000,7 +1010,10 @@ __global__ __launch_bounds__(256) void crt_kernel(CrtLaunch p) {
       if (row < p.M && (!p.syrk || row >= col)) {
         int64_t xh;
         uint64_t xl;
-        crt(p.T, rw, e, xh, xl);
+        if (p.T.L <= kLimb37Max)
+          crt<3, 37>(p.T, rw, e, xh, xl);
+        else
+          crt<4, 32>(p.T, rw, e, xh, xl);
         const double d = (double)xh * 18446744073709551616.0 + (double)xl;
         const double c = ldexp(d, -(p.sig_a[row] + sb));
         contrib += (p.syrk && row != col) ? 2.0 * c * c : c * c;
@@ -1149,12 +1162,15 @@ Tables make_tables(int L) {
   typedef unsigned __int128 u128;
   u128 P = 1;
   for (int l = 0; l < L; ++l) P *= (u128)kModuli[l];
-  for (int j = 0; j < 4; ++j) T.P[j] = (double)(uint32_t)(P >> (32 * j));
+  const int nl = L <= kLimb37Max ? 3 : 4, lb = L <= kLimb37Max ? 37 : 32;
+  const u128 lmask = ((u128)1 << lb) - 1;
+  for (int j = 0; j < nl; ++j) T.P[j] = (double)(uint64_t)((P >> (lb * j)) & lmask);
   for (int l = 0; l < L; ++l) {
     const u128 Pl = P / (u128)kModuli[l];
     const int inv = modinv((int)(Pl % (u128)kModuli[l]), kModuli[l]);
     const u128 W = Pl * (u128)inv;  // < P
-    for (int j = 0; j < 4; ++j) T.W[l][j] = (double)(uint32_t)(W >> (32 * j));
+    for (int j = 0; j < nl; ++j)
+      T.W[l][j] = (double)(uint64_t)((W >> (lb * j)) & lmask);
     // W / P = inv / m_l exactly; rounded once to FP64.
     T.wfrac[l] = (double)inv / (double)kModuli[l];
   }

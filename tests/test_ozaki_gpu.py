@@ -145,10 +145,11 @@ def test_device_entry_points_reject_host_pointers(skg, ctx):
         assert e.value.code == skg.ERR_ARG
 
 
-@pytest.mark.parametrize("L", [8, 12, 15, 16])
+@pytest.mark.parametrize("L", [8, 12, 13, 14, 15, 16])
 def test_moduli_counts_bit_exact(skg, ctx, rng, L):
-    """Modulus counts: up to 14 moduli the two-limb residue split, above it
-    the three-limb split (|x| up to 2^60)."""
+    """Modulus counts: up to 14 moduli the two-limb residue split and the
+    three-limb (37-bit) CRT, above it the three-limb residue split (|x| up
+    to 2^60) and the four-limb (32-bit) CRT."""
     K, N = 257, 40
     X = c3_like(rng, K, N)
     C = np.zeros(N * N)
