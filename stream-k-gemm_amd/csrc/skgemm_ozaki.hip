@@ -768,7 +768,7 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
   // by at most one from floor(acc / m), so one correction each way suffices.
   auto write_residues = [&](int l, int out_plane, int m0, int n0) {
     const int m = G.mods[l];
-    const double inv = G.inv_mods[l];
+    const double md = (double)m, inv = G.inv_mods[l];
     constexpr int TP = BM + 16;
     // Accumulator element v of lane x is row 8(v/4) + 4(x/32) + v%4,
     // column x%32 of its 32x32 block (measured on gfx950).
@@ -782,8 +782,10 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
           uint32_t word = 0;
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            const int a = acc[i][j][v4 * 4 + u];
-            int r = a - m * (int)floor((double)a * inv);
+            // a - m floor(a fl(1/m)) as one exact FMA (|m q| < 2^40): no
+            // quarter-rate 32-bit integer multiply.
+            const double ad = (double)acc[i][j][v4 * 4 + u];
+            int r = (int)fma(-md, floor(ad * inv), ad);
             r += (r < 0) ? m : 0;
             r -= (r >= m) ? m : 0;
             word |= (uint32_t)r << (8 * u);
