@@ -379,15 +379,16 @@ __global__ __launch_bounds__(1024) void norm_split_k_kernel(
   }
 }
 
-// Operand contiguous along o: workgroup of 64 o x 128 k.  Thread (o, 32
-// consecutive k) loads along o (coalesced), then each plane's 64 x 128-byte
-// block is transposed through LDS so that every 8 lanes store one whole
-// 128-byte row segment.  LDS row pitch 144 bytes: the per-lane dword
-// writes (lanes = 64 rows) and the 16-byte row reads are conflict-free.
+// Operand contiguous along o: workgroup of 64 o x 128 k, 512 threads.
+// Thread (o, 16 consecutive k) loads along o (coalesced), then each plane's
+// 64 x 128-byte block is transposed through LDS so that every 8 lanes store
+// one whole 128-byte row segment.  LDS row pitch 144 bytes: the per-lane
+// dword writes (lanes = 64 rows) and the 16-byte row reads are
+// conflict-free.
 constexpr int kTPitch = 144;
 
 template <bool WIDE>
-__global__ __launch_bounds__(256) void split_o_kernel(
+__global__ __launch_bounds__(512) void split_o_kernel(
     Operand op, int K, const int* sigma, Tables T, int8_t* planes,
     int64_t plane, int kpad, int opad) {
   __shared__ __align__(16) uint8_t tile[2][64 * kTPitch];
@@ -396,38 +397,33 @@ __global__ __launch_bounds__(256) void split_o_kernel(
   const int kb = kblocks - 1 - (blockIdx.x - ob * kblocks);
   const int ol = threadIdx.x & 63, kq = threadIdx.x >> 6;
   const int o = ob * 64 + ol;
-  const int k0 = kb * 128 + kq * 32;
+  const int k0 = kb * 128 + kq * 16;
   const bool in_o = o < op.extent;
   const int s = in_o ? sigma[o] : 0;
-  Split p[32];
+  Split p[16];
 #pragma unroll
-  for (int j = 0; j < 32; ++j) {
+  for (int j = 0; j < 16; ++j) {
     const int k = k0 + j;
     const double a = (in_o && k < K) ? op.ptr[op_addr(op, o, k)] : 0.0;
     p[j] = split26<WIDE>(rint(ldexp(a, s)));
   }
-  // Store side: chunk q = threadIdx.x + 256 i (i = 0, 1) is row q / 8,
-  // 16-byte chunk q % 8 of the 64 x 128 block.
+  // Store side: thread q is row q / 8, 16-byte chunk q % 8 of the
+  // 64 x 128 block.
   const int r0 = threadIdx.x >> 3, c0 = threadIdx.x & 7;
   int8_t* dst0 = planes + (int64_t)(ob * 64 + r0) * kpad + kb * 128 + c0 * 16;
-  const int64_t row32 = (int64_t)32 * kpad;
   for (int l = 0; l < T.L; ++l) {
     const double m = T.md[l], inv = T.inv_m[l], c = T.c26[l], c2 = T.c52[l];
     uint8_t* t = tile[l & 1];
-    uint32_t* trow = reinterpret_cast<uint32_t*>(t + ol * kTPitch + kq * 32);
+    uint32_t* trow = reinterpret_cast<uint32_t*>(t + ol * kTPitch + kq * 16);
 #pragma unroll
-    for (int q = 0; q < 8; ++q)
+    for (int q = 0; q < 4; ++q)
       trow[q] = pack4(residue<WIDE>(p[4 * q], m, inv, c, c2),
                       residue<WIDE>(p[4 * q + 1], m, inv, c, c2),
                       residue<WIDE>(p[4 * q + 2], m, inv, c, c2),
                       residue<WIDE>(p[4 * q + 3], m, inv, c, c2));
     __syncthreads();
-    const uint4 v0 = *reinterpret_cast<const uint4*>(t + r0 * kTPitch + c0 * 16);
-    const uint4 v1 =
-        *reinterpret_cast<const uint4*>(t + (r0 + 32) * kTPitch + c0 * 16);
-    int8_t* d = dst0 + l * plane;
-    *reinterpret_cast<uint4*>(d) = v0;
-    *reinterpret_cast<uint4*>(d + row32) = v1;
+    const uint4 v = *reinterpret_cast<const uint4*>(t + r0 * kTPitch + c0 * 16);
+    *reinterpret_cast<uint4*>(dst0 + l * plane) = v;
     // tile[l & 1] is rewritten two planes later, after the next barrier.
   }
 }
@@ -1300,10 +1296,10 @@ int prepare_operand(skgemm_ctx* c, const Operand& op, int K, const Tables& T,
     }
     const dim3 grid((unsigned)((int64_t)(opad / 64) * (kpad / 128)));
     if (wide)
-      hipLaunchKernelGGL(split_o_kernel<true>, grid, dim3(256), 0, c->stream,
+      hipLaunchKernelGGL(split_o_kernel<true>, grid, dim3(512), 0, c->stream,
                          op, K, sigma, T, planes, plane, kpad, opad);
     else
-      hipLaunchKernelGGL(split_o_kernel<false>, grid, dim3(256), 0, c->stream,
+      hipLaunchKernelGGL(split_o_kernel<false>, grid, dim3(512), 0, c->stream,
                          op, K, sigma, T, planes, plane, kpad, opad);
   }
   const hipError_t e = hipGetLastError();
