@@ -610,6 +610,14 @@ int fill_problem(skgemm_ctx* c, Problem* p, int kind, const Plan& pl, int M,
   p->iters_per_tile = (int)iters_per_tile(pl.cfg, K);
   // Tile order: keep the smaller operand cached and read the larger once.
   p->row_order = (kind == 0 && (int64_t)M >= (int64_t)N) ? 1 : 0;
+  // Short M, long N, no strip: bands of ceil(tiles_n / 8) tile columns, so
+  // that each XCD's share of the tiles is one band and the tile rows that
+  // read one B panel run on one XCD together (c3 rotation 2302 -> 2196 us).
+  // With a strip the column order stays: each workgroup's tiles are then a
+  // whole column, strip tile included, and the split needs no fix-up.
+  if (kind == 0 && p->row_order == 0 && p->tiles_m > 1 && p->tiles_n >= 16 &&
+      p->xrow == 0 && p->xcol == 0)
+    p->row_order = (p->tiles_n + 7) / 8;
   p->kchunks = pl.kchunks;
   p->kci = (p->iters_per_tile + pl.kchunks - 1) / pl.kchunks;
   p->num_tiles = tiles_for(kind, pl.cfg, M, N);
@@ -659,9 +667,16 @@ static const int64_t* split_table(skgemm_ctx* c, const Problem& p, int cfg,
       while (ti * (ti + 1) / 2 > tile) --ti;
       while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
       tj = tile - ti * (ti + 1) / 2;
-    } else if (p.row_order) {
+    } else if (p.row_order == 1) {
       ti = tile / p.tiles_n;
       tj = tile % p.tiles_n;
+    } else if (p.row_order >= 2) {
+      const int64_t per = (int64_t)p.tiles_m * p.row_order;
+      const int64_t bnd = tile / per, r = tile - bnd * per;
+      const int64_t bw = std::min<int64_t>(p.row_order,
+                                           p.tiles_n - bnd * p.row_order);
+      ti = r / bw;
+      tj = bnd * p.row_order + r % bw;
     } else {
       ti = tile % p.tiles_m;
       tj = tile / p.tiles_m;

@@ -43,7 +43,9 @@ struct Problem {
   double alpha, beta, gamma;
   int tiles_m, tiles_n;
   int iters_per_tile;  // ceil(K / BK), at least 1
-  int row_order;       // GEMM tile order: 1 = tj fastest (A panels read once)
+  // GEMM tile order: 0 = ti fastest, 1 = tj fastest (A panels read once),
+  // >= 2 = bands of row_order tile columns, tj fastest within a band.
+  int row_order;
   // K is cut into kchunks chunks of kci BK-steps; a unit is (chunk, tile).
   // With kchunks > 1 each unit writes its raw sum to part[c] (row-major
   // M x N, ld = N) and a reduce pass forms D.

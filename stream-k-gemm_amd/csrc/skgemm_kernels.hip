@@ -802,9 +802,17 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
     int ti, tj;
     if (syrk) {
       tri_tile(tile, ti, tj);
-    } else if (row_order) {
+    } else if (row_order == 1) {
       ti = (int)(tile / tiles_n);
       tj = (int)(tile % tiles_n);
+    } else if (row_order >= 2) {
+      // Bands of row_order tile columns, tj fastest within a band.
+      const int64_t per = (int64_t)tiles_m * row_order;
+      const int bnd = (int)(tile / per);
+      const int r = (int)(tile - bnd * per);
+      const int bw = min(row_order, tiles_n - bnd * row_order);
+      ti = r / bw;
+      tj = bnd * row_order + r % bw;
     } else {
       ti = (int)(tile % tiles_m);
       tj = (int)(tile / tiles_m);

@@ -472,3 +472,25 @@ def test_strip_c2_rotation_column_major(skg, ctx, rng):
         finally:
             ctx.set_tuning(-1, 0)
         assert rel_fro(decode_d(D, 1, Nd, nb, Nd), ref) < TOL, grid
+
+
+def test_banded_tile_order(skg, ctx, rng):
+    """Short M, long N, no strip: tiles walk bands of ceil(tiles_n / 8)
+    tile columns (here 2 x 17 tiles of 128, bands of 3, the last one
+    partial), under the default grid, split tiles and K chunks."""
+    M, N, K = 200, 2100, 300
+    A = rng.uniform(-1, 1, (M, K))
+    B = rng.uniform(-1, 1, (K, N))
+    ref = A @ B + 0.5
+    try:
+        for grid, kchunks in ((0, 1), (7, 1), (37, 1), (0, 3)):
+            ctx.set_tuning(8, grid)
+            ctx.set_kchunks(kchunks)
+            D = np.zeros(M * N)
+            ctx.dgemm_host(0, 0, 0, M, N, K, 1.0,
+                           np.ascontiguousarray(A.ravel()), K,
+                           np.ascontiguousarray(B.ravel()), N, 0.0, D, N, 0.5)
+            assert rel_fro(D.reshape(M, N), ref) < TOL, (grid, kchunks)
+    finally:
+        ctx.set_tuning(-1, 0)
+        ctx.set_kchunks(0)
