@@ -112,6 +112,23 @@ int skgemm_synchronize(skgemm_ctx* ctx);
  * error instead of a hang. */
 int skgemm_set_spin_limit(skgemm_ctx* ctx, int64_t limit);
 
+/* Which entries a SYRK on this context writes (plain, split form, batched
+ * and their host forms):
+ *   SKGEMM_SYRK_LOWER (default): the lower triangle i >= j of the row-major
+ *     C only; nothing above the diagonal is written.
+ *   SKGEMM_SYRK_DIAG_BLOCKS: the lower triangle plus the whole of every
+ *     16 x 16 diagonal block (i / 16 == j / 16), as the reference's store
+ *     does: it writes full 16 x 16 blocks of every tile pair tm1 <= tm2 of
+ *     its column-major C, the diagonal pairs included
+ *     (include/sme_syrk.hpp:57-109,137-149; syrk_crsp.cpp:137-244,306-348).
+ *     Entries above the diagonal inside those blocks get the symmetric
+ *     value alpha (A^T A)_ij + beta C_ij.  Unlike the reference, nothing
+ *     outside the N x N matrix is written when N % 16 != 0.
+ * The C++ facade's sme_syrk<double>::compute selects the second mode. */
+#define SKGEMM_SYRK_LOWER 0
+#define SKGEMM_SYRK_DIAG_BLOCKS 1
+int skgemm_set_syrk_store(skgemm_ctx* ctx, int mode);
+
 /* Device facts used by the scheduler: CU count, XCD count, LDS per CU. */
 int skgemm_device_info(skgemm_ctx* ctx, int* num_cus, int* num_xcds,
                        int* lds_bytes_per_cu, int* clock_mhz);
@@ -154,7 +171,8 @@ int skgemm_dgemm_host(skgemm_ctx* ctx, int layout_a, int layout_b,
  * row-major N x N matrix C (== the reference's column-major upper
  * triangle), A is K x N in layout layout_a (B-operand convention: ROW_MAJOR
  * k*ld+n, COL_MAJOR k+n*ld, PACK16 = P16).  Entries above the diagonal are
- * never written.  include/sme_syrk.hpp:137-149, syrk_crsp.cpp:306-348.
+ * not written, except inside the 16 x 16 diagonal blocks under
+ * SKGEMM_SYRK_DIAG_BLOCKS (skgemm_set_syrk_store).  include/sme_syrk.hpp:137-149, syrk_crsp.cpp:306-348.
  * Unlike the reference (quirk: alpha applied to the last KC block only,
  * include/sme_syrk.hpp:123-133) alpha scales the whole product. */
 int skgemm_dsyrk(skgemm_ctx* ctx, int layout_a, int N, int K, double alpha,

@@ -22,11 +22,13 @@
 // result back.  A failing call cannot report through a void signature, so
 // it prints the library's error text and aborts.
 //
-// Differences (supersets): any M, N, K (the reference truncates to
-// 16*floor(M/16) rows and 32*floor(N/32) columns, gemm_crsp.cpp:197-198);
-// sme_syrk applies alpha to the whole product (the reference applies it to
-// the last KC block only, include/sme_syrk.hpp:123-133) and never writes
-// outside the M x M matrix (include/sme_syrk.hpp:66-108 does when M % 16).
+// Differences: sme_syrk applies alpha to the whole product (the reference
+// applies it to the last KC block only, include/sme_syrk.hpp:123-133; the
+// two agree for alpha = 1 or K <= 256) and never writes outside the M x M
+// matrix (include/sme_syrk.hpp:66-108 does when M % 16 != 0).  The gemm
+// entry points keep the reference's truncation to 16*floor(M/16) rows and
+// 32*floor(N/32) columns (gemm_crsp.cpp:197-198,375-376) with gamma on the
+// remaining entries; the C ABI itself computes every row and column.
 #pragma once
 
 #include <chrono>
@@ -45,7 +47,10 @@ inline skgemm_ctx* ctx() {
   static thread_local skgemm_ctx* c = nullptr;
   if (c == nullptr) {
     const char* dev = std::getenv("SKGEMM_DEVICE");
-    if (skgemm_create(dev ? std::atoi(dev) : 0, &c) != SKGEMM_OK) {
+    if (skgemm_create(dev ? std::atoi(dev) : 0, &c) != SKGEMM_OK ||
+        // sme_syrk writes whole 16 x 16 diagonal blocks, as the reference's
+        // tile store does (include/sme_syrk.hpp:57-109).
+        skgemm_set_syrk_store(c, SKGEMM_SYRK_DIAG_BLOCKS) != SKGEMM_OK) {
       std::fprintf(stderr, "skgemm: %s\n", skgemm_last_error());
       std::abort();
     }
@@ -90,6 +95,15 @@ class gemm<double> {
   static constexpr int ZA_TILE_M = 16;  // PA panel width (gemm_crsp.cpp:62)
   static constexpr int ZA_TILE_N = 32;  // PB panel width (gemm_crsp.cpp:63)
 
+  // Rows >= Mt and columns >= Nt of the row-major M x N D: the reference's
+  // reduction sums zero partials there (gemm_crsp.cpp:285-310), giving
+  // alpha * 0 + gamma (one exact product, so fused or not it is the same).
+  static void edges(int M, int N, int Mt, int Nt, T* D, T alpha, T gamma) {
+    const T v = alpha * 0.0 + gamma;
+    for (int i = 0; i < M; ++i)
+      for (int j = (i < Mt ? Nt : 0); j < N; ++j) D[(size_t)i * N + j] = v;
+  }
+
   // gemm_crsp.cpp:261-311.  C_local is the reference's per-thread workspace;
   // the GPU path keeps its workspace inside the library, so it is unused.
   static void compute_streamk_packed_core(const int M, const int N,
@@ -100,15 +114,24 @@ class gemm<double> {
     compute_streamk_packed(M, N, K, A_packed, B_packed, D, alpha, gamma);
   }
 
-  // gemm_crsp.cpp:316-338: D = alpha * A * B + gamma on PA / PB.
+  // gemm_crsp.cpp:316-338: D = alpha * A * B + gamma on PA / PB.  PA holds
+  // floor(M/16) panels and PB floor(N/32) (gemm_crsp.cpp:375-376), so the
+  // product covers the first 16*floor(M/16) rows and 32*floor(N/32)
+  // columns; the reference's reduction still writes every element of D
+  // (gemm_crsp.cpp:285-310), with a zero sum outside those panels, i.e.
+  // alpha * 0 + gamma there.
   static void compute_streamk_packed(const int M, const int N, const int K,
                                      const T* A_packed, const T* B_packed,
                                      T* D, const T alpha, const T gamma) {
-    skgemm_facade::check(
-        skgemm_dgemm_host(skgemm_facade::ctx(), SKGEMM_PACK16, SKGEMM_PACK32,
-                          SKGEMM_ROW_MAJOR, M, N, K, alpha, A_packed, K,
-                          B_packed, K, 0.0, D, N, gamma),
-        "gemm::compute_streamk_packed");
+    const int Mt = (M / ZA_TILE_M) * ZA_TILE_M, Nt = (N / ZA_TILE_N) * ZA_TILE_N;
+    if (Mt > 0 && Nt > 0)
+      skgemm_facade::check(
+          skgemm_dgemm_host(skgemm_facade::ctx(), SKGEMM_PACK16, SKGEMM_PACK32,
+                            SKGEMM_ROW_MAJOR, Mt, Nt, K, alpha, A_packed,
+                            K > 0 ? K : 1, B_packed, K > 0 ? K : 1, 0.0, D, N,
+                            gamma),
+          "gemm::compute_streamk_packed");
+    edges(M, N, Mt, Nt, D, alpha, gamma);
   }
 
   // gemm_opt.cpp:314-337 (same contract, M-tile decomposition there).
@@ -141,17 +164,25 @@ class gemm<double> {
   }
 
   // gemm_crsp.cpp:369-386: A column-major M x K, B row-major K x N, D
-  // row-major M x N, D = alpha * A * B + gamma.  The GPU reads the unpacked
-  // layouts directly, so no packing pass is needed.
+  // row-major M x N, D = alpha * A * B + gamma.  The reference packs A and B
+  // (truncating to whole panels) and runs compute_streamk_packed, so the
+  // same rows and columns come out: the product on the first
+  // 16*floor(M/16) rows and 32*floor(N/32) columns, alpha * 0 + gamma on
+  // the rest.  The GPU reads the unpacked layouts directly (no packing
+  // pass); skgemm_dgemm_host computes every row and column for a caller
+  // that wants them.
   static void compute_test(const int M, const int N, const int K,
                            const T* A_colmajor, const T* B_rowmajor, T* D,
                            const T alpha, const T gamma) {
-    skgemm_facade::check(
-        skgemm_dgemm_host(skgemm_facade::ctx(), SKGEMM_COL_MAJOR,
-                          SKGEMM_ROW_MAJOR, SKGEMM_ROW_MAJOR, M, N, K, alpha,
-                          A_colmajor, M > 0 ? M : 1, B_rowmajor,
-                          N > 0 ? N : 1, 0.0, D, N > 0 ? N : 1, gamma),
-        "gemm::compute_test");
+    const int Mt = (M / ZA_TILE_M) * ZA_TILE_M, Nt = (N / ZA_TILE_N) * ZA_TILE_N;
+    if (Mt > 0 && Nt > 0)
+      skgemm_facade::check(
+          skgemm_dgemm_host(skgemm_facade::ctx(), SKGEMM_COL_MAJOR,
+                            SKGEMM_ROW_MAJOR, SKGEMM_ROW_MAJOR, Mt, Nt, K,
+                            alpha, A_colmajor, M, B_rowmajor, N, 0.0, D, N,
+                            gamma),
+          "gemm::compute_test");
+    edges(M, N, Mt, Nt, D, alpha, gamma);
   }
 
   // gemm_crsp.cpp:388-430: ns per call, operands resident on the device,
@@ -168,10 +199,14 @@ class gemm<double> {
     skgemm_facade::check(skgemm_dev_alloc(c, sizeof(T) * M * N, 1, &dD), "alloc");
     skgemm_memcpy_h2d(c, dA, A_colmajor, sizeof(T) * M * K);
     skgemm_memcpy_h2d(c, dB, B_rowmajor, sizeof(T) * K * N);
+    // Same rows and columns as compute_test (whole panels; gamma on the
+    // rest after the copy back).
+    const int Mt = (M / ZA_TILE_M) * ZA_TILE_M, Nt = (N / ZA_TILE_N) * ZA_TILE_N;
     auto run = [&] {
+      if (Mt == 0 || Nt == 0) return;
       skgemm_facade::check(
           skgemm_dgemm(c, SKGEMM_COL_MAJOR, SKGEMM_ROW_MAJOR, SKGEMM_ROW_MAJOR,
-                       M, N, K, alpha, (const T*)dA, M, (const T*)dB, N, 0.0,
+                       Mt, Nt, K, alpha, (const T*)dA, M, (const T*)dB, N, 0.0,
                        (T*)dD, N, gamma),
           "gemm::compute_benchmark");
     };
@@ -185,6 +220,7 @@ class gemm<double> {
     }
     skgemm_facade::check(skgemm_synchronize(c), "synchronize");
     skgemm_memcpy_d2h(c, D, dD, sizeof(T) * M * N);
+    edges(M, N, Mt, Nt, D, alpha, gamma);
     skgemm_dev_free(c, dA);
     skgemm_dev_free(c, dB);
     skgemm_dev_free(c, dD);
@@ -203,9 +239,11 @@ class sme_syrk<double> {
   static constexpr int TILE_M = 16;
 
   // include/sme_syrk.hpp:137-149: C = alpha * A^T * A + beta * C on the
-  // upper triangle (m1 <= m2) of the M x M column-major C; A is P16
-  // [ceil(M/16)][K][16].  Column-major upper == row-major lower, which is
-  // what the C ABI writes.
+  // upper triangle (m1 <= m2) of the M x M column-major C and on the whole
+  // of every 16 x 16 diagonal block, as the reference's store writes them
+  // (:57-109); A is P16 [ceil(M/16)][K][16].  Column-major upper ==
+  // row-major lower, which is what the C ABI writes; the facade's context
+  // selects SKGEMM_SYRK_DIAG_BLOCKS for the diagonal blocks.
   static void compute(const int K, const int M, const T* A_packed, T* C,
                       const T alpha, const T beta) {
     skgemm_facade::check(

@@ -208,7 +208,8 @@ static inline void ukernel_16x32(int k_len, const double* __restrict a,
 // reload/store per KC block) and :261-311 (k_chunk = ceil(K/T), per-thread
 // full M x N partial, thread-ordered reduction, D = alpha*sum + gamma).
 // Works on the packed layouts PA [M/16][K][16], PB [N/32][K][32]; like the
-// reference it covers only 16*floor(M/16) rows and 32*floor(N/32) columns.
+// reference the product covers only 16*floor(M/16) rows and 32*floor(N/32)
+// columns, and the reduction writes alpha * 0 + gamma to the rest of D.
 // ---------------------------------------------------------------------------
 void oracle_gemm_crsp_packed(int M, int N, int K, const double* A_packed,
                              const double* B_packed, double* D, double alpha,
@@ -246,11 +247,13 @@ void oracle_gemm_crsp_packed(int M, int N, int K, const double* A_packed,
       }
     }
   }
-  // gemm_crsp.cpp:285-310: sum partials in thread order 0..T-1, then
-  // D = alpha*sum + gamma (fused multiply-add, as svmla).
+  // gemm_crsp.cpp:285-310: for every i < M, j < N sum the partials in
+  // thread order 0..T-1, then D = alpha*sum + gamma (fused multiply-add, as
+  // svmla).  Outside the 16*floor(M/16) x 32*floor(N/32) panels no thread
+  // wrote a partial, so the sum is zero and D = alpha * 0 + gamma there.
 #pragma omp parallel for num_threads(nthreads)
-  for (int i = 0; i < tiles_m * 16; ++i)
-    for (int j = 0; j < tiles_n * 32; ++j) {
+  for (int i = 0; i < M; ++i)
+    for (int j = 0; j < N; ++j) {
       double sum = 0.0;
       for (int t = 0; t < nthreads; ++t) sum += C_local[t][(long)i * N + j];
       D[(long)i * N + j] = std::fma(sum, alpha, gamma);
@@ -332,7 +335,10 @@ void oracle_syrk_reference(int K, int M, const double* A_packed, double* C,
 
 // sme_syrk.hpp:111-149 / syrk_crsp.cpp:246-348: dynamic schedule over tile
 // pairs tm1 <= tm2, 16x16 blocks accumulated per KC=256 block and stored with
-// a running beta.  Two deliberate departures, both documented:
+// a running beta.  Every pair's block is stored whole (sme_syrk.hpp:57-109,
+// syrk_crsp.cpp:137-244), so the diagonal pairs tm1 == tm2 also define the
+// entries m1 > m2 of their 16 x 16 block.  Two deliberate departures, both
+// documented:
 //  * Q3: the reference applies alpha only to the last KC block
 //    (sme_syrk.hpp:123-133); this restatement computes the intended
 //    alpha*A^T*A + beta*C (identical when alpha == 1, the only value the
@@ -369,7 +375,7 @@ void oracle_syrk_tiles(int K, int M, const double* A_packed, double* C,
         if (col >= M) continue;
         for (int i = 0; i < 16; ++i) {
           const int row = tm1 * 16 + i;
-          if (row >= M || row > col) continue;
+          if (row >= M) continue;  // whole 16 x 16 block, clipped to M
           double* c = C + row + (long)col * M;
           *c = (beta == 0.0) ? alpha * acc[i][j] : alpha * acc[i][j] + beta * *c;
         }
@@ -680,6 +686,35 @@ void oracle_stencil_apply(int ni, int nj, int nk, int nb, int F,
           out[((long)(b / out_tile) * K + g) * out_tile + b % out_tile] = res;
         }
       }
+}
+
+// ---------------------------------------------------------------------------
+// Split form (Ozaki scheme II), exact integer product and its conversion to
+// FP64 (the step oracle/ozaki.py's exact_product + to_double restates with
+// Python integers), for sizes where Python integers are too slow.  xa (M x K)
+// and xb (N x K) hold the scaled integers rint(x 2^sigma), |x| < 2^55, row
+// per free index.  The sum of K products is exact in 128-bit arithmetic:
+// every partial sum is bounded by ||xa_i|| ||xb_j|| (Cauchy-Schwarz on the
+// prefix), which the scaling keeps below 2^110.  out[i N + j] =
+// ldexp(fl(fl(hi) 2^64 + fl(lo)), -(sa_i + sb_j)) with hi = C >> 64 (signed)
+// and lo = C mod 2^64, the conversion the library's CRT pass performs.
+// No counterpart in the reference (its path is FP64 only).
+// ---------------------------------------------------------------------------
+void oracle_ozaki_exact(int M, int N, int K, const int64_t* xa,
+                        const int64_t* xb, const int* sa, const int* sb,
+                        double* out) {
+#pragma omp parallel for collapse(2) schedule(dynamic, 16)
+  for (int i = 0; i < M; ++i)
+    for (int j = 0; j < N; ++j) {
+      const int64_t* a = xa + (long)i * K;
+      const int64_t* b = xb + (long)j * K;
+      __int128 acc = 0;
+      for (int k = 0; k < K; ++k) acc += (__int128)a[k] * b[k];
+      const int64_t hi = (int64_t)(acc >> 64);
+      const uint64_t lo = (uint64_t)acc;
+      const double d = (double)hi * 18446744073709551616.0 + (double)lo;
+      out[(long)i * N + j] = std::ldexp(d, -(sa[i] + sb[j]));
+    }
 }
 
 }  // extern "C"

@@ -64,6 +64,7 @@ def load(path=None):
         "oracle_trp_3d": [I, I, P, P, P, I],
         "oracle_dgemm": [I, I, I, I, I, I, D, P, L64, P, L64, D, P, L64, D, I],
         "oracle_dsyrk": [I, I, I, D, P, L64, D, P, L64, I],
+        "oracle_ozaki_exact": [I, I, I, P, P, P, P, P],
     }
     for name, args in sig.items():
         f = getattr(L, name)

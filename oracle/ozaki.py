@@ -102,3 +102,36 @@ def product(Xa, Xb, L=14):
     xb = scaled_integers(Xb, sb)
     cint = exact_product(xa, xb)
     return to_double(cint, sa, sb), sa, sb, cint
+
+
+def exact_to_double(xa, xb, sa, sb):
+    """exact_product + to_double in 128-bit C arithmetic (oracle.cpp
+    oracle_ozaki_exact), for sizes where Python integers are too slow.
+    xa: M x K, xb: N x K integer-valued float64 (|x| < 2^55)."""
+    from oracle import oracle
+    a = np.ascontiguousarray(xa).astype(np.int64)
+    b = np.ascontiguousarray(xb).astype(np.int64)
+    assert np.array_equal(a, xa) and np.array_equal(b, xb)
+    M, K = a.shape
+    N = b.shape[0]
+    sa = np.ascontiguousarray(sa, np.int32)
+    sb = np.ascontiguousarray(sb, np.int32)
+    out = np.empty(M * N, np.float64)
+    oracle.load().oracle_ozaki_exact(M, N, K, a.ctypes.data, b.ctypes.data,
+                                     sa.ctypes.data, sb.ctypes.data,
+                                     out.ctypes.data)
+    return out.reshape(M, N)
+
+
+def product_rows(Xa, Xb, L=14, norm2_a=None, norm2_b=None):
+    """The split-form product of the rows Xa (M x K) with every row of Xb
+    (N x K), computed as product() does, through exact_to_double.  norm2_a /
+    norm2_b: squared row norms when they are known for the full operand
+    (the scale of a row depends on that row alone)."""
+    m, h = tables(L)
+    na = np.einsum("ij,ij->i", Xa, Xa) if norm2_a is None else norm2_a
+    nb = np.einsum("ij,ij->i", Xb, Xb) if norm2_b is None else norm2_b
+    sa, sb = sigma(na, h), sigma(nb, h)
+    xa = scaled_integers(Xa, sa)
+    xb = scaled_integers(Xb, sb)
+    return exact_to_double(xa, xb, sa, sb), sa, sb

@@ -284,6 +284,15 @@ extern "C" int skgemm_set_spin_limit(skgemm_ctx* c, int64_t limit) {
   return SKGEMM_OK;
 }
 
+extern "C" int skgemm_set_syrk_store(skgemm_ctx* c, int mode) {
+  SKG_TRY(check_ctx(c));
+  if (mode != SKGEMM_SYRK_LOWER && mode != SKGEMM_SYRK_DIAG_BLOCKS)
+    return set_error(SKGEMM_ERR_ARG, "SYRK store mode must be %d or %d",
+                     SKGEMM_SYRK_LOWER, SKGEMM_SYRK_DIAG_BLOCKS);
+  c->syrk_store = mode;
+  return SKGEMM_OK;
+}
+
 // ---- memory -----------------------------------------------------------------
 
 extern "C" int skgemm_dev_alloc(skgemm_ctx* c, size_t bytes, int zero,
@@ -601,7 +610,7 @@ int fill_problem(skgemm_ctx* c, Problem* p, int kind, const Plan& pl, int M,
   p->M = M;
   p->N = N;
   p->K = K;
-  p->syrk = kind == 1;
+  p->syrk = kind == 1 ? (c->syrk_store ? 2 : 1) : 0;
   p->xcol = strip_cols(pl.cfg, kind, N);
   p->xrow = strip_rows(pl.cfg, kind, M, N);
   p->tiles_m = p->xrow ? M / t.bm : (M + t.bm - 1) / t.bm;
@@ -661,26 +670,8 @@ static const int64_t* split_table(skgemm_ctx* c, const Problem& p, int cfg,
   const int wb = (t.bm / t.wm / 16) * (t.bn / t.wn / 16);
   std::vector<int> wt((size_t)p.num_tiles);
   for (int64_t tile = 0; tile < p.num_tiles; ++tile) {
-    int64_t ti, tj;
-    if (p.syrk) {
-      ti = (int64_t)((std::sqrt(8.0 * (double)tile + 1.0) - 1.0) * 0.5);
-      while (ti * (ti + 1) / 2 > tile) --ti;
-      while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
-      tj = tile - ti * (ti + 1) / 2;
-    } else if (p.row_order == 1) {
-      ti = tile / p.tiles_n;
-      tj = tile % p.tiles_n;
-    } else if (p.row_order >= 2) {
-      const int64_t per = (int64_t)p.tiles_m * p.row_order;
-      const int64_t bnd = tile / per, r = tile - bnd * per;
-      const int64_t bw = std::min<int64_t>(p.row_order,
-                                           p.tiles_n - bnd * p.row_order);
-      ti = r / bw;
-      tj = bnd * p.row_order + r % bw;
-    } else {
-      ti = tile % p.tiles_m;
-      tj = tile / p.tiles_m;
-    }
+    int ti, tj;
+    tile_coords(tile, p.syrk, p.row_order, p.tiles_m, p.tiles_n, ti, tj);
     const bool strip = (p.xrow && ti == p.tiles_m - 1) ||
                        (p.xcol && tj == p.tiles_n - 1);
     wt[tile] = wb + (strip ? 1 : 0) + (strip && p.syrk && ti == tj ? 1 : 0);

@@ -45,6 +45,8 @@ struct skgemm_ctx {
   int flags_n = 0;
   int* status = nullptr;
   int64_t spin_limit = 1ll << 26;
+  // SYRK outputs: SKGEMM_SYRK_LOWER or SKGEMM_SYRK_DIAG_BLOCKS.
+  int syrk_store = SKGEMM_SYRK_LOWER;
   // Per-chunk sums of K-chunked launches.
   double* part = nullptr;
   size_t part_n = 0;

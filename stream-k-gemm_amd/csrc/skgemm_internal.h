@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 
 namespace skg {
@@ -39,7 +40,8 @@ struct Problem {
   Operand a, b;
   Output d;
   int M, N, K;
-  int syrk;            // 1: tiles of the lower triangle only, diagonal masked
+  int syrk;            // 1: tiles of the lower triangle only, diagonal masked;
+                       // 2: the same, storing whole 16 x 16 diagonal blocks
   double alpha, beta, gamma;
   int tiles_m, tiles_n;
   int iters_per_tile;  // ceil(K / BK), at least 1
@@ -61,6 +63,48 @@ struct Problem {
   int64_t iter_begin;  // first flattened MAC-iteration index
   int64_t num_tiles;
 };
+
+// SYRK store predicate.  syrk == 1: the lower triangle (row >= col) of the
+// row-major C.  syrk == 2: that plus the whole of every 16 x 16 diagonal
+// block, which is what the reference's tile store writes
+// (include/sme_syrk.hpp:57-109, syrk_crsp.cpp:137-244: full 16 x 16 blocks
+// of the tile pairs tm1 <= tm2, the diagonal pairs included).
+__device__ __forceinline__ bool syrk_keep(int syrk, int row, int col) {
+  return syrk == 0 || row >= col || (syrk == 2 && (row >> 4) == (col >> 4));
+}
+
+// Flattened tile index -> tile coordinates (ti, tj) of one problem, shared
+// by the kernel and the host's cost-weighted split so both walk the same
+// order.  SYRK: the row-major lower triangle (ti >= tj), t = ti (ti + 1) / 2
+// + tj.  GEMM: row_order 0 = ti fastest, 1 = tj fastest, >= 2 = bands of
+// row_order tile columns (the last band may be narrower), tj fastest
+// within a band, bands in order.
+__host__ __device__ __forceinline__ void tile_coords(int64_t t, int syrk,
+                                                     int row_order,
+                                                     int tiles_m, int tiles_n,
+                                                     int& ti, int& tj) {
+  if (syrk) {
+    int64_t i = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while (i * (i + 1) / 2 > t) --i;
+    while ((i + 1) * (i + 2) / 2 <= t) ++i;
+    ti = (int)i;
+    tj = (int)(t - i * (i + 1) / 2);
+  } else if (row_order == 1) {
+    ti = (int)(t / tiles_n);
+    tj = (int)(t % tiles_n);
+  } else if (row_order >= 2) {
+    const int64_t per = (int64_t)tiles_m * row_order;
+    const int bnd = (int)(t / per);
+    const int r = (int)(t - bnd * per);
+    const int left = tiles_n - bnd * row_order;
+    const int bw = row_order < left ? row_order : left;
+    ti = r / bw;
+    tj = bnd * row_order + r % bw;
+  } else {
+    ti = (int)(t % tiles_m);
+    tj = (int)(t / tiles_m);
+  }
+}
 
 // Launch-wide state of one Stream-K launch.
 struct Launch {

@@ -245,15 +245,6 @@ struct Stage {
   }
 };
 
-// Row-major lower-triangle tile index t -> (ti, tj), ti >= tj.
-__device__ __forceinline__ void tri_tile(int64_t t, int& ti, int& tj) {
-  int64_t i = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-  while (i * (i + 1) / 2 > t) --i;
-  while ((i + 1) * (i + 2) / 2 <= t) ++i;
-  ti = (int)i;
-  tj = (int)(t - i * (i + 1) / 2);
-}
-
 // Wave w -> sub-tile (wm, wn).  Hardware places wave w on SIMD w % 4.
 // wn = w / WM keeps the WM waves of one sub-tile column on distinct SIMDs;
 // the rotation by wn (WM / 2) puts the WN waves of one sub-tile row on
@@ -800,23 +791,7 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
     const int ke = (int)((end - t0) < kci ? (end - t0) : kci);
 
     int ti, tj;
-    if (syrk) {
-      tri_tile(tile, ti, tj);
-    } else if (row_order == 1) {
-      ti = (int)(tile / tiles_n);
-      tj = (int)(tile % tiles_n);
-    } else if (row_order >= 2) {
-      // Bands of row_order tile columns, tj fastest within a band.
-      const int64_t per = (int64_t)tiles_m * row_order;
-      const int bnd = (int)(tile / per);
-      const int r = (int)(tile - bnd * per);
-      const int bw = min(row_order, tiles_n - bnd * row_order);
-      ti = r / bw;
-      tj = bnd * row_order + r % bw;
-    } else {
-      ti = (int)(tile % tiles_m);
-      tj = (int)(tile / tiles_m);
-    }
+    tile_coords(tile, syrk, row_order, tiles_m, tiles_n, ti, tj);
     const int m0 = ti * BM, n0 = tj * BN;
     // Strip tiles: the last tile column (row) of a problem whose N (M)
     // remainder of at most 16 rides on it.
@@ -998,7 +973,7 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
               const int row = row0 + 4 * v;
-              if (row < M && col < N && (!syrk || row >= col))
+              if (row < M && col < N && syrk_keep(syrk, row, col))
                 pb[(int64_t)row * N + col] = acc[i][j][v];
             }
           }
@@ -1015,7 +990,7 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const int row = m0 + BM + lq + 4 * v;
-            if (row < M && col < N && row >= col)
+            if (row < M && col < N && syrk_keep(syrk, row, col))
               pb[(int64_t)row * N + col] = cacc[v];
           }
         }
@@ -1034,7 +1009,7 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const int row = row0 + 4 * v;
-            if (row < M && col < N && (!syrk || row >= col)) {
+            if (row < M && col < N && syrk_keep(syrk, row, col)) {
               double val = alpha * acc[i][j][v];
               if (beta != 0.0) val = fma(beta, dst[v * rstep], val);
               dst[v * rstep] = val + gamma;
@@ -1060,7 +1035,7 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int row = m0 + BM + lq + 4 * v;
-          if (row < M && col < N && row >= col) {
+          if (row < M && col < N && syrk_keep(syrk, row, col)) {
             double val = alpha * cacc[v];
             if (beta != 0.0) val = fma(beta, dst[v * rstep], val);
             dst[v * rstep] = val + gamma;
@@ -1188,21 +1163,26 @@ size_t lds_bytes_dgemm_sk(int cfg) {
 // ---------------------------------------------------------------------------
 __global__ void chunk_reduce_kernel(Problem p) {
   const int col = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int row = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (row >= p.M || col >= p.N || (p.syrk && row < col)) return;
+  if (col >= p.N) return;
   if (p.pred != nullptr && *p.pred == 0) return;
   const int64_t mn = (int64_t)p.M * p.N;
-  const gdouble* src = (const gdouble*)(p.part) + (int64_t)row * p.N + col;
-  double sum = 0.0;
-  for (int c = 0; c < p.kchunks; ++c) sum += src[c * mn];
-  gdouble* dst = (gdouble*)(p.d.ptr) + out_addr(p.d, row, col);
-  double val = p.alpha * sum;
-  if (p.beta != 0.0) val = fma(p.beta, *dst, val);
-  *dst = val + p.gamma;
+  // Rows grid-stride in steps of 4 gridDim.y (gridDim.y <= 65535).
+  for (int row = blockIdx.y * 4 + (threadIdx.x >> 6); row < p.M;
+       row += 4 * gridDim.y) {
+    if (!syrk_keep(p.syrk, row, col)) continue;
+    const gdouble* src = (const gdouble*)(p.part) + (int64_t)row * p.N + col;
+    double sum = 0.0;
+    for (int c = 0; c < p.kchunks; ++c) sum += src[c * mn];
+    gdouble* dst = (gdouble*)(p.d.ptr) + out_addr(p.d, row, col);
+    double val = p.alpha * sum;
+    if (p.beta != 0.0) val = fma(p.beta, *dst, val);
+    *dst = val + p.gamma;
+  }
 }
 
 hipError_t launch_chunk_reduce(const Problem& p, hipStream_t s) {
-  dim3 grid((p.N + 63) / 64, (p.M + 3) / 4);
+  const int rows = (p.M + 3) / 4;
+  dim3 grid((p.N + 63) / 64, rows < 65535 ? rows : 65535);
   hipLaunchKernelGGL(chunk_reduce_kernel, grid, dim3(256), 0, s, p);
   return hipGetLastError();
 }

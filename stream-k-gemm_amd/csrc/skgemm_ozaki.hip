@@ -973,10 +973,15 @@ __global__ __launch_bounds__(256) void crt_kernel(CrtLaunch p) {
   double contrib = 0.0;
   for (int64_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
     const int bc = (int)(b / nbr), br = (int)(b - (int64_t)bc * nbr);
-    if (p.syrk && br * 256 + 255 < bc * 4) continue;
+    // First column whose entries this SYRK keeps at or below a row: the
+    // column itself, or the start of its 16-block when whole diagonal
+    // blocks are stored (syrk == 2).
+    const int cb = p.syrk == 2 ? ((bc * 4) & ~15) : bc * 4;
+    if (p.syrk && br * 256 + 255 < cb) continue;
     const int row0 = br * 256 + (threadIdx.x & 63) * 4;
     const int col = bc * 4 + (threadIdx.x >> 6);
-    if (row0 >= p.M || col >= p.N || (p.syrk && row0 + 3 < col)) continue;
+    const int cmin = p.syrk == 2 ? (col & ~15) : col;
+    if (row0 >= p.M || col >= p.N || (p.syrk && row0 + 3 < cmin)) continue;
     const uint8_t* src = p.R + (int64_t)col * p.mp + row0;
     uint32_t rw[kMaxMod];
 #pragma unroll
@@ -1005,7 +1010,7 @@ __global__ __launch_bounds__(256) void crt_kernel(CrtLaunch p) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int row = row0 + e;
-      if (row < p.M && (!p.syrk || row >= col)) {
+      if (row < p.M && syrk_keep(p.syrk, row, col)) {
         int64_t xh;
         uint64_t xl;
         if (p.T.L <= kLimb37Max)
@@ -1014,7 +1019,9 @@ __global__ __launch_bounds__(256) void crt_kernel(CrtLaunch p) {
           crt<4, 32>(p.T, rw, e, xh, xl);
         const double d = (double)xh * 18446744073709551616.0 + (double)xl;
         const double c = ldexp(d, -(p.sig_a[row] + sb));
-        contrib += (p.syrk && row != col) ? 2.0 * c * c : c * c;
+        // ||C||_F^2 from the lower triangle (off-diagonal entries twice).
+        if (!p.syrk || row >= col)
+          contrib += (p.syrk && row != col) ? 2.0 * c * c : c * c;
         double* dst = p.d.ptr + out_addr(p.d, row, col);
         double val = p.alpha * c;
         if (p.beta != 0.0) val = fma(p.beta, *dst, val);
@@ -1631,7 +1638,7 @@ int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
   P.d = pr.d;
   P.M = M;
   P.N = N;
-  P.syrk = kind == 1;
+  P.syrk = kind == 1 ? (c->syrk_store ? 2 : 1) : 0;
   P.alpha = pr.alpha;
   P.beta = pr.beta;
   P.gamma = pr.gamma;

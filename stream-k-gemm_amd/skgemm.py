@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libskgemm.so")
 
 ROW_MAJOR, COL_MAJOR, PACK16, PACK32 = 0, 1, 2, 3
 FP64, OZAKI_INT8 = 0, 1
+SYRK_LOWER, SYRK_DIAG_BLOCKS = 0, 1
 OK = 0
 ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_TIMEOUT, ERR_NOTSUP, ERR_STATE = -1, -2, -3, -4, -5, -6
 
@@ -24,6 +25,7 @@ ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_TIMEOUT, ERR_NOTSUP, ERR_STATE = -1, -2, -3, -4
 EXPORTS = [
     "skgemm_last_error", "skgemm_version", "skgemm_create", "skgemm_destroy",
     "skgemm_set_stream", "skgemm_synchronize", "skgemm_set_spin_limit",
+    "skgemm_set_syrk_store",
     "skgemm_device_info", "skgemm_dev_alloc", "skgemm_dev_free",
     "skgemm_memcpy_h2d", "skgemm_memcpy_d2h", "skgemm_dgemm",
     "skgemm_dgemm_host", "skgemm_dsyrk", "skgemm_dsyrk_host",
@@ -83,6 +85,7 @@ def lib():
         "skgemm_set_stream": (c_int, [P, P]),
         "skgemm_synchronize": (c_int, [P]),
         "skgemm_set_spin_limit": (c_int, [P, c_int64]),
+        "skgemm_set_syrk_store": (c_int, [P, c_int]),
         "skgemm_device_info": (c_int, [P, P, P, P, P]),
         "skgemm_dev_alloc": (c_int, [P, c_size_t, c_int, ctypes.POINTER(P)]),
         "skgemm_dev_free": (c_int, [P, P]),
@@ -218,6 +221,10 @@ class Context:
 
     def set_spin_limit(self, n):
         check(lib().skgemm_set_spin_limit(self.h, n))
+
+    def set_syrk_store(self, mode):
+        """SYRK_LOWER or SYRK_DIAG_BLOCKS (whole 16 x 16 diagonal blocks)."""
+        check(lib().skgemm_set_syrk_store(self.h, mode))
 
     def device_info(self):
         v = [c_int() for _ in range(4)]

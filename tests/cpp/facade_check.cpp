@@ -77,8 +77,59 @@ int main() {
                                                       D2.data(), alpha, gamma);
     expect(ns > 0.0, "gemm::compute_benchmark ns per call", ns);
   }
+  // Ragged M = 20, N = 40 with buffers of the reference's sizes: PA holds
+  // floor(M/16) = 1 panel, PB floor(N/32) = 1 (gemm_crsp.cpp:375-376).  The
+  // product covers rows < 16 and columns < 32; every other entry is
+  // alpha * 0 + gamma, as the reference's reduction writes it
+  // (gemm_crsp.cpp:285-310).
+  {
+    const int M = 20, N = 40, K = 300;
+    const double alpha = -1.5, gamma = 0.75;
+    std::vector<double> A((size_t)M * K), B((size_t)K * N);
+    for (int k = 0; k < K; ++k)
+      for (int i = 0; i < M; ++i) A[i + (size_t)k * M] = 0.01 * (i + 1) + 0.001 * (k + 1);
+    for (int k = 0; k < K; ++k)
+      for (int j = 0; j < N; ++j) B[(size_t)k * N + j] = 0.001 * (k + 1) + 0.01 * (j + 1);
+    std::vector<double> PA((size_t)(M / 16) * K * 16), PB((size_t)(N / 32) * K * 32);
+    gemm<double>::pack_A(M, K, A.data(), PA.data());
+    gemm<double>::pack_B(N, K, B.data(), PB.data());
+    const long double S1 = (long double)K * (K + 1) / 2,
+                      S2 = (long double)K * (K + 1) * (2 * K + 1) / 6;
+    for (int form = 0; form < 3; ++form) {
+      std::vector<double> D((size_t)M * N, -99.0);
+      if (form == 0)
+        gemm<double>::compute_streamk_packed(M, N, K, PA.data(), PB.data(), D.data(),
+                                             alpha, gamma);
+      else if (form == 1)
+        gemm<double>::compute_test(M, N, K, A.data(), B.data(), D.data(), alpha, gamma);
+      else
+        gemm<double>::compute_benchmark(M, N, K, A.data(), B.data(), D.data(), alpha,
+                                        gamma, 1, 1);
+      double worst = 0.0, edge = 0.0;
+      for (int i = 0; i < M; ++i)
+        for (int j = 0; j < N; ++j) {
+          const double got = D[(size_t)i * N + j];
+          if (i < 16 && j < 32) {
+            const long double I = i + 1, J = j + 1;
+            const long double want =
+                alpha * (1e-5L * (I + J) * S1 + 1e-4L * I * J * K + 1e-6L * S2) + gamma;
+            worst = std::fmax(worst, (double)std::fabs((got - want) / want));
+          } else {
+            edge = std::fmax(edge, std::fabs(got - gamma));
+          }
+        }
+      static const char* names[3] = {"compute_streamk_packed", "compute_test",
+                                     "compute_benchmark"};
+      char what[96];
+      std::snprintf(what, sizeof what, "gemm::%s M=20 N=40 panels", names[form]);
+      expect(worst < 1e-13, what, worst);
+      std::snprintf(what, sizeof what, "gemm::%s M=20 N=40 gamma edges", names[form]);
+      expect(edge == 0.0, what, edge);
+    }
+  }
   // sme_syrk<double>::compute: P16 input, C = alpha A^T A + beta C on the
-  // column-major upper triangle; entries below it untouched.
+  // column-major upper triangle and on the whole of every 16 x 16 diagonal
+  // block (include/sme_syrk.hpp:57-109); other entries below it untouched.
   {
     const int K = 3000, M = 70, t = (M + 15) / 16;
     std::vector<double> Ad((size_t)K * M), P((size_t)t * K * 16, 0.0);
@@ -95,7 +146,7 @@ int main() {
     for (int m2 = 0; m2 < M; ++m2)
       for (int m1 = 0; m1 < M; ++m1) {
         const double got = C[m1 + (size_t)m2 * M];
-        if (m1 <= m2) {
+        if (m1 <= m2 || m1 / 16 == m2 / 16) {
           long double acc = 0;
           for (int k = 0; k < K; ++k) acc += (long double)Ad[(size_t)k * M + m1] * Ad[(size_t)k * M + m2];
           const double want = (double)(1.5L * acc + 0.5L);
@@ -105,7 +156,7 @@ int main() {
         }
       }
     expect(worst < 1e-13, "sme_syrk::compute vs naive (long double)", worst);
-    expect(untouched == 0.0, "sme_syrk::compute leaves m1 > m2 alone", untouched);
+    expect(untouched == 0.0, "sme_syrk::compute leaves m1 > m2 off the blocks", untouched);
   }
   // sme_gemm_trp_3d<double>::compute with Q = I: the band-packed output
   // holds the grid-packed input, re-laid out.

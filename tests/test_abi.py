@@ -208,3 +208,183 @@ def test_stencil_argument_checks_without_device(skg):
                                   1) == skg.ERR_STATE
     assert lib.skgemm_stencil_apply(None, 4, 4, 4, 16, 2, None, None, c, c, c,
                                     0.0, None, 32) == skg.ERR_STATE
+
+
+# ---- tile orders (tile_coords, shared by the kernel and the host split) ----
+
+@pytest.fixture(scope="module")
+def tile_order_exe(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("toc") / "tile_order_check")
+    subprocess.run([os.environ.get("HIPCC", "/opt/rocm/bin/hipcc"),
+                    "-std=c++17", "-O1", "--offload-arch=gfx950",
+                    "-D__HIP_PLATFORM_AMD__",
+                    os.path.join(ROOT, "tests", "cpp", "tile_order_check.cpp"),
+                    "-o", out], check=True, capture_output=True, text=True)
+    return out
+
+
+def tile_coords_of(exe, syrk, order, tm, tn):
+    out = subprocess.run([exe, str(syrk), str(order), str(tm), str(tn)],
+                         check=True, capture_output=True, text=True).stdout
+    return [tuple(int(v) for v in line.split()) for line in out.splitlines()]
+
+
+@pytest.mark.parametrize("order,tm,tn", [
+    (0, 1, 1), (0, 5, 3), (1, 5, 3), (1, 1, 40), (2, 2, 17), (3, 2, 17),
+    (3, 1, 16), (4, 3, 16), (8, 2, 64), (5, 4, 23), (7, 6, 20), (16, 2, 16),
+    (3, 7, 2)])
+def test_gemm_tile_orders_are_bijections(tile_order_exe, order, tm, tn):
+    """Each GEMM tile order visits every (ti, tj) of the tm x tn grid once:
+    ti fastest (0), tj fastest (1), and bands of `order` tile columns with
+    tj fastest inside a band (>= 2), the last band narrower when order does
+    not divide tn, with tm > 1."""
+    seq = tile_coords_of(tile_order_exe, 0, order, tm, tn)
+    assert sorted(seq) == [(i, j) for i in range(tm) for j in range(tn)]
+    if order == 0:
+        assert seq == [(i, j) for j in range(tn) for i in range(tm)]
+    elif order == 1:
+        assert seq == [(i, j) for i in range(tm) for j in range(tn)]
+    else:
+        want = []
+        for b0 in range(0, tn, order):
+            cols = range(b0, min(b0 + order, tn))
+            want += [(i, j) for i in range(tm) for j in cols]
+        assert seq == want
+
+
+@pytest.mark.parametrize("tm", [1, 2, 4, 9, 33])
+def test_syrk_tile_order_is_the_lower_triangle(tile_order_exe, tm):
+    seq = tile_coords_of(tile_order_exe, 1, 0, tm, tm)
+    assert seq == [(i, j) for i in range(tm) for j in range(i + 1)]
+
+
+# ---- Stream-K fix-up flag protocol under arbitrary interleavings ----------
+
+class FixupMemory:
+    def __init__(self, G):
+        self.slot = [None] * G      # partial-tile slot of each workgroup
+        self.flag = [0] * G         # 1: that workgroup's partial is ready
+        self.status = 0             # bit 0: a wait hit its spin limit
+        self.out = {}               # tile -> value written
+
+
+def workgroup(w, G, ntiles, ipt, mem, spin_limit, starts):
+    """sk_dgemm_kernel's per-workgroup loop (skgemm_kernels.hip, fix-up
+    section): a generator that yields at every shared-memory access, so a
+    scheduler can interleave workgroups arbitrarily.  The value of a
+    partial is the sum of its iteration indices, so a tile's exact result is
+    the sum over the tile's iterations."""
+    total = ntiles * ipt
+    it, end = starts(w), starts(w + 1)
+    while it < end:
+        tile = it // ipt
+        t0 = tile * ipt
+        kb, ke = it - t0, min(end - t0, ipt)
+        acc = sum(range(t0 + kb, t0 + ke))
+        yield
+        if kb != 0:
+            # spill: stores complete (s_waitcnt 0, barrier), then the flag
+            assert mem.slot[w] is None, "a slot is written once per launch"
+            mem.slot[w] = acc
+            yield
+            mem.flag[w] = 1
+            yield
+            it = t0 + ke
+            continue
+        if ke < ipt:
+            w2 = w + 1
+            while w2 < G and starts(w2) < t0 + ipt:
+                if starts(w2 + 1) == starts(w2):
+                    w2 += 1
+                    continue
+                n = 0
+                while mem.flag[w2] == 0:
+                    yield
+                    n += 1
+                    if n > spin_limit:
+                        mem.status |= 1
+                        break
+                mem.flag[w2] = 0          # consumed (cleared by its consumer)
+                yield
+                acc += mem.slot[w2] if mem.slot[w2] is not None else 0
+                yield
+                w2 += 1
+        mem.out[tile] = acc
+        yield
+        it = t0 + ke
+
+
+def run_launch(G, ntiles, ipt, mem, rng, spin_limit=1 << 40, hold=None,
+               hold_steps=0):
+    """Random interleaving of all G workgroups (all resident at once).  A
+    workgroup in `hold` is not scheduled during the first hold_steps steps
+    (a late peer).  Returns the number of scheduler steps."""
+    total = ntiles * ipt
+    starts = lambda x: wg_start(x, total, G)
+    mem.slot = [None] * G
+    mem.out = {}
+    live = {w: workgroup(w, G, ntiles, ipt, mem, spin_limit, starts)
+            for w in range(G)}
+    steps = 0
+    while live:
+        ready = [w for w in live
+                 if hold is None or w not in hold or steps >= hold_steps]
+        w = ready[int(rng.integers(len(ready)))] if ready else None
+        if w is None:
+            steps += 1
+            continue
+        try:
+            next(live[w])
+        except StopIteration:
+            del live[w]
+        steps += 1
+        assert steps < 10 ** 6, "no progress: a wait that never ends"
+    return steps
+
+
+@pytest.mark.parametrize("ntiles,ipt,G", [(1, 12, 5), (3, 7, 4), (10, 3, 7),
+                                          (2, 40, 9), (5, 5, 25), (7, 9, 3)])
+def test_fixup_protocol_random_interleavings(ntiles, ipt, G):
+    """Every tile's result is exact, every flag is consumed and cleared by
+    the end of the launch (so a replayed graph needs no memset), and no
+    schedule deadlocks: an owner only waits on later workgroups, whose
+    partial is the first unit of their range."""
+    rng = np.random.default_rng(ntiles * 1000 + ipt * 10 + G)
+    G = min(G, ntiles * ipt)
+    mem = FixupMemory(G)
+    for trial in range(200):
+        run_launch(G, ntiles, ipt, mem, rng)
+        assert mem.status == 0
+        assert mem.flag == [0] * G
+        for t in range(ntiles):
+            assert mem.out[t] == sum(range(t * ipt, (t + 1) * ipt)), (trial, t)
+
+
+def test_fixup_timeout_then_reset():
+    """A peer that is late beyond the owner's spin limit: the owner gives up
+    (status bit 0, which the host reports as SKGEMM_ERR_TIMEOUT), every
+    workgroup still finishes, and the late peer's flag is left set.  Without
+    the host's reset (skgemm_ctx::check_status zeroes the flags on a
+    timeout) the next launch's owner would take that stale flag and add a
+    slot before its new partial is there; with the reset the next launch is
+    exact."""
+    rng = np.random.default_rng(7)
+    ntiles, ipt, G = 1, 20, 2      # owner w0, peer w1 on the one tile
+    mem = FixupMemory(G)
+    run_launch(G, ntiles, ipt, mem, rng, spin_limit=2, hold={1},
+               hold_steps=50)
+    assert mem.status == 1
+    assert mem.flag[1] == 1                           # stale flag
+    exact = sum(range(ipt))
+    assert mem.out[0] != exact
+    # Next launch without the reset: the owner may consume the stale flag
+    # before the peer rewrote its slot (peer held back so it is certain).
+    stale = FixupMemory(G)
+    stale.flag = list(mem.flag)
+    run_launch(G, ntiles, ipt, stale, rng, hold={1}, hold_steps=50)
+    assert stale.out[0] != exact
+    # With the reset (status and flags zeroed), any interleaving is exact.
+    mem.status, mem.flag = 0, [0] * G
+    for _ in range(50):
+        run_launch(G, ntiles, ipt, mem, rng)
+        assert mem.status == 0 and mem.out[0] == exact and mem.flag == [0, 0]

@@ -100,15 +100,45 @@ def test_gemm_beta_gamma_placement(orc, rng):
     assert rel_fro(D, 0.5 * A @ B - 2.0 * C0 + 0.25) < 1e-15
 
 
-def test_crsp_partial_panels_untouched(orc):
-    """Quirk Q1: the crsp restatement covers 16*floor(M/16) x 32*floor(N/32)."""
+def closed_form(M, N, K):
+    """A[i,k] = 0.01(i+1) + 0.001(k+1), B[k,j] = 0.001(k+1) + 0.01(j+1)
+    (gemm_crsp.cpp:451-460): (AB)_ij = 1e-5 (I+J) S1 + 1e-4 I J K + 1e-6 S2
+    with I = i+1, J = j+1, S1 = sum k, S2 = sum k^2 over k = 1..K."""
+    I = np.arange(1, M + 1, dtype=np.float64)[:, None]
+    J = np.arange(1, N + 1, dtype=np.float64)[None, :]
+    S1 = K * (K + 1) / 2.0
+    S2 = K * (K + 1) * (2 * K + 1) / 6.0
+    return 1e-5 * (I + J) * S1 + 1e-4 * I * J * K + 1e-6 * S2
+
+
+def test_crsp_partial_panels_gamma(orc):
+    """Quirk Q1 in gemm_crsp: the product covers 16*floor(M/16) rows and
+    32*floor(N/32) columns, but the reduction (gemm_crsp.cpp:285-310) loops
+    over every i < M, j < N with zero partials outside the panels (calloc'd
+    workspace, :326), so those entries get alpha * 0 + gamma.  Inside the
+    panels the closed form holds."""
+    M, N, K = 40, 70, 16
+    alpha, gamma = 2.0, 0.375
+    A, B = orc.gemm_test_inputs(M, N, K)
+    D = np.full(M * N, -7.0)
+    orc.gemm_crsp(M, N, K, orc.pack_A(M, K, A), orc.pack_B(N, K, B), alpha,
+                  gamma, nthreads=2, D=D)
+    D = D.reshape(M, N)
+    assert np.all(D[32:, :] == gamma) and np.all(D[:32, 64:] == gamma)
+    want = alpha * closed_form(32, 64, K) + gamma
+    assert np.max(np.abs(D[:32, :64] - want) / np.abs(want)) < 1e-14
+
+
+def test_opt_partial_panels_untouched(orc):
+    """gemm_opt writes D in place per tile (gemm_opt.cpp:275-287): entries
+    outside its 16*floor(M/16) x 32*floor(N/32) tiles are never written."""
     M, N, K = 40, 70, 16
     A, B = orc.gemm_test_inputs(M, N, K)
     D = np.full(M * N, -7.0)
-    orc.gemm_crsp(M, N, K, orc.pack_A(M, K, A), orc.pack_B(N, K, B), 1.0, 0.0,
-                  nthreads=2, D=D)
+    orc.gemm_opt(M, N, K, orc.pack_A(M, K, A), orc.pack_B(N, K, B), 1.0, 0.0,
+                 nthreads=2, D=D)
     D = D.reshape(M, N)
-    assert np.all(D[32:, :] == -7.0) and np.all(D[:, 64:] == -7.0)
+    assert np.all(D[32:, :] == -7.0) and np.all(D[:32, 64:] == -7.0)
     assert np.all(D[:32, :64] != -7.0)
 
 
@@ -137,12 +167,27 @@ def test_syrk_reference_cases(orc, t):
     ints = np.rint(unpack_p16(ap, K, M) * 100).astype(np.int64)
     exact = (ints.T @ ints) / 1e4
     iu = np.triu_indices(M)
-    for C in (orc.syrk_reference(K, M, ap, 1.0, 0.0),
-              orc.syrk_tiles(K, M, ap, 1.0, 0.0, nthreads=4)):
+    blk = diag_block_lower(M)
+    for tiles, C in ((False, orc.syrk_reference(K, M, ap, 1.0, 0.0)),
+                     (True, orc.syrk_tiles(K, M, ap, 1.0, 0.0, nthreads=4))):
         Cm = C.reshape(M, M).T
         err = np.abs(Cm[iu] - exact[iu]) / np.maximum(1.0, np.abs(exact[iu]))
         assert err.max() < 1e-13  # reference tolerance is 1e-9
-        assert np.all(np.tril(Cm, -1) == 0.0)  # lower never written
+        if tiles:
+            # the tile store writes whole diagonal 16 x 16 blocks
+            # (sme_syrk.hpp:57-109): m1 > m2 there holds (A^T A)[m1, m2]
+            e2 = np.abs(Cm[blk] - exact[blk]) / np.maximum(1.0, np.abs(exact[blk]))
+            assert e2.max() < 1e-13
+            assert np.all(Cm[~blk & ~np.triu(np.ones((M, M), bool))] == 0.0)
+        else:
+            assert np.all(np.tril(Cm, -1) == 0.0)  # syrk_reference: upper only
+
+
+def diag_block_lower(M):
+    """Mask [m1, m2] of the entries m1 > m2 inside the 16 x 16 diagonal
+    blocks (m1 // 16 == m2 // 16)."""
+    i = np.arange(M)
+    return (i[:, None] > i[None, :]) & (i[:, None] // 16 == i[None, :] // 16)
 
 
 def test_syrk_alpha_beta_and_ragged(orc, rng):
@@ -161,8 +206,10 @@ def test_syrk_alpha_beta_and_ragged(orc, rng):
     iu = np.triu_indices(M)
     ref = 0.75 * (A.T @ A) - 1.5 * C0
     assert np.max(np.abs(Cm[iu] - ref[iu])) < 1e-13
-    il = np.tril_indices(M, -1)
-    assert np.array_equal(Cm[il], C0[il])     # strict lower untouched
+    blk = diag_block_lower(M)                 # m1 > m2 in a diagonal block
+    assert np.max(np.abs(Cm[blk] - ref[blk])) < 1e-13
+    rest = ~blk & ~np.triu(np.ones((M, M), bool))
+    assert np.array_equal(Cm[rest], C0[rest])  # the rest of the lower part
 
 
 def test_syrk_orthonormal_columns_give_identity(orc, rng):

@@ -75,3 +75,24 @@ def test_error_against_double_double(orc, rng, L, tol):
     # fewer moduli leave less headroom: the error grows as predicted
     C12, _, _, _ = ozaki.product(X.T.copy(), X.T.copy(), 12)
     assert rel_fro(C12[il], ref[il]) > rel_fro(C[il], ref[il])
+
+
+def test_exact_to_double_matches_python_integers(rng):
+    """The 128-bit C product and conversion equal the Python-integer
+    restatement bit for bit (c3-like rows at the full scaling headroom,
+    mixed signs)."""
+    for L in (13, 14, 16):
+        X = c3_like(rng, 700, 24)
+        C, sa, sb, _ = ozaki.product(X.T.copy(), X.T.copy(), L)
+        m, h = ozaki.tables(L)
+        xa = ozaki.scaled_integers(X.T, sa)
+        got = ozaki.exact_to_double(xa, xa, sa, sb)
+        assert np.array_equal(got, C)
+        rows = [0, 5, 23]
+        got2, _, _ = ozaki.product_rows(X.T[rows].copy(), X.T.copy(), L)
+        assert np.array_equal(got2, C[rows])
+    A = rng.standard_normal((9, 333)) * 10.0 ** rng.uniform(0, 5, 9)[:, None]
+    B = rng.standard_normal((11, 333))
+    C, sa, sb, _ = ozaki.product(A, B, 14)
+    got, _, _ = ozaki.product_rows(A, B, 14)
+    assert np.array_equal(got, C)

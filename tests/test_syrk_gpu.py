@@ -1,6 +1,8 @@
 """GPU parity of the FP64 Stream-K SYRK (lower-triangle tiles only) against
 the oracle through the C ABI.  Bar: 1e-13 relative Frobenius on the lower
-triangle; entries above the diagonal are never written."""
+triangle.  Default store: entries above the diagonal are never written;
+SYRK_DIAG_BLOCKS (the reference's store, include/sme_syrk.hpp:57-109) also
+writes the whole of every 16 x 16 diagonal block."""
 import numpy as np
 import pytest
 
@@ -11,10 +13,21 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-13
 
 
+def diag_blocks_upper(N):
+    """Mask [i, j] of row-major entries j > i inside the 16 x 16 diagonal
+    blocks (what SYRK_DIAG_BLOCKS adds to the lower triangle)."""
+    r = np.arange(N)
+    return (r[None, :] > r[:, None]) & (r[None, :] // 16 == r[:, None] // 16)
+
+
 @pytest.mark.parametrize("t", range(5))
 def test_reference_cases_p16(skg, ctx, orc, t):
     """syrk_crsp.cpp:474-493 through sme_syrk's layouts: P16 input, M x M
-    column-major upper triangle (== row-major lower) output, glibc inputs."""
+    column-major upper triangle (== row-major lower) output, glibc inputs.
+    Default store: the rest is untouched.  SYRK_DIAG_BLOCKS: the whole
+    output of the oracle's restatement of the tile store (whole 16 x 16
+    blocks of every tile pair tm1 <= tm2, sme_syrk.hpp:57-109), the
+    strict lower part off the diagonal blocks untouched."""
     K, M = SYRK_CASES[t]
     ap = orc.init_A_packed(K, M, 42 + t)
     ref = orc.syrk_reference(K, M, ap, 1.0, 0.0)      # column-major upper
@@ -28,6 +41,88 @@ def test_reference_cases_p16(skg, ctx, orc, t):
     err = np.abs(got - want) / np.maximum(1.0, np.abs(want))
     assert err.max() < 1e-12                           # reference: 1e-9
     assert np.all(C.reshape(M, M).T[np.tril_indices(M, -1)] == 0.1)
+    # The reference's store: whole diagonal blocks.
+    full = orc.syrk_tiles(K, M, ap, 1.0, 0.0, C=np.full(M * M, 0.1))
+    Cb = np.full(M * M, 0.1)
+    ctx.set_syrk_store(skg.SYRK_DIAG_BLOCKS)
+    try:
+        ctx.dsyrk_host(skg.PACK16, M, K, 1.0, ap, K, 0.0, Cb, M)
+    finally:
+        ctx.set_syrk_store(skg.SYRK_LOWER)
+    written = full != 0.1
+    assert rel_fro(Cb[written], full[written]) < TOL
+    assert np.all(Cb[~written] == 0.1)
+    blk = diag_blocks_upper(M).T                       # [m1, m2], m1 > m2
+    assert written.reshape(M, M).T[blk].all()
+
+
+def _diag_block_ref(A, C0, alpha, beta):
+    N = A.shape[1]
+    ref = alpha * (A.T @ A) + beta * C0
+    keep = np.tril(np.ones((N, N), bool)) | diag_blocks_upper(N)
+    return ref, keep
+
+
+@pytest.mark.parametrize("N,K,tile,kchunks", [(137, 531, -1, 0),
+                                              (137, 531, -1, 3),
+                                              (200, 64, 12, 1),
+                                              (130, 520, 12, 3),
+                                              (136, 70, 11, 1),
+                                              (64, 300, 1, 2),
+                                              (9, 40, -1, 0)])
+def test_diag_blocks_plain(skg, ctx, rng, N, K, tile, kchunks):
+    """SYRK_DIAG_BLOCKS on the FP64 kernel: ragged N (the last block clipped
+    to N), K chunks (the chunk-reduce pass), strip tiles with the corner
+    block, alpha and beta; every other entry above the diagonal untouched."""
+    A = rng.uniform(-1, 1, (K, N))
+    C0 = rng.uniform(-1, 1, (N, N))
+    ref, keep = _diag_block_ref(A, C0, 0.5, -1.25)
+    ctx.set_syrk_store(skg.SYRK_DIAG_BLOCKS)
+    try:
+        for layout in (0, 1, 2):
+            ctx.set_tuning(tile, 0)
+            ctx.set_kchunks(kchunks)
+            Ab, lda = encode_b(A, layout, 2)
+            C = C0.copy().ravel()
+            ctx.dsyrk_host(layout, N, K, 0.5, Ab, lda, -1.25, C, N)
+            Cm = C.reshape(N, N)
+            assert rel_fro(Cm[keep], ref[keep]) < TOL, layout
+            assert np.array_equal(Cm[~keep], C0[~keep]), layout
+    finally:
+        ctx.set_syrk_store(skg.SYRK_LOWER)
+        ctx.set_tuning(-1, 0)
+        ctx.set_kchunks(0)
+
+
+def test_diag_blocks_split_and_batched(skg, ctx, rng):
+    """SYRK_DIAG_BLOCKS through the split form (CRT pass) and a batched
+    launch (plain FP64 and split): the same entries as the plain call."""
+    N, K = 150, 2000
+    A = rng.uniform(-1, 1, (K, N))
+    C0 = rng.uniform(-1, 1, (N, N))
+    ref, keep = _diag_block_ref(A, C0, 1.0, 0.5)
+    ctx.set_syrk_store(skg.SYRK_DIAG_BLOCKS)
+    try:
+        C = C0.copy().ravel()
+        bound = ctx.dsyrk_ozaki_host(0, N, K, 1.0, np.ascontiguousarray(A.ravel()),
+                                     N, 0.5, C, N)
+        Cm = C.reshape(N, N)
+        assert 0 < bound < 1e-12
+        assert rel_fro(Cm[keep], ref[keep]) < TOL
+        assert np.array_equal(Cm[~keep], C0[~keep])
+        dA = skg.DeviceArray.from_host(ctx, A)
+        for precision in (skg.FP64, skg.OZAKI_INT8):
+            dC = skg.DeviceArray.from_host(ctx, C0)
+            p = skg.Problem(kind=1, layout_a=0, layout_b=0, layout_d=0, M=N,
+                            N=N, K=K, alpha=1.0, beta=0.5, gamma=0.0,
+                            A=dA.ptr, lda=N, B=dA.ptr, ldb=N, D=dC.ptr, ldd=N)
+            ctx.batched([p], precision=precision)
+            ctx.synchronize()
+            Cm = dC.download().reshape(N, N)
+            assert rel_fro(Cm[keep], ref[keep]) < TOL, precision
+            assert np.array_equal(Cm[~keep], C0[~keep]), precision
+    finally:
+        ctx.set_syrk_store(skg.SYRK_LOWER)
 
 
 @pytest.mark.parametrize("layout", [0, 1, 2, 3])
