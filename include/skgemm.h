@@ -182,6 +182,25 @@ int skgemm_dsyrk_host(skgemm_ctx* ctx, int layout_a, int N, int K,
                       double alpha, const double* A, int64_t lda, double beta,
                       double* C, int64_t ldc);
 
+/* Allocation-free form of skgemm_dsyrk (same arguments, same result to
+ * 1e-13 relative Frobenius: the same plan, on the even Stream-K split).  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The call makes no hipMalloc, hipFree
+ * or other allocating call: the fix-up partials, flags and K-chunk sums use
+ * a 128 MiB device array of the library's, one per device, held until the
+ * process ends.  A problem whose plan needs more (flags + grid partial
+ * tiles + kchunks * N * N doubles; e.g. N = 2048 with K chunks) is rejected
+ * with SKGEMM_ERR_NOMEM and a message giving both sizes;
+ * skgemm_set_kchunks(ctx, 1) lowers the need.  All such calls on one device
+ * share the array: calls that overlap in time on one device (several
+ * streams or contexts) race and must be serialised by the caller.
+ * Asynchronous on the context's stream; a fix-up timeout is reported by the
+ * next skgemm_synchronize. */
+int skgemm_dsyrk_noalloc(skgemm_ctx* ctx, int layout_a, int N, int K,
+                         double alpha, const double* A, int64_t lda,
+                         double beta, double* C, int64_t ldc);
+
 /* ---- Ozaki split form (INT8 slices, INT32 accumulation, FP64 sum) ------ */
 
 /* Same contracts as skgemm_dgemm / skgemm_dsyrk (beta/gamma supported) with

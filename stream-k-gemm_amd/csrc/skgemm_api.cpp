@@ -236,6 +236,9 @@ extern "C" int skgemm_create(int device, skgemm_ctx** out) {
   c->last0 = c->ev0;
   c->last1 = c->ev1;
   c->spin_limit = 1ll << 26;
+  // Device address of the allocation-free entry point's scratch array on
+  // this device (once per device; a failure is reported by that entry point).
+  noalloc_init(device);
   *out = c;
   return SKGEMM_OK;
 }
@@ -605,7 +608,7 @@ Plan plan(skgemm_ctx* c, int kind, int la, int lb, int M, int N, int K) {
 
 // Fill a Problem for the Stream-K kernel (tile counts depend on the cfg).
 int fill_problem(skgemm_ctx* c, Problem* p, int kind, const Plan& pl, int M,
-                 int N, int K) {
+                 int N, int K, bool alloc_part) {
   const TileCfg& t = tile_cfg(pl.cfg);
   p->M = M;
   p->N = N;
@@ -631,7 +634,7 @@ int fill_problem(skgemm_ctx* c, Problem* p, int kind, const Plan& pl, int M,
   p->kci = (p->iters_per_tile + pl.kchunks - 1) / pl.kchunks;
   p->num_tiles = tiles_for(kind, pl.cfg, M, N);
   p->part = nullptr;
-  if (pl.kchunks > 1) {
+  if (pl.kchunks > 1 && alloc_part) {
     SKG_TRY(c->ensure_part((size_t)pl.kchunks * M * N));
     p->part = c->part;
   }
@@ -714,7 +717,7 @@ static const int64_t* split_table(skgemm_ctx* c, const Problem& p, int cfg,
 
 // Launch one problem (already described) with the given plan.
 int run_single(skgemm_ctx* c, Problem& p, const Plan& plan_in,
-               const int* pred, bool timed) {
+               const int* pred, bool timed, const CallerBuffers* cb) {
   Plan pl = plan_in;
   if (tile_cfg(pl.cfg).ns >= 2 && !(p.a.vec2 && p.b.vec2)) {
     // LDS-DMA tiles move 16-byte pairs: unaligned operands take the
@@ -738,10 +741,21 @@ int run_single(skgemm_ctx* c, Problem& p, const Plan& plan_in,
   L.num_problems = 1;
   L.total_iters = p.num_tiles * p.kchunks * p.kci;
   L.grid = pl.grid;
-  L.starts = split_table(c, p, pl.cfg, pl.grid);
-  SKG_TRY(c->ensure_ws((size_t)pl.grid * slot_doubles(pl.cfg), pl.grid));
-  L.ws = c->ws;
-  L.flags = c->flags;
+  if (cb != nullptr) {
+    // Buffers the caller provides (nothing is allocated): the even split,
+    // whose starts need no device table.
+    if ((size_t)pl.grid * slot_doubles(pl.cfg) > cb->ws_doubles ||
+        pl.grid > cb->nflags)
+      return set_error(SKGEMM_ERR_NOMEM, "internal: caller buffers too small");
+    L.starts = nullptr;
+    L.ws = cb->ws;
+    L.flags = cb->flags;
+  } else {
+    L.starts = split_table(c, p, pl.cfg, pl.grid);
+    SKG_TRY(c->ensure_ws((size_t)pl.grid * slot_doubles(pl.cfg), pl.grid));
+    L.ws = c->ws;
+    L.flags = c->flags;
+  }
   L.status = c->status;
   L.spin_limit = c->spin_limit;
   if (timed) c->record_begin();

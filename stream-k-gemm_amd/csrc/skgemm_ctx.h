@@ -136,8 +136,10 @@ bool transpose_launch(int& la, int& lb, int& ld_, int& M, int& N,
                       int64_t& ldb);
 int check_ctx(skgemm_ctx* c);
 int check_dims(int M, int N, int K);
+// alloc_part = false leaves p->part NULL when the plan has K chunks (the
+// caller provides the chunk buffer).
 int fill_problem(skgemm_ctx* c, Problem* p, int kind, const Plan& pl, int M,
-                 int N, int K);
+                 int N, int K, bool alloc_part = true);
 // pred: nullptr, or a device flag; the launch does its work only when the
 // flag is non-zero (decided on the device).  timed: record the launch in
 // the context's timing ring.
@@ -154,8 +156,20 @@ int build_plain_table(skgemm_ctx* c, const skgemm_problem* q, int n,
 // dtab: the rows in device memory; dhdr: device {row count, iterations}.
 int launch_plain_table(skgemm_ctx* c, const PlainTable& t, const Problem* dtab,
                        const int64_t* dhdr);
+// Fix-up buffers a caller provides instead of the context's (no
+// allocation, no weighted-split table: the even split).
+struct CallerBuffers {
+  double* ws;
+  size_t ws_doubles;
+  int* flags;
+  int nflags;
+};
 int run_single(skgemm_ctx* c, Problem& p, const Plan& pl,
-               const int* pred = nullptr, bool timed = true);
+               const int* pred = nullptr, bool timed = true,
+               const CallerBuffers* cb = nullptr);
+// Looks up the device address of the allocation-free entry point's scratch
+// array on `device` (current), once per device (skgemm_noalloc.hip).
+int noalloc_init(int device);
 int prepare_gemm(int la, int lb, int ld_, int M, int N, int K, const double* A,
                  int64_t lda, const double* B, int64_t ldb, double* D,
                  int64_t ldd, Problem* p);

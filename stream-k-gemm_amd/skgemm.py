@@ -29,6 +29,7 @@ EXPORTS = [
     "skgemm_device_info", "skgemm_dev_alloc", "skgemm_dev_free",
     "skgemm_memcpy_h2d", "skgemm_memcpy_d2h", "skgemm_dgemm",
     "skgemm_dgemm_host", "skgemm_dsyrk", "skgemm_dsyrk_host",
+    "skgemm_dsyrk_noalloc",
     "skgemm_dgemm_ozaki", "skgemm_dsyrk_ozaki", "skgemm_dgemm_ozaki_host",
     "skgemm_dsyrk_ozaki_host", "skgemm_pack", "skgemm_transpose",
     "skgemm_nloc_create", "skgemm_nloc_apply", "skgemm_nloc_destroy",
@@ -101,6 +102,8 @@ def lib():
                                  c_double, P, c_int64]),
         "skgemm_dsyrk_host": (c_int, [P, c_int, c_int, c_int, c_double, P,
                                       c_int64, c_double, P, c_int64]),
+        "skgemm_dsyrk_noalloc": (c_int, [P, c_int, c_int, c_int, c_double, P,
+                                         c_int64, c_double, P, c_int64]),
         "skgemm_dgemm_ozaki": (c_int, [P, c_int, c_int, c_int, c_int, c_int,
                                        c_int, c_double, P, c_int64, P, c_int64,
                                        c_double, P, c_int64, c_double, c_int,
@@ -272,6 +275,10 @@ class Context:
     def dsyrk(self, la, N, K, alpha, A, lda, beta, C, ldc):
         check(lib().skgemm_dsyrk(self.h, la, N, K, alpha, _ptr(A), lda, beta,
                                  _ptr(C), ldc))
+
+    def dsyrk_noalloc(self, la, N, K, alpha, A, lda, beta, C, ldc):
+        check(lib().skgemm_dsyrk_noalloc(self.h, la, N, K, alpha, _ptr(A),
+                                         lda, beta, _ptr(C), ldc))
 
     def dgemm_ozaki(self, la, lb, ld_, M, N, K, alpha, A, lda, B, ldb, beta, D,
                     ldd, gamma=0.0, slices=0):
