@@ -207,19 +207,30 @@ int skgemm_dsyrk_noalloc(skgemm_ctx* ctx, int layout_a, int N, int K,
  * the product computed in the Ozaki split form (scheme II, CRT variant):
  * power-of-two scaling of rows of A and columns of B from their 2-norms,
  * each scaled operand cut into `slices` INT8 residue planes modulo
- * pairwise-coprime moduli <= 256 (8..16; 0 = 14: each scaled row keeps
- * about 53 bits of its norm), one INT8 MFMA product per plane with exact
- * INT32 accumulation, and an exact CRT reconstruction rounded once to FP64.
- * The result does not depend on the schedule (bit-identical for any plan).
+ * pairwise-coprime moduli <= 256, one INT8 MFMA product per plane with
+ * exact INT32 accumulation (K longer than 131072 is cut into chunks whose
+ * residues are added modulo each modulus), and an exact CRT reconstruction
+ * rounded once to FP64.  The result does not depend on the schedule
+ * (bit-identical for any plan).
+ * slices: 8..16 moduli; 0 = 14 (each scaled row keeps about 54 bits of its
+ * norm); SKGEMM_SLICES_AUTO = the fewest moduli whose bound (below) is at
+ * most 1e-13, the contract tolerance, or plain FP64 when 16 moduli cannot
+ * reach it.  SYRK chooses them before the products from the column norms
+ * (||C||_F >= sqrt(sum_i C_ii^2)); GEMM, which has no such lower bound,
+ * starts at 14 and takes one more while the a-posteriori bound misses it
+ * (at most three passes).  The certifying mode reads norms back (it is
+ * synchronous and cannot be captured into a graph); skgemm_split_info
+ * gives the count it used.
  * *err_bound (may be NULL; non-NULL makes the call synchronous) receives a
  * rigorous a-posteriori bound on ||C - AB||_F / ||AB||_F from the integer
  * rounding of the scaled inputs (worst case over the rounding errors, so
- * it is loose: 3.4e-13 on Psi 65536 x 1024 with 14 moduli where the
- * measured error is ~1e-15), or -1 when a row of A or column of B could not
- * be scaled (squared norm not finite or outside [2^-1000, 2^1000]) and the
- * call was computed by the plain FP64 path instead.  Device pointers.
- * Needs K <= 131072.  No counterpart in the reference (its path is FP64
- * only). */
+ * it is loose: 1.7e-13 on Psi 65536 x 1024 with 14 moduli where the
+ * measured error is ~5e-16), or -1 when the call was computed by the plain
+ * FP64 path instead: a row of A or column of B could not be scaled (squared
+ * norm not finite or outside [2^-1000, 2^1000]), or, certifying, 16 moduli
+ * could not reach 1e-13.  Device pointers.  No counterpart in the
+ * reference (its path is FP64 only). */
+#define SKGEMM_SLICES_AUTO (-1)
 int skgemm_dgemm_ozaki(skgemm_ctx* ctx, int layout_a, int layout_b,
                        int layout_d, int M, int N, int K, double alpha,
                        const double* A, int64_t lda, const double* B,
@@ -414,6 +425,17 @@ int skgemm_last_kernel_ms(skgemm_ctx* ctx, double* ms);
  * skgemm_timing_enable clears the last-launch time. */
 int skgemm_timing_enable(skgemm_ctx* ctx, int capacity);
 int skgemm_timing_read(skgemm_ctx* ctx, double* ms, int max, int* count);
+
+/* Split-form stage timing: with on != 0, every split-form call records one
+ * event pair per stage in the timing ring instead of one per call: (1) the
+ * norms, scales and residue planes of the operands, (2) the INT8 products,
+ * (3) the CRT pass with the predicated FP64 fallback; skgemm_timing_read
+ * then returns three durations per such call, in that order.  (The
+ * pipelined SYRK of SKGEMM_OZ_PIPE records one pair per call.) */
+int skgemm_timing_stages(skgemm_ctx* ctx, int on);
+/* Modulus count of the last split-form call on ctx (0 when it was computed
+ * by the plain FP64 path). */
+int skgemm_split_info(skgemm_ctx* ctx, int* moduli);
 
 /* MFMA peak probe: back-to-back independent MFMAs on every SIMD, no memory
  * traffic.  which: 0 = v_mfma_f64_16x16x4_f64, 1 = v_mfma_i32_32x32x32_i8.

@@ -3,7 +3,7 @@ the library's skgemm_dgemm_ozaki / skgemm_dsyrk_ozaki.  TEST INFRASTRUCTURE
 ONLY (see oracle.py); no counterpart in the reference.
 
 The restatement follows the same steps with numpy / Python integers:
-  sigma_o = h - 1 - ceil(e_o / 2),  ||x_o||^2 = f 2^e_o, h = floor(log2 P / 2) - 1
+  sigma_o = h - 1 - ceil(e_o / 2),  ||x_o||^2 = f 2^e_o, h = floor(log2 P / 2 - 0.05)
   x'_o = rint(x_o 2^sigma_o)          (exact FP64 integers)
   C'   = A' B'                        (exact, Python integers -- no CRT)
   C    = (double(C' >> 64) * 2^64 + double(C' mod 2^64)) * 2^-(sa_i + sb_j)
@@ -26,7 +26,7 @@ MODULI = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211,
 def tables(L):
     m = MODULI[:L]
     bits = sum(math.log2(x) for x in m)
-    h = int(math.floor(bits / 2.0)) - 1
+    h = int(math.floor(bits / 2.0 - 0.05))
     return m, h
 
 

@@ -1142,6 +1142,19 @@ extern "C" int skgemm_set_kchunks(skgemm_ctx* c, int kchunks) {
   return SKGEMM_OK;
 }
 
+extern "C" int skgemm_timing_stages(skgemm_ctx* c, int on) {
+  SKG_TRY(check_ctx(c));
+  c->stage_timing = on != 0;
+  return SKGEMM_OK;
+}
+
+extern "C" int skgemm_split_info(skgemm_ctx* c, int* moduli) {
+  SKG_TRY(check_ctx(c));
+  if (moduli == nullptr) return set_error(SKGEMM_ERR_ARG, "moduli is NULL");
+  *moduli = c->oz_last_moduli;
+  return SKGEMM_OK;
+}
+
 extern "C" int skgemm_timing_enable(skgemm_ctx* c, int capacity) {
   SKG_TRY(check_ctx(c));
   if (capacity < 0) return set_error(SKGEMM_ERR_ARG, "capacity < 0");

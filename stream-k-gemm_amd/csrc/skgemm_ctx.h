@@ -81,6 +81,10 @@ struct skgemm_ctx {
   // Split form: never pick a schedule whose workgroups wait on each other
   // (set on serving slots, whose launches run concurrently).
   bool oz_no_wait = false;
+  // Modulus count of the last split-form call (0: computed in plain FP64).
+  int oz_last_moduli = 0;
+  // skgemm_timing_stages: split-form calls record one event pair per stage.
+  bool stage_timing = false;
 
   int check_status();
   int ensure_ws(size_t ws_doubles, int nflags);

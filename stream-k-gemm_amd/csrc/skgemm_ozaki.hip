@@ -44,7 +44,9 @@ constexpr int kMaxMod = 16;
 // 217 = 7*31, the others prime.
 static const int kModuli[kMaxMod] = {256, 255, 253, 251, 247, 241, 239, 233,
                                      229, 227, 223, 217, 211, 199, 197, 193};
-constexpr int kMaxKpad = 131072;  // Kpad * 128^2 <= 2^31
+// Products one INT32 sum may hold: kMaxKpad * 128^2 <= 2^31.  Longer K is
+// cut into chunks of at most this many (their residues are added modulo m).
+constexpr int kMaxKpad = 131072;
 
 struct Tables {
   int L;
@@ -193,11 +195,11 @@ __global__ __launch_bounds__(256) void norm_o_finish_kernel(
 // x = xh 2^26 + xl exactly (|xl| <= 2^25, |xh| <= 2^28); then
 // y = xh (2^26 mod m) + xl is exact with |y| < 2^36, and for odd m the FP64
 // quotient rint(y * fl(1/m)) is the nearest integer to y / m (its error is
-// below 2^-24 while y / m lies at least 1/(2m) from a half-integer), so
+// below 2^-23 while y / m lies at least 1/(2m) from a half-integer), so
 // r = y - q m is the symmetric residue in [-(m-1)/2, (m-1)/2].  For m = 256
 // the result is in [-128, 128]; +128 and -128 share the byte 0x80, which is
 // the same residue.  Five FP64 operations per residue, all exact.
-// With more than 14 moduli h exceeds 54 and |x| reaches 2^76: then
+// With more than 14 moduli h exceeds 55 and |x| reaches 2^76: then
 // x = x2 2^52 + xh 2^26 + xl (all limbs <= 2^25 in magnitude) and
 // y = x2 (2^52 mod m) + xh (2^26 mod m) + xl, |y| < 2^33, one more FMA.
 struct Split {
@@ -1163,7 +1165,11 @@ Tables make_tables(int L) {
     T.c52[l] = sym_pow2_mod(52, kModuli[l]);
     bits += std::log2((double)kModuli[l]);
   }
-  T.h = (int)std::floor(bits / 2.0) - 1;
+  // Scaled rows have ||x'|| < 2^(h-1) (plus the rounding, sqrt(K)/2), so
+  // |C'_ij| < 2^(2h-2) (1 + 2^-20) by Cauchy-Schwarz, which must stay below
+  // P / 4 = 2^(bits-2) for the CRT: h <= bits / 2 less a margin of 0.05
+  // bits (2^0.1 > 1 + 2^-20).  L = 14: bits = 110.16, h = 55.
+  T.h = (int)std::floor(bits / 2.0 - 0.05);
   typedef unsigned __int128 u128;
   u128 P = 1;
   for (int l = 0; l < L; ++l) P *= (u128)kModuli[l];
@@ -1245,6 +1251,7 @@ namespace {
 
 enum {
   kPlanesA = 0, kPlanesB, kRes, kNormA, kNormB, kSigA, kSigB, kScalars,
+  kSavedD,
   kPartA, kPartB
 };
 
@@ -1259,13 +1266,16 @@ int norm_segments(int O, int K) {
 }
 
 // Scale + split one operand (free index o, contraction k) into planes.
+// norms_ready: norm2, sigma and the fallback flag are already on the device
+// (the certifying mode computed them first); only the residues are formed.
 int prepare_operand(skgemm_ctx* c, const Operand& op, int K, const Tables& T,
                     int8_t* planes, int64_t plane, int kpad, int opad,
-                    double* norm2, int* sigma, int* fallback, int part_buf) {
+                    double* norm2, int* sigma, int* fallback, int part_buf,
+                    bool norms_ready) {
   const int O = op.extent;
-  const bool wide = T.h > 54;
+  const bool wide = T.h > 55;  // |x'| < 2^54: two 26-bit limbs suffice
   const char* fuse = std::getenv("SKGEMM_OZ_FUSE");
-  if (op.kcontig && !(fuse != nullptr && fuse[0] == '0')) {
+  if (op.kcontig && !norms_ready && !(fuse != nullptr && fuse[0] == '0')) {
     // One pass: norm, scale and residues per row.
     if (wide)
       hipLaunchKernelGGL(norm_split_k_kernel<true>, dim3(opad), dim3(1024), 0,
@@ -1276,7 +1286,7 @@ int prepare_operand(skgemm_ctx* c, const Operand& op, int K, const Tables& T,
                          0, c->stream, op, K, T.h, T, planes, plane, kpad,
                          norm2, sigma, fallback);
   } else if (op.kcontig) {
-    if (O > 0)
+    if (O > 0 && !norms_ready)
       hipLaunchKernelGGL(norm_k_kernel, dim3(O), dim3(256), 0, c->stream, op,
                          K, T.h, norm2, sigma, fallback);
     const int nseg = (kpad + kSplitSeg - 1) / kSplitSeg;
@@ -1288,7 +1298,7 @@ int prepare_operand(skgemm_ctx* c, const Operand& op, int K, const Tables& T,
       hipLaunchKernelGGL(split_k_kernel<false>, grid, dim3(256), 0, c->stream,
                          op, K, sigma, T, planes, plane, kpad, 0, nseg);
   } else {
-    if (O > 0) {
+    if (O > 0 && !norms_ready) {
       const int nseg = norm_segments(O, K);
       const int kseg = (K + nseg - 1) / nseg;
       double* part = (double*)c->extra->get(part_buf,
@@ -1325,7 +1335,16 @@ struct OzPlan {
 // "cfg,dp[,grid[,kc]]" overrides (tuning sweeps); a grid forced with
 // skgemm_set_tuning selects Stream-K on that grid, except on contexts that
 // must not wait (oz_no_wait), where it caps the data-parallel grid.
-OzPlan choose_plan(skgemm_ctx* c, int kind, int M, int N, int L, int kpad) {
+// K chunks a data-parallel plan needs so that no unit sums more than
+// kMaxKpad products (each INT32 sum stays exact); above kMaxKpad only
+// data-parallel plans are possible (a Stream-K unit spans all of K).
+static int min_chunks(int kpad, int bk) {
+  const int ipt = kpad / bk, per = kMaxKpad / bk;
+  return (ipt + per - 1) / per;
+}
+
+OzPlan choose_plan_any(skgemm_ctx* c, int kind, int M, int N, int L,
+                       int kpad) {
   auto tiles_for = [&](int cfg) {
     const GemmCfg& g = kGemmCfgs[cfg];
     const int64_t tm = (M + g.bm - 1) / g.bm, tn = (N + g.bn - 1) / g.bn;
@@ -1336,7 +1355,10 @@ OzPlan choose_plan(skgemm_ctx* c, int kind, int M, int N, int L, int kpad) {
     if (std::sscanf(env, "%d,%d,%d,%d", &cfg, &dp, &grid, &kc) >= 2 &&
         cfg >= 0 && cfg < kNumGemmCfgs) {
       const int ipt = kpad / kGemmCfgs[cfg].bk;
-      kc = dp ? std::max(1, std::min(kc, ipt)) : 1;
+      if (kpad > kMaxKpad) dp = 1;
+      kc = dp ? std::max(min_chunks(kpad, kGemmCfgs[cfg].bk),
+                         std::max(1, std::min(kc, ipt)))
+              : 1;
       const int cap = gemm_occupancy(cfg) * c->num_cus;
       const int64_t U = (int64_t)L * tiles_for(cfg) * kc;
       int g = grid > 0 ? std::min(grid, cap) : cap;
@@ -1344,15 +1366,17 @@ OzPlan choose_plan(skgemm_ctx* c, int kind, int M, int N, int L, int kpad) {
       return OzPlan{cfg, dp ? 1 : 0, std::max(1, g), kc};
     }
   }
-  if (c->forced_grid > 0 && !c->oz_no_wait)  // Stream-K on that many WGs
-    return OzPlan{1, 0, std::min(c->forced_grid,
+  if (c->forced_grid > 0 && !c->oz_no_wait && kpad <= kMaxKpad)
+    return OzPlan{1, 0, std::min(c->forced_grid,   // Stream-K on that many
                                  gemm_occupancy(1) * c->num_cus), 1};
   OzPlan best{1, 0, 0, 1};
   double best_score = 0.0;
   for (int cfg : {5, 6}) {
     const int cap = gemm_occupancy(cfg) * c->num_cus;
     const int ipt = kpad / kGemmCfgs[cfg].bk;
-    for (int kc = 1; kc <= 16 && (kc == 1 || ipt / kc >= 32); ++kc) {
+    const int k0 = min_chunks(kpad, kGemmCfgs[cfg].bk);
+    for (int kc = k0; kc <= std::max(16, k0) && (kc == k0 || ipt / kc >= 32);
+         ++kc) {
       const int64_t U = (int64_t)L * tiles_for(cfg) * kc;
       const int64_t rounds = (U + cap - 1) / cap;
       const double eff = (double)U / (double)(rounds * cap);
@@ -1367,10 +1391,11 @@ OzPlan choose_plan(skgemm_ctx* c, int kind, int M, int N, int L, int kpad) {
       }
     }
   }
-  if (!best.dp && c->oz_no_wait) {
-    const int64_t U = (int64_t)L * tiles_for(6);
+  if (!best.dp && (c->oz_no_wait || kpad > kMaxKpad)) {
+    const int k0 = min_chunks(kpad, kGemmCfgs[6].bk);
+    const int64_t U = (int64_t)L * tiles_for(6) * k0;
     best = OzPlan{6, 1, (int)std::min<int64_t>(gemm_occupancy(6) * c->num_cus, U),
-                  1};
+                  k0};
   }
   if (best.dp) {
     if (c->forced_grid > 0) best.grid = std::min(best.grid, c->forced_grid);
@@ -1379,14 +1404,40 @@ OzPlan choose_plan(skgemm_ctx* c, int kind, int M, int N, int L, int kpad) {
   return OzPlan{1, 0, gemm_occupancy(1) * c->num_cus, 1};
 }
 
+OzPlan choose_plan(skgemm_ctx* c, int kind, int M, int N, int L, int kpad) {
+  OzPlan op = choose_plan_any(c, kind, M, N, L, kpad);
+  // Whatever chose it, no unit may sum more than kMaxKpad products.
+  if (kpad > kMaxKpad) {
+    op.dp = 1;
+    op.kc = std::max(op.kc, min_chunks(kpad, kGemmCfgs[op.cfg].bk));
+  }
+  return op;
+}
+
 // A-posteriori bound on ||C - AB||_F / ||AB||_F from the integer rounding
 // of step 1: with |A 2^sigma - A'| <= 1/2 elementwise,
 // |C_ij - (AB)_ij| <= 2^-(sa_i+sb_j) (sqrt(K)/2 (|A'_i| + |B'_j|) + K/4),
 // |A'_i| <= 2^sa_i ||A_i|| + sqrt(K)/2; the Frobenius norm of that bound
 // matrix is a sum of separable terms.
+long double bound_numerator(const std::vector<double>& na,
+                            const std::vector<int>& sa,
+                            const std::vector<double>& nb,
+                            const std::vector<int>& sb, int K);
+
 double error_bound(const std::vector<double>& na, const std::vector<int>& sa,
                    const std::vector<double>& nb, const std::vector<int>& sb,
                    int K, double fro2) {
+  const long double num = bound_numerator(na, sa, nb, sb, K);
+  const long double den = std::sqrt((long double)fro2) - num;
+  if (!(den > 0)) return INFINITY;
+  return (double)(num / den) + 4.5e-16;  // + final roundings
+}
+
+// sqrt(sum_ij (e_i n_j + n_i e_j + e_i e_j)^2) in the notation above.
+long double bound_numerator(const std::vector<double>& na,
+                            const std::vector<int>& sa,
+                            const std::vector<double>& nb,
+                            const std::vector<int>& sb, int K) {
   const long double c = 0.5L * std::sqrt((long double)K), d = 0.25L * K;
   long double A0 = 0, A1 = 0, A2 = 0, B0 = 0, B1 = 0, B2 = 0;
   for (size_t i = 0; i < na.size(); ++i) {
@@ -1405,10 +1456,23 @@ double error_bound(const std::vector<double>& na, const std::vector<int>& sa,
   }
   const long double s = c * c * (A2 * B0 + A0 * B2 + 2 * A1 * B1) +
                         2 * c * d * (A1 * B0 + A0 * B1) + d * d * A0 * B0;
-  const long double num = std::sqrt(std::max(s, 0.0L));
-  const long double den = std::sqrt((long double)fro2) - num;
-  if (!(den > 0)) return INFINITY;
-  return (double)(num / den) + 4.5e-16;  // + final roundings
+  return std::sqrt(std::max(s, 0.0L));
+}
+
+// Scale exponents from squared norms for headroom h (sigma_of on the host;
+// the same frexp exponent, so the same integers as the device's).
+std::vector<int> host_sigma(const std::vector<double>& n2, int h) {
+  std::vector<int> s(n2.size());
+  for (size_t i = 0; i < n2.size(); ++i) {
+    if (!(n2[i] > 0.0)) {
+      s[i] = 0;
+      continue;
+    }
+    int e;
+    std::frexp(n2[i], &e);
+    s[i] = h - 1 - ((e + 1) >> 1);
+  }
+  return s;
 }
 
 }  // namespace
@@ -1423,9 +1487,6 @@ int ozaki_reserve(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
     return set_error(SKGEMM_ERR_ARG, "slices (moduli) must be 0 or 8..%d",
                      kMaxMod);
   const int kpad = (int)round_up(std::max(K, 1), 128);
-  if (kpad > kMaxKpad)
-    return set_error(SKGEMM_ERR_NOTSUP,
-                     "split form needs K <= %d for exact INT32 sums", kMaxKpad);
   if (c->extra == nullptr) {
     c->extra = new skgemm_ctx::Extra();
     c->extra->gen = &c->mem_gen;
@@ -1457,20 +1518,18 @@ int ozaki_reserve(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
   return c->ensure_ws((size_t)pl.grid * slot_doubles(pl.cfg), pl.grid);
 }
 
-// Shared driver.  kind 0: C = A B; kind 1: C = Psi^T Psi (B == A).
-int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
-              int slices, double* err_bound) {
+namespace {
+
+// The split-form sequence with L moduli.  kind 0: C = A B; kind 1:
+// C = Psi^T Psi (B == A).  norms_ready: the norms, scales and fallback flag
+// of A (and B) are already in the context's buffers.
+int run_moduli(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
+               int L, double* err_bound, bool norms_ready,
+               bool timing_begun = false) {
   const Operand& A = pr.a;
   const Operand& B = pr.b;
   const int M = pr.M, N = pr.N, K = pr.K;
-  int L = slices == 0 ? 14 : slices;
-  if (L < 8 || L > kMaxMod)
-    return set_error(SKGEMM_ERR_ARG, "slices (moduli) must be 0 or 8..%d",
-                     kMaxMod);
   const int kpad = (int)round_up(std::max(K, 1), 128);
-  if (kpad > kMaxKpad)
-    return set_error(SKGEMM_ERR_NOTSUP,
-                     "split form needs K <= %d for exact INT32 sums", kMaxKpad);
   if (hipSetDevice(c->device) != hipSuccess)
     return set_error(SKGEMM_ERR_HIP, "hipSetDevice");
   if (c->extra == nullptr) {
@@ -1508,7 +1567,7 @@ int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
                                   skgemm_ctx::Extra::kMaxPipe,
                               kSplitSeg);
       pipe = (kpad + chunk_k - 1) / chunk_k;
-      if (pipe < 2 || !X->pipe_ready()) pipe = 1;
+      if (pipe < 2 || chunk_k > kMaxKpad || !X->pipe_ready()) pipe = 1;
     }
   }
   uint8_t* R = (uint8_t*)X->get(
@@ -1528,11 +1587,18 @@ int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
   const int crt_grid = (int)std::max<int64_t>(
       1, std::min<int64_t>(kCrtGrid, crt_blocks));
 
-  c->record_begin();
-  // Cleared by a kernel, not a memset node, so the sequence stays valid
-  // inside a captured graph.
-  hipError_t e = launch_zero_ints(fallback, 1, c->stream);
-  if (e != hipSuccess) return hip_error(e, "clear split-form flag");
+  if (norms_ready) pipe = 1;
+  // Stage timing (skgemm_timing_stages): one event pair per stage.
+  const bool stages = c->stage_timing && pipe == 1;
+  // timing_begun: the certifying mode's norm pass opened the event pair.
+  if (!timing_begun) c->record_begin();
+  hipError_t e = hipSuccess;
+  if (!norms_ready) {
+    // Cleared by a kernel, not a memset node, so the sequence stays valid
+    // inside a captured graph.
+    e = launch_zero_ints(fallback, 1, c->stream);
+    if (e != hipSuccess) return hip_error(e, "clear split-form flag");
+  }
   int rc = SKGEMM_OK;
   if (pipe > 1) {
     // Norms of the whole operand first (the scale of a row needs all of K).
@@ -1540,13 +1606,17 @@ int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
                        A, K, T.h, n2a, sga, fallback);
   } else {
     rc = prepare_operand(c, A, K, T, planesA, pa, kpad, opa, n2a, sga,
-                         fallback, kPartA);
+                         fallback, kPartA, norms_ready);
   }
   if (rc != SKGEMM_OK) return rc;
   if (kind == 0) {
     rc = prepare_operand(c, B, K, T, planesB, pb, kpad, opb, n2b, sgb,
-                         fallback, kPartB);
+                         fallback, kPartB, norms_ready);
     if (rc != SKGEMM_OK) return rc;
+  }
+  if (stages) {
+    c->record_end();
+    c->record_begin();
   }
 
   GemmLaunch G{};
@@ -1597,7 +1667,7 @@ int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
     G.grid = (int)std::max<int64_t>(1, std::min<int64_t>(op.grid, G.units));
     const int nseg = (kpad + kSplitSeg - 1) / kSplitSeg;
     const int spc = chunk_k / kSplitSeg;  // segments per chunk
-    const bool wide = T.h > 54;
+    const bool wide = T.h > 55;  // |x'| < 2^54: two 26-bit limbs suffice
     for (int ch = 0; ch < pipe; ++ch) {
       const int seg0 = ch * spc, nsegc = std::min(spc, nseg - seg0);
       const dim3 grid((unsigned)((int64_t)opa * nsegc));
@@ -1624,6 +1694,10 @@ int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
   } else {
     e = launch_gemm(op.cfg, G, c->stream);
     if (e != hipSuccess) return hip_error(e, "ozaki gemm launch");
+  }
+  if (stages) {
+    c->record_end();
+    c->record_begin();
   }
 
   CrtLaunch P{};
@@ -1683,7 +1757,155 @@ int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
     for (double v : parts) f2 += v;
     *err_bound = fb ? -1.0 : error_bound(na, sa, nb, sb, K, f2);
   }
+  c->oz_last_moduli = L;
   return SKGEMM_OK;
+}
+
+// Plain FP64 in place of the split form (the certifying mode's last resort).
+int run_plain(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
+              double* err_bound) {
+  Problem fp = pr;
+  const Plan pl = plan(c, kind, la, lb, pr.M, pr.N, pr.K);
+  SKG_TRY(fill_problem(c, &fp, kind, pl, pr.M, pr.N, pr.K));
+  fp.alpha = pr.alpha;
+  fp.beta = pr.beta;
+  fp.gamma = pr.gamma;
+  SKG_TRY(run_single(c, fp, pl));
+  c->oz_last_moduli = 0;
+  if (err_bound) *err_bound = -1.0;
+  return SKGEMM_OK;
+}
+
+// The tolerance the certifying mode (SKGEMM_SLICES_AUTO) guarantees.
+constexpr double kCertifyTarget = 1e-13;
+
+// Certifying mode.  SYRK: the norms first (read back), then the fewest
+// moduli whose a-priori bound, with ||C||_F >= sqrt(sum_i C_ii^2) =
+// sqrt(sum_i ||a_i||^4), is at most kCertifyTarget, and the sequence on the
+// norms already computed.  GEMM (no such lower bound): 14 moduli, then one
+// more while the a-posteriori bound misses the target (D restored from a
+// copy first when beta != 0).  Plain FP64 when 16 moduli cannot certify.
+int run_certify(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
+                double* err_bound) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (c->capturing || hipStreamIsCapturing(c->stream, &cs) != hipSuccess ||
+      cs != hipStreamCaptureStatusNone)
+    return set_error(SKGEMM_ERR_NOTSUP,
+                     "the certifying split form reads norms back and cannot "
+                     "be captured; give a modulus count");
+  const int M = pr.M, K = pr.K;
+  hipError_t e;
+  if (kind == 1) {
+    if (c->extra == nullptr) {
+      c->extra = new skgemm_ctx::Extra();
+      c->extra->gen = &c->mem_gen;
+    }
+    auto* X = c->extra;
+    const int opa = (int)round_up(M, 256);
+    double* n2a = (double*)X->get(kNormA, sizeof(double) * opa);
+    int* sga = (int*)X->get(kSigA, sizeof(int) * opa);
+    char* scal = (char*)X->get(kScalars, 8 + sizeof(double) * kCrtGrid);
+    if (!n2a || !sga || !scal)
+      return set_error(SKGEMM_ERR_NOMEM, "split-form workspace");
+    int* fallback = (int*)scal;
+    const Tables T14 = make_tables(14);
+    c->record_begin();  // the norm pass counts in the call's (first stage's) time
+    if ((e = launch_zero_ints(fallback, 1, c->stream)) != hipSuccess)
+      return hip_error(e, "clear split-form flag");
+    const Operand& A = pr.a;
+    if (A.kcontig) {
+      hipLaunchKernelGGL(norm_k_kernel, dim3(M), dim3(256), 0, c->stream, A, K,
+                         T14.h, n2a, sga, fallback);
+    } else {
+      const int nseg = norm_segments(M, K);
+      const int kseg = (K + nseg - 1) / nseg;
+      double* part = (double*)X->get(kPartA, sizeof(double) * 2 * nseg * M);
+      if (part == nullptr) return set_error(SKGEMM_ERR_NOMEM, "norm partials");
+      hipLaunchKernelGGL(norm_o_partial_kernel, dim3((M + 255) / 256, nseg),
+                         dim3(256), 0, c->stream, A, K, kseg, part,
+                         part + (int64_t)nseg * M);
+      hipLaunchKernelGGL(norm_o_finish_kernel, dim3((M + 255) / 256),
+                         dim3(256), 0, c->stream, M, nseg, T14.h, part,
+                         part + (int64_t)nseg * M, n2a, sga, fallback);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return hip_error(e, "norms");
+    std::vector<double> n2(M);
+    int fb = 0;
+    if ((e = hipMemcpyAsync(n2.data(), n2a, sizeof(double) * M,
+                            hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(&fb, fallback, sizeof(int), hipMemcpyDeviceToHost,
+                            c->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+      return hip_error(e, "norm readback");
+    if (fb) return run_moduli(c, kind, la, lb, pr, 14, err_bound, false);
+    long double diag2 = 0;
+    for (double v : n2) diag2 += (long double)v * v;
+    for (int L = 8; L <= kMaxMod; ++L) {
+      const std::vector<int> sg = host_sigma(n2, make_tables(L).h);
+      const long double num = bound_numerator(n2, sg, n2, sg, K);
+      const long double den = std::sqrt(diag2) - num;
+      if (!(den > 0) || (double)(num / den) + 4.5e-16 > kCertifyTarget)
+        continue;
+      if ((e = hipMemcpyAsync(sga, sg.data(), sizeof(int) * M,
+                              hipMemcpyHostToDevice, c->stream)) != hipSuccess ||
+          (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+        return hip_error(e, "scale upload");
+      return run_moduli(c, kind, la, lb, pr, L, err_bound, true, true);
+    }
+    return run_plain(c, kind, la, lb, pr, err_bound);
+  }
+  // GEMM: keep D when beta != 0, so that a second attempt starts from it.
+  void* saved = nullptr;
+  int64_t nd = 0;
+  if (pr.beta != 0.0) {
+    // Elements the output spans: its largest offset is that of (M-1, N-1)
+    // in every layout (a panel stride exceeds the panel's contents).
+    const Output& d = pr.d;
+    const int n = pr.N - 1;
+    nd = (int64_t)(n >> d.pshift) * d.pstride +
+         (int64_t)(n & ((1 << d.pshift) - 1)) * d.ns +
+         (int64_t)(M - 1) * d.ms + 1;
+    if (c->extra == nullptr) {
+      c->extra = new skgemm_ctx::Extra();
+      c->extra->gen = &c->mem_gen;
+    }
+    saved = c->extra->get(kSavedD, sizeof(double) * nd);
+    if (saved == nullptr) return set_error(SKGEMM_ERR_NOMEM, "saved D");
+    if ((e = hipMemcpyAsync(saved, pr.d.ptr, sizeof(double) * nd,
+                            hipMemcpyDeviceToDevice, c->stream)) != hipSuccess)
+      return hip_error(e, "save D");
+  }
+  for (int L = 14; L <= kMaxMod; ++L) {
+    if (L > 14 && saved != nullptr &&
+        (e = hipMemcpyAsync(pr.d.ptr, saved, sizeof(double) * nd,
+                            hipMemcpyDeviceToDevice, c->stream)) != hipSuccess)
+      return hip_error(e, "restore D");
+    double b = 0.0;
+    SKG_TRY(run_moduli(c, kind, la, lb, pr, L, &b, false));
+    if (b < 0.0 || b <= kCertifyTarget) {
+      if (err_bound) *err_bound = b;
+      return SKGEMM_OK;
+    }
+  }
+  if (saved != nullptr &&
+      (e = hipMemcpyAsync(pr.d.ptr, saved, sizeof(double) * nd,
+                          hipMemcpyDeviceToDevice, c->stream)) != hipSuccess)
+    return hip_error(e, "restore D");
+  return run_plain(c, kind, la, lb, pr, err_bound);
+}
+
+}  // namespace
+
+int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
+              int slices, double* err_bound) {
+  if (slices == SKGEMM_SLICES_AUTO)
+    return run_certify(c, kind, la, lb, pr, err_bound);
+  const int L = slices == 0 ? 14 : slices;
+  if (L < 8 || L > kMaxMod)
+    return set_error(SKGEMM_ERR_ARG,
+                     "slices (moduli) must be %d (certifying), 0 or 8..%d",
+                     SKGEMM_SLICES_AUTO, kMaxMod);
+  return run_moduli(c, kind, la, lb, pr, L, err_bound, false);
 }
 
 }  // namespace skg

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "libskgemm.so")
 ROW_MAJOR, COL_MAJOR, PACK16, PACK32 = 0, 1, 2, 3
 FP64, OZAKI_INT8 = 0, 1
 SYRK_LOWER, SYRK_DIAG_BLOCKS = 0, 1
+SLICES_AUTO = -1
 OK = 0
 ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_TIMEOUT, ERR_NOTSUP, ERR_STATE = -1, -2, -3, -4, -5, -6
 
@@ -38,6 +39,7 @@ EXPORTS = [
     "skgemm_num_tiles",
     "skgemm_tile_info", "skgemm_autotune", "skgemm_load_tuning",
     "skgemm_query_plan", "skgemm_timing_enable", "skgemm_timing_read",
+    "skgemm_timing_stages", "skgemm_split_info",
     "skgemm_last_kernel_ms", "skgemm_peak_probe",
     "skgemm_copy_probe", "skgemm_fill_uniform", "skgemm_fill_normal",
     "skgemm_scale_columns_pow10", "skgemm_server_create",
@@ -143,6 +145,8 @@ def lib():
                                       c_int, P, P, P, P, P]),
         "skgemm_timing_enable": (c_int, [P, c_int]),
         "skgemm_timing_read": (c_int, [P, P, c_int, P]),
+        "skgemm_timing_stages": (c_int, [P, c_int]),
+        "skgemm_split_info": (c_int, [P, P]),
         "skgemm_last_kernel_ms": (c_int, [P, P]),
         "skgemm_peak_probe": (c_int, [P, c_int, P]),
         "skgemm_copy_probe": (c_int, [P, c_size_t, P]),
@@ -261,6 +265,15 @@ class Context:
                                        ctypes.byref(n)))
         return buf[:n.value].copy()
 
+    def timing_stages(self, on=True):
+        check(lib().skgemm_timing_stages(self.h, 1 if on else 0))
+
+    def split_moduli(self):
+        """Modulus count of the last split-form call (0: plain FP64)."""
+        v = c_int()
+        check(lib().skgemm_split_info(self.h, ctypes.byref(v)))
+        return v.value
+
     def last_kernel_ms(self):
         d = _out_double()
         check(lib().skgemm_last_kernel_ms(self.h, ctypes.byref(d)))
@@ -281,20 +294,22 @@ class Context:
                                          lda, beta, _ptr(C), ldc))
 
     def dgemm_ozaki(self, la, lb, ld_, M, N, K, alpha, A, lda, B, ldb, beta, D,
-                    ldd, gamma=0.0, slices=0):
+                    ldd, gamma=0.0, slices=0, bound=True):
         e = _out_double()
         check(lib().skgemm_dgemm_ozaki(self.h, la, lb, ld_, M, N, K, alpha,
                                        _ptr(A), lda, _ptr(B), ldb, beta,
                                        _ptr(D), ldd, gamma, slices,
-                                       ctypes.byref(e)))
-        return e.value
+                                       ctypes.byref(e) if bound else None))
+        return e.value if bound else None
 
-    def dsyrk_ozaki(self, la, N, K, alpha, A, lda, beta, C, ldc, slices=0):
+    def dsyrk_ozaki(self, la, N, K, alpha, A, lda, beta, C, ldc, slices=0,
+                    bound=True):
+        """Returns the error bound (bound=True: synchronous), else None."""
         e = _out_double()
         check(lib().skgemm_dsyrk_ozaki(self.h, la, N, K, alpha, _ptr(A), lda,
                                        beta, _ptr(C), ldc, slices,
-                                       ctypes.byref(e)))
-        return e.value
+                                       ctypes.byref(e) if bound else None))
+        return e.value if bound else None
 
     def dgemm_ozaki_host(self, la, lb, ld_, M, N, K, alpha, A, lda, B, ldb,
                          beta, D, ldd, gamma=0.0, slices=0):

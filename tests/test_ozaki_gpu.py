@@ -121,14 +121,154 @@ def test_rotation_split_accuracy(skg, ctx, orc, rng):
     assert err < 1e-13 and err <= bound
 
 
-def test_k_limit_is_reported(skg, ctx):
-    lib = skg.lib()
-    A = np.zeros(140000 * 4)
-    C = np.zeros(16)
-    with pytest.raises(skg.SkgemmError) as e:
-        ctx.dsyrk_ozaki_host(0, 4, 140000, 1.0, A, 4, 0.0, C, 4)
-    assert e.value.code == skg.ERR_NOTSUP
-    del lib
+@pytest.mark.parametrize("K,N,layout", [(140000, 40, 0), (175616, 64, 1),
+                                         (262400, 24, 2)])
+def test_long_k_in_chunks_bit_exact(skg, ctx, rng, K, N, layout):
+    """K beyond the 131072 products one INT32 sum holds exactly: the plan
+    cuts K into chunks and the CRT pass adds their residues modulo each
+    modulus.  Bit for bit against the exact-integer restatement, for the
+    chooser's plan and for plans that ask for fewer chunks than needed (the
+    planner raises them); the bound as for short K."""
+    X = c3_like(rng, K, N)
+    Xb, ld = encode_b(X, layout)
+    ref, _, _ = ozaki.product_rows(X.T.copy(), X.T.copy(), 14)
+    il = np.tril_indices(N)
+    try:
+        for plan in (None, "6,1", "5,1,0,1", "1,0"):
+            if plan is None:
+                os.environ.pop("SKGEMM_OZ_PLAN", None)
+            else:
+                os.environ["SKGEMM_OZ_PLAN"] = plan
+            print("K", K, "N", N, "plan", plan, flush=True)
+            C = np.zeros(N * N)
+            bound = ctx.dsyrk_ozaki_host(layout, N, K, 1.0, Xb, ld, 0.0, C, N)
+            assert np.array_equal(C.reshape(N, N)[il], ref[il]), plan
+            assert 0 < bound < 1e-12
+    finally:
+        os.environ.pop("SKGEMM_OZ_PLAN", None)
+
+
+def test_reference_sweep_shape_split(skg, ctx, orc):
+    """The reference's SYRK sweep shape (sub_syrk.sh:26-29: M = 616,
+    K = Nd = 175616) in the split form on Psi normal(0, 1/sqrt(Nd)) with
+    10^U(0,6) column scales: sampled rows against the double-double oracle
+    and, bit for bit, against the exact-integer restatement; certifying
+    mode reaches the 1e-13 bound."""
+    Nd, Ns = 175616, 616
+    psi = skg.DeviceArray(ctx, Nd * Ns)
+    ctx.fill_normal(psi.ptr, Nd * Ns, 21, 1.0 / np.sqrt(Nd))
+    ctx.scale_columns_pow10(psi.ptr, Nd, Ns, Nd, 22, 6.0)
+    ms = skg.DeviceArray(ctx, Ns * Ns, zero=True)
+    bound = ctx.dsyrk_ozaki(skg.COL_MAJOR, Ns, Nd, 1.0, psi.ptr, Nd, 0.0,
+                            ms.ptr, Ns, slices=14)
+    C = ms.download().reshape(Ns, Ns)
+    bound_c = ctx.dsyrk_ozaki(skg.COL_MAJOR, Ns, Nd, 1.0, psi.ptr, Nd, 0.0,
+                              ms.ptr, Ns, slices=skg.SLICES_AUTO)
+    Lc = ctx.split_moduli()
+    Cc = ms.download().reshape(Ns, Ns)
+    P = psi.download().reshape(Ns, Nd)
+    psi.free()
+    ms.free()
+    rows = np.sort(np.random.default_rng(23).choice(Ns, 12, replace=False))
+    sub = np.ascontiguousarray(P[rows])
+    ref = np.zeros(len(rows) * Ns)
+    orc.dgemm(0, 1, 0, len(rows), Ns, Nd, 1.0, sub.ravel(), Nd, P.ravel(), Nd,
+              0.0, ref, Ns, 0.0, dd=True)
+    ref = ref.reshape(len(rows), Ns)
+    low = lambda X: np.concatenate([X[i, :r + 1] for i, r in enumerate(rows)])
+    err = rel_fro(low(C[rows]), low(ref))
+    err_c = rel_fro(low(Cc[rows]), low(ref))
+    print("175616 x 616: err %.3e bound %.3e; certifying %d moduli err %.3e "
+          "bound %.3e" % (err, bound, Lc, err_c, bound_c), flush=True)
+    assert err < 1e-13 and err <= bound
+    assert 8 <= Lc <= 16 and 0 < bound_c <= 1e-13 and err_c <= bound_c
+    norms = np.einsum("ij,ij->i", P, P)
+    for L, X in ((14, C), (Lc, Cc)):
+        exact, _, _ = ozaki.product_rows(sub, P, L, norm2_a=norms[rows],
+                                         norm2_b=norms)
+        assert np.array_equal(low(X[rows]), low(exact)), L
+
+
+def test_certifying_syrk(skg, ctx, orc, rng):
+    """SKGEMM_SLICES_AUTO on a SYRK: the fewest moduli whose bound is at most
+    1e-13, chosen from the column norms before the products; the result is
+    bit for bit the split form at that count, within the bound of the
+    double-double oracle; a count below it would not certify."""
+    K, N = 16384, 256
+    X = c3_like(rng, K, N)
+    Xc = np.asfortranarray(X).ravel(order="F").copy()
+    C = np.zeros(N * N)
+    bound = ctx.dsyrk_ozaki_host(skg.COL_MAJOR, N, K, 1.0, Xc, K, 0.0, C, N,
+                                 slices=skg.SLICES_AUTO)
+    L = ctx.split_moduli()
+    assert 8 <= L <= 16 and 0 < bound <= 1e-13
+    ref, _, _ = ozaki.product_rows(X.T.copy(), X.T.copy(), L)
+    il = np.tril_indices(N)
+    assert np.array_equal(C.reshape(N, N)[il], ref[il])
+    dd = np.zeros(N * N)
+    orc.dsyrk(0, N, K, 1.0, np.ascontiguousarray(X.ravel()), N, 0.0, dd, N,
+              dd=True)
+    assert rel_fro(C.reshape(N, N)[il], dd.reshape(N, N)[il]) <= bound
+    # one modulus fewer: its a-posteriori bound misses 1e-13
+    C2 = np.zeros(N * N)
+    b2 = ctx.dsyrk_ozaki_host(skg.COL_MAJOR, N, K, 1.0, Xc, K, 0.0, C2, N,
+                              slices=L - 1)
+    assert b2 > 1e-13
+
+
+def test_certifying_gemm(skg, ctx, rng):
+    """SKGEMM_SLICES_AUTO on a GEMM: an ordinary product certifies at 14
+    moduli; a product that cancels to rounding level (B in the null space
+    of A's rows) cannot be certified by 16 moduli and is computed by the
+    plain FP64 path (bound -1, modulus count 0), from the original D when
+    beta != 0 (D is restored before every further attempt)."""
+    M, N, K = 64, 48, 512
+    A = rng.uniform(-1, 1, (M, K))
+    B = rng.uniform(-1, 1, (K, N))
+    D = np.zeros(M * N)
+    b = ctx.dgemm_ozaki_host(0, 0, 0, M, N, K, 1.0, np.ascontiguousarray(A.ravel()),
+                             K, np.ascontiguousarray(B.ravel()), N, 0.0, D, N,
+                             slices=skg.SLICES_AUTO)
+    assert ctx.split_moduli() == 14 and 0 < b <= 1e-13
+    Q, _ = np.linalg.qr(A.T)                       # K x M, rows of A's span
+    Bn = B - Q @ (Q.T @ B)                         # A Bn ~ 1e-15
+    C0 = rng.uniform(-1, 1, (M, N))
+    D1 = C0.copy().ravel()
+    D2 = C0.copy().ravel()
+    Ab = np.ascontiguousarray(A.ravel())
+    Bb = np.ascontiguousarray(Bn.ravel())
+    b = ctx.dgemm_ozaki_host(0, 0, 0, M, N, K, 1.5, Ab, K, Bb, N, 0.5, D1, N,
+                             0.25, slices=skg.SLICES_AUTO)
+    assert b == -1.0 and ctx.split_moduli() == 0
+    ctx.dgemm_host(0, 0, 0, M, N, K, 1.5, Ab, K, Bb, N, 0.5, D2, N, 0.25)
+    assert np.array_equal(D1, D2)
+
+
+def test_stage_timing(skg, ctx, rng):
+    """skgemm_timing_stages: three event pairs per split-form call (residue
+    pass, INT8 products, CRT), one per call otherwise."""
+    K, N = 8192, 512
+    dA = skg.DeviceArray.from_host(ctx, c3_like(rng, K, N))
+    dC = skg.DeviceArray(ctx, N * N, zero=True)
+    ctx.timing_enable(32)
+    try:
+        ctx.timing_stages(True)
+        for _ in range(3):
+            ctx.dsyrk_ozaki(skg.ROW_MAJOR, N, K, 1.0, dA.ptr, N, 0.0, dC.ptr,
+                            N, bound=False)
+        t = ctx.timing_read()
+        assert len(t) == 9 and np.all(t > 0)
+        ctx.timing_stages(False)
+        for _ in range(2):
+            ctx.dsyrk_ozaki(skg.ROW_MAJOR, N, K, 1.0, dA.ptr, N, 0.0, dC.ptr,
+                            N, bound=False)
+        t2 = ctx.timing_read()
+        assert len(t2) == 2
+        assert abs(np.median(t2) - np.median(t.reshape(3, 3).sum(1))) < \
+            0.5 * np.median(t2)
+    finally:
+        ctx.timing_stages(False)
+        ctx.timing_enable(0)
 
 
 def test_device_entry_points_reject_host_pointers(skg, ctx):

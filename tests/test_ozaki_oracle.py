@@ -21,7 +21,13 @@ def test_moduli_pairwise_coprime_and_headroom():
         for j in range(i):
             assert math.gcd(m[i], m[j]) == 1
     _, h = ozaki.tables(14)
-    assert h == 54  # log2(P) = 110.16 for the first 14 moduli
+    assert h == 55  # log2(P) = 110.16 for the first 14 moduli
+    # every modulus count keeps 2^(2h-2) below P / 4 with room for the
+    # rounding of the scaled rows
+    for L in range(8, 17):
+        mL, hL = ozaki.tables(L)
+        bits = sum(math.log2(x) for x in mL)
+        assert 2 * hL - 2 <= bits - 2 - 0.1 and 2 * hL + 2 > bits - 2
 
 
 def test_crt_reconstructs_exact_product(rng):
