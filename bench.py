@@ -137,8 +137,9 @@ def cpu_baseline(psi_host, Nd, Ns, gpu_plain, gpu_split):
         step()
         times.append(time.perf_counter() - t0)
     mean_s = float(np.mean(times))
-    # C is column-major upper == row-major lower
-    cpu = C.reshape(Ns, Ns).T
+    # C is column-major upper: C[m1 + m2 Ns] with m1 <= m2, i.e. the
+    # row-major lower triangle of C.reshape(Ns, Ns), as on the GPU
+    cpu = C.reshape(Ns, Ns)
     il = np.tril_indices(Ns)
 
     def rel(x):

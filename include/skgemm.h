@@ -213,19 +213,21 @@ int skgemm_dsyrk_noalloc(skgemm_ctx* ctx, int layout_a, int N, int K,
  * rounded once to FP64.  The result does not depend on the schedule
  * (bit-identical for any plan).
  * slices: 8..16 moduli; 0 = 14 (each scaled row keeps about 54 bits of its
- * norm); SKGEMM_SLICES_AUTO = the fewest moduli whose bound (below) is at
- * most 1e-13, the contract tolerance, or plain FP64 when 16 moduli cannot
- * reach it.  SYRK chooses them before the products from the column norms
- * (||C||_F >= sqrt(sum_i C_ii^2)); GEMM, which has no such lower bound,
- * starts at 14 and takes one more while the a-posteriori bound misses it
- * (at most three passes).  The certifying mode reads norms back (it is
- * synchronous and cannot be captured into a graph); skgemm_split_info
- * gives the count it used.
+ * norm); SKGEMM_SLICES_AUTO = certifying: 14 moduli, then more while the
+ * bound (below) exceeds 1e-13, the contract tolerance (the count its
+ * scaling predicts, then one at a time; D is restored between attempts
+ * when beta != 0), or plain FP64 when 16 moduli cannot reach it.  The
+ * certifying mode reads its bound back (it is synchronous and cannot be
+ * captured into a graph); skgemm_split_info gives the count it used.
+ * Each row of A (column of B) is scaled by s = 2^e m / 8, m in 8..15, the
+ * largest such value that keeps its 2-norm below 2^(h-1) (h = 55 at 14
+ * moduli), and rounded to integers.
  * *err_bound (may be NULL; non-NULL makes the call synchronous) receives a
- * rigorous a-posteriori bound on ||C - AB||_F / ||AB||_F from the integer
- * rounding of the scaled inputs (worst case over the rounding errors, so
- * it is loose: 1.7e-13 on Psi 65536 x 1024 with 14 moduli where the
- * measured error is ~5e-16), or -1 when the call was computed by the plain
+ * rigorous a-posteriori bound on ||C - AB||_F / ||AB||_F from the norms of
+ * the actual rounding errors of the scaled inputs (7.4e-14 on Psi 65536 x
+ * 1024 with 14 moduli, where the measured error is ~1.4e-16; Cauchy-Schwarz
+ * on the error vectors makes it pessimistic by about sqrt(K)), or -1 when
+ * the call was computed by the plain
  * FP64 path instead: a row of A or column of B could not be scaled (squared
  * norm not finite or outside [2^-1000, 2^1000]), or, certifying, 16 moduli
  * could not reach 1e-13.  Device pointers.  No counterpart in the

@@ -696,9 +696,11 @@ void oracle_stencil_apply(int ni, int nj, int nk, int nb, int F,
 // per free index.  The sum of K products is exact in 128-bit arithmetic:
 // every partial sum is bounded by ||xa_i|| ||xb_j|| (Cauchy-Schwarz on the
 // prefix), which the scaling keeps below 2^110.  out[i N + j] =
-// ldexp(fl(fl(hi) 2^64 + fl(lo)), -(sa_i + sb_j)) with hi = C >> 64 (signed)
-// and lo = C mod 2^64, the conversion the library's CRT pass performs.
-// No counterpart in the reference (its path is FP64 only).
+// ldexp(fl(d / (m_a m_b)), -(e_a + e_b - 6)) with d = fl(fl(hi) 2^64 +
+// fl(lo)), hi = C >> 64 (signed), lo = C mod 2^64, and the scale codes
+// sa_i = 8 e_a + m_a - 8 (s = 2^(e - 3) m): the conversion and unscaling
+// the library's CRT pass performs.  No counterpart in the reference (its
+// path is FP64 only).
 // ---------------------------------------------------------------------------
 void oracle_ozaki_exact(int M, int N, int K, const int64_t* xa,
                         const int64_t* xb, const int* sa, const int* sb,
@@ -713,7 +715,9 @@ void oracle_ozaki_exact(int M, int N, int K, const int64_t* xa,
       const int64_t hi = (int64_t)(acc >> 64);
       const uint64_t lo = (uint64_t)acc;
       const double d = (double)hi * 18446744073709551616.0 + (double)lo;
-      out[(long)i * N + j] = std::ldexp(d, -(sa[i] + sb[j]));
+      const int ma = 8 + (sa[i] & 7), mb = 8 + (sb[j] & 7);
+      out[(long)i * N + j] =
+          std::ldexp(d / (double)(ma * mb), -((sa[i] >> 3) + (sb[j] >> 3) - 6));
     }
 }
 

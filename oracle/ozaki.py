@@ -3,10 +3,14 @@ the library's skgemm_dgemm_ozaki / skgemm_dsyrk_ozaki.  TEST INFRASTRUCTURE
 ONLY (see oracle.py); no counterpart in the reference.
 
 The restatement follows the same steps with numpy / Python integers:
-  sigma_o = h - 1 - ceil(e_o / 2),  ||x_o||^2 = f 2^e_o, h = floor(log2 P / 2 - 0.05)
-  x'_o = rint(x_o 2^sigma_o)          (exact FP64 integers)
+  h = floor(log2 P / 2 - 0.05); for ||x_o||^2 = g 2^e (g in [0.5, 1)) the
+  scale s_o = 2^(sigma - 3) m, m in 8..15 the largest with
+  fl(g m^2) <= 64 (1 - 2^-19) (e even, sigma = h - 1 - e/2) or
+  <= 128 (1 - 2^-19) (e odd, sigma = h - (e + 3)/2), coded 8 sigma + m - 8
+  x'_o = rint(fl(x_o s_o))            (exact FP64 integers)
   C'   = A' B'                        (exact, Python integers -- no CRT)
-  C    = (double(C' >> 64) * 2^64 + double(C' mod 2^64)) * 2^-(sa_i + sb_j)
+  C    = ldexp(fl(d / (m_a m_b)), -(sigma_a + sigma_b - 6)),
+         d = fl(fl(C' >> 64) * 2^64 + fl(C' mod 2^64))
 which is the value the GPU's residue / INT8-product / CRT pipeline must
 reproduce bit for bit whenever |C'| < P / 2 (guaranteed by the scaling).
 
@@ -30,19 +34,53 @@ def tables(L):
     return m, h
 
 
-def sigma(norm2, h):
-    """Scale exponents from squared norms (same formula as the GPU)."""
+def scale_code(norm2, h):
+    """Scale codes 8 sigma + (m - 8) from squared norms (the GPU's
+    scale_code: exact steps and one rounded product fl(g m^2))."""
     norm2 = np.asarray(norm2, np.float64)
-    e = np.frexp(norm2)[1].astype(np.int64)
-    s = h - 1 - ((e + 1) >> 1)
-    s[~(norm2 > 0)] = 0
-    return s
+    out = np.zeros(norm2.shape, np.int64)
+    lim_even = 64.0 * (1.0 - 2.0 ** -19)
+    lim_odd = 128.0 * (1.0 - 2.0 ** -19)
+    for i, v in np.ndenumerate(norm2):
+        if not (v > 0.0):
+            continue
+        g, e = math.frexp(float(v))
+        odd = (e & 1) != 0
+        sig = h - ((e + 3) >> 1) if odd else h - 1 - (e >> 1)
+        lim = lim_odd if odd else lim_even
+        if not odd and not (g * 64.0 <= lim):
+            sig -= 1
+            lim = 256.0 * (1.0 - 2.0 ** -19)
+        m = 15
+        while m > 8 and not (g * float(m * m) <= lim):
+            m -= 1
+        out[i] = 8 * sig + (m - 8)
+    return out
 
 
-def scaled_integers(X, sig):
-    """X: O x K float64 (free index first).  Returns exact integer-valued
-    float64 array rint(X * 2^sigma_o)."""
-    return np.rint(np.ldexp(X, np.asarray(sig)[:, None]))
+def sigma(norm2, h):
+    """Scale codes (kept under the old name for callers)."""
+    return scale_code(norm2, h)
+
+
+def code_exp(code):
+    return np.asarray(code, np.int64) >> 3
+
+
+def code_mant(code):
+    return 8 + (np.asarray(code, np.int64) & 7)
+
+
+def scale_of(code):
+    """s = 2^(sigma - 3) m, exact."""
+    return np.ldexp(code_mant(code).astype(np.float64), code_exp(code) - 3)
+
+
+def scaled_integers(X, code):
+    """X: O x K float64 (free index first).  Returns the exact
+    integer-valued float64 array rint(fl(X_o * s_o))."""
+    s = scale_of(code)
+    return np.rint(X * s[:, None])
 
 
 def residues(xint, m):
@@ -79,16 +117,19 @@ def crt(res_list, m):
 
 
 def to_double(cint, sa, sb):
-    """The GPU's conversion of the exact integer product to FP64."""
+    """The GPU's conversion of the exact integer product to FP64 and its
+    unscaling by s_a s_b = 2^(sigma_a + sigma_b - 6) m_a m_b."""
     O1, O2 = cint.shape
     out = np.empty((O1, O2), np.float64)
     for i in range(O1):
+        ea, ma = int(code_exp(sa[i])), int(code_mant(sa[i]))
         for j in range(O2):
+            eb, mb = int(code_exp(sb[j])), int(code_mant(sb[j]))
             v = int(cint[i, j])
             hi = v >> 64
             lo = v & ((1 << 64) - 1)
             d = float(hi) * 18446744073709551616.0 + float(lo)
-            out[i, j] = math.ldexp(d, -(int(sa[i]) + int(sb[j])))
+            out[i, j] = math.ldexp(d / float(ma * mb), -(ea + eb - 6))
     return out
 
 

@@ -91,11 +91,36 @@ __device__ __forceinline__ int64_t out_addr(const Output& d, int m, int n) {
          (int64_t)m * d.ms;
 }
 
-__device__ __forceinline__ int sigma_of(double norm2, int h) {
+// Scale of a row from its squared norm: s = 2^(sigma - 3) m, m in 8..15,
+// the largest with norm2 s^2 <= 2^(2h-2) (1 - 2^-19), i.e. ||x s|| stays
+// below 2^(h-1) with room for the rounding of the norm, stored as the code
+// 8 sigma + (m - 8).  With norm2 = g 2^e (g in [0.5, 1)): e even gives
+// sigma = h - 1 - e/2 and the test fl(g m^2) <= 64 (1 - 2^-19), e odd
+// sigma = h - (e + 3)/2 and fl(g m^2) <= 128 (1 - 2^-19); m = 8 is the
+// power-of-two scale 2^sigma, and the eight steps put every row within 1/8
+// of an octave of the bound.  Exact operations and one rounded product,
+// so host and device codes agree for the same norm2.  A zero row takes
+// code 0 (s = 1).
+__host__ __device__ inline int scale_code(double norm2, int h) {
   if (!(norm2 > 0.0)) return 0;
   int e;
-  frexp(norm2, &e);
-  return h - 1 - ((e + 1) >> 1);  // ceil(e / 2) for any sign of e
+  const double g = frexp(norm2, &e);
+  const bool odd = (e & 1) != 0;
+  int sig = odd ? h - ((e + 3) >> 1) : h - 1 - (e >> 1);
+  double lim = odd ? 128.0 * (1.0 - 0x1p-19) : 64.0 * (1.0 - 0x1p-19);
+  if (!odd && !(g * 64.0 <= lim)) {  // g within 2^-19 of 1: one step down
+    --sig;
+    lim = 256.0 * (1.0 - 0x1p-19);
+  }
+  int m = 15;
+  while (m > 8 && !(g * (double)(m * m) <= lim)) --m;
+  return 8 * sig + (m - 8);
+}
+__host__ __device__ inline int code_exp(int code) { return code >> 3; }
+__host__ __device__ inline int code_mant(int code) { return 8 + (code & 7); }
+// s = 2^(sigma - 3) m, exact.
+__host__ __device__ inline double scale_of(int code) {
+  return ldexp((double)code_mant(code), code_exp(code) - 3);
 }
 
 // Rows whose squared norm leaves [2^-1000, 2^1000] (or is not finite, or
@@ -113,7 +138,7 @@ __device__ __forceinline__ void finish_norm(double s, double mx, int o, int h,
                                             double* norm2, int* sigma,
                                             int* fallback) {
   norm2[o] = s;
-  sigma[o] = sigma_of(s, h);
+  sigma[o] = scale_code(s, h);
   if (needs_fallback(s, mx)) atomicOr(fallback, 1);
 }
 
@@ -242,16 +267,25 @@ constexpr int kSplitSeg = 4096;
 
 // Residues of row elements k0 .. k0 + 3 (zero past K or outside the
 // operand), one dword per plane.
+// x' = rint(fl(a s)) and the rounding error a s - x', formed by one FMA
+// (the exact error, rounded once); *d2 accumulates its square.
+__device__ __forceinline__ double scaled_int(double a, double sc, double* d2) {
+  const double xr = rint(a * sc);
+  const double d = fma(a, sc, -xr);
+  *d2 = fma(d, d, *d2);
+  return xr;
+}
+
 template <bool WIDE>
 __device__ __forceinline__ void split4(const double* row, bool in_o, int K,
-                                       int s, const Tables& T, int8_t* dst,
-                                       int64_t plane, int k0) {
+                                       double sc, const Tables& T, int8_t* dst,
+                                       int64_t plane, int k0, double* d2) {
   Split p[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int k = k0 + j;
     const double a = (in_o && k < K) ? row[k] : 0.0;
-    p[j] = split26<WIDE>(rint(ldexp(a, s)));
+    p[j] = split26<WIDE>(scaled_int(a, sc, d2));
   }
   for (int l = 0; l < T.L; ++l) {
     const double m = T.md[l], inv = T.inv_m[l], c = T.c26[l], c2 = T.c52[l];
@@ -263,21 +297,50 @@ __device__ __forceinline__ void split4(const double* row, bool in_o, int K,
   }
 }
 
+// Sum over the workgroup of one value per thread (fixed order), for
+// thread 0.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+#pragma unroll
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  return red[0];
+}
+
+// dpart[seg][o]: the squared rounding error of row o over segment seg.
 template <bool WIDE>
 __global__ __launch_bounds__(256) void split_k_kernel(
     Operand op, int K, const int* sigma, Tables T, int8_t* planes,
-    int64_t plane, int kpad, int seg0, int nsegc) {
+    int64_t plane, int kpad, int seg0, int nsegc, double* dpart, int opad) {
+  __shared__ double red[256];
   const int o = blockIdx.x / nsegc;
   const int seg = seg0 + nsegc - 1 - (blockIdx.x - o * nsegc);
   const bool in_o = o < op.extent;
-  const int s = in_o ? sigma[o] : 0;
+  const double sc = scale_of(in_o ? sigma[o] : 0);
   const double* row = in_o ? op.ptr + op_addr(op, o, 0) : op.ptr;
   int8_t* dst = planes + (int64_t)o * kpad;
+  double d2 = 0.0;
   for (int step = 3; step >= 0; --step) {
     const int k0 = seg * kSplitSeg + step * 1024 + threadIdx.x * 4;
     if (k0 >= kpad) continue;
-    split4<WIDE>(row, in_o, K, s, T, dst, plane, k0);
+    split4<WIDE>(row, in_o, K, sc, T, dst, plane, k0, &d2);
   }
+  const double tot = block_sum<256>(d2, red);
+  if (threadIdx.x == 0) dpart[(int64_t)seg * opad + o] = tot;
+}
+
+// dd2[o] = sum of the parts of row o in part order.
+__global__ __launch_bounds__(256) void delta_finish_kernel(
+    int O, int nparts, const double* dpart, int opad, double* dd2) {
+  const int o = blockIdx.x * 256 + threadIdx.x;
+  if (o >= O) return;
+  double t = 0.0;
+  for (int q = 0; q < nparts; ++q) t += dpart[(int64_t)q * opad + o];
+  dd2[o] = t;
 }
 
 // Operand contiguous along k, norm and residues in one kernel: one
@@ -291,7 +354,7 @@ __global__ __launch_bounds__(256) void split_k_kernel(
 template <bool WIDE>
 __global__ __launch_bounds__(1024) void norm_split_k_kernel(
     Operand op, int K, int h, Tables T, int8_t* planes, int64_t plane,
-    int kpad, double* norm2, int* sigma, int* fallback) {
+    int kpad, double* norm2, int* sigma, int* fallback, double* dd2) {
   __shared__ double red[2][1024];
   __shared__ int s_sig;
   const int o = blockIdx.x;
@@ -343,7 +406,7 @@ __global__ __launch_bounds__(1024) void norm_split_k_kernel(
     }
     if (tid == 0) {
       finish_norm(red[0][0], red[1][0], o, h, norm2, sigma, fallback);
-      s_sig = sigma_of(red[0][0], h);
+      s_sig = scale_code(red[0][0], h);
     }
     __syncthreads();
     s = s_sig;
@@ -358,13 +421,16 @@ __global__ __launch_bounds__(1024) void norm_split_k_kernel(
   };
   double cur[4], nxt[4] = {0.0, 0.0, 0.0, 0.0};
   int k0 = (nseg - 1) * kSplitSeg + tid * 4;
+  const double sc = scale_of(s);
+  double d2 = 0.0;
   load4(k0, cur);
   for (int seg = nseg - 1; seg >= 0; --seg) {
     if (seg > 0) load4(k0 - kSplitSeg, nxt);
     if (k0 < kpad) {
       Split p[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) p[j] = split26<WIDE>(rint(ldexp(cur[j], s)));
+      for (int j = 0; j < 4; ++j)
+        p[j] = split26<WIDE>(scaled_int(cur[j], sc, &d2));
       for (int l = 0; l < T.L; ++l) {
         const double m = T.md[l], inv = T.inv_m[l], c = T.c26[l],
                      c2 = T.c52[l];
@@ -379,6 +445,8 @@ __global__ __launch_bounds__(1024) void norm_split_k_kernel(
     for (int j = 0; j < 4; ++j) cur[j] = nxt[j];
     k0 -= kSplitSeg;
   }
+  const double tot = block_sum<1024>(d2, red[0]);
+  if (tid == 0) dd2[o] = tot;
 }
 
 // Operand contiguous along o: workgroup of 64 o x 128 k, 512 threads.
@@ -389,11 +457,13 @@ __global__ __launch_bounds__(1024) void norm_split_k_kernel(
 // conflict-free.
 constexpr int kTPitch = 144;
 
+// dpart[kb][o]: the squared rounding error of row o over k block kb.
 template <bool WIDE>
 __global__ __launch_bounds__(512) void split_o_kernel(
     Operand op, int K, const int* sigma, Tables T, int8_t* planes,
-    int64_t plane, int kpad, int opad) {
+    int64_t plane, int kpad, int opad, double* dpart) {
   __shared__ __align__(16) uint8_t tile[2][64 * kTPitch];
+  __shared__ double red[8][64];
   const int kblocks = kpad / 128;
   const int ob = blockIdx.x / kblocks;
   const int kb = kblocks - 1 - (blockIdx.x - ob * kblocks);
@@ -401,13 +471,22 @@ __global__ __launch_bounds__(512) void split_o_kernel(
   const int o = ob * 64 + ol;
   const int k0 = kb * 128 + kq * 16;
   const bool in_o = o < op.extent;
-  const int s = in_o ? sigma[o] : 0;
+  const double sc = scale_of(in_o ? sigma[o] : 0);
   Split p[16];
+  double d2 = 0.0;
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     const int k = k0 + j;
     const double a = (in_o && k < K) ? op.ptr[op_addr(op, o, k)] : 0.0;
-    p[j] = split26<WIDE>(rint(ldexp(a, s)));
+    p[j] = split26<WIDE>(scaled_int(a, sc, &d2));
+  }
+  red[kq][ol] = d2;
+  __syncthreads();
+  if (kq == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) t += red[q][ol];
+    dpart[(int64_t)kb * opad + o] = t;
   }
   // Store side: thread q is row q / 8, 16-byte chunk q % 8 of the
   // 64 x 128 block.
@@ -978,8 +1057,8 @@ __global__ __launch_bounds__(256) void crt_kernel(CrtLaunch p) {
     // First column whose entries this SYRK keeps at or below a row: the
     // column itself, or the start of its 16-block when whole diagonal
     // blocks are stored (syrk == 2).
-    const int cb = p.syrk == 2 ? ((bc * 4) & ~15) : bc * 4;
-    if (p.syrk && br * 256 + 255 < cb) continue;
+    const int cfirst = p.syrk == 2 ? ((bc * 4) & ~15) : bc * 4;
+    if (p.syrk && br * 256 + 255 < cfirst) continue;
     const int row0 = br * 256 + (threadIdx.x & 63) * 4;
     const int col = bc * 4 + (threadIdx.x >> 6);
     const int cmin = p.syrk == 2 ? (col & ~15) : col;
@@ -1008,7 +1087,7 @@ __global__ __launch_bounds__(256) void crt_kernel(CrtLaunch p) {
         rw[l] = (s0 % m) | ((s1 % m) << 8) | ((s2 % m) << 16) | ((s3 % m) << 24);
       }
     }
-    const int sb = p.sig_b[col];
+    const int cb = p.sig_b[col];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int row = row0 + e;
@@ -1020,7 +1099,12 @@ __global__ __launch_bounds__(256) void crt_kernel(CrtLaunch p) {
         else
           crt<4, 32>(p.T, rw, e, xh, xl);
         const double d = (double)xh * 18446744073709551616.0 + (double)xl;
-        const double c = ldexp(d, -(p.sig_a[row] + sb));
+        // Unscale by s_a s_b = 2^(ea + eb - 6) ma mb: one rounded division
+        // by the integer ma mb (<= 225), then an exact power of two.
+        const int ca = p.sig_a[row];
+        const double c =
+            ldexp(d / (double)(code_mant(ca) * code_mant(cb)),
+                  -(code_exp(ca) + code_exp(cb) - 6));
         // ||C||_F^2 from the lower triangle (off-diagonal entries twice).
         if (!p.syrk || row >= col)
           contrib += (p.syrk && row != col) ? 2.0 * c * c : c * c;
@@ -1041,6 +1125,51 @@ __global__ __launch_bounds__(256) void crt_kernel(CrtLaunch p) {
 }
 
 constexpr int kCrtGrid = 4096;
+
+// Inputs of the error bound reduced on the device, so that a call reads
+// back eight doubles instead of every row's norms: out = {sum n^2, sum e^2,
+// sum n e} of A's rows, the same of B's columns, ||C||_F^2 (the CRT
+// workgroups' parts in order), and the fallback flag.  n = sqrt(norm2 g),
+// e = sqrt(delta2 g) / s with g = 1 + (K + 2) 2^-52 and a factor 1 + 2^-51
+// per rounded operation (the bound's rigour does not rest on sqrt or the
+// division being correctly rounded); the host applies the summation factor.
+// One workgroup, fixed order.
+__global__ __launch_bounds__(1024) void bound_sums_kernel(
+    const double* n2a, const int* ca, const double* dda, int M,
+    const double* n2b, const int* cb, const double* ddb, int N, int K,
+    const double* fro2, int nparts, const int* fallback, double* out) {
+  __shared__ double red[1024];
+  const double u2 = 0x1p-51, g = 1.0 + (K + 2.0) * 0x1p-52;
+  double v[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < M; i += 1024) {
+    const double n = sqrt(n2a[i] * g) * (1.0 + u2);
+    const double e = sqrt(dda[i] * g) * (1.0 + u2) / scale_of(ca[i]) *
+                     (1.0 + u2);
+    v[0] += n * n;
+    v[1] += e * e;
+    v[2] += n * e;
+  }
+  for (int j = threadIdx.x; j < N; j += 1024) {
+    const double n = sqrt(n2b[j] * g) * (1.0 + u2);
+    const double e = sqrt(ddb[j] * g) * (1.0 + u2) / scale_of(cb[j]) *
+                     (1.0 + u2);
+    v[3] += n * n;
+    v[4] += e * e;
+    v[5] += n * e;
+  }
+  for (int q = threadIdx.x; q < nparts; q += 1024) v[6] += fro2[q];
+  for (int t = 0; t < 7; ++t) {
+    red[threadIdx.x] = v[t];
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[t] = red[0];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[7] = (double)*fallback;
+}
 
 // ---- host ----------------------------------------------------------------------
 
@@ -1194,7 +1323,7 @@ Tables make_tables(int L) {
 // ---- context extras ------------------------------------------------------------
 
 struct skgemm_ctx::Extra {
-  static constexpr int kBufs = 12;
+  static constexpr int kBufs = 17;
   void* buf[kBufs] = {nullptr};
   size_t n[kBufs] = {0};
   uint64_t* gen = nullptr;  // the context's mem_gen
@@ -1215,6 +1344,14 @@ struct skgemm_ctx::Extra {
         return false;
       }
     return true;
+  }
+  // Pinned host buffer for the error bound's eight sums.
+  double* pinned = nullptr;
+  double* pinned_sums() {
+    if (pinned == nullptr &&
+        hipHostMalloc((void**)&pinned, 8 * sizeof(double), 0) != hipSuccess)
+      pinned = nullptr;
+    return pinned;
   }
   void* get(int i, size_t bytes) {
     if (bytes > n[i]) {
@@ -1240,6 +1377,7 @@ void skgemm_ctx::free_extra() {
   if (extra->aux) hipStreamDestroy(extra->aux);
   for (int i = 0; i < Extra::kBufs; ++i)
     if (extra->buf[i]) hipFree(extra->buf[i]);
+  if (extra->pinned) hipHostFree(extra->pinned);
   delete extra;
   extra = nullptr;
 }
@@ -1251,7 +1389,7 @@ namespace {
 
 enum {
   kPlanesA = 0, kPlanesB, kRes, kNormA, kNormB, kSigA, kSigB, kScalars,
-  kSavedD,
+  kSavedD, kDeltaA, kDeltaB, kDpartA, kDpartB, kBoundSums,
   kPartA, kPartB
 };
 
@@ -1271,32 +1409,39 @@ int norm_segments(int O, int K) {
 int prepare_operand(skgemm_ctx* c, const Operand& op, int K, const Tables& T,
                     int8_t* planes, int64_t plane, int kpad, int opad,
                     double* norm2, int* sigma, int* fallback, int part_buf,
-                    bool norms_ready) {
+                    bool norms_ready, double* dd2, int dpart_buf) {
   const int O = op.extent;
   const bool wide = T.h > 55;  // |x'| < 2^54: two 26-bit limbs suffice
   const char* fuse = std::getenv("SKGEMM_OZ_FUSE");
   if (op.kcontig && !norms_ready && !(fuse != nullptr && fuse[0] == '0')) {
-    // One pass: norm, scale and residues per row.
+    // One pass: norm, scale, residues and rounding-error norm per row.
     if (wide)
       hipLaunchKernelGGL(norm_split_k_kernel<true>, dim3(opad), dim3(1024), 0,
                          c->stream, op, K, T.h, T, planes, plane, kpad, norm2,
-                         sigma, fallback);
+                         sigma, fallback, dd2);
     else
       hipLaunchKernelGGL(norm_split_k_kernel<false>, dim3(opad), dim3(1024),
                          0, c->stream, op, K, T.h, T, planes, plane, kpad,
-                         norm2, sigma, fallback);
+                         norm2, sigma, fallback, dd2);
   } else if (op.kcontig) {
     if (O > 0 && !norms_ready)
       hipLaunchKernelGGL(norm_k_kernel, dim3(O), dim3(256), 0, c->stream, op,
                          K, T.h, norm2, sigma, fallback);
     const int nseg = (kpad + kSplitSeg - 1) / kSplitSeg;
+    double* dpart =
+        (double*)c->extra->get(dpart_buf, sizeof(double) * nseg * opad);
+    if (dpart == nullptr) return set_error(SKGEMM_ERR_NOMEM, "error norms");
     const dim3 grid((unsigned)((int64_t)opad * nseg));
     if (wide)
       hipLaunchKernelGGL(split_k_kernel<true>, grid, dim3(256), 0, c->stream,
-                         op, K, sigma, T, planes, plane, kpad, 0, nseg);
+                         op, K, sigma, T, planes, plane, kpad, 0, nseg, dpart,
+                         opad);
     else
       hipLaunchKernelGGL(split_k_kernel<false>, grid, dim3(256), 0, c->stream,
-                         op, K, sigma, T, planes, plane, kpad, 0, nseg);
+                         op, K, sigma, T, planes, plane, kpad, 0, nseg, dpart,
+                         opad);
+    hipLaunchKernelGGL(delta_finish_kernel, dim3((opad + 255) / 256),
+                       dim3(256), 0, c->stream, opad, nseg, dpart, opad, dd2);
   } else {
     if (O > 0 && !norms_ready) {
       const int nseg = norm_segments(O, K);
@@ -1311,13 +1456,20 @@ int prepare_operand(skgemm_ctx* c, const Operand& op, int K, const Tables& T,
                          0, c->stream, O, nseg, T.h, part,
                          part + (int64_t)nseg * O, norm2, sigma, fallback);
     }
-    const dim3 grid((unsigned)((int64_t)(opad / 64) * (kpad / 128)));
+    const int kblocks = kpad / 128;
+    double* dpart =
+        (double*)c->extra->get(dpart_buf, sizeof(double) * kblocks * opad);
+    if (dpart == nullptr) return set_error(SKGEMM_ERR_NOMEM, "error norms");
+    const dim3 grid((unsigned)((int64_t)(opad / 64) * kblocks));
     if (wide)
       hipLaunchKernelGGL(split_o_kernel<true>, grid, dim3(512), 0, c->stream,
-                         op, K, sigma, T, planes, plane, kpad, opad);
+                         op, K, sigma, T, planes, plane, kpad, opad, dpart);
     else
       hipLaunchKernelGGL(split_o_kernel<false>, grid, dim3(512), 0, c->stream,
-                         op, K, sigma, T, planes, plane, kpad, opad);
+                         op, K, sigma, T, planes, plane, kpad, opad, dpart);
+    hipLaunchKernelGGL(delta_finish_kernel, dim3((opad + 255) / 256),
+                       dim3(256), 0, c->stream, opad, kblocks, dpart, opad,
+                       dd2);
   }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_error(e, "ozaki split");
@@ -1414,65 +1566,45 @@ OzPlan choose_plan(skgemm_ctx* c, int kind, int M, int N, int L, int kpad) {
   return op;
 }
 
-// A-posteriori bound on ||C - AB||_F / ||AB||_F from the integer rounding
-// of step 1: with |A 2^sigma - A'| <= 1/2 elementwise,
-// |C_ij - (AB)_ij| <= 2^-(sa_i+sb_j) (sqrt(K)/2 (|A'_i| + |B'_j|) + K/4),
-// |A'_i| <= 2^sa_i ||A_i|| + sqrt(K)/2; the Frobenius norm of that bound
-// matrix is a sum of separable terms.
-long double bound_numerator(const std::vector<double>& na,
-                            const std::vector<int>& sa,
-                            const std::vector<double>& nb,
-                            const std::vector<int>& sb, int K);
+// A-posteriori bound on ||C - AB||_F / ||AB||_F.  Row i of A is scaled by
+// s_i and rounded, x'_i = s_i a_i - d_i, column j of B by t_j, y'_j =
+// t_j b_j - g_j.  Then x'_i . y'_j / (s_i t_j) - a_i . b_j =
+// -(a_i . g_j) / t_j - (d_i . b_j) / s_i + (d_i . g_j) / (s_i t_j), so with
+// n_i >= ||a_i||, e_i >= ||d_i|| / s_i (and m_j, f_j for B)
+//   |C_ij - (AB)_ij| <= n_i f_j + e_i m_j + e_i f_j,
+// whose squared Frobenius norm expands into products of row sums and
+// column sums.  ||d_i||^2 is the residue pass's sum of the rounding errors
+// (each the exact error rounded once by an FMA), the norms the norm pass's
+// sums of squares; both lie within (K + 2) u of their values, which the
+// factor g covers.  Converting C' to FP64 and unscaling add two roundings.
+struct RowSums {
+  double nn = 0, ee = 0, ne = 0;  // sum n^2, sum e^2, sum n e
+};
 
-double error_bound(const std::vector<double>& na, const std::vector<int>& sa,
-                   const std::vector<double>& nb, const std::vector<int>& sb,
-                   int K, double fro2) {
-  const long double num = bound_numerator(na, sa, nb, sb, K);
-  const long double den = std::sqrt((long double)fro2) - num;
+double bound_numerator(const RowSums& A, const RowSums& B) {
+  const double s = A.nn * B.ee + A.ee * B.nn + A.ee * B.ee +
+                   2.0 * (A.ne * B.ne + A.ne * B.ee + A.ee * B.ne);
+  return std::sqrt(std::max(s, 0.0)) * (1.0 + 0x1p-40);
+}
+
+// From the device's sums (bound_sums_kernel): the summation of up to R
+// positive terms in FP64 is covered by the factor 1 + (R + 4) 2^-52.
+double error_bound(const double* sums, int M, int N, int nparts) {
+  const double u = 0x1p-52;
+  const double fa = 1.0 + (M + 4.0) * u, fb = 1.0 + (N + 4.0) * u;
+  RowSums A, B;
+  A.nn = sums[0] * fa;
+  A.ee = sums[1] * fa;
+  A.ne = sums[2] * fa;
+  B.nn = sums[3] * fb;
+  B.ee = sums[4] * fb;
+  B.ne = sums[5] * fb;
+  const double num = bound_numerator(A, B);
+  const double fro = std::sqrt(sums[6]) * (1.0 - (nparts + 4.0) * u) *
+                     (1.0 - 0x1p-30);
+  const double den = fro - num;
   if (!(den > 0)) return INFINITY;
-  return (double)(num / den) + 4.5e-16;  // + final roundings
-}
-
-// sqrt(sum_ij (e_i n_j + n_i e_j + e_i e_j)^2) in the notation above.
-long double bound_numerator(const std::vector<double>& na,
-                            const std::vector<int>& sa,
-                            const std::vector<double>& nb,
-                            const std::vector<int>& sb, int K) {
-  const long double c = 0.5L * std::sqrt((long double)K), d = 0.25L * K;
-  long double A0 = 0, A1 = 0, A2 = 0, B0 = 0, B1 = 0, B2 = 0;
-  for (size_t i = 0; i < na.size(); ++i) {
-    const long double al = std::ldexp(1.0L, -sa[i]);
-    const long double a = std::ldexp(std::sqrt((long double)na[i]), sa[i]) + c;
-    A0 += al * al;
-    A1 += al * al * a;
-    A2 += al * al * a * a;
-  }
-  for (size_t j = 0; j < nb.size(); ++j) {
-    const long double be = std::ldexp(1.0L, -sb[j]);
-    const long double b = std::ldexp(std::sqrt((long double)nb[j]), sb[j]) + c;
-    B0 += be * be;
-    B1 += be * be * b;
-    B2 += be * be * b * b;
-  }
-  const long double s = c * c * (A2 * B0 + A0 * B2 + 2 * A1 * B1) +
-                        2 * c * d * (A1 * B0 + A0 * B1) + d * d * A0 * B0;
-  return std::sqrt(std::max(s, 0.0L));
-}
-
-// Scale exponents from squared norms for headroom h (sigma_of on the host;
-// the same frexp exponent, so the same integers as the device's).
-std::vector<int> host_sigma(const std::vector<double>& n2, int h) {
-  std::vector<int> s(n2.size());
-  for (size_t i = 0; i < n2.size(); ++i) {
-    if (!(n2[i] > 0.0)) {
-      s[i] = 0;
-      continue;
-    }
-    int e;
-    std::frexp(n2[i], &e);
-    s[i] = h - 1 - ((e + 1) >> 1);
-  }
-  return s;
+  return num / den * (1.0 + 0x1p-40) + 9e-16;  // + the final roundings
 }
 
 }  // namespace
@@ -1509,6 +1641,14 @@ int ozaki_reserve(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
     ok = ok && X->get(kPartA, sizeof(double) * 2 * norm_segments(M, K) * M);
   if (kind == 0 && !pr.b.kcontig && N > 0)
     ok = ok && X->get(kPartB, sizeof(double) * 2 * norm_segments(N, K) * N);
+  // Rounding-error norms and their per-block parts (kpad / 128 blocks, at
+  // least the 4096-wide segments of the k-contiguous pass).
+  const size_t kbl = (size_t)kpad / 128;
+  ok = ok && X->get(kDeltaA, sizeof(double) * opa) &&
+       X->get(kDpartA, sizeof(double) * kbl * opa);
+  if (kind == 0)
+    ok = ok && X->get(kDeltaB, sizeof(double) * opb) &&
+         X->get(kDpartB, sizeof(double) * kbl * opb);
   if (!ok) return set_error(SKGEMM_ERR_NOMEM, "split-form workspace");
   if (!op.dp) {
     const int rc = c->ensure_ws((size_t)op.grid * gc.bm * gc.bn / 2 + 1, op.grid);
@@ -1576,10 +1716,14 @@ int run_moduli(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
   double* n2b = kind == 1 ? n2a : (double*)X->get(kNormB, sizeof(double) * opb);
   int* sga = (int*)X->get(kSigA, sizeof(int) * opa);
   int* sgb = kind == 1 ? sga : (int*)X->get(kSigB, sizeof(int) * opb);
+  // Squared rounding errors of the scaled rows (for the error bound).
+  double* dda = (double*)X->get(kDeltaA, sizeof(double) * opa);
+  double* ddb = kind == 1 ? dda : (double*)X->get(kDeltaB, sizeof(double) * opb);
   // [0, 4): fallback flag; [8, 8 + 8 kCrtGrid): per-workgroup parts of
   // ||C||_F^2 written by the CRT kernel.
   char* scal = (char*)X->get(kScalars, 8 + sizeof(double) * kCrtGrid);
-  if (!planesA || !planesB || !R || !n2a || !n2b || !sga || !sgb || !scal)
+  if (!planesA || !planesB || !R || !n2a || !n2b || !sga || !sgb || !scal ||
+      !dda || !ddb)
     return set_error(SKGEMM_ERR_NOMEM, "split-form workspace");
   int* fallback = (int*)scal;
   double* fro2 = (double*)(scal + 8);
@@ -1606,12 +1750,12 @@ int run_moduli(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
                        A, K, T.h, n2a, sga, fallback);
   } else {
     rc = prepare_operand(c, A, K, T, planesA, pa, kpad, opa, n2a, sga,
-                         fallback, kPartA, norms_ready);
+                         fallback, kPartA, norms_ready, dda, kDpartA);
   }
   if (rc != SKGEMM_OK) return rc;
   if (kind == 0) {
     rc = prepare_operand(c, B, K, T, planesB, pb, kpad, opb, n2b, sgb,
-                         fallback, kPartB, norms_ready);
+                         fallback, kPartB, norms_ready, ddb, kDpartB);
     if (rc != SKGEMM_OK) return rc;
   }
   if (stages) {
@@ -1668,17 +1812,20 @@ int run_moduli(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
     const int nseg = (kpad + kSplitSeg - 1) / kSplitSeg;
     const int spc = chunk_k / kSplitSeg;  // segments per chunk
     const bool wide = T.h > 55;  // |x'| < 2^54: two 26-bit limbs suffice
+    double* dpart =
+        (double*)X->get(kDpartA, sizeof(double) * nseg * opa);
+    if (dpart == nullptr) return set_error(SKGEMM_ERR_NOMEM, "error norms");
     for (int ch = 0; ch < pipe; ++ch) {
       const int seg0 = ch * spc, nsegc = std::min(spc, nseg - seg0);
       const dim3 grid((unsigned)((int64_t)opa * nsegc));
       if (wide)
         hipLaunchKernelGGL(split_k_kernel<true>, grid, dim3(256), 0,
                            c->stream, A, K, sga, T, planesA, pa, kpad, seg0,
-                           nsegc);
+                           nsegc, dpart, opa);
       else
         hipLaunchKernelGGL(split_k_kernel<false>, grid, dim3(256), 0,
                            c->stream, A, K, sga, T, planesA, pa, kpad, seg0,
-                           nsegc);
+                           nsegc, dpart, opa);
       if ((e = hipGetLastError()) != hipSuccess)
         return hip_error(e, "ozaki split");
       if ((e = hipEventRecord(X->ev[ch], c->stream)) != hipSuccess ||
@@ -1688,6 +1835,8 @@ int run_moduli(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
       e = launch_gemm(op.cfg, G, X->aux);
       if (e != hipSuccess) return hip_error(e, "ozaki gemm launch");
     }
+    hipLaunchKernelGGL(delta_finish_kernel, dim3((opa + 255) / 256),
+                       dim3(256), 0, c->stream, opa, nseg, dpart, opa, dda);
     if ((e = hipEventRecord(X->ev[pipe], X->aux)) != hipSuccess ||
         (e = hipStreamWaitEvent(c->stream, X->ev[pipe], 0)) != hipSuccess)
       return hip_error(e, "ozaki pipeline");
@@ -1734,28 +1883,21 @@ int run_moduli(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
   c->record_end();
 
   if (err_bound != nullptr) {
-    std::vector<double> na(M), nb(N), parts(crt_grid);
-    std::vector<int> sa(M), sb(N);
-    int fb = 0;
-    if ((e = hipMemcpyAsync(na.data(), n2a, sizeof(double) * M,
-                            hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(sa.data(), sga, sizeof(int) * M,
-                            hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(nb.data(), n2b, sizeof(double) * N,
-                            hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(sb.data(), sgb, sizeof(int) * N,
-                            hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(&fb, fallback, sizeof(int), hipMemcpyDeviceToHost,
-                            c->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(parts.data(), fro2, sizeof(double) * crt_grid,
+    double* sums = (double*)X->get(kBoundSums, sizeof(double) * 8);
+    double* host = X->pinned_sums();
+    if (sums == nullptr || host == nullptr)
+      return set_error(SKGEMM_ERR_NOMEM, "error-bound sums");
+    hipLaunchKernelGGL(bound_sums_kernel, dim3(1), dim3(1024), 0, c->stream,
+                       n2a, sga, dda, M, n2b, sgb, ddb, N, K, fro2, crt_grid,
+                       fallback, sums);
+    if ((e = hipGetLastError()) != hipSuccess ||
+        (e = hipMemcpyAsync(host, sums, sizeof(double) * 8,
                             hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(c->stream)) != hipSuccess)
       return hip_error(e, "error-bound readback");
     rc = c->check_status();
     if (rc != SKGEMM_OK) return rc;
-    double f2 = 0.0;
-    for (double v : parts) f2 += v;
-    *err_bound = fb ? -1.0 : error_bound(na, sa, nb, sb, K, f2);
+    *err_bound = host[7] != 0.0 ? -1.0 : error_bound(host, M, N, crt_grid);
   }
   c->oz_last_moduli = L;
   return SKGEMM_OK;
@@ -1779,82 +1921,28 @@ int run_plain(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
 // The tolerance the certifying mode (SKGEMM_SLICES_AUTO) guarantees.
 constexpr double kCertifyTarget = 1e-13;
 
-// Certifying mode.  SYRK: the norms first (read back), then the fewest
-// moduli whose a-priori bound, with ||C||_F >= sqrt(sum_i C_ii^2) =
-// sqrt(sum_i ||a_i||^4), is at most kCertifyTarget, and the sequence on the
-// norms already computed.  GEMM (no such lower bound): 14 moduli, then one
-// more while the a-posteriori bound misses the target (D restored from a
-// copy first when beta != 0).  Plain FP64 when 16 moduli cannot certify.
+// Certifying mode: 14 moduli first (the default count, whose fused norm and
+// residue pass reads the operand once more than the residues need); while
+// the rigorous a-posteriori bound misses kCertifyTarget, the count the
+// bound's scaling predicts (the bound falls by 2^(h(L) - h(14)) with the
+// headroom h), then one more at a time, with D restored from a copy first
+// when beta != 0; plain FP64 when 16 moduli cannot certify.
 int run_certify(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
                 double* err_bound) {
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (c->capturing || hipStreamIsCapturing(c->stream, &cs) != hipSuccess ||
       cs != hipStreamCaptureStatusNone)
     return set_error(SKGEMM_ERR_NOTSUP,
-                     "the certifying split form reads norms back and cannot "
-                     "be captured; give a modulus count");
-  const int M = pr.M, K = pr.K;
+                     "the certifying split form reads its bound back and "
+                     "cannot be captured; give a modulus count");
+  const int M = pr.M;
   hipError_t e;
-  if (kind == 1) {
-    if (c->extra == nullptr) {
-      c->extra = new skgemm_ctx::Extra();
-      c->extra->gen = &c->mem_gen;
-    }
-    auto* X = c->extra;
-    const int opa = (int)round_up(M, 256);
-    double* n2a = (double*)X->get(kNormA, sizeof(double) * opa);
-    int* sga = (int*)X->get(kSigA, sizeof(int) * opa);
-    char* scal = (char*)X->get(kScalars, 8 + sizeof(double) * kCrtGrid);
-    if (!n2a || !sga || !scal)
-      return set_error(SKGEMM_ERR_NOMEM, "split-form workspace");
-    int* fallback = (int*)scal;
-    const Tables T14 = make_tables(14);
-    c->record_begin();  // the norm pass counts in the call's (first stage's) time
-    if ((e = launch_zero_ints(fallback, 1, c->stream)) != hipSuccess)
-      return hip_error(e, "clear split-form flag");
-    const Operand& A = pr.a;
-    if (A.kcontig) {
-      hipLaunchKernelGGL(norm_k_kernel, dim3(M), dim3(256), 0, c->stream, A, K,
-                         T14.h, n2a, sga, fallback);
-    } else {
-      const int nseg = norm_segments(M, K);
-      const int kseg = (K + nseg - 1) / nseg;
-      double* part = (double*)X->get(kPartA, sizeof(double) * 2 * nseg * M);
-      if (part == nullptr) return set_error(SKGEMM_ERR_NOMEM, "norm partials");
-      hipLaunchKernelGGL(norm_o_partial_kernel, dim3((M + 255) / 256, nseg),
-                         dim3(256), 0, c->stream, A, K, kseg, part,
-                         part + (int64_t)nseg * M);
-      hipLaunchKernelGGL(norm_o_finish_kernel, dim3((M + 255) / 256),
-                         dim3(256), 0, c->stream, M, nseg, T14.h, part,
-                         part + (int64_t)nseg * M, n2a, sga, fallback);
-    }
-    if ((e = hipGetLastError()) != hipSuccess) return hip_error(e, "norms");
-    std::vector<double> n2(M);
-    int fb = 0;
-    if ((e = hipMemcpyAsync(n2.data(), n2a, sizeof(double) * M,
-                            hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(&fb, fallback, sizeof(int), hipMemcpyDeviceToHost,
-                            c->stream)) != hipSuccess ||
-        (e = hipStreamSynchronize(c->stream)) != hipSuccess)
-      return hip_error(e, "norm readback");
-    if (fb) return run_moduli(c, kind, la, lb, pr, 14, err_bound, false);
-    long double diag2 = 0;
-    for (double v : n2) diag2 += (long double)v * v;
-    for (int L = 8; L <= kMaxMod; ++L) {
-      const std::vector<int> sg = host_sigma(n2, make_tables(L).h);
-      const long double num = bound_numerator(n2, sg, n2, sg, K);
-      const long double den = std::sqrt(diag2) - num;
-      if (!(den > 0) || (double)(num / den) + 4.5e-16 > kCertifyTarget)
-        continue;
-      if ((e = hipMemcpyAsync(sga, sg.data(), sizeof(int) * M,
-                              hipMemcpyHostToDevice, c->stream)) != hipSuccess ||
-          (e = hipStreamSynchronize(c->stream)) != hipSuccess)
-        return hip_error(e, "scale upload");
-      return run_moduli(c, kind, la, lb, pr, L, err_bound, true, true);
-    }
-    return run_plain(c, kind, la, lb, pr, err_bound);
+  if (c->extra == nullptr) {
+    c->extra = new skgemm_ctx::Extra();
+    c->extra->gen = &c->mem_gen;
   }
-  // GEMM: keep D when beta != 0, so that a second attempt starts from it.
+  auto* X = c->extra;
+  // D is kept when beta != 0, so that a further attempt starts from it.
   void* saved = nullptr;
   int64_t nd = 0;
   if (pr.beta != 0.0) {
@@ -1865,32 +1953,40 @@ int run_certify(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
     nd = (int64_t)(n >> d.pshift) * d.pstride +
          (int64_t)(n & ((1 << d.pshift) - 1)) * d.ns +
          (int64_t)(M - 1) * d.ms + 1;
-    if (c->extra == nullptr) {
-      c->extra = new skgemm_ctx::Extra();
-      c->extra->gen = &c->mem_gen;
-    }
-    saved = c->extra->get(kSavedD, sizeof(double) * nd);
+    saved = X->get(kSavedD, sizeof(double) * nd);
     if (saved == nullptr) return set_error(SKGEMM_ERR_NOMEM, "saved D");
     if ((e = hipMemcpyAsync(saved, pr.d.ptr, sizeof(double) * nd,
                             hipMemcpyDeviceToDevice, c->stream)) != hipSuccess)
       return hip_error(e, "save D");
   }
-  for (int L = 14; L <= kMaxMod; ++L) {
-    if (L > 14 && saved != nullptr &&
+  auto restore = [&]() -> int {
+    if (saved != nullptr &&
         (e = hipMemcpyAsync(pr.d.ptr, saved, sizeof(double) * nd,
                             hipMemcpyDeviceToDevice, c->stream)) != hipSuccess)
       return hip_error(e, "restore D");
-    double b = 0.0;
+    return SKGEMM_OK;
+  };
+  const int h14 = make_tables(14).h;
+  double b = 0.0;
+  SKG_TRY(run_moduli(c, kind, la, lb, pr, 14, &b, false));
+  if (b < 0.0 || b <= kCertifyTarget) {  // -1: computed by plain FP64
+    if (err_bound) *err_bound = b;
+    return SKGEMM_OK;
+  }
+  // Predicted count: the smallest whose extra headroom brings b down.
+  int L = 15;
+  while (L < kMaxMod &&
+         b * std::ldexp(1.0, -(make_tables(L).h - h14)) > 0.9 * kCertifyTarget)
+    ++L;
+  for (; L <= kMaxMod; ++L) {
+    SKG_TRY(restore());
     SKG_TRY(run_moduli(c, kind, la, lb, pr, L, &b, false));
-    if (b < 0.0 || b <= kCertifyTarget) {
+    if (b <= kCertifyTarget) {
       if (err_bound) *err_bound = b;
       return SKGEMM_OK;
     }
   }
-  if (saved != nullptr &&
-      (e = hipMemcpyAsync(pr.d.ptr, saved, sizeof(double) * nd,
-                          hipMemcpyDeviceToDevice, c->stream)) != hipSuccess)
-    return hip_error(e, "restore D");
+  SKG_TRY(restore());
   return run_plain(c, kind, la, lb, pr, err_bound);
 }
 

@@ -27,11 +27,15 @@ wl = importlib.util.module_from_spec(_spec)
 _spec.loader.exec_module(wl)
 
 
-def near_pow2(x):
-    """Squared norms within 1e-12 of a power of two: the host's and the
-    device's summation orders may give different exponents there."""
-    m, _ = np.frexp(np.asarray(x, np.float64))
-    return (np.abs(m - 0.5) < 1e-12) | (np.abs(m - 1.0) < 1e-12)
+def near_pow2(x, L=14):
+    """Squared norms whose scale code changes under a relative change of
+    1e-12: the host's and the device's summation orders may give different
+    codes there (the code is a step function of the norm)."""
+    _, h = ozaki.tables(L)
+    x = np.asarray(x, np.float64)
+    c = ozaki.scale_code(x, h)
+    return ((ozaki.scale_code(x * (1 + 1e-12), h) != c) |
+            (ozaki.scale_code(x * (1 - 1e-12), h) != c))
 
 
 def syrk_rows_dd(orc, P, rows):

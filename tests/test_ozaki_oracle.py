@@ -53,9 +53,35 @@ def test_scaling_bounds_the_integer_product(rng):
     x = ozaki.scaled_integers(X.T, s)
     cint = ozaki.exact_product(x, x)
     assert max(abs(int(v)) for v in cint.ravel()) <= P // 4
-    # and the integer rows use the headroom: ||x'_o|| > 2^(h-3)
+    # and the integer rows use the headroom to within one eighth of an
+    # octave: 2^(h-1) 8/9 < ||x'_o|| <= 2^(h-1) + sqrt(K)/2
     norms = np.sqrt(np.einsum("ij,ij->i", x, x))
-    assert np.all(norms > 2.0 ** (h - 3)) and np.all(norms <= 2.0 ** h)
+    assert np.all(norms > 2.0 ** (h - 1) * 8 / 9)
+    assert np.all(norms <= 2.0 ** (h - 1) + np.sqrt(2000) / 2)
+
+
+def test_scale_code_is_the_largest_admissible_step():
+    """scale_code against its definition in exact rational arithmetic: s
+    = 2^(sigma-3) m (m in 8..15) is the largest value of that grid with
+    norm2 s^2 <= 2^(2h-2) (1 - 2^-19), up to the one rounding of fl(g m^2)
+    the code makes (it can only err towards the smaller step)."""
+    from fractions import Fraction as F
+    rng = np.random.default_rng(9)
+    h = 55
+    lim = F(2) ** (2 * h - 2) * (1 - F(1, 2 ** 19))
+    vals = np.concatenate([10.0 ** rng.uniform(-250, 250, 3000),
+                           np.ldexp(1.0, rng.integers(-500, 500, 200)),
+                           np.nextafter(np.ldexp(1.0, rng.integers(-50, 50, 200)), 0)])
+    codes = ozaki.scale_code(vals, h)
+    for v, c in zip(vals, codes):
+        e, m = int(c) >> 3, 8 + (int(c) & 7)
+        s = F(m, 8) * F(2) ** e
+        n2 = F(float(v))
+        assert n2 * s * s <= lim * (1 + F(1, 2 ** 50))
+        # the next grid value up is not admissible
+        nxt = F(m + 1, 8) * F(2) ** e if m < 15 else F(2) ** (e + 1)
+        assert n2 * nxt * nxt > lim * (1 - F(1, 2 ** 50))
+    assert ozaki.scale_code([0.0], h)[0] == 0
 
 
 def test_small_integer_inputs_are_exact(rng):
