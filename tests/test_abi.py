@@ -166,6 +166,16 @@ def test_committed_traffic_figures_follow_their_counters():
         want = (e["fetch_kib_raw"] * ff + e["write_kib_raw"] * wf) * 1024
         assert abs(e["bytes_per_launch"] - want) <= 1e-4 * want, key
         assert e["bytes_per_launch"] > e["algorithmic_bytes"]
+    # the split-form entries carry the factors of their own run
+    c3 = d["calibration_c3"]
+    ff3 = c3["copy_kernel_bytes_each_way"] / 1024 / c3["fetch_kib_raw"]
+    wf3 = c3["copy_kernel_bytes_each_way"] / 1024 / c3["write_kib_raw"]
+    for key in ("int8_products", "residue_pass"):
+        e = d[key]
+        assert abs(e["fetch_factor"] - ff3) < 1e-3 and abs(e["write_factor"] - wf3) < 1e-3
+        want = (e["fetch_kib_raw"] * ff3 + e["write_kib_raw"] * wf3) * 1024
+        assert abs(e["bytes_per_launch"] - want) <= 1e-4 * want, key
+        assert e["bytes_per_launch"] > e["algorithmic_bytes"]
 
 
 @pytest.mark.parametrize("G", [1, 7, 8, 9, 37, 256, 511, 512, 1024])
