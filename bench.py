@@ -661,6 +661,43 @@ def bench_serving(skg, ctx, c4_tflops, seconds, precision):
     return out
 
 
+def bench_int8_library(reps):
+    """The vendor library's INT8 GEMM (torch._int_mm, hipBLASLt on ROCm) on
+    shapes beside the split form's products: one 1024 x 1024 x 65536
+    product, the MACs of 14 of them stacked (14336 x 1024 x 65536), and a
+    large square (8192^3).  A scale for the INT8 roofline fraction, not a
+    path of this library."""
+    import torch
+    out = {"source": "torch._int_mm, int8 x int8 -> int32, A row-major, "
+                     "B column-major, uniform int8 operands"}
+    g = torch.Generator(device="cuda").manual_seed(5)
+    for M, N, K in ((1024, 1024, 65536), (14336, 1024, 65536),
+                    (8192, 8192, 8192)):
+        name = "%dx%dx%d" % (M, N, K)
+        try:
+            a = torch.randint(-128, 128, (M, K), dtype=torch.int8,
+                              device="cuda", generator=g)
+            b = torch.randint(-128, 128, (N, K), dtype=torch.int8,
+                              device="cuda", generator=g).t()
+            for _ in range(3):
+                torch._int_mm(a, b)
+            t0 = torch.cuda.Event(enable_timing=True)
+            t1 = torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            t0.record()
+            for _ in range(reps):
+                torch._int_mm(a, b)
+            t1.record()
+            torch.cuda.synchronize()
+            ms = t0.elapsed_time(t1) / reps
+            out[name] = {"ms": ms, "tops": 2.0 * M * N * K / (ms * 1e-3) / 1e12}
+            del a, b
+        except Exception as e:  # the library may lack a kernel for a shape
+            out[name] = {"error": str(e)[:200]}
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -673,7 +710,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip every section but the headline")
     ap.add_argument("--extra", default="c2,c1,rotation,projection,nloc,"
-                                       "stencil,layouts,batched,serving",
+                                       "stencil,layouts,batched,serving,"
+                                       "int8_library",
                     help="comma list of extra sections to run")
     ap.add_argument("--tuning-cache", default=None,
                     help="autotuner cache file: read if it exists (no "
@@ -751,6 +789,9 @@ def main():
     if "layouts" in sections:
         log("layout conversions (65536x1024) ...")
         extra["layouts"] = bench_layouts(skg, ctx, 10)
+    if "int8_library" in sections:
+        log("library INT8 GEMM for scale ...")
+        extra["int8_library"] = bench_int8_library(10)
     if "batched" in sections or "serving" in sections:
         log("batched (64 systems) ...")
         extra["batched"] = bench_batched(skg, ctx, 5)
