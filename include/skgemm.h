@@ -112,6 +112,19 @@ int skgemm_synchronize(skgemm_ctx* ctx);
  * error instead of a hang. */
 int skgemm_set_spin_limit(skgemm_ctx* ctx, int64_t limit);
 
+/* Stream-K fix-up of the plain FP64 kernel on this context (single,
+ * batched and host-pointer forms).  0 (default): the workgroup that starts
+ * a split tile waits for the partials of the workgroups that finish it,
+ * which needs every workgroup of the launch resident at once.  1
+ * (wait-free): every contributor stores its partial and counts itself in;
+ * the last to arrive adds the partials in the same order and writes the
+ * tile, so nothing waits and launches on several streams may share the
+ * device in any proportion.  Results are bit-identical between the modes;
+ * the wait-free mode holds twice the partial-tile workspace.  No
+ * counterpart in the reference (its threads join at an OpenMP barrier,
+ * gemm_crsp.cpp:285). */
+int skgemm_set_waitfree(skgemm_ctx* ctx, int on);
+
 /* Which entries a SYRK on this context writes (plain, split form, batched
  * and their host forms):
  *   SKGEMM_SYRK_LOWER (default): the lower triangle i >= j of the row-major

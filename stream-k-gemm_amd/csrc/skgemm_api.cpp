@@ -280,6 +280,13 @@ extern "C" int skgemm_device_info(skgemm_ctx* c, int* num_cus, int* num_xcds,
   return SKGEMM_OK;
 }
 
+extern "C" int skgemm_set_waitfree(skgemm_ctx* c, int on) {
+  SKG_TRY(check_ctx(c));
+  if (on != 0 && on != 1) return set_error(SKGEMM_ERR_ARG, "on must be 0 or 1");
+  c->waitfree = on != 0;
+  return SKGEMM_OK;
+}
+
 extern "C" int skgemm_set_spin_limit(skgemm_ctx* c, int64_t limit) {
   SKG_TRY(check_ctx(c));
   if (limit < 1) return set_error(SKGEMM_ERR_ARG, "spin limit must be >= 1");
@@ -377,6 +384,10 @@ int skgemm_ctx::check_status() {
                      (long long)spin_limit);
   }
   return SKGEMM_OK;
+}
+
+size_t skgemm_ctx::ws_doubles(int cfg, int grid) const {
+  return (size_t)grid * slot_doubles(cfg) * (waitfree ? 2 : 1);
 }
 
 int skgemm_ctx::ensure_ws(size_t ws_doubles, int nflags) {
@@ -744,7 +755,7 @@ int run_single(skgemm_ctx* c, Problem& p, const Plan& plan_in,
   if (cb != nullptr) {
     // Buffers the caller provides (nothing is allocated): the even split,
     // whose starts need no device table.
-    if ((size_t)pl.grid * slot_doubles(pl.cfg) > cb->ws_doubles ||
+    if (c->ws_doubles(pl.cfg, pl.grid) > cb->ws_doubles ||
         pl.grid > cb->nflags)
       return set_error(SKGEMM_ERR_NOMEM, "internal: caller buffers too small");
     L.starts = nullptr;
@@ -752,12 +763,13 @@ int run_single(skgemm_ctx* c, Problem& p, const Plan& plan_in,
     L.flags = cb->flags;
   } else {
     L.starts = split_table(c, p, pl.cfg, pl.grid);
-    SKG_TRY(c->ensure_ws((size_t)pl.grid * slot_doubles(pl.cfg), pl.grid));
+    SKG_TRY(c->ensure_ws(c->ws_doubles(pl.cfg, pl.grid), pl.grid));
     L.ws = c->ws;
     L.flags = c->flags;
   }
   L.status = c->status;
   L.spin_limit = c->spin_limit;
+  L.waitfree = c->waitfree ? 1 : 0;
   if (timed) c->record_begin();
   SKG_HIP(launch_dgemm_sk(pl.cfg, L, c->stream), "Stream-K launch");
   if (p.kchunks > 1)

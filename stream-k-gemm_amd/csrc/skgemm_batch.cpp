@@ -189,7 +189,7 @@ int build_plain_table(skgemm_ctx* c, const skgemm_problem* q, int n,
 int launch_plain_table(skgemm_ctx* c, const PlainTable& tb,
                        const Problem* dtab, const int64_t* dhdr) {
   if (tb.rows.empty()) return SKGEMM_OK;
-  SKG_TRY(c->ensure_ws((size_t)tb.grid * slot_doubles(tb.cfg), tb.grid));
+  SKG_TRY(c->ensure_ws(c->ws_doubles(tb.cfg, tb.grid), tb.grid));
   Launch L{};
   L.problems = dtab;
   L.hdr = dhdr;
@@ -200,6 +200,7 @@ int launch_plain_table(skgemm_ctx* c, const PlainTable& tb,
   L.flags = c->flags;
   L.status = c->status;
   L.spin_limit = c->spin_limit;
+  L.waitfree = c->waitfree ? 1 : 0;
   SKG_HIP(launch_dgemm_sk(tb.cfg, L, c->stream), "batched launch");
   return SKGEMM_OK;
 }
@@ -222,7 +223,7 @@ int run_plain(skgemm_ctx* c, const skgemm_problem* q, int n, int use_graph) {
     b->pending = false;
   }
   std::memcpy(b->htab, tb.rows.data(), sizeof(Problem) * used);
-  SKG_TRY(c->ensure_ws((size_t)grid * slot_doubles(cfg), grid));
+  SKG_TRY(c->ensure_ws(c->ws_doubles(cfg, grid), grid));
   b->hhdr[0] = used;
   b->hhdr[1] = iters;
   SKG_HIP(hipMemcpyAsync(b->dtab, b->htab, sizeof(Problem) * used,
@@ -244,6 +245,7 @@ int run_plain(skgemm_ctx* c, const skgemm_problem* q, int n, int use_graph) {
   L.flags = c->flags;
   L.status = c->status;
   L.spin_limit = c->spin_limit;
+  L.waitfree = c->waitfree ? 1 : 0;
   c->record_begin();
   if (!use_graph) {
     SKG_HIP(launch_dgemm_sk(cfg, L, c->stream), "batched launch");

@@ -45,6 +45,9 @@ struct skgemm_ctx {
   int flags_n = 0;
   int* status = nullptr;
   int64_t spin_limit = 1ll << 26;
+  // Last-arriver fix-up (no waits between workgroups): twice the slots.
+  bool waitfree = false;
+  size_t ws_doubles(int cfg, int grid) const;
   // SYRK outputs: SKGEMM_SYRK_LOWER or SKGEMM_SYRK_DIAG_BLOCKS.
   int syrk_store = SKGEMM_SYRK_LOWER;
   // Per-chunk sums of K-chunked launches.

@@ -117,10 +117,13 @@ struct Launch {
   // Single launches: first flattened iteration of each workgroup (grid + 1
   // entries, a cost-weighted split), or nullptr for the even split.
   const int64_t* starts;
-  double* ws;           // grid partial tiles of BM*BN doubles
-  int* flags;           // grid flags: 0 empty, 1 partial ready
+  double* ws;           // grid (wait-free: 2 grid) partial tiles
+  int* flags;           // grid flags: 0 empty, 1 partial ready (wait-free:
+                        // arrival counters of split units, by first
+                        // contributor)
   int* status;          // bit 0: a fix-up wait hit its spin limit
   int64_t spin_limit;   // s_sleep rounds before a wait gives up
+  int waitfree;         // 1: last-arriver fix-up, no inter-workgroup waits
 };
 
 // Tile configurations (index = kernel table entry).

@@ -5,7 +5,9 @@
 // evenly over OpenMP threads with one full M x N partial per thread,
 // gemm_crsp.cpp:261-311):
 //   * One persistent grid of G workgroups, G <= the number that fit on the
-//     chip at once, so every fix-up wait has a resident producer.
+//     chip at once, so every fix-up wait has a resident producer (or, in
+//     the wait-free mode, any G: the last contributor to a split tile
+//     finishes it and nothing waits).
 //   * The MAC-iteration space (tiles x ceil(K/BK)) of one or many problems
 //     is split into G contiguous, equal ranges.  A range may start and end
 //     inside a tile.  The workgroup that computes a tile's first iteration
@@ -713,6 +715,7 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
   constexpr int64_t kSlot = (int64_t)BM * BN + (XS ? 8 * NT : 0);
 
   extern __shared__ __align__(16) double lds[];
+  __shared__ int s_last;  // wait-free fix-up: this workgroup arrived last
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -877,10 +880,8 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
     // Order: a spiller's stores complete (s_waitcnt 0) before the barrier
     // and the flag store; the owner's data loads issue after the barrier
     // that follows its observation of the flag.
-    bool write_out = true;
-    if (kb != 0) {
-      // Not the owner: spill the partial to this workgroup's slot.
-      gdouble* slot = (gdouble*)(L.ws) + (int64_t)w * kSlot;
+    gdouble* const wsd = (gdouble*)(L.ws);
+    auto spill = [&](gdouble* slot) {
 #pragma unroll
       for (int i = 0; i < MB; ++i)
 #pragma unroll
@@ -899,6 +900,78 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
       }
+    };
+    // acc (+)= the partial in `slot`; `first` assigns instead of adding.
+    auto gather = [&](gdouble* slot, bool first) {
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const double x = __hip_atomic_load(
+                slot + ((i * NB + j) * 4 + v) * NT + tid, __ATOMIC_RELAXED,
+                __HIP_MEMORY_SCOPE_AGENT);
+            acc[i][j][v] = first ? x : acc[i][j][v] + x;
+          }
+      if (XS != 0 && (xt || yt)) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const double x = __hip_atomic_load(
+              slot + (MB * NB * 4 + v) * NT + tid, __ATOMIC_RELAXED,
+              __HIP_MEMORY_SCOPE_AGENT);
+          const double y = __hip_atomic_load(
+              slot + (MB * NB * 4 + 4 + v) * NT + tid, __ATOMIC_RELAXED,
+              __HIP_MEMORY_SCOPE_AGENT);
+          xacc[v] = first ? x : xacc[v] + x;
+          cacc[v] = first ? y : cacc[v] + y;
+        }
+      }
+    };
+    bool write_out = true;
+    if (L.waitfree && (kb != 0 || ke < kci)) {
+      // Wait-free fix-up: every contributor of a split unit stores its
+      // partial, then counts itself in on the counter of the unit's first
+      // contributor wo (flags[wo]); the last to arrive resets the counter,
+      // adds the partials in contributor order (the waiting fix-up's
+      // order, so the bits are the same) and writes the tile.  No
+      // workgroup waits for another, so a launch needs no co-resident
+      // workgroups.  The first contributor's partial goes to the upper
+      // half of the slots (its range may also begin with another unit's
+      // tail, which takes slot w).
+      const int64_t tile_end = t0 + kci;
+      int wo = w;
+      if (kb != 0) {
+        wo = w - 1;
+        while (wstart(wo) > t0) --wo;  // wstart(0) = 0 <= t0
+      }
+      int ncontrib = 0;
+      for (int w2 = wo; w2 < G && wstart(w2) < tile_end; ++w2)
+        ncontrib += wstart(w2 + 1) != wstart(w2);
+      spill(wsd + (int64_t)(w == wo ? G + w : w) * kSlot);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (tid == 0) {
+        const int before = __hip_atomic_fetch_add(
+            &L.flags[wo], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int last = before == ncontrib - 1;
+        if (last)
+          __hip_atomic_store(&L.flags[wo], 0, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        s_last = last;
+      }
+      __syncthreads();
+      write_out = s_last != 0;
+      if (write_out) {
+        gather(wsd + (int64_t)(G + wo) * kSlot, true);
+        for (int w2 = wo + 1; w2 < G && wstart(w2) < tile_end; ++w2) {
+          if (wstart(w2 + 1) == wstart(w2)) continue;  // empty range
+          gather(wsd + (int64_t)w2 * kSlot, false);
+        }
+      }
+    } else if (kb != 0) {
+      // Not the owner: spill the partial to this workgroup's slot.
+      spill(wsd + (int64_t)w * kSlot);
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
       if (tid == 0)
@@ -924,27 +997,7 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
                              __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
-        gdouble* slot = (gdouble*)(L.ws) + (int64_t)w2 * kSlot;
-#pragma unroll
-        for (int i = 0; i < MB; ++i)
-#pragma unroll
-          for (int j = 0; j < NB; ++j)
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-              acc[i][j][v] += __hip_atomic_load(
-                  slot + ((i * NB + j) * 4 + v) * NT + tid, __ATOMIC_RELAXED,
-                  __HIP_MEMORY_SCOPE_AGENT);
-        if (XS != 0 && (xt || yt)) {
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            xacc[v] += __hip_atomic_load(slot + (MB * NB * 4 + v) * NT + tid,
-                                         __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-            cacc[v] += __hip_atomic_load(
-                slot + (MB * NB * 4 + 4 + v) * NT + tid, __ATOMIC_RELAXED,
-                __HIP_MEMORY_SCOPE_AGENT);
-          }
-        }
+        gather(wsd + (int64_t)w2 * kSlot, false);
       }
     }
 

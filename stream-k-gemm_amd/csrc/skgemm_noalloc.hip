@@ -78,7 +78,7 @@ extern "C" int skgemm_dsyrk_noalloc(skgemm_ctx* c, int la, int N, int K,
   p.gamma = 0.0;
   // Scratch layout: flags | fix-up partial tiles | K-chunk sums.
   const size_t flags_b = round256(sizeof(int) * (size_t)pl.grid);
-  const size_t ws_n = (size_t)pl.grid * slot_doubles(pl.cfg);
+  const size_t ws_n = c->ws_doubles(pl.cfg, pl.grid);
   const size_t ws_b = round256(sizeof(double) * ws_n);
   const size_t part_b =
       pl.kchunks > 1 ? sizeof(double) * (size_t)pl.kchunks * N * N : 0;

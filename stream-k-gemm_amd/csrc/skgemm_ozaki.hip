@@ -1655,7 +1655,7 @@ int ozaki_reserve(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
     if (rc != SKGEMM_OK) return rc;
   }
   Plan pl = plan(c, kind, la, lb, M, N, K);
-  return c->ensure_ws((size_t)pl.grid * slot_doubles(pl.cfg), pl.grid);
+  return c->ensure_ws(c->ws_doubles(pl.cfg, pl.grid), pl.grid);
 }
 
 namespace {

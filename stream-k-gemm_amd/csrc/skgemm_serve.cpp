@@ -10,12 +10,13 @@
 // earlier launches run joins the next launch; a completion thread polls the
 // slots' events and stamps each request's completion time.
 //
-// Slots run concurrently on separate default-priority streams.  A plain
-// FP64 slot runs the persistent Stream-K kernel on capacity / max_inflight
-// workgroups of one tile shape, so the launches in flight together never
-// exceed what the device holds at once and every fix-up wait ends; split
-// slots use only data-parallel INT8 schedules (no waits between
-// workgroups).
+// Slots run concurrently on separate default-priority streams.  No launch
+// of a slot waits between its workgroups: plain FP64 slots run the
+// persistent Stream-K kernel with the wait-free (last-arriver) fix-up and
+// split slots only data-parallel INT8 schedules, so every launch takes the
+// whole device's worth of workgroups, a launch running alone has all of it,
+// and launches in flight together share the CUs as their workgroups
+// retire.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -307,9 +308,10 @@ extern "C" int skgemm_server_create(int device, size_t hbm_budget, int max_batch
         hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess)
       rc = set_error(SKGEMM_ERR_HIP, "server event");
     if (rc == SKGEMM_OK) {
-      // Launches in flight together fit the device at once.
-      const int grid = std::max(8, sl.ctx->capacity(kServeCfg) / max_inflight);
-      rc = skgemm_set_tuning(sl.ctx, kServeCfg, grid);
+      // Nothing waits across workgroups, so each launch may take the
+      // whole device whatever else is in flight.
+      rc = skgemm_set_tuning(sl.ctx, kServeCfg, sl.ctx->capacity(kServeCfg));
+      sl.ctx->waitfree = true;
       sl.ctx->oz_no_wait = true;
     }
     if (rc != SKGEMM_OK) {

@@ -370,6 +370,120 @@ def test_fixup_protocol_random_interleavings(ntiles, ipt, G):
             assert mem.out[t] == sum(range(t * ipt, (t + 1) * ipt)), (trial, t)
 
 
+def workgroup_waitfree(w, G, ntiles, ipt, mem, starts):
+    """The wait-free (last-arriver) fix-up of sk_dgemm_kernel
+    (skgemm_kernels.hip, L.waitfree): every contributor of a split tile
+    stores its partial (the tile's first contributor wo in slot G + wo,
+    the others in their own slot), then adds one to flags[wo]; the one that
+    saw ncontrib - 1 resets the counter and adds the partials in contributor
+    order.  A partial is (sum of its iterations, (contributor,)), so the
+    order of the additions is checked along with the value."""
+    it, end = starts(w), starts(w + 1)
+    while it < end:
+        tile = it // ipt
+        t0 = tile * ipt
+        kb, ke = it - t0, min(end - t0, ipt)
+        acc = (sum(range(t0 + kb, t0 + ke)), (w,))
+        yield
+        if kb != 0 or ke < ipt:
+            wo = w
+            if kb != 0:
+                wo = w - 1
+                while starts(wo) > t0:
+                    wo -= 1
+            contrib = [x for x in range(wo, G)
+                       if starts(x) < t0 + ipt and starts(x + 1) != starts(x)]
+            mine = G + w if w == wo else w
+            assert mem.slot[mine] is None, "a slot is written once per launch"
+            mem.slot[mine] = acc
+            yield                                   # stores complete
+            before = mem.flag[wo]
+            mem.flag[wo] = before + 1               # one atomic add
+            last = before == len(contrib) - 1
+            if last:
+                mem.flag[wo] = 0
+            yield
+            if not last:
+                it = t0 + ke
+                continue
+            total, order = 0, ()
+            for x in contrib:
+                v = mem.slot[G + x if x == wo else x]
+                assert v is not None, "a contributor's partial is missing"
+                total, order = total + v[0], order + v[1]
+                yield
+            acc = (total, order)
+        mem.out[tile] = acc
+        yield
+        it = t0 + ke
+
+
+def run_launch_waitfree(G, ntiles, ipt, mem, rng, resident):
+    """At most `resident` workgroups run at once, admitted in a random
+    order (a launch sharing the device with another one); the rest start
+    only as running ones finish."""
+    total = ntiles * ipt
+    starts = lambda x: wg_start(x, total, G)
+    mem.slot = [None] * (2 * G)
+    mem.out = {}
+    waiting = list(rng.permutation(G))
+    live = {}
+    steps = 0
+    while live or waiting:
+        while waiting and len(live) < resident:
+            x = int(waiting.pop())
+            live[x] = workgroup_waitfree(x, G, ntiles, ipt, mem, starts)
+        w = list(live)[int(rng.integers(len(live)))]
+        try:
+            next(live[w])
+        except StopIteration:
+            del live[w]
+        steps += 1
+        assert steps < 10 ** 6, "no progress"
+    return steps
+
+
+@pytest.mark.parametrize("ntiles,ipt,G", [(1, 12, 5), (3, 7, 4), (10, 3, 7),
+                                          (2, 40, 9), (5, 5, 25), (7, 9, 3)])
+@pytest.mark.parametrize("resident", [1, 2, 1000])
+def test_waitfree_fixup_any_residency(ntiles, ipt, G, resident):
+    """The wait-free fix-up finishes under any interleaving and with any
+    number of workgroups resident (one at a time included, where the
+    waiting fix-up's owner would wait forever for a peer that cannot
+    start), every tile is exact, the partials are added in contributor
+    order (the waiting fix-up's order, so the two modes give the same
+    bits), and every counter is back at zero."""
+    rng = np.random.default_rng(ntiles * 1000 + ipt * 10 + G + resident)
+    G = min(G, ntiles * ipt)
+    total = ntiles * ipt
+    mem = FixupMemory(G)
+    for trial in range(100):
+        run_launch_waitfree(G, ntiles, ipt, mem, rng, resident)
+        assert mem.flag == [0] * G
+        for t in range(ntiles):
+            got, order = mem.out[t]
+            assert got == sum(range(t * ipt, (t + 1) * ipt)), (trial, t)
+            want = tuple(x for x in range(G)
+                         if wg_start(x, total, G) < (t + 1) * ipt
+                         and wg_start(x + 1, total, G) > t * ipt)
+            assert order == want, (trial, t)
+
+
+def test_waiting_fixup_needs_coresidency():
+    """The contrast: with one workgroup resident at a time and the owner
+    admitted first, the waiting fix-up's owner spins until its limit (the
+    reason the waiting mode caps a launch at what the device holds)."""
+    rng = np.random.default_rng(3)
+    ntiles, ipt, G = 1, 20, 2
+    total = ntiles * ipt
+    starts = lambda x: wg_start(x, total, G)
+    mem = FixupMemory(G)
+    owner = workgroup(0, G, ntiles, ipt, mem, 100, starts)
+    for _ in owner:       # runs alone: the peer never starts
+        pass
+    assert mem.status == 1
+
+
 def test_fixup_timeout_then_reset():
     """A peer that is late beyond the owner's spin limit: the owner gives up
     (status bit 0, which the host reports as SKGEMM_ERR_TIMEOUT), every
