@@ -4,21 +4,30 @@
 // No counterpart in the reference, whose path is FP64 only.
 //
 // C = A B (A: M x K, B: K x N; for SYRK A = Psi^T, B = Psi):
-//  1. scale: sigma_o = h - 1 - ceil(e_o / 2) with ||row/column o||_2^2 =
-//     f 2^e_o, so that A'_o = rint(A_o 2^sigma_o) has ||A'_o||_2 <= 2^h and
-//     |(A'B')_ij| <= 2^(2h) <= P/4, P = product of the moduli (Cauchy-Schwarz);
+//  1. scale: each row of A / column of B gets s_o = 2^(sigma - 3) m,
+//     m in 8..15, the largest with ||x_o s_o|| < 2^(h-1) from its squared
+//     norm (scale_code), so that A'_o = rint(A_o s_o) and
+//     |(A'B')_ij| < 2^(2h-2) (1 + 2^-20) < P/4, P = product of the moduli
+//     (Cauchy-Schwarz); the same pass sums the squared rounding errors
+//     (fma(a, s, -rint(a s)), exact before one rounding);
 //  2. split: for L pairwise-coprime moduli m_l <= 256, the symmetric
 //     residues of A' and B' (in [-128, 127]) as INT8 planes, K contiguous;
 //  3. L INT8 products on v_mfma_i32_32x32x32_i8 with exact INT32
 //     accumulation (|sum| <= Kpad * 127^2 < 2^31 for the odd moduli; for
 //     m = 256 a wrap modulo 2^32 keeps the residue), reduced mod m_l, in one
-//     launch over (modulus, tile) units, data-parallel or Stream-K;
+//     launch over (modulus, K chunk, tile) units, data-parallel or
+//     Stream-K; K beyond 131072 is cut into chunks whose residues are added
+//     modulo m_l;
 //  4. CRT: x = sum_l r_l W_l - q P in exact 128-bit arithmetic (q from an
-//     FP64 estimate), one rounding to FP64, exact power-of-two unscaling,
-//     epilogue alpha*C + beta*D + gamma.
+//     FP64 estimate), one rounding to FP64, division by the two scales'
+//     mantissas and an exact power-of-two unscaling, epilogue
+//     alpha*C + beta*D + gamma.
 // Integer steps are exact, so a result does not depend on the partition of
 // the work (bit-identical for any grid).  The only approximation is step 1's
-// rounding to integers; its a-posteriori bound is returned with the result.
+// rounding to integers; its rigorous a-posteriori bound (from the norms and
+// the rounding-error norms) is returned with the result.  The certifying
+// mode (SKGEMM_SLICES_AUTO) starts at 14 moduli and adds moduli until that
+// bound is at most 1e-13, else computes in plain FP64.
 
 #include <hip/hip_runtime.h>
 
