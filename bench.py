@@ -445,8 +445,9 @@ def bench_nloc(skg, ctx, reps):
     projectors of 512..1024 grid points (compact spheres) and 9, 13 or 16
     columns, seeded.  HBM-bound: the algorithmic traffic is psi read at the
     touched points plus result read and written there (3 x 8 x points x
-    nb bytes); the implementation also writes and reads the gathered rows
-    and the backward products (4 x 8 x rows x nb more)."""
+    nb bytes); the implementation reads psi once per projector row (the
+    repeats of overlapping projectors mostly from L2) and writes and reads
+    the forward parts of the 256-row chunks (2 x 8 x parts x nb)."""
     import time
     K, nb, count = 32768, 520, 64
     rng = np.random.default_rng(6)
@@ -477,7 +478,8 @@ def bench_nloc(skg, ctx, reps):
     for x in (psi, res):
         x.free()
     alg = 3.0 * 8 * points * nb
-    moved = alg + 4.0 * 8 * rows * nb
+    parts = sum(-(-r // 256) * c for r, c in zip(nrows, ncols))
+    moved = 8.0 * nb * (rows + 2 * points + 2 * parts)
     flop = 4.0 * nb * sum(r * c for r, c in zip(nrows, ncols))
     return {"workload": "nloc projectors: Nd 32768, 520 bands, 64 projectors "
                         "(512..1024 points, 9/13/16 columns), 16->32 packed",

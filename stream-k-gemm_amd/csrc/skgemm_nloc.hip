@@ -1,26 +1,30 @@
 // skgemm_nloc.hip - non-local projector application on band-packed orbitals,
-// result += V_nloc psi (include/nloc_packed_opt2.hpp:229-655), as a client of
-// the batched Stream-K scheduler.
+// result += V_nloc psi (include/nloc_packed_opt2.hpp:229-655).
 //
 // Per projector p (grid indices idx_p, nrow_p of them; chi_p nrow_p x
 // ncol_p; gamma_p):  X_p = dv chi_p^T psi[idx_p, :],  X_p *= gamma_p (first
 // projector of each atom only),  result[idx_p, :] += chi_p X_p.
-// Device sequence, all on the context's stream:
-//   1. forward products fused with the gather: X_p = dv chi_p^T psi[idx_p,
-//      :] read straight from psi16 at the projector's grid points, times
-//      gamma where the atom rule applies (times 1.0, exact, elsewhere); one
-//      workgroup per (projector, 64 bands);
-//   2. backward products: one persistent batched Stream-K launch, one
-//      problem per projector (temp_p = chi_p X_p); its problem table is
-//      described once per band count and kept in device memory, so an
-//      application is three launches and no host work;
-//   3. scatter: result[g, b] += temp[R, b] for every row R with grid index
-//      g, each grid point by one thread in projector order (the rows that
-//      touch g are listed in a CSR table built at create time), so shared
-//      grid points need no atomics and the sums keep the reference's order.
-// HBM-bound: the forward pass reads nrow_total x nb doubles of psi (repeats
-// from overlapping projectors mostly hit L2), the backward products write
-// and the scatter reads them; the products are small (ncol <= a few tens).
+// Device sequence, all on the context's stream, no host work:
+//   1. forward parts: the projectors' rows are cut into chunks of at most
+//      256; one workgroup per (chunk, 64 bands) gathers its rows of psi
+//      straight from the 16-band panels (32 rows per wave, all loads in
+//      flight together) and forms the chunk's part of chi_p^T psi[idx_p, :];
+//   2. backward products: one workgroup per (chunk, 64 bands) forms X_p for
+//      its bands, (dv sum of the chunk parts, in chunk order) gamma_p (times
+//      1.0, exact, where the atom rule does not apply), in registers, and
+//      writes the chunk's rows of temp_p = chi_p X_p;
+//   3. scatter: one wave per (touched grid point g, 64 bands) adds the temp
+//      rows of g in projector order (a CSR table built at create time), so
+//      shared grid points need no atomics and the sums keep the reference's
+//      order.
+// HBM-bound: psi is read once per projector row (repeats of overlapping
+// projectors mostly hit L2), temp written and read once, result read and
+// written once per touched point.  (Forming the products inside the
+// scatter instead reads X_p, ncol values per band, for every touched
+// (point, row), 13x the bytes of one temp row at 13 columns: on 32768
+// points x 520 bands with 64 projectors that scatter alone took 305 us,
+// against 285 us for this whole sequence.)
+
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,80 +38,172 @@
 namespace skg {
 namespace {
 
-// Forward products fused with the gather:
-//   X[coff_p + c][b] = (dv sum_r chi_p[r][c] psi(index[roff_p + r], b)) factor
-// One workgroup per (projector, 64 bands) (grid x: projectors, y: groups
-// of 64 bands), 8 waves.  Rows go in chunks of 64: the chunk's chi values
-// are staged in LDS (row-major, read as broadcasts) and wave w takes rows w,
-// w + 8, ... of the chunk (8 gathered loads in flight), accumulating 16
-// columns per lane; the 8 partial sums are added in wave order through LDS.
-// Columns past 16 take further passes.  chi_p is column-major nrow x ncol.
 struct NlocProj {
   int64_t roff, choff, coff;
   int nrow, ncol;
+  int chunk0, nchunks;  // this projector's forward chunks
+  int64_t poff;         // its first row of forward parts (chunk-major)
 };
-constexpr int kNlocCols = 16, kNlocWaves = 8, kNlocChunk = 64;
+struct NlocChunk {
+  int proj, r0, nr;
+};
+constexpr int kFwdWaves = 8, kFwdRowsPerWave = 32;
+constexpr int kFwdRows = kFwdWaves * kFwdRowsPerWave;  // rows per chunk
+constexpr int kFwdCols = 16;                            // columns per pass
 
-__global__ __launch_bounds__(kNlocWaves * 64) void nloc_project_kernel(
+// Forward parts: part[poff_p + (k - chunk0_p) ncol_p + c][b] =
+//   sum over the chunk's rows r of chi_p[r][c] psi(index[roff_p + r], b).
+// Wave w takes rows w * 32 .. w * 32 + 31 of the chunk and issues its 32
+// psi loads at once, on clamped addresses (a band past nb reads band
+// nb - 1, a row past the chunk its last row), before the chunk's chi
+// values are staged in LDS ([column][row], zero past the chunk's rows and
+// the projector's columns, so the clamped rows add nothing); chi values
+// are read as broadcasts.  The 8 waves' sums are added in wave order
+// through LDS, 8 columns at a time.  chi_p is column-major nrow x ncol.
+// (A non-finite psi value at a chunk's last row can turn an infinite part
+// into NaN: the clamped copies add 0 x inf.)
+__global__ __launch_bounds__(kFwdWaves * 64, 4) void nloc_forward_kernel(
     const double* __restrict__ psi16, int Nd, int nb,
     const int* __restrict__ index, const double* __restrict__ chi,
-    const NlocProj* __restrict__ proj, const double* __restrict__ factor,
-    double dv, double* __restrict__ X) {
-  __shared__ double part[kNlocWaves][kNlocCols][64];
-  __shared__ double chs[kNlocChunk][kNlocCols];
+    const NlocProj* __restrict__ proj, const NlocChunk* __restrict__ chunks,
+    double* __restrict__ part) {
+  // chi of the chunk ([kFwdRows][kFwdCols], a row's 16 values read as
+  // eight 16-byte broadcasts), then the wave sums ([kFwdWaves][8][64]):
+  // 32 KB either way.
+  __shared__ __align__(16) double sh[kFwdCols * kFwdRows];
+  static_assert(kFwdWaves * 8 * 64 <= kFwdCols * kFwdRows, "LDS reuse");
   const int tid = threadIdx.x;
-  const int lane = tid & 63, w = tid >> 6;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int b = blockIdx.y * 64 + lane;
-  const NlocProj P = proj[blockIdx.x];
-  const bool bok = b < nb;
-  const double* src = psi16 + (int64_t)(b >> 4) * Nd * 16 + (b & 15);
+  const NlocChunk C = chunks[blockIdx.x];
+  const NlocProj P = proj[C.proj];
+  const int bc = min(b, nb - 1);
+  const double* src = psi16 + (int64_t)(bc >> 4) * Nd * 16 + (bc & 15);
   const int* idx = index + P.roff;
-  constexpr int kPer = kNlocChunk / kNlocWaves;  // rows per wave and chunk
-  for (int c0 = 0; c0 < P.ncol; c0 += kNlocCols) {
-    const int nc = P.ncol - c0 < kNlocCols ? P.ncol - c0 : kNlocCols;
-    const double* ch = chi + P.choff + (int64_t)c0 * P.nrow;
-    double acc[kNlocCols];
+  const int rb = C.r0 + w * kFwdRowsPerWave;  // this wave's first row
+  const int rlast = C.r0 + C.nr - 1;
+  double v[kFwdRowsPerWave];
 #pragma unroll
-    for (int c = 0; c < kNlocCols; ++c) acc[c] = 0.0;
-    for (int r0 = 0; r0 < P.nrow; r0 += kNlocChunk) {
-      __syncthreads();  // the previous chunk's chi reads are done
-      for (int e = tid; e < kNlocChunk * kNlocCols; e += kNlocWaves * 64) {
-        const int r = e / kNlocCols, c = e % kNlocCols;
-        chs[r][c] =
-            (c < nc && r0 + r < P.nrow) ? ch[(int64_t)c * P.nrow + r0 + r] : 0.0;
-      }
-      double v[kPer];
+  for (int u = 0; u < kFwdRowsPerWave; ++u)
+    v[u] = src[(int64_t)idx[min(rb + u, rlast)] * 16];
+  double* dst = part + (P.poff + (int64_t)(blockIdx.x - P.chunk0) * P.ncol) * nb;
+  for (int c0 = 0; c0 < P.ncol; c0 += kFwdCols) {
+    const int nc = min(kFwdCols, P.ncol - c0);
+    if (c0 > 0) __syncthreads();  // the previous pass's sums are read
+    for (int e = tid; e < kFwdCols * kFwdRows; e += kFwdWaves * 64) {
+      const int r = e / kFwdCols, c = e % kFwdCols;
+      sh[e] = (c < nc && r < C.nr)
+                  ? chi[P.choff + (int64_t)(c0 + c) * P.nrow + C.r0 + r]
+                  : 0.0;
+    }
+    __syncthreads();
+    double acc[kFwdCols];
 #pragma unroll
-      for (int u = 0; u < kPer; ++u) {
-        const int r = r0 + w + kNlocWaves * u;
-        v[u] = (bok && r < P.nrow) ? src[(int64_t)idx[r] * 16] : 0.0;
+    for (int c = 0; c < kFwdCols; ++c) acc[c] = 0.0;
+    const double2* ch =
+        reinterpret_cast<const double2*>(sh + w * kFwdRowsPerWave * kFwdCols);
+#pragma unroll
+    for (int u = 0; u < kFwdRowsPerWave; ++u) {
+#pragma unroll
+      for (int q = 0; q < kFwdCols / 2; ++q) {
+        const double2 x = ch[u * (kFwdCols / 2) + q];
+        acc[2 * q] = fma(x.x, v[u], acc[2 * q]);
+        acc[2 * q + 1] = fma(x.y, v[u], acc[2 * q + 1]);
       }
+    }
+    // The sums are complete here: without this the compiler sinks the
+    // second half's products below the first half's reduction and keeps
+    // (spills) their chi values across it.
+#pragma unroll
+    for (int c = 0; c < kFwdCols; ++c) asm volatile("" : "+v"(acc[c]));
+#pragma unroll
+    for (int h = 0; h < kFwdCols / 8; ++h) {
+      __syncthreads();  // chi (h = 0) or the previous half's sums are read
+#pragma unroll
+      for (int c = 0; c < 8; ++c) sh[(w * 8 + c) * 64 + lane] = acc[h * 8 + c];
       __syncthreads();
+      const int c = h * 8 + w;  // wave w adds up column h * 8 + w
+      if (c < nc) {
+        double sum = sh[(c % 8) * 64 + lane];
 #pragma unroll
-      for (int u = 0; u < kPer; ++u)
-#pragma unroll
-        for (int c = 0; c < kNlocCols; ++c)
-          acc[c] = fma(chs[w + kNlocWaves * u][c], v[u], acc[c]);
-    }
-#pragma unroll
-    for (int c = 0; c < kNlocCols; ++c) part[w][c][lane] = acc[c];
-    __syncthreads();
-    for (int c = w; c < nc; c += kNlocWaves) {
-      double sum = part[0][c][lane];
-#pragma unroll
-      for (int q = 1; q < kNlocWaves; ++q) sum += part[q][c][lane];
-      if (bok) {
-        const int64_t col = P.coff + c0 + c;
-        X[col * nb + b] = (dv * sum) * factor[col];
+        for (int q = 1; q < kFwdWaves; ++q)
+          sum += sh[(q * 8 + c % 8) * 64 + lane];
+        if (b < nb) dst[(int64_t)(c0 + c) * nb + b] = sum;
       }
     }
-    __syncthreads();
   }
 }
 
-// One lane per band, 64 consecutive bands per wave (a 16-band panel row is
-// 128 contiguous bytes).  Grid point u (sorted list `point`) receives the rows ptr[u] .. ptr[u+1]
-// of temp in projector order.
+// temp[roff_p + r][b] = sum_c chi_p[r][c] X_p[c][b] for the chunk's rows,
+// with X_p[c][b] = (dv sum_j part[poff_p + j ncol_p + c][b]) factor (j in
+// chunk order) formed in registers for this lane's band.  Wave w takes the
+// chunk's rows w * 32 .. w * 32 + 31; chi is staged in LDS as in the
+// forward pass and read as broadcasts.
+__global__ __launch_bounds__(kFwdWaves * 64) void nloc_backward_kernel(
+    const double* __restrict__ part, const NlocProj* __restrict__ proj,
+    const NlocChunk* __restrict__ chunks, const double* __restrict__ factor,
+    double dv, const double* __restrict__ chi, int nb,
+    double* __restrict__ temp) {
+  __shared__ __align__(16) double sh[kFwdCols * kFwdRows];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int b = blockIdx.y * 64 + lane;
+  const int bc = min(b, nb - 1);
+  const NlocChunk C = chunks[blockIdx.x];
+  const NlocProj P = proj[C.proj];
+  const int nr = min(kFwdRowsPerWave, C.nr - w * kFwdRowsPerWave);
+  double* dst = temp + (P.roff + C.r0 + w * kFwdRowsPerWave) * nb + b;
+  for (int c0 = 0; c0 < P.ncol; c0 += kFwdCols) {
+    const int nc = min(kFwdCols, P.ncol - c0);
+    // The 16 columns' parts of one chunk are loaded together, chunk after
+    // chunk (each column's sum in chunk order).
+    double xv[kFwdCols];
+#pragma unroll
+    for (int c = 0; c < kFwdCols; ++c) xv[c] = 0.0;
+    const double* src = part + P.poff * nb + bc;
+    for (int j = 0; j < P.nchunks; ++j) {
+      double x[kFwdCols];
+#pragma unroll
+      for (int c = 0; c < kFwdCols; ++c)
+        x[c] = src[((int64_t)j * P.ncol + min(c0 + c, P.ncol - 1)) * nb];
+#pragma unroll
+      for (int c = 0; c < kFwdCols; ++c) xv[c] += x[c];
+    }
+#pragma unroll
+    for (int c = 0; c < kFwdCols; ++c)
+      xv[c] = (dv * xv[c]) * factor[P.coff + min(c0 + c, P.ncol - 1)];
+    if (c0 > 0) __syncthreads();  // the previous pass's chi is read
+    for (int e = tid; e < kFwdCols * kFwdRows; e += kFwdWaves * 64) {
+      const int r = e / kFwdCols, c = e % kFwdCols;
+      sh[e] = (c < nc && r < C.nr)
+                  ? chi[P.choff + (int64_t)(c0 + c) * P.nrow + C.r0 + r]
+                  : 0.0;
+    }
+    __syncthreads();
+    // One row at a time (its 16 chi values are eight 16-byte broadcasts);
+    // a projector of more than 16 columns continues from the sums its
+    // earlier passes stored.
+    const double2* ch =
+        reinterpret_cast<const double2*>(sh + w * kFwdRowsPerWave * kFwdCols);
+    for (int u = 0; u < nr; ++u) {
+      double t = (c0 > 0 && b < nb) ? dst[(int64_t)u * nb] : 0.0;
+#pragma unroll
+      for (int q = 0; q < kFwdCols / 2; ++q) {
+        const double2 x = ch[u * (kFwdCols / 2) + q];
+        t = fma(x.x, xv[2 * q], t);
+        t = fma(x.y, xv[2 * q + 1], t);
+      }
+      if (b < nb) dst[(int64_t)u * nb] = t;
+    }
+  }
+}
+
+// result[g][b] += the temp rows of grid point g in projector order.  One
+// lane per band, a wave per (grid point, 64 bands); grid point u (sorted
+// list `point`) receives rows rows_of[ptr[u] .. ptr[u+1]), loaded four at a
+// time from clamped positions and added in order.
 __global__ void nloc_scatter_kernel(const double* __restrict__ temp, int nb,
                                     const int* __restrict__ point,
                                     const int* __restrict__ ptr,
@@ -115,16 +211,25 @@ __global__ void nloc_scatter_kernel(const double* __restrict__ temp, int nb,
                                     int64_t npoints, int Nd, int tshift,
                                     double* __restrict__ result) {
   const int b = blockIdx.x * 64 + (threadIdx.x & 63);
-  if (b >= nb) return;
+  const int bc = min(b, nb - 1);
   const int T = 1 << tshift;
-  for (int64_t u = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6); u < npoints;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int64_t u = (int64_t)blockIdx.y * 4 + wv; u < npoints;
        u += (int64_t)gridDim.y * 4) {
     double* dst =
-        result + ((int64_t)(b >> tshift) * Nd + point[u]) * T + (b & (T - 1));
+        result + ((int64_t)(bc >> tshift) * Nd + point[u]) * T + (bc & (T - 1));
     double acc = *dst;
-    for (int j = ptr[u]; j < ptr[u + 1]; ++j)
-      acc += temp[(int64_t)rows_of[j] * nb + b];
-    *dst = acc;
+    const int j0 = ptr[u], j1 = ptr[u + 1];
+    for (int j = j0; j < j1; j += 4) {
+      double x[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        x[k] = temp[(int64_t)rows_of[min(j + k, j1 - 1)] * nb + bc];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (j + k < j1) acc += x[k];
+    }
+    if (b < nb) *dst = acc;
   }
 }
 
@@ -141,36 +246,28 @@ using namespace skg;
 struct skgemm_nloc {
   skgemm_ctx* ctx = nullptr;
   int Nd = 0, num_proj = 0;
-  std::vector<int> nrow, ncol;
-  std::vector<int64_t> roff, coff, choff;  // row, column, chi offsets
-  int64_t rows = 0, cols = 0;
+  int64_t rows = 0, cols = 0, nchunks = 0, parts = 0;
   // device copies
   int* index = nullptr;       // rows
   double* chi = nullptr;      // sum nrow * ncol
   double* factor = nullptr;   // cols
-  NlocProj* proj = nullptr;   // num_proj (rows, chi and column offsets)
+  NlocProj* proj = nullptr;   // num_proj
+  NlocChunk* chunks = nullptr;  // nchunks
   int* point = nullptr;       // npoints
   int* ptr = nullptr;         // npoints + 1
-  int* rows_of = nullptr;     // rows
+  int* rows_of = nullptr;     // rows touching each point, in order
   int64_t npoints = 0;
   // workspace, grown with nb
-  double* X = nullptr;
-  double* temp = nullptr;
+  double* part = nullptr;     // parts x nb
+  double* temp = nullptr;     // rows x nb
   int ws_nb = 0;
-  // The two grouped products, described for `table_nb` bands and kept in
-  // device memory (their rows depend on nb and the workspace only).
-  PlainTable bw;
-  Problem* d_bw = nullptr;
-  int64_t* d_hdr = nullptr;  // {rows, iterations} of bw
-  int table_nb = -1, table_cfg = -2, table_grid = -2;
 };
 
 static void nloc_free(skgemm_nloc* h) {
   for (void* p : {(void*)h->index, (void*)h->chi, (void*)h->factor,
-                  (void*)h->proj,
-                  (void*)h->point, (void*)h->ptr, (void*)h->rows_of,
-                  (void*)h->X, (void*)h->temp, (void*)h->d_bw,
-                  (void*)h->d_hdr})
+                  (void*)h->proj, (void*)h->chunks, (void*)h->point,
+                  (void*)h->ptr, (void*)h->rows_of, (void*)h->part,
+                  (void*)h->temp})
     if (p) hipFree(p);
 }
 
@@ -201,69 +298,76 @@ extern "C" int skgemm_nloc_create(skgemm_ctx* c, int Nd, int num_proj,
   if (num_proj > 0 && (nrow == nullptr || ncol == nullptr ||
                        atom_index == nullptr))
     return set_error(SKGEMM_ERR_ARG, "nloc: nrow, ncol, atom_index needed");
+  std::vector<int64_t> roff(num_proj + 1, 0), coff(num_proj + 1, 0),
+      choff(num_proj + 1, 0);
+  for (int p = 0; p < num_proj; ++p) {
+    if (nrow[p] < 0 || ncol[p] < 0)
+      return set_error(SKGEMM_ERR_ARG, "nloc: projector %d has a negative size",
+                       p);
+    roff[p + 1] = roff[p] + nrow[p];
+    coff[p + 1] = coff[p] + ncol[p];
+    choff[p + 1] = choff[p] + (int64_t)nrow[p] * ncol[p];
+  }
+  const int64_t rows = roff[num_proj], cols = coff[num_proj];
+  if (rows >= INT32_MAX || cols >= INT32_MAX)
+    return set_error(SKGEMM_ERR_NOTSUP, "nloc: more than 2^31 projector rows");
+  if ((rows > 0 && index_all == nullptr) ||
+      (choff[num_proj] > 0 && chi_all == nullptr) ||
+      (cols > 0 && gamma_all == nullptr))
+    return set_error(SKGEMM_ERR_ARG, "nloc: index, chi or gamma is NULL");
+  for (int64_t R = 0; R < rows; ++R)
+    if (index_all[R] < 0 || index_all[R] >= Nd)
+      return set_error(SKGEMM_ERR_ARG, "nloc: grid index %d out of [0, %d)",
+                       index_all[R], Nd);
   auto* h = new skgemm_nloc();
   h->ctx = c;
   h->Nd = Nd;
   h->num_proj = num_proj;
-  h->roff.assign(num_proj + 1, 0);
-  h->coff.assign(num_proj + 1, 0);
-  h->choff.assign(num_proj + 1, 0);
-  for (int p = 0; p < num_proj; ++p) {
-    if (nrow[p] < 0 || ncol[p] < 0) {
-      delete h;
-      return set_error(SKGEMM_ERR_ARG, "nloc: projector %d has a negative size",
-                       p);
-    }
-    h->nrow.push_back(nrow[p]);
-    h->ncol.push_back(ncol[p]);
-    h->roff[p + 1] = h->roff[p] + nrow[p];
-    h->coff[p + 1] = h->coff[p] + ncol[p];
-    h->choff[p + 1] = h->choff[p] + (int64_t)nrow[p] * ncol[p];
-  }
-  h->rows = h->roff[num_proj];
-  h->cols = h->coff[num_proj];
-  if (h->rows >= INT32_MAX || h->cols >= INT32_MAX) {
-    delete h;
-    return set_error(SKGEMM_ERR_NOTSUP, "nloc: more than 2^31 projector rows");
-  }
-  if ((h->rows > 0 && index_all == nullptr) ||
-      (h->choff[num_proj] > 0 && chi_all == nullptr) ||
-      (h->cols > 0 && gamma_all == nullptr)) {
-    delete h;
-    return set_error(SKGEMM_ERR_ARG, "nloc: index, chi or gamma is NULL");
-  }
-  for (int64_t R = 0; R < h->rows; ++R)
-    if (index_all[R] < 0 || index_all[R] >= Nd) {
-      delete h;
-      return set_error(SKGEMM_ERR_ARG, "nloc: grid index %d out of [0, %d)",
-                       index_all[R], Nd);
-    }
+  h->rows = rows;
+  h->cols = cols;
   // Row factors: gamma for the first projector of each atom (the reference
   // starts from the first projector's atom_index - 1), 1.0 elsewhere.
-  std::vector<double> factor((size_t)h->cols, 1.0);
+  std::vector<double> factor((size_t)cols, 1.0);
   if (num_proj > 0) {
     int atom = atom_index[0] - 1;
     for (int p = 0; p < num_proj; ++p) {
       if (atom == atom_index[p]) continue;
       atom = atom_index[p];
       for (int j = 0; j < ncol[p]; ++j)
-        factor[h->coff[p] + j] = gamma_all[h->coff[p] + j];
+        factor[coff[p] + j] = gamma_all[coff[p] + j];
     }
   }
+  // Projector table and forward chunks (projectors without columns or rows
+  // get none: they contribute nothing).
+  std::vector<NlocProj> meta((size_t)num_proj);
+  std::vector<NlocChunk> chunks;
+  int64_t poff = 0;
+  for (int p = 0; p < num_proj; ++p) {
+    NlocProj m{roff[p], choff[p], coff[p], nrow[p], ncol[p], 0, 0, poff};
+    m.chunk0 = (int)chunks.size();
+    if (ncol[p] > 0)
+      for (int r0 = 0; r0 < nrow[p]; r0 += kFwdRows)
+        chunks.push_back(NlocChunk{p, r0, std::min(kFwdRows, nrow[p] - r0)});
+    m.nchunks = (int)chunks.size() - m.chunk0;
+    poff += (int64_t)m.nchunks * ncol[p];
+    meta[p] = m;
+  }
+  h->nchunks = (int64_t)chunks.size();
+  h->parts = poff;
   // Scatter table: grid points in ascending order, each with its rows in
-  // projector order (then row order), skipping projectors without columns
-  // (they contribute nothing).
+  // projector order (then row order), skipping projectors without columns.
   std::vector<std::pair<int, int64_t>> touch;
-  touch.reserve((size_t)h->rows);
+  touch.reserve((size_t)rows);
   for (int p = 0; p < num_proj; ++p) {
     if (ncol[p] == 0) continue;
     for (int r = 0; r < nrow[p]; ++r)
-      touch.push_back({index_all[h->roff[p] + r], h->roff[p] + r});
+      touch.push_back({index_all[roff[p] + r], roff[p] + r});
   }
   std::stable_sort(touch.begin(), touch.end(),
                    [](const std::pair<int, int64_t>& a,
                       const std::pair<int, int64_t>& b) { return a.first < b.first; });
   std::vector<int> point, ptr(1, 0), rows_of;
+  rows_of.reserve(touch.size());
   for (size_t i = 0; i < touch.size(); ++i) {
     if (i == 0 || touch[i].first != touch[i - 1].first) {
       point.push_back(touch[i].first);
@@ -277,80 +381,30 @@ extern "C" int skgemm_nloc_create(skgemm_ctx* c, int Nd, int num_proj,
                ? SKGEMM_OK
                : set_error(SKGEMM_ERR_HIP, "hipSetDevice");
   if (rc == SKGEMM_OK)
-    rc = upload(&h->index, index_all, (size_t)h->rows, "index", c->stream);
+    rc = upload(&h->index, index_all, (size_t)rows, "index", c->stream);
   if (rc == SKGEMM_OK)
-    rc = upload(&h->chi, chi_all, (size_t)h->choff[num_proj], "chi", c->stream);
+    rc = upload(&h->chi, chi_all, (size_t)choff[num_proj], "chi", c->stream);
   if (rc == SKGEMM_OK)
     rc = upload(&h->factor, factor.data(), factor.size(), "gamma", c->stream);
-  std::vector<NlocProj> meta((size_t)num_proj);
-  for (int p = 0; p < num_proj; ++p)
-    meta[p] = NlocProj{h->roff[p], h->choff[p], h->coff[p], nrow[p], ncol[p]};
   if (rc == SKGEMM_OK)
     rc = upload(&h->proj, meta.data(), meta.size(), "projector table",
                 c->stream);
-  if (rc == SKGEMM_OK) rc = upload(&h->point, point.data(), point.size(), "points", c->stream);
+  if (rc == SKGEMM_OK)
+    rc = upload(&h->chunks, chunks.data(), chunks.size(), "chunk table",
+                c->stream);
+  if (rc == SKGEMM_OK)
+    rc = upload(&h->point, point.data(), point.size(), "points", c->stream);
   if (rc == SKGEMM_OK && h->npoints > 0)
     rc = upload(&h->ptr, ptr.data(), ptr.size(), "point table", c->stream);
   if (rc == SKGEMM_OK)
-    rc = upload(&h->rows_of, rows_of.data(), rows_of.size(), "row table", c->stream);
+    rc = upload(&h->rows_of, rows_of.data(), rows_of.size(), "row list",
+                c->stream);
   if (rc != SKGEMM_OK) {
     nloc_free(h);
     delete h;
     return rc;
   }
   *out = h;
-  return SKGEMM_OK;
-}
-
-// Describe the backward products temp_p (nrow x nb, row-major) = chi_p
-// X_p (chi_p column-major nrow x ncol, X_p row-major ncol x nb) and upload
-// their table.
-static int nloc_tables(skgemm_nloc* h, int nb) {
-  skgemm_ctx* c = h->ctx;
-  std::vector<skgemm_problem> bw;
-  for (int p = 0; p < h->num_proj; ++p) {
-    const int nr = h->nrow[p], nc = h->ncol[p];
-    if (nc == 0 || nr == 0) continue;
-    skgemm_problem g{};
-    g.kind = 0;
-    g.layout_a = SKGEMM_COL_MAJOR;
-    g.layout_b = SKGEMM_ROW_MAJOR;
-    g.layout_d = SKGEMM_ROW_MAJOR;
-    g.M = nr;
-    g.N = nb;
-    g.K = nc;
-    g.alpha = 1.0;
-    g.A = h->chi + h->choff[p];
-    g.lda = nr;
-    g.B = h->X + h->coff[p] * nb;
-    g.ldb = nb;
-    g.D = h->temp + h->roff[p] * nb;
-    g.ldd = nb;
-    bw.push_back(g);
-  }
-  h->table_nb = -1;
-  SKG_TRY(build_plain_table(c, bw.data(), (int)bw.size(), &h->bw));
-  SKG_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-  if (h->d_bw) hipFree(h->d_bw);
-  h->d_bw = nullptr;
-  const size_t nbk = std::max<size_t>(h->bw.rows.size(), 1);
-  if (hipMalloc(&h->d_bw, nbk * sizeof(Problem)) != hipSuccess ||
-      (h->d_hdr == nullptr &&
-       hipMalloc(&h->d_hdr, 2 * sizeof(int64_t)) != hipSuccess))
-    return set_error(SKGEMM_ERR_NOMEM, "nloc problem table");
-  const int64_t hdr[2] = {(int64_t)h->bw.rows.size(), h->bw.iters};
-  // On the context's stream, completed before returning (hdr is local).
-  SKG_HIP(hipMemcpyAsync(h->d_bw, h->bw.rows.data(),
-                         h->bw.rows.size() * sizeof(Problem),
-                         hipMemcpyHostToDevice, c->stream),
-          "nloc table upload");
-  SKG_HIP(hipMemcpyAsync(h->d_hdr, hdr, sizeof(hdr), hipMemcpyHostToDevice,
-                         c->stream),
-          "nloc table upload");
-  SKG_HIP(hipStreamSynchronize(c->stream), "nloc table upload");
-  h->table_nb = nb;
-  h->table_cfg = c->forced_cfg;
-  h->table_grid = c->forced_grid;
   return SKGEMM_OK;
 }
 
@@ -372,28 +426,29 @@ extern "C" int skgemm_nloc_apply(skgemm_nloc* h, int nb, double dv,
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
   if (nb > h->ws_nb) {
     SKG_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-    for (double** p : {&h->X, &h->temp}) {
+    for (double** p : {&h->part, &h->temp}) {
       if (*p) hipFree(*p);
       *p = nullptr;
     }
     h->ws_nb = 0;
-    if (hipMalloc(&h->X, (size_t)h->cols * nb * sizeof(double)) != hipSuccess ||
+    if (hipMalloc(&h->part, (size_t)std::max<int64_t>(h->parts, 1) * nb *
+                                sizeof(double)) != hipSuccess ||
         hipMalloc(&h->temp, (size_t)h->rows * nb * sizeof(double)) != hipSuccess)
       return set_error(SKGEMM_ERR_NOMEM, "nloc workspace for %d bands", nb);
     h->ws_nb = nb;
     ++c->mem_gen;
   }
-  if (nb != h->table_nb || h->ws_nb != nb || c->forced_cfg != h->table_cfg ||
-      c->forced_grid != h->table_grid) {
-    SKG_TRY(nloc_tables(h, nb));
-  }
-  hipLaunchKernelGGL(nloc_project_kernel,
-                     dim3((unsigned)h->num_proj, (nb + 63) / 64),
-                     dim3(kNlocWaves * 64), 0,
+  const unsigned bgroups = (unsigned)((nb + 63) / 64);
+  if (h->nchunks == 0) return SKGEMM_OK;  // no projector has columns
+  const dim3 cgrid((unsigned)h->nchunks, bgroups);
+  hipLaunchKernelGGL(nloc_forward_kernel, cgrid, dim3(kFwdWaves * 64), 0,
                      c->stream, psi16, h->Nd, nb, h->index, h->chi, h->proj,
-                     h->factor, dv, h->X);
+                     h->chunks, h->part);
   SKG_HIP(hipGetLastError(), "nloc forward launch");
-  SKG_TRY(launch_plain_table(c, h->bw, h->d_bw, h->d_hdr));
+  hipLaunchKernelGGL(nloc_backward_kernel, cgrid, dim3(kFwdWaves * 64), 0,
+                     c->stream, h->part, h->proj, h->chunks, h->factor, dv,
+                     h->chi, nb, h->temp);
+  SKG_HIP(hipGetLastError(), "nloc backward launch");
   hipLaunchKernelGGL(nloc_scatter_kernel, rows_grid(nb, h->npoints), dim3(256),
                      0, c->stream, h->temp, nb, h->point, h->ptr, h->rows_of,
                      h->npoints, h->Nd, result_tile == 32 ? 5 : 4, result);
