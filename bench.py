@@ -663,7 +663,28 @@ def bench_serving(skg, ctx, c4_tflops, seconds, precision):
     return out
 
 
-def bench_int8_library(reps):
+def bench_int8_library(reps, timeout_s=180):
+    """The vendor library's INT8 GEMM, measured in a child process: torch's
+    own HIP runtime does not find the device in a process where this
+    library's runtime already holds it, so the child imports torch alone.
+    A failure of the child is recorded, never fatal to the bench."""
+    cmd = [sys.executable, os.path.abspath(__file__), "--int8-library-child",
+           "--int8-reps", str(reps)]
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True,
+                           timeout=timeout_s)
+    except subprocess.TimeoutExpired:
+        return {"error": "timed out after %d s" % timeout_s}
+    if r.returncode != 0:
+        return {"error": "child exit %d: %s" % (r.returncode,
+                                                 r.stderr.strip()[-300:])}
+    try:
+        return json.loads(r.stdout.strip().splitlines()[-1])
+    except (ValueError, IndexError):
+        return {"error": "unreadable child output: %s" % r.stdout[-300:]}
+
+
+def int8_library_child(reps):
     """The vendor library's INT8 GEMM (torch._int_mm, hipBLASLt on ROCm) on
     shapes beside the split form's products: one 1024 x 1024 x 65536
     product, the MACs of 14 of them stacked (14336 x 1024 x 65536), and a
@@ -719,7 +740,14 @@ def main():
                     help="autotuner cache file: read if it exists (no "
                          "tuning runs), else written by the tuning runs")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--int8-library-child", action="store_true",
+                    help=argparse.SUPPRESS)
+    ap.add_argument("--int8-reps", type=int, default=10,
+                    help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.int8_library_child:
+        print(json.dumps(int8_library_child(args.int8_reps)))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(a row-sharded SYRK would add one all-reduce of Ms; see the "
@@ -770,39 +798,48 @@ def main():
     extra = {}
     sections = set() if args.no_extra else set(
         x for x in args.extra.split(",") if x)
+    def section(key, label, fn):
+        # A section that raises is recorded with its error; the headline
+        # line is printed whatever an extra does.
+        log(label + " ...")
+        try:
+            extra[key] = fn()
+        except Exception as e:  # noqa: BLE001
+            log("%s failed: %s" % (key, e))
+            extra[key] = {"error": str(e)[:300]}
+
     if "c2" in sections:
-        log("c2 step (SYRK + rotation, 32768 x 520) ...")
-        extra["c2_step"] = bench_c2_step(skg, ctx, 20, 5, args.tuning_cache)
+        section("c2_step", "c2 step (SYRK + rotation, 32768 x 520)",
+                lambda: bench_c2_step(skg, ctx, 20, 5, args.tuning_cache))
     if "c1" in sections:
-        log("c1 GEMM 1024^3 ...")
-        extra["c1_gemm"] = bench_c1(skg, ctx, 20)
+        section("c1_gemm", "c1 GEMM 1024^3", lambda: bench_c1(skg, ctx, 20))
     if "rotation" in sections:
-        log("c3 rotation, plain and split ...")
-        extra["c3_rotation"] = bench_split_rotation(skg, ctx, 10)
+        section("c3_rotation", "c3 rotation, plain and split",
+                lambda: bench_split_rotation(skg, ctx, 10))
     if "projection" in sections:
-        log("projection GEMM (616x616x175616) ...")
-        extra["projection"] = bench_projection(skg, ctx, 10)
+        section("projection", "projection GEMM (616x616x175616)",
+                lambda: bench_projection(skg, ctx, 10))
     if "nloc" in sections:
-        log("nloc projectors (c2 grid) ...")
-        extra["nloc"] = bench_nloc(skg, ctx, 20)
+        section("nloc", "nloc projectors (c2 grid)",
+                lambda: bench_nloc(skg, ctx, 20))
     if "stencil" in sections:
-        log("stencil (32^3, 520 bands) ...")
-        extra["stencil"] = bench_stencil(skg, ctx, 10)
+        section("stencil", "stencil (32^3, 520 bands)",
+                lambda: bench_stencil(skg, ctx, 10))
     if "layouts" in sections:
-        log("layout conversions (65536x1024) ...")
-        extra["layouts"] = bench_layouts(skg, ctx, 10)
+        section("layouts", "layout conversions (65536x1024)",
+                lambda: bench_layouts(skg, ctx, 10))
     if "int8_library" in sections:
-        log("library INT8 GEMM for scale ...")
-        extra["int8_library"] = bench_int8_library(10)
+        section("int8_library", "library INT8 GEMM for scale (child process)",
+                lambda: bench_int8_library(10))
     if "batched" in sections or "serving" in sections:
-        log("batched (64 systems) ...")
-        extra["batched"] = bench_batched(skg, ctx, 5)
+        section("batched", "batched (64 systems)",
+                lambda: bench_batched(skg, ctx, 5))
     latency = {"c3_split_device_ms_p50": h["split_device_ms_p50"],
                "c3_split_device_ms_p99": h["split_device_ms_p99"],
                "c3_plain_device_ms_p50": h["plain_device_ms_p50"],
                "c3_plain_device_ms_p99": h["plain_device_ms_p99"],
                "source": "HIP events per call on the launch stream"}
-    if "serving" in sections:
+    if "serving" in sections and "error" not in extra.get("batched", {}):
         log("serving (200 systems, Poisson), plain FP64 ...")
         rate_tflops = extra["batched"]["plain_fp64_graph"]["tflops"]
         sp = bench_serving(skg, ctx, rate_tflops, args.serve_seconds, skg.FP64)
