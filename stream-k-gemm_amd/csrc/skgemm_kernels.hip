@@ -902,18 +902,24 @@ __global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
       }
     };
     // acc (+)= the partial in `slot`; `first` assigns instead of adding.
+    // The four loads of an accumulator block are issued before any of them
+    // is added (one round trip per block, not per element).
     auto gather = [&](gdouble* slot, bool first) {
 #pragma unroll
       for (int i = 0; i < MB; ++i)
 #pragma unroll
-        for (int j = 0; j < NB; ++j)
+        for (int j = 0; j < NB; ++j) {
+          double x[4];
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const double x = __hip_atomic_load(
+          for (int v = 0; v < 4; ++v)
+            x[v] = __hip_atomic_load(
                 slot + ((i * NB + j) * 4 + v) * NT + tid, __ATOMIC_RELAXED,
                 __HIP_MEMORY_SCOPE_AGENT);
-            acc[i][j][v] = first ? x : acc[i][j][v] + x;
-          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            acc[i][j][v] = first ? x[v] : acc[i][j][v] + x[v];
+        }
       if (XS != 0 && (xt || yt)) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {

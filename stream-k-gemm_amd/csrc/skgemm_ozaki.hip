@@ -620,9 +620,13 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
   constexpr int RPD = 1024 / BK;                       // rows per DMA
   constexpr int NDA = BM / RPD, NDB = BN / RPD;        // DMAs per stage
   constexpr int DPW = (NDA + NDB) / (WM * WN);         // DMAs per wave
-  // PD >= 3: the same with PD LDS stages (PD - 1 steps in flight).
-  constexpr bool DMA = PD == 0 || PD >= 3;
-  constexpr int NS = PD == 0 ? 2 : PD;  // LDS stages of the DMA path
+  // PD 3, 4: the same with PD LDS stages (PD - 1 steps in flight).
+  // PD 8..11: PD - 6 LDS stages (steps issued PD - 7 steps ahead), the
+  // barrier that publishes step k + 1 placed before the last sub-step of
+  // step k (fragments read across it).
+  constexpr bool EB = PD >= 8 && PD <= 11;
+  constexpr bool DMA = PD == 0 || PD == 3 || PD == 4 || EB;
+  constexpr int NS = (PD == 3 || PD == 4) ? PD : EB ? PD - 6 : 2;
   static_assert(!DMA || (NDA + NDB) % (WM * WN) == 0, "DMA split");
   int dma_off[DMA ? DPW : 1];
   if (DMA) {
@@ -751,7 +755,103 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
       __syncthreads();
       return;
     }
-    if constexpr (PD >= 3) {
+    if constexpr (EB) {
+      // NS stages hold steps k .. k + NS - 1; step k + NS is issued into
+      // step k's stage once every wave has read all of step k.  The
+      // barrier that publishes step k + 1 (and frees step k's stage, whose
+      // fragments are all in registers by then) comes before the last
+      // sub-step of step k, so that sub-step's MFMAs keep the matrix core
+      // busy across it while step k + 1's first fragments are read.
+      // Issues past the unit's end are clamped to its last step (in
+      // bounds, never read), so exactly (NS - 2) DPW copies are younger
+      // than step k + 1's at its barrier.
+      constexpr int S = BK / 32;  // MFMA sub-steps per step
+      static_assert(S >= 2 && S % 2 == 0, "sub-steps");
+      constexpr int kAhead = (NS - 2) * DPW;
+      constexpr int kWait = (kAhead & 15) | ((kAhead >> 4) << 14) |
+                            (7 << 4) | (0 << 8);  // vmcnt(kAhead) lgkmcnt(0)
+      auto issue = [&](int buf, int step) {
+        step = step < ke ? step : ke - 1;
+        const __amdgpu_buffer_rsrc_t sa = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<int8_t*>(baseA + (int64_t)step * BK), (short)0,
+            kNumRecords, kRsrcWord3);
+        const __amdgpu_buffer_rsrc_t sb = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<int8_t*>(baseB + (int64_t)step * BK), (short)0,
+            kNumRecords, kRsrcWord3);
+        int8_t* base = lds + buf * (BM + BN) * BK;
+#pragma unroll
+        for (int d = 0; d < DPW; ++d) {
+          const int q = wave * DPW + d;  // wave-uniform
+          if (q < NDA)
+            lds_dma16(sa, base + q * 1024, dma_off[d]);
+          else
+            lds_dma16(sb, base + BM * BK + (q - NDA) * 1024, dma_off[d]);
+        }
+      };
+      auto frag = [&](v4i(&a)[MB], v4i(&b)[NB], int buf, int s) {
+        const int8_t* la = lds + buf * (BM + BN) * BK;
+        const int8_t* lb = la + BM * BK;
+        const int c = 2 * s + (lane >> 5);
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+          a[i] = *reinterpret_cast<const v4i*>(
+              la + lds_off<BK>(wm * TM + i * 32 + (lane & 31), c));
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+          b[j] = *reinterpret_cast<const v4i*>(
+              lb + lds_off<BK>(wn * TN + j * 32 + (lane & 31), c));
+      };
+      auto mma = [&](const v4i(&a)[MB], const v4i(&b)[NB]) {
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                a[i], b[j], acc[i][j], 0, 0, 0);
+      };
+      v4i fa0[MB], fb0[NB], fa1[MB], fb1[NB];
+#pragma unroll
+      for (int s0 = 0; s0 < NS; ++s0) issue(s0, kk + s0);
+      // step kk landed: the NS - 1 younger steps' copies may still fly
+      {
+        constexpr int kFirst = (NS - 1) * DPW;
+        constexpr int kW0 = (kFirst & 15) | ((kFirst >> 4) << 14) |
+                            (7 << 4) | (15 << 8);
+        __builtin_amdgcn_s_waitcnt(kW0);
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+      }
+      frag(fa0, fb0, 0, 0);
+      int stage = 0;
+      for (; kk < ke; ++kk) {
+#pragma unroll
+        for (int s = 0; s + 1 < S; ++s) {
+          if (s % 2 == 0) {
+            frag(fa1, fb1, stage, s + 1);
+            if (!idle) mma(fa0, fb0);
+          } else {
+            frag(fa0, fb0, stage, s + 1);
+            if (!idle) mma(fa1, fb1);
+          }
+        }
+        // Step k + 1's copies and this wave's fragment reads are done;
+        // after the barrier everyone's are.
+        __builtin_amdgcn_s_waitcnt(kWait);
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        const int next = stage + 1 == NS ? 0 : stage + 1;
+        if (kk + 1 < ke) frag(fa0, fb0, next, 0);
+        if (!idle) mma(fa1, fb1);  // sub-step S - 1 (S even)
+        issue(stage, kk + NS);
+        stage = next;
+      }
+      __builtin_amdgcn_s_waitcnt(0);  // clamped tail copies
+      __syncthreads();
+      return;
+    }
+    if constexpr (PD == 3 || PD == 4) {
       // NS stages: steps kk + 1 .. kk + NS - 1 are in flight while step kk
       // is multiplied out.  Steps past the unit's end are clamped to its
       // last step (in bounds, never read), so exactly (NS - 2) DPW copies
@@ -971,15 +1071,22 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
         __syncthreads();
         const __amdgpu_buffer_rsrc_t slot = __builtin_amdgcn_make_buffer_rsrc(
             G.ws + (int64_t)w2 * (BM * BN), (short)0, kNumRecords, kRsrcWord3);
+        // One accumulator block's loads are all issued before any is
+        // added (one round trip per block).
 #pragma unroll
         for (int i = 0; i < MB; ++i)
 #pragma unroll
-          for (int j = 0; j < NB; ++j)
+          for (int j = 0; j < NB; ++j) {
+            int x[16];
 #pragma unroll
             for (int v = 0; v < 16; ++v)
-              acc[i][j][v] += __builtin_amdgcn_raw_buffer_load_b32(
+              x[v] = __builtin_amdgcn_raw_buffer_load_b32(
                   slot, tid * 4, ((i * NB + j) * 16 + v) * NT * 4,
                   kAgentCoherent);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][j][v] += x[v];
+          }
       }
     }
     if (write_out) write_residues(l, l, m0, n0);
@@ -1195,6 +1302,18 @@ struct GemmCfg {
 // 7: 128x128, 4 waves of 64x64,  BK 64,  LDS DMA, 4 stages (64 KB LDS)
 // 8: 128x128, 4 waves of 64x64,  BK 128, LDS DMA, 3 stages (96 KB LDS)
 // 9: 256x256, 8 waves of 64x128, BK 64,  LDS DMA, 3 stages (96 KB LDS)
+// Early barrier (the barrier publishing step k + 1 sits before the last
+// sub-step of step k):
+// 10: 256x256, 8 waves of 128x64, BK 128, LDS DMA, 2 stages (128 KB LDS)
+// 11: 256x256, 8 waves of 64x128, BK 128, LDS DMA, 2 stages (128 KB LDS)
+// 12: 128x128, 4 waves of 64x64,  BK 128, LDS DMA, 2 stages (64 KB LDS)
+// 13: 256x256, 8 waves of 64x128, BK 64,  LDS DMA, 4 stages (128 KB LDS)
+// 14: 256x256, 8 waves of 64x128, BK 64,  LDS DMA, 5 stages (160 KB LDS)
+// 15: 128x128, 4 waves of 64x64,  BK 64,  LDS DMA, 4 stages (64 KB LDS)
+// 16: 128x128, 4 waves of 64x64,  BK 64,  LDS DMA, 5 stages (80 KB LDS)
+// 17: 256x256, 4 waves of 128x128, BK 64, LDS DMA, 4 stages (128 KB LDS)
+// 18: 256x256, 4 waves of 128x128, BK 64, LDS DMA, 5 stages (160 KB LDS)
+// 19: 256x256, 4 waves of 128x128, BK 128, LDS DMA, 2 stages (128 KB LDS)
 static const GemmCfg kGemmCfgs[] = {{256, 256, 4, 2, 64, 2},
                                     {128, 128, 2, 2, 128, 2},
                                     {128, 128, 2, 1, 128, 2},
@@ -1204,8 +1323,18 @@ static const GemmCfg kGemmCfgs[] = {{256, 256, 4, 2, 64, 2},
                                     {128, 128, 2, 2, 128, 0},
                                     {128, 128, 2, 2, 64, 4},
                                     {128, 128, 2, 2, 128, 3},
-                                    {256, 256, 4, 2, 64, 3}};
-constexpr int kNumGemmCfgs = 10;
+                                    {256, 256, 4, 2, 64, 3},
+                                    {256, 256, 2, 4, 128, 8},
+                                    {256, 256, 4, 2, 128, 8},
+                                    {128, 128, 2, 2, 128, 8},
+                                    {256, 256, 4, 2, 64, 10},
+                                    {256, 256, 4, 2, 64, 11},
+                                    {128, 128, 2, 2, 64, 10},
+                                    {128, 128, 2, 2, 64, 11},
+                                    {256, 256, 2, 2, 64, 10},
+                                    {256, 256, 2, 2, 64, 11},
+                                    {256, 256, 2, 2, 128, 8}};
+constexpr int kNumGemmCfgs = 20;
 
 #define SKG_OZ_CFGS(X)          \
   X(0, 256, 256, 4, 2, 64, 2)   \
@@ -1217,7 +1346,17 @@ constexpr int kNumGemmCfgs = 10;
   X(6, 128, 128, 2, 2, 128, 0)  \
   X(7, 128, 128, 2, 2, 64, 4)   \
   X(8, 128, 128, 2, 2, 128, 3)  \
-  X(9, 256, 256, 4, 2, 64, 3)
+  X(9, 256, 256, 4, 2, 64, 3)   \
+  X(10, 256, 256, 2, 4, 128, 8) \
+  X(11, 256, 256, 4, 2, 128, 8) \
+  X(12, 128, 128, 2, 2, 128, 8) \
+  X(13, 256, 256, 4, 2, 64, 10)  \
+  X(14, 256, 256, 4, 2, 64, 11)  \
+  X(15, 128, 128, 2, 2, 64, 10)  \
+  X(16, 128, 128, 2, 2, 64, 11)  \
+  X(17, 256, 256, 2, 2, 64, 10)  \
+  X(18, 256, 256, 2, 2, 64, 11)  \
+  X(19, 256, 256, 2, 2, 128, 8)
 
 static const void* gemm_fn(int cfg) {
 #define SKG_FN_CASE(i, bm, bn, wm, wn, bk, pd) \
@@ -1233,7 +1372,9 @@ static const void* gemm_fn(int cfg) {
 
 static size_t gemm_lds(int cfg) {
   const GemmCfg& g = kGemmCfgs[cfg];
-  const size_t stages = g.pd >= 3 ? g.pd : 2;
+  const size_t stages = (g.pd == 3 || g.pd == 4)    ? g.pd
+                        : (g.pd >= 8 && g.pd <= 11) ? g.pd - 6
+                                                    : 2;
   const size_t pipe = stages * (g.bm + g.bn) * g.bk;
   const size_t tile = (size_t)g.bn * (g.bm + 16);  // residue transpose
   return std::max(pipe, tile);
