@@ -222,8 +222,8 @@ int skgemm_dsyrk_noalloc(skgemm_ctx* ctx, int layout_a, int N, int K,
  * each scaled operand cut into `slices` INT8 residue planes modulo
  * pairwise-coprime moduli <= 256, one INT8 MFMA product per plane with
  * exact INT32 accumulation (K longer than 131072 is cut into chunks whose
- * residues are added modulo each modulus), and an exact CRT reconstruction
- * rounded once to FP64.  The result does not depend on the schedule
+ * residues are added before the CRT; K up to 32 such chunks, 4194304, else
+ * SKGEMM_ERR_NOTSUP), and an exact CRT reconstruction rounded once to FP64.  The result does not depend on the schedule
  * (bit-identical for any plan).
  * slices: 8..16 moduli; 0 = 14 (each scaled row keeps about 54 bits of its
  * norm); SKGEMM_SLICES_AUTO = certifying: 14 moduli, then more while the

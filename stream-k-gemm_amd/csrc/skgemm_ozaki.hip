@@ -532,7 +532,7 @@ struct GemmLaunch {
   int L;
   int mods[kMaxMod];
   double inv_mods[kMaxMod];
-  uint8_t* R;  // [L][N][mp] residues in [0, m), column-major
+  uint8_t* R;  // [L][N][mp] symmetric residues as signed bytes, column-major
   int64_t r_plane;
   int mp;      // column pitch of R: M rounded up to 16
   int grid;
@@ -950,8 +950,12 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
   // pitch BM + 16: dword writes of 4 rows and 16-byte reads are free of bank
   // conflicts) and stored as 16-byte row runs of a column.  Padding rows and
   // the upper part of diagonal SYRK tiles are stored too; the CRT pass
-  // never reads them.  acc is exact in FP64 and floor(acc * fl(1/m)) is off
-  // by at most one from floor(acc / m), so one correction each way suffices.
+  // never reads them.  The residue is the symmetric one, a - m rint(a / m),
+  // stored as a signed byte: acc is exact in FP64 and, for odd m, a / m
+  // lies at least 1 / (2m) from a half-integer while fl(a fl(1/m)) is
+  // within 2^-21 of it, so rint gives rint(a / m) and r lies in
+  // [-(m-1)/2, (m-1)/2]; for m = 256 any r in [-128, 128] is the byte of
+  // the right class (128 and -128 share it).
   auto write_residues = [&](int l, int out_plane, int m0, int n0) {
     const int m = G.mods[l];
     const double md = (double)m, inv = G.inv_mods[l];
@@ -968,13 +972,11 @@ __global__ __launch_bounds__(WM* WN * 64) void gemm_i8_kernel(GemmLaunch G) {
           uint32_t word = 0;
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            // a - m floor(a fl(1/m)) as one exact FMA (|m q| < 2^40): no
-            // quarter-rate 32-bit integer multiply.
+            // a - m rint(a fl(1/m)) as one exact FMA (|m q| < 2^40): no
+            // quarter-rate 32-bit integer multiply, no correction.
             const double ad = (double)acc[i][j][v4 * 4 + u];
-            int r = (int)fma(-md, floor(ad * inv), ad);
-            r += (r < 0) ? m : 0;
-            r -= (r >= m) ? m : 0;
-            word |= (uint32_t)r << (8 * u);
+            const int r = (int)fma(-md, rint(ad * inv), ad);
+            word |= ((uint32_t)r & 255u) << (8 * u);
           }
           const int row = wm * TM + i * 32 + v4 * 8 + (lane >> 5) * 4;
           *reinterpret_cast<uint32_t*>(lds + col * TP + row) = word;
@@ -1111,49 +1113,83 @@ struct CrtLaunch {
   double* fro2;  // per workgroup: its part of ||C||_F^2 (SYRK: both halves)
 };
 
+// Chunk sums the CRT takes without a reduction modulo m_l.
+constexpr int kCrtMaxChunks = 32;
+
 // x = S - q P with S = sum_l r_l W_l, in NL limbs of B bits held in FP64
 // accumulators: up to 14 moduli (P < 2^111) three 37-bit limbs, above that
-// four 32-bit limbs.  Every product r_l W_lj < 2^45 (2^40) and every limb
-// sum < 2^49 (2^45), so all of it is exact FMA arithmetic (full-rate FP64
+// four 32-bit limbs, all of it exact FMA arithmetic (full-rate FP64
 // instead of quarter-rate 64-bit integer multiplies).  q = floor(S / P +
 // 1/2) comes from the FP64 estimate f, within 2^-36 of the true value,
 // whose fractional part lies in [1/4, 3/4] because |x| <= P / 4.  Carries
-// are propagated exactly with floor(b 2^-B).  r[l] holds the residue in
-// byte e of word l.  Returns x = hi 2^64 + lo with lo unsigned.
-template <int NL, int B>
-__device__ __forceinline__ void crt(const Tables& T, const uint32_t* rw, int e,
-                                    int64_t& hi, uint64_t& lo) {
-  double a[NL], f = 0.5;
+// are propagated exactly with floor(b 2^-B).  r[l] is any integer of the
+// residue class of x mod m_l with |r[l]| <= 128 kCrtMaxChunks (a residue
+// byte, or the plain sum of K chunks' residue bytes: S changes by a
+// multiple of P and q absorbs it); then every product stays below 2^50
+// (2^45) and every limb sum below 2^53 (2^49), so the arithmetic is exact.
+// Returns x = hi 2^64 + lo with lo unsigned, for the four rows of a group
+// at once, modulus by modulus (each modulus's
+// table entries are read once and used for all four rows, so they need not
+// stay live in scalar registers).  Residue l of row e: CH false, the signed
+// byte e of rw[l]; CH true, the 16-bit lane of ev[l] (rows 0, 2) or od[l]
+// (rows 1, 3) less `bias` (sums of biased bytes r + 128 over K chunks).
+template <int NL, int B, bool CH>
+__device__ __forceinline__ void crt4(const Tables& T, const uint32_t* rw,
+                                     const uint32_t* ev, const uint32_t* od,
+                                     int bias, int64_t* hi, uint64_t* lo) {
+  double a[4][NL], f[4];
 #pragma unroll
-  for (int j = 0; j < NL; ++j) a[j] = 0.0;
+  for (int e = 0; e < 4; ++e) {
+    f[e] = 0.5;
+#pragma unroll
+    for (int j = 0; j < NL; ++j) a[e][j] = 0.0;
+  }
 #pragma unroll
   for (int l = 0; l < kMaxMod; ++l) {
     if (l < T.L) {
-      const double r = (double)((rw[l] >> (8 * e)) & 255u);
+      double w[NL];
 #pragma unroll
-      for (int j = 0; j < NL; ++j) a[j] = fma(r, T.W[l][j], a[j]);
-      f = fma(r, T.wfrac[l], f);
+      for (int j = 0; j < NL; ++j) w[j] = T.W[l][j];
+      const double wf = T.wfrac[l];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        int ri;
+        if constexpr (CH) {
+          const uint32_t v = (e & 1) ? od[l] : ev[l];
+          ri = (int)((e & 2) ? (v >> 16) : (v & 0xFFFFu)) - bias;
+        } else {
+          ri = (int)(int8_t)(uint8_t)(rw[l] >> (8 * e));
+        }
+        const double rl = (double)ri;
+#pragma unroll
+        for (int j = 0; j < NL; ++j) a[e][j] = fma(rl, w[j], a[e][j]);
+        f[e] = fma(rl, wf, f[e]);
+      }
     }
   }
-  const double q = floor(f);
   const double two = (double)(1ull << B), inv = 1.0 / (double)(1ull << B);
-  double b[NL];
 #pragma unroll
-  for (int j = 0; j < NL; ++j) b[j] = fma(-q, T.P[j], a[j]);
+  for (int e = 0; e < 4; ++e) {
+    const double q = floor(f[e]);
+    double b[NL];
 #pragma unroll
-  for (int j = 0; j + 1 < NL; ++j) {
-    const double c = floor(b[j] * inv);
-    b[j] = fma(-c, two, b[j]);
-    b[j + 1] += c;
-  }
-  if constexpr (NL == 4) {
-    lo = (uint64_t)(uint32_t)b[0] | ((uint64_t)(uint32_t)b[1] << 32);
-    hi = (int64_t)(((uint64_t)(int64_t)b[3] << 32) | (uint64_t)(uint32_t)b[2]);
-  } else {
-    // x = b2 2^74 + b1 2^37 + b0 with b0, b1 in [0, 2^37).
-    const uint64_t l0 = (uint64_t)b[0], l1 = (uint64_t)b[1];
-    lo = l0 | (l1 << 37);
-    hi = (int64_t)b[2] * 1024 + (int64_t)(l1 >> 27);
+    for (int j = 0; j < NL; ++j) b[j] = fma(-q, T.P[j], a[e][j]);
+#pragma unroll
+    for (int j = 0; j + 1 < NL; ++j) {
+      const double c = floor(b[j] * inv);
+      b[j] = fma(-c, two, b[j]);
+      b[j + 1] += c;
+    }
+    if constexpr (NL == 4) {
+      lo[e] = (uint64_t)(uint32_t)b[0] | ((uint64_t)(uint32_t)b[1] << 32);
+      hi[e] = (int64_t)(((uint64_t)(int64_t)b[3] << 32) |
+                        (uint64_t)(uint32_t)b[2]);
+    } else {
+      // x = b2 2^74 + b1 2^37 + b0 with b0, b1 in [0, 2^37).
+      const uint64_t l0 = (uint64_t)b[0], l1 = (uint64_t)b[1];
+      lo[e] = l0 | (l1 << 37);
+      hi[e] = (int64_t)b[2] * 1024 + (int64_t)(l1 >> 27);
+    }
   }
 }
 
@@ -1161,7 +1197,9 @@ __device__ __forceinline__ void crt(const Tables& T, const uint32_t* rw, int e,
 // column (one dword of every residue plane: all loads issued together).
 // Grid-stride; SYRK skips blocks strictly above the diagonal.  Each
 // workgroup writes its part of ||C||_F^2 to fro2[blockIdx.x] (summed on the
-// host in a fixed order).
+// host in a fixed order).  CHUNKS: the residue planes of p.kc > 1 K chunks
+// are added first.
+template <bool CHUNKS>
 __global__ __launch_bounds__(256) void crt_kernel(CrtLaunch p) {
   __shared__ double red[256];
   if (*p.fallback != 0) return;
@@ -1186,35 +1224,45 @@ __global__ __launch_bounds__(256) void crt_kernel(CrtLaunch p) {
       rw[l] = l < p.T.L
                   ? *reinterpret_cast<const uint32_t*>(src + l * p.r_plane)
                   : 0u;
-    if (p.kc > 1) {
-      // Sum the chunks' residues bytewise (each sum < 2^16), reduce mod m_l.
-      for (int l = 0; l < p.T.L; ++l) {
-        uint32_t s0 = rw[l] & 255u, s1 = (rw[l] >> 8) & 255u,
-                 s2 = (rw[l] >> 16) & 255u, s3 = rw[l] >> 24;
-        for (int c = 1; c < p.kc; ++c) {
-          const uint32_t v = *reinterpret_cast<const uint32_t*>(
-              src + ((int64_t)c * p.T.L + l) * p.r_plane);
-          s0 += v & 255u;
-          s1 += (v >> 8) & 255u;
-          s2 += (v >> 16) & 255u;
-          s3 += v >> 24;
+    // With K chunks: each chunk's residue is a signed byte; biased to
+    // r + 128 and split into the 16-bit lanes of an even-row and an odd-row
+    // word, kc <= 257 sums stay below 2^16 (the plain sum is another member
+    // of the class, which the CRT takes as it is).
+    uint32_t ev[CHUNKS ? kMaxMod : 1], od[CHUNKS ? kMaxMod : 1];
+    if constexpr (CHUNKS) {
+#pragma unroll
+      for (int l = 0; l < kMaxMod; ++l) {
+        const uint32_t v = rw[l] ^ 0x80808080u;
+        ev[l] = v & 0x00FF00FFu;
+        od[l] = (v >> 8) & 0x00FF00FFu;
+      }
+      for (int c = 1; c < p.kc; ++c) {
+#pragma unroll
+        for (int l = 0; l < kMaxMod; ++l) {
+          if (l < p.T.L) {
+            const uint32_t v =
+                *reinterpret_cast<const uint32_t*>(
+                    src + ((int64_t)c * p.T.L + l) * p.r_plane) ^
+                0x80808080u;
+            ev[l] += v & 0x00FF00FFu;
+            od[l] += (v >> 8) & 0x00FF00FFu;
+          }
         }
-        const uint32_t m = (uint32_t)p.T.m[l];
-        rw[l] = (s0 % m) | ((s1 % m) << 8) | ((s2 % m) << 16) | ((s3 % m) << 24);
       }
     }
+    const int bias = 128 * p.kc;
     const int cb = p.sig_b[col];
+    int64_t xh4[4];
+    uint64_t xl4[4];
+    if (p.T.L <= kLimb37Max)
+      crt4<3, 37, CHUNKS>(p.T, rw, ev, od, bias, xh4, xl4);
+    else
+      crt4<4, 32, CHUNKS>(p.T, rw, ev, od, bias, xh4, xl4);
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int row = row0 + e;
       if (row < p.M && syrk_keep(p.syrk, row, col)) {
-        int64_t xh;
-        uint64_t xl;
-        if (p.T.L <= kLimb37Max)
-          crt<3, 37>(p.T, rw, e, xh, xl);
-        else
-          crt<4, 32>(p.T, rw, e, xh, xl);
-        const double d = (double)xh * 18446744073709551616.0 + (double)xl;
+        const double d = (double)xh4[e] * 18446744073709551616.0 + (double)xl4[e];
         // Unscale by s_a s_b = 2^(ea + eb - 6) ma mb: one rounded division
         // by the integer ma mb (<= 225), then an exact power of two.
         const int ca = p.sig_a[row];
@@ -1708,13 +1756,18 @@ OzPlan choose_plan_any(skgemm_ctx* c, int kind, int M, int N, int L,
 
 OzPlan choose_plan(skgemm_ctx* c, int kind, int M, int N, int L, int kpad) {
   OzPlan op = choose_plan_any(c, kind, M, N, L, kpad);
-  // Whatever chose it, no unit may sum more than kMaxKpad products.
+  // Whatever chose it, no unit may sum more than kMaxKpad products, and
+  // the CRT takes at most kCrtMaxChunks chunk residues.
+  op.kc = std::min(op.kc, kCrtMaxChunks);
   if (kpad > kMaxKpad) {
     op.dp = 1;
     op.kc = std::max(op.kc, min_chunks(kpad, kGemmCfgs[op.cfg].bk));
   }
   return op;
 }
+
+// K the split form accepts: kCrtMaxChunks chunks of kMaxKpad products.
+constexpr int64_t kMaxSplitK = (int64_t)kCrtMaxChunks * kMaxKpad;
 
 // A-posteriori bound on ||C - AB||_F / ||AB||_F.  Row i of A is scaled by
 // s_i and rounded, x'_i = s_i a_i - d_i, column j of B by t_j, y'_j =
@@ -1764,6 +1817,9 @@ namespace skg {
 int ozaki_reserve(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
                   int slices) {
   const int M = pr.M, N = pr.N, K = pr.K;
+  if ((int64_t)K > kMaxSplitK)
+    return set_error(SKGEMM_ERR_NOTSUP, "split form: K = %d exceeds %lld", K,
+                     (long long)kMaxSplitK);
   const int L = slices == 0 ? 14 : slices;
   if (L < 8 || L > kMaxMod)
     return set_error(SKGEMM_ERR_ARG, "slices (moduli) must be 0 or 8..%d",
@@ -2016,7 +2072,12 @@ int run_moduli(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
   P.beta = pr.beta;
   P.gamma = pr.gamma;
   P.fro2 = fro2;
-  hipLaunchKernelGGL(crt_kernel, dim3(crt_grid), dim3(256), 0, c->stream, P);
+  if (P.kc > 1)
+    hipLaunchKernelGGL(crt_kernel<true>, dim3(crt_grid), dim3(256), 0,
+                       c->stream, P);
+  else
+    hipLaunchKernelGGL(crt_kernel<false>, dim3(crt_grid), dim3(256), 0,
+                       c->stream, P);
   if ((e = hipGetLastError()) != hipSuccess) return hip_error(e, "crt launch");
 
   // Plain FP64 fallback, decided on the device: the launch does its work
@@ -2144,6 +2205,10 @@ int run_certify(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
 
 int ozaki_run(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
               int slices, double* err_bound) {
+  if ((int64_t)pr.K > kMaxSplitK)
+    return set_error(SKGEMM_ERR_NOTSUP,
+                     "split form: K = %d exceeds %lld (%d chunks of %d)",
+                     pr.K, (long long)kMaxSplitK, kCrtMaxChunks, kMaxKpad);
   if (slices == SKGEMM_SLICES_AUTO)
     return run_certify(c, kind, la, lb, pr, err_bound);
   const int L = slices == 0 ? 14 : slices;

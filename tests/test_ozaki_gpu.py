@@ -121,6 +121,19 @@ def test_rotation_split_accuracy(skg, ctx, orc, rng):
     assert err < 1e-13 and err <= bound
 
 
+def test_split_k_limit(skg, ctx):
+    """K beyond 32 chunks of 131072 products (the CRT's chunk-sum range) is
+    refused before any launch."""
+    K = 32 * 131072 + 1
+    a = skg.DeviceArray(ctx, 64)
+    c = skg.DeviceArray(ctx, 64)
+    with pytest.raises(skg.SkgemmError) as e:
+        ctx.dsyrk_ozaki(skg.COL_MAJOR, 8, K, 1.0, a.ptr, K, 0.0, c.ptr, 8)
+    assert e.value.code == skg.ERR_NOTSUP
+    a.free()
+    c.free()
+
+
 @pytest.mark.parametrize("K,N,layout", [(140000, 40, 0), (175616, 64, 1),
                                          (262400, 24, 2)])
 def test_long_k_in_chunks_bit_exact(skg, ctx, rng, K, N, layout):
@@ -134,7 +147,10 @@ def test_long_k_in_chunks_bit_exact(skg, ctx, rng, K, N, layout):
     ref, _, _ = ozaki.product_rows(X.T.copy(), X.T.copy(), 14)
     il = np.tril_indices(N)
     try:
-        for plan in (None, "6,1", "5,1,0,1", "1,0"):
+        # "12,1,0,32": the most chunks the CRT adds without a reduction
+        # (their plain sum of signed residue bytes); "14,1,0,9": the
+        # five-stage 256 x 256 tile.
+        for plan in (None, "6,1", "5,1,0,1", "1,0", "12,1,0,32", "14,1,0,9"):
             if plan is None:
                 os.environ.pop("SKGEMM_OZ_PLAN", None)
             else:
