@@ -1351,17 +1351,10 @@ struct GemmCfg {
 // 8: 128x128, 4 waves of 64x64,  BK 128, LDS DMA, 3 stages (96 KB LDS)
 // 9: 256x256, 8 waves of 64x128, BK 64,  LDS DMA, 3 stages (96 KB LDS)
 // Early barrier (the barrier publishing step k + 1 sits before the last
-// sub-step of step k):
-// 10: 256x256, 8 waves of 128x64, BK 128, LDS DMA, 2 stages (128 KB LDS)
-// 11: 256x256, 8 waves of 64x128, BK 128, LDS DMA, 2 stages (128 KB LDS)
-// 12: 128x128, 4 waves of 64x64,  BK 128, LDS DMA, 2 stages (64 KB LDS)
-// 13: 256x256, 8 waves of 64x128, BK 64,  LDS DMA, 4 stages (128 KB LDS)
-// 14: 256x256, 8 waves of 64x128, BK 64,  LDS DMA, 5 stages (160 KB LDS)
-// 15: 128x128, 4 waves of 64x64,  BK 64,  LDS DMA, 4 stages (64 KB LDS)
-// 16: 128x128, 4 waves of 64x64,  BK 64,  LDS DMA, 5 stages (80 KB LDS)
-// 17: 256x256, 4 waves of 128x128, BK 64, LDS DMA, 4 stages (128 KB LDS)
-// 18: 256x256, 4 waves of 128x128, BK 64, LDS DMA, 5 stages (160 KB LDS)
-// 19: 256x256, 4 waves of 128x128, BK 128, LDS DMA, 2 stages (128 KB LDS)
+// sub-step of step k; measured beside the rest, selectable plans):
+// 10: 256x256, 8 waves of 64x128, BK 128, LDS DMA, 2 stages (128 KB LDS)
+// 11: 128x128, 4 waves of 64x64,  BK 128, LDS DMA, 2 stages (64 KB LDS)
+// 12: 256x256, 8 waves of 64x128, BK 64,  LDS DMA, 5 stages (160 KB LDS)
 static const GemmCfg kGemmCfgs[] = {{256, 256, 4, 2, 64, 2},
                                     {128, 128, 2, 2, 128, 2},
                                     {128, 128, 2, 1, 128, 2},
@@ -1372,17 +1365,10 @@ static const GemmCfg kGemmCfgs[] = {{256, 256, 4, 2, 64, 2},
                                     {128, 128, 2, 2, 64, 4},
                                     {128, 128, 2, 2, 128, 3},
                                     {256, 256, 4, 2, 64, 3},
-                                    {256, 256, 2, 4, 128, 8},
                                     {256, 256, 4, 2, 128, 8},
                                     {128, 128, 2, 2, 128, 8},
-                                    {256, 256, 4, 2, 64, 10},
-                                    {256, 256, 4, 2, 64, 11},
-                                    {128, 128, 2, 2, 64, 10},
-                                    {128, 128, 2, 2, 64, 11},
-                                    {256, 256, 2, 2, 64, 10},
-                                    {256, 256, 2, 2, 64, 11},
-                                    {256, 256, 2, 2, 128, 8}};
-constexpr int kNumGemmCfgs = 20;
+                                    {256, 256, 4, 2, 64, 11}};
+constexpr int kNumGemmCfgs = 13;
 
 #define SKG_OZ_CFGS(X)          \
   X(0, 256, 256, 4, 2, 64, 2)   \
@@ -1395,16 +1381,9 @@ constexpr int kNumGemmCfgs = 20;
   X(7, 128, 128, 2, 2, 64, 4)   \
   X(8, 128, 128, 2, 2, 128, 3)  \
   X(9, 256, 256, 4, 2, 64, 3)   \
-  X(10, 256, 256, 2, 4, 128, 8) \
-  X(11, 256, 256, 4, 2, 128, 8) \
-  X(12, 128, 128, 2, 2, 128, 8) \
-  X(13, 256, 256, 4, 2, 64, 10)  \
-  X(14, 256, 256, 4, 2, 64, 11)  \
-  X(15, 128, 128, 2, 2, 64, 10)  \
-  X(16, 128, 128, 2, 2, 64, 11)  \
-  X(17, 256, 256, 2, 2, 64, 10)  \
-  X(18, 256, 256, 2, 2, 64, 11)  \
-  X(19, 256, 256, 2, 2, 128, 8)
+  X(10, 256, 256, 4, 2, 128, 8) \
+  X(11, 128, 128, 2, 2, 128, 8) \
+  X(12, 256, 256, 4, 2, 64, 11)
 
 static const void* gemm_fn(int cfg) {
 #define SKG_FN_CASE(i, bm, bn, wm, wn, bk, pd) \

@@ -147,10 +147,11 @@ def test_long_k_in_chunks_bit_exact(skg, ctx, rng, K, N, layout):
     ref, _, _ = ozaki.product_rows(X.T.copy(), X.T.copy(), 14)
     il = np.tril_indices(N)
     try:
-        # "12,1,0,32": the most chunks the CRT adds without a reduction
-        # (their plain sum of signed residue bytes); "14,1,0,9": the
-        # five-stage 256 x 256 tile.
-        for plan in (None, "6,1", "5,1,0,1", "1,0", "12,1,0,32", "14,1,0,9"):
+        # "11,1,0,32": the most chunks the CRT adds without a reduction
+        # (their plain sum of signed residue bytes); "12,1,0,9": the
+        # five-stage 256 x 256 tile; "10,1,0,5": the early-barrier one.
+        for plan in (None, "6,1", "5,1,0,1", "1,0", "11,1,0,32", "12,1,0,9",
+                     "10,1,0,5"):
             if plan is None:
                 os.environ.pop("SKGEMM_OZ_PLAN", None)
             else:
