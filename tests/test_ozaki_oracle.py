@@ -128,3 +128,118 @@ def test_exact_to_double_matches_python_integers(rng):
     C, sa, sb, _ = ozaki.product(A, B, 14)
     got, _, _ = ozaki.product_rows(A, B, 14)
     assert np.array_equal(got, C)
+
+
+# ---- the device's residue and CRT arithmetic, emulated in exact FP64 ------
+# The GPU's INT8-product epilogue and CRT are FP64 sequences whose
+# exactness rests on magnitude arguments; these tests run the same
+# sequences with every FP64 operation rounded exactly as the device rounds
+# it (fma through exact rationals) and compare with integer arithmetic.
+
+def _fma(x, y, z):
+    from fractions import Fraction
+    return float(Fraction(x) * Fraction(y) + Fraction(z))
+
+
+def _tables(L):
+    """The CRT tables the library builds (make_tables): limbs of P and of
+    W_l = (P / m_l) ((P / m_l)^-1 mod m_l), 37-bit limbs up to 14 moduli,
+    32-bit above; wfrac_l = inv_l / m_l rounded once."""
+    m, _ = ozaki.tables(L)
+    P = 1
+    for x in m:
+        P *= x
+    nl, lb = (3, 37) if L <= 14 else (4, 32)
+    mask = (1 << lb) - 1
+    limbs = lambda v: [float((v >> (lb * j)) & mask) for j in range(nl)]
+    W, wf = [], []
+    for ml in m:
+        Pl = P // ml
+        inv = pow(Pl % ml, -1, ml)
+        W.append(limbs(Pl * inv))
+        wf.append(inv / ml)
+    return m, P, limbs(P), W, wf, nl, lb
+
+
+def _device_crt(r, Pj, W, wf, nl, lb):
+    """crt4 for one element: limb sums, quotient estimate, carries, then
+    x = hi 2^64 + lo as the kernel assembles it."""
+    a = [0.0] * nl
+    f = 0.5
+    for l, rl in enumerate(r):
+        rl = float(rl)
+        for j in range(nl):
+            a[j] = _fma(rl, W[l][j], a[j])
+        f = _fma(rl, wf[l], f)
+    q = float(np.floor(f))
+    two, inv = float(1 << lb), 1.0 / float(1 << lb)
+    b = [_fma(-q, Pj[j], a[j]) for j in range(nl)]
+    for j in range(nl - 1):
+        c = float(np.floor(b[j] * inv))
+        b[j] = _fma(-c, two, b[j])
+        b[j + 1] = b[j + 1] + c
+    if nl == 4:
+        lo = (int(b[0]) & 0xFFFFFFFF) | ((int(b[1]) & 0xFFFFFFFF) << 32)
+        hi = (int(b[3]) << 32) | (int(b[2]) & 0xFFFFFFFF)
+    else:
+        l0, l1 = int(b[0]), int(b[1])
+        lo = (l0 | (l1 << 37)) & ((1 << 64) - 1)
+        hi = int(b[2]) * 1024 + (l1 >> 27)
+    return hi * (1 << 64) + lo
+
+
+@pytest.mark.parametrize("L", [14, 16])
+def test_device_crt_exact_on_chunk_sums(rng, L):
+    """The CRT recovers x (|x| <= P / 4) from any representatives of its
+    residue classes with |r_l| <= 128 x 32: symmetric residue bytes, the
+    plain sums of up to 32 K chunks' residue bytes, and the extreme
+    representatives of that range."""
+    m, P, Pj, W, wf, nl, lb = _tables(L)
+    lim = 128 * 32
+    xs = [int(v) for v in rng.integers(-2 ** 62, 2 ** 62, 6)]
+    xs += [P // 4, -(P // 4), 0, 1, -1, P // 4 - 12345]
+    xs += [int(rng.integers(0, 2 ** 60)) * (P // 2 ** 62) // 4 for _ in range(4)]
+    for x in xs:
+        sym = [((x + ml // 2) % ml) - ml // 2 for ml in m]
+        cases = [sym]
+        # chunk sums: other members of each class inside [-lim, lim]
+        cases.append([s + ml * int(rng.integers(-((lim - ml) // ml),
+                                                (lim - ml) // ml + 1))
+                      for s, ml in zip(sym, m)])
+        # extremes: the largest-magnitude members, random signs
+        ext = []
+        for s, ml in zip(sym, m):
+            up = s + ml * ((lim - s) // ml)
+            dn = s - ml * ((lim + s) // ml)
+            ext.append(up if rng.random() < 0.5 else dn)
+        cases.append(ext)
+        for r in cases:
+            assert all(abs(v) <= lim for v in r)
+            assert all((v - x) % ml == 0 for v, ml in zip(r, m))
+            assert _device_crt(r, Pj, W, wf, nl, lb) == x
+
+
+def test_device_residue_epilogue_symmetric(rng):
+    """The INT8 products' epilogue: r = fma(-m, rint(a fl(1/m)), a) on an
+    exact INT32 sum a is the symmetric residue of a for odd m (no
+    correction step), and for m = 256 a member of [-128, 128] whose byte
+    is a's class."""
+    m, _ = ozaki.tables(16)
+    edges = [0, 1, -1, 2 ** 31 - 1, -2 ** 31, 2 ** 30, -2 ** 30 + 7]
+    for ml in m:
+        inv = 1.0 / ml
+        half = [k * ml + d for k in (-(2 ** 31) // ml + 1, -3, 0, 5,
+                                     (2 ** 31 - 1) // ml - 1)
+                for d in (-(ml // 2), ml // 2, (ml - 1) // 2, -((ml - 1) // 2))]
+        vals = edges + half + [int(v) for v in rng.integers(-2 ** 31, 2 ** 31, 2000)]
+        for a in vals:
+            if not -2 ** 31 <= a < 2 ** 31:
+                continue
+            q = round(float(a) * inv)      # rint: ties to even
+            r = _fma(-float(ml), float(q), float(a))
+            assert r == a - ml * q
+            r = int(r)
+            if ml % 2:
+                assert abs(r) <= (ml - 1) // 2 and (r - a) % ml == 0
+            else:
+                assert -128 <= r <= 128 and ((r & 255) - a) % 256 == 0
