@@ -1301,7 +1301,8 @@ constexpr int kCrtGrid = 4096;
 __global__ __launch_bounds__(1024) void bound_sums_kernel(
     const double* n2a, const int* ca, const double* dda, int M,
     const double* n2b, const int* cb, const double* ddb, int N, int K,
-    const double* fro2, int nparts, const int* fallback, double* out) {
+    const double* fro2, int nparts, const int* fallback, const int* status,
+    double* out) {
   __shared__ double red[1024];
   const double u2 = 0x1p-51, g = 1.0 + (K + 2.0) * 0x1p-52;
   double v[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -1332,7 +1333,10 @@ __global__ __launch_bounds__(1024) void bound_sums_kernel(
     if (threadIdx.x == 0) out[t] = red[0];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[7] = (double)*fallback;
+  if (threadIdx.x == 0) {
+    out[7] = (double)*fallback;
+    out[8] = (double)*status;  // the context's fix-up status, same readback
+  }
 }
 
 // ---- host ----------------------------------------------------------------------
@@ -1526,9 +1530,18 @@ struct skgemm_ctx::Extra {
   double* pinned = nullptr;
   double* pinned_sums() {
     if (pinned == nullptr &&
-        hipHostMalloc((void**)&pinned, 8 * sizeof(double), 0) != hipSuccess)
+        hipHostMalloc((void**)&pinned, 16 * sizeof(double), 0) != hipSuccess)
       pinned = nullptr;
     return pinned;
+  }
+  // Marks the bound readback in the stream; polled instead of a blocking
+  // stream synchronisation (no wake-up latency between calls).
+  hipEvent_t read_ev = nullptr;
+  hipEvent_t readback_event() {
+    if (read_ev == nullptr &&
+        hipEventCreateWithFlags(&read_ev, hipEventDisableTiming) != hipSuccess)
+      read_ev = nullptr;
+    return read_ev;
   }
   void* get(int i, size_t bytes) {
     if (bytes > n[i]) {
@@ -1551,6 +1564,7 @@ void skgemm_ctx::free_extra() {
   if (extra->aux) hipStreamSynchronize(extra->aux);
   for (auto& e : extra->ev)
     if (e) hipEventDestroy(e);
+  if (extra->read_ev) hipEventDestroy(extra->read_ev);
   if (extra->aux) hipStreamDestroy(extra->aux);
   for (int i = 0; i < Extra::kBufs; ++i)
     if (extra->buf[i]) hipFree(extra->buf[i]);
@@ -2073,20 +2087,28 @@ int run_moduli(skgemm_ctx* c, int kind, int la, int lb, const Problem& pr,
   c->record_end();
 
   if (err_bound != nullptr) {
-    double* sums = (double*)X->get(kBoundSums, sizeof(double) * 8);
+    double* sums = (double*)X->get(kBoundSums, sizeof(double) * 9);
     double* host = X->pinned_sums();
-    if (sums == nullptr || host == nullptr)
+    hipEvent_t rev = X->readback_event();
+    if (sums == nullptr || host == nullptr || rev == nullptr)
       return set_error(SKGEMM_ERR_NOMEM, "error-bound sums");
     hipLaunchKernelGGL(bound_sums_kernel, dim3(1), dim3(1024), 0, c->stream,
                        n2a, sga, dda, M, n2b, sgb, ddb, N, K, fro2, crt_grid,
-                       fallback, sums);
+                       fallback, c->status, sums);
     if ((e = hipGetLastError()) != hipSuccess ||
-        (e = hipMemcpyAsync(host, sums, sizeof(double) * 8,
+        (e = hipMemcpyAsync(host, sums, sizeof(double) * 9,
                             hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
-        (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+        (e = hipEventRecord(rev, c->stream)) != hipSuccess)
       return hip_error(e, "error-bound readback");
-    rc = c->check_status();
-    if (rc != SKGEMM_OK) return rc;
+    while ((e = hipEventQuery(rev)) == hipErrorNotReady) {
+    }
+    if (e != hipSuccess) return hip_error(e, "error-bound readback");
+    // A fix-up wait that gave up (status != 0): the context's handler
+    // clears the flags and reports it.
+    if (host[8] != 0.0) {
+      rc = c->check_status();
+      if (rc != SKGEMM_OK) return rc;
+    }
     *err_bound = host[7] != 0.0 ? -1.0 : error_bound(host, M, N, crt_grid);
   }
   c->oz_last_moduli = L;
