@@ -447,7 +447,8 @@ def bench_nloc(skg, ctx, reps):
     touched points plus result read and written there (3 x 8 x points x
     nb bytes); the implementation reads psi once per projector row (the
     repeats of overlapping projectors mostly from L2) and writes and reads
-    the forward parts of the 256-row chunks (2 x 8 x parts x nb)."""
+    the forward parts of the 256-row chunks and X (2 x 8 x (parts + columns)
+    x nb)."""
     import time
     K, nb, count = 32768, 520, 64
     rng = np.random.default_rng(6)
@@ -479,7 +480,7 @@ def bench_nloc(skg, ctx, reps):
         x.free()
     alg = 3.0 * 8 * points * nb
     parts = sum(-(-r // 256) * c for r, c in zip(nrows, ncols))
-    moved = 8.0 * nb * (rows + 2 * points + 2 * parts)
+    moved = 8.0 * nb * (rows + 2 * points + 2 * parts + 2 * sum(ncols))
     flop = 4.0 * nb * sum(r * c for r, c in zip(nrows, ncols))
     return {"workload": "nloc projectors: Nd 32768, 520 bands, 64 projectors "
                         "(512..1024 points, 9/13/16 columns), 16->32 packed",

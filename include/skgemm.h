@@ -324,9 +324,9 @@ int skgemm_batched(skgemm_ctx* ctx, const skgemm_problem* problems, int count,
 
 /* result += V_nloc psi on band-packed orbitals, the reference's
  * nloc_project_vectors_packed (16-band-packed psi -> 32-band-packed
- * result, :229-447) and nloc_project_vectors_packed_16to16 (:453-655), as a
- * client of the batched scheduler: per projector p, X_p = dv chi_p^T
- * psi[idx_p, :], X_p *= gamma_p, result[idx_p, :] += chi_p X_p.
+ * result, :229-447) and nloc_project_vectors_packed_16to16 (:453-655): per
+ * projector p, X_p = dv chi_p^T psi[idx_p, :], X_p *= gamma_p,
+ * result[idx_p, :] += chi_p X_p.
  * A projector set on Nd grid points is uploaded once.  Host arrays,
  * projectors concatenated in order: nrow[p] grid indices in
  * index_all[sum_{q<p} nrow_q ..] (in [0, Nd)), chi_p column-major

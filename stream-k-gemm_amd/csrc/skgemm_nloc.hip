@@ -9,21 +9,24 @@
 //      256; one workgroup per (chunk, 64 bands) gathers its rows of psi
 //      straight from the 16-band panels (32 rows per wave, all loads in
 //      flight together) and forms the chunk's part of chi_p^T psi[idx_p, :];
-//   2. backward products: one workgroup per (chunk, 64 bands) forms X_p for
-//      its bands, (dv sum of the chunk parts, in chunk order) gamma_p (times
-//      1.0, exact, where the atom rule does not apply), in registers, and
-//      writes the chunk's rows of temp_p = chi_p X_p;
-//   3. scatter: one wave per (touched grid point g, 64 bands) adds the temp
-//      rows of g in projector order (a CSR table built at create time), so
-//      shared grid points need no atomics and the sums keep the reference's
-//      order.
+//   2. X_p = (dv sum of the chunk parts, in chunk order) gamma_p (times 1.0,
+//      exact, where the atom rule does not apply), one thread per (column,
+//      band);
+//   3. backward products and scatter together: the touched grid points,
+//      ascending, are cut at create time into blocks of 32; one wave per
+//      (block, 64 bands) holds the block's result values in LDS and adds
+//      each row's sum_c chi_p[r][c] X_p[c][b] to its point's value, the rows
+//      in projector order, projector by projector with X_p in registers.
+//      Shared grid points need no atomics, the sums keep the reference's
+//      order and no per-row products are stored.
 // HBM-bound: psi is read once per projector row (repeats of overlapping
-// projectors mostly hit L2), temp written and read once, result read and
-// written once per touched point.  (Forming the products inside the
-// scatter instead reads X_p, ncol values per band, for every touched
-// (point, row), 13x the bytes of one temp row at 13 columns: on 32768
-// points x 520 bands with 64 projectors that scatter alone took 305 us,
-// against 285 us for this whole sequence.)
+// projectors mostly hit L2), result read and written once per touched
+// point; the forward parts and X (columns x bands) are small.
+// (Before: the backward kernel stored every row's products, 8 x rows x nb
+// bytes, for a scatter kernel to add them up: 92 + 109 us on 32768 points x
+// 520 bands with 64 projectors.  A first fused form, one wave per point
+// reading X from LDS row by row, took 0.64 ms: each point's result load,
+// row entries, chi loads and FMA chain ran one after another.)
 
 #include <hip/hip_runtime.h>
 
@@ -46,196 +49,293 @@ struct NlocProj {
 };
 struct NlocChunk {
   int proj, r0, nr;
+  int64_t f0;  // its chi values in the forward A layout, in blocks of 4096
 };
+typedef double v4d __attribute__((ext_vector_type(4)));
+// A segment of a point block's rows: rows e0 .. e1 of the block-ordered row
+// list, all of one projector (X columns coff .., ncol of them), touching
+// distinct points of the block; its chi rows are stored from a0 on, in
+// units of 256 doubles (one 16-row block and 16-column pass: row block m,
+// pass q at a0 + m ceil(ncol / 16) + q).
+struct NlocSeg {
+  int e0, e1, coff, ncol;
+  int64_t a0;
+};
+// A block of touched points u0 .. u0 + np - 1 (ascending grid index) and its
+// segments s0 .. s1 - 1.
+struct NlocBlk {
+  int u0, np, s0, s1;
+};
+constexpr int kBlkPoints = 32;  // points per block of the apply kernel
 constexpr int kFwdWaves = 8, kFwdRowsPerWave = 32;
 constexpr int kFwdRows = kFwdWaves * kFwdRowsPerWave;  // rows per chunk
-constexpr int kFwdCols = 16;                            // columns per pass
 
 // Forward parts: part[poff_p + (k - chunk0_p) ncol_p + c][b] =
 //   sum over the chunk's rows r of chi_p[r][c] psi(index[roff_p + r], b).
-// Wave w takes rows w * 32 .. w * 32 + 31 of the chunk and issues its 32
-// psi loads at once, on clamped addresses (a band past nb reads band
-// nb - 1, a row past the chunk its last row), before the chunk's chi
-// values are staged in LDS ([column][row], zero past the chunk's rows and
-// the projector's columns, so the clamped rows add nothing); chi values
-// are read as broadcasts.  The 8 waves' sums are added in wave order
-// through LDS, 8 columns at a time.  chi_p is column-major nrow x ncol.
-// (A non-finite psi value at a chunk's last row can turn an infinite part
-// into NaN: the clamped copies add 0 x inf.)
+// Wave w takes rows 32 w .. 32 w + 31 of the chunk as eight steps of
+// v_mfma_f64_16x16x4 per 16-band block (A = 16 chi columns x 4 rows, B = 4
+// psi rows x 16 bands): lane l gathers psi at row 4 s + l / 16, bands
+// 16 j + l % 16 of the workgroup's 64 (32 loads issued at once, on clamped
+// rows and bands) and reads its chi values from a copy the host stores in
+// the A layout (zero past the chunk's rows and the projector's columns, so
+// the clamped rows add nothing).  The 8 waves' sums go through LDS and are
+// added in wave order.  (A non-finite psi value at a chunk's last row can
+// turn an infinite part into NaN: the clamped copies add 0 x inf.)
+// (Before: FMAs with each row's 16 chi values read from LDS as broadcasts,
+// 1 KB returned to the wave per 16-byte broadcast, which bound the kernel by
+// LDS bandwidth: 76 us on 49665 rows x 520 bands.)
 __global__ __launch_bounds__(kFwdWaves * 64, 4) void nloc_forward_kernel(
     const double* __restrict__ psi16, int Nd, int nb,
-    const int* __restrict__ index, const double* __restrict__ chi,
+    const int* __restrict__ index, const double* __restrict__ chif,
     const NlocProj* __restrict__ proj, const NlocChunk* __restrict__ chunks,
     double* __restrict__ part) {
-  // chi of the chunk ([kFwdRows][kFwdCols], a row's 16 values read as
-  // eight 16-byte broadcasts), then the wave sums ([kFwdWaves][8][64]):
-  // 32 KB either way.
-  __shared__ __align__(16) double sh[kFwdCols * kFwdRows];
-  static_assert(kFwdWaves * 8 * 64 <= kFwdCols * kFwdRows, "LDS reuse");
+  __shared__ double red[kFwdWaves * 16 * 64];  // each wave's 16 x 64 sums
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int b = blockIdx.y * 64 + lane;
+  const int b0 = blockIdx.y * 64;
   const NlocChunk C = chunks[blockIdx.x];
   const NlocProj P = proj[C.proj];
-  const int bc = min(b, nb - 1);
-  const double* src = psi16 + (int64_t)(bc >> 4) * Nd * 16 + (bc & 15);
   const int* idx = index + P.roff;
-  const int rb = C.r0 + w * kFwdRowsPerWave;  // this wave's first row
   const int rlast = C.r0 + C.nr - 1;
-  double v[kFwdRowsPerWave];
+  double v[8][4];
 #pragma unroll
-  for (int u = 0; u < kFwdRowsPerWave; ++u)
-    v[u] = src[(int64_t)idx[min(rb + u, rlast)] * 16];
+  for (int s = 0; s < 8; ++s) {
+    const int g = idx[min(C.r0 + 32 * w + 4 * s + (lane >> 4), rlast)];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int bb = min(b0 + 16 * j + (lane & 15), nb - 1);
+      v[s][j] = psi16[((int64_t)(bb >> 4) * Nd + g) * 16 + (bb & 15)];
+    }
+  }
   double* dst = part + (P.poff + (int64_t)(blockIdx.x - P.chunk0) * P.ncol) * nb;
-  for (int c0 = 0; c0 < P.ncol; c0 += kFwdCols) {
-    const int nc = min(kFwdCols, P.ncol - c0);
-    if (c0 > 0) __syncthreads();  // the previous pass's sums are read
-    for (int e = tid; e < kFwdCols * kFwdRows; e += kFwdWaves * 64) {
-      const int r = e / kFwdCols, c = e % kFwdCols;
-      sh[e] = (c < nc && r < C.nr)
-                  ? chi[P.choff + (int64_t)(c0 + c) * P.nrow + C.r0 + r]
-                  : 0.0;
+  const int kp = (P.ncol + 15) >> 4;
+  for (int q = 0; q < kp; ++q) {
+    const double2* cf = reinterpret_cast<const double2*>(
+        chif + (C.f0 + q) * (kFwdRows * 16) + (w * 64 + lane) * 8);
+    double a[8];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const double2 x = cf[h];
+      a[2 * h] = x.x;
+      a[2 * h + 1] = x.y;
     }
+    v4d d[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) d[j] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        d[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], v[s][j], d[j], 0, 0,
+                                                    0);
+    if (q > 0) __syncthreads();  // the previous pass's sums are read
+    // lane l holds column 4 u + l / 16, band 16 j + l % 16
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        red[(w * 16 + 4 * u + (lane >> 4)) * 64 + 16 * j + (lane & 15)] =
+            d[j][u];
     __syncthreads();
-    double acc[kFwdCols];
+    // wave w adds up columns 2 w and 2 w + 1 over the waves, in wave order
 #pragma unroll
-    for (int c = 0; c < kFwdCols; ++c) acc[c] = 0.0;
-    const double2* ch =
-        reinterpret_cast<const double2*>(sh + w * kFwdRowsPerWave * kFwdCols);
+    for (int h = 0; h < 2; ++h) {
+      const int c = 2 * w + h;
+      double sum = red[c * 64 + lane];
 #pragma unroll
-    for (int u = 0; u < kFwdRowsPerWave; ++u) {
-#pragma unroll
-      for (int q = 0; q < kFwdCols / 2; ++q) {
-        const double2 x = ch[u * (kFwdCols / 2) + q];
-        acc[2 * q] = fma(x.x, v[u], acc[2 * q]);
-        acc[2 * q + 1] = fma(x.y, v[u], acc[2 * q + 1]);
-      }
-    }
-    // The sums are complete here: without this the compiler sinks the
-    // second half's products below the first half's reduction and keeps
-    // (spills) their chi values across it.
-#pragma unroll
-    for (int c = 0; c < kFwdCols; ++c) asm volatile("" : "+v"(acc[c]));
-#pragma unroll
-    for (int h = 0; h < kFwdCols / 8; ++h) {
-      __syncthreads();  // chi (h = 0) or the previous half's sums are read
-#pragma unroll
-      for (int c = 0; c < 8; ++c) sh[(w * 8 + c) * 64 + lane] = acc[h * 8 + c];
-      __syncthreads();
-      const int c = h * 8 + w;  // wave w adds up column h * 8 + w
-      if (c < nc) {
-        double sum = sh[(c % 8) * 64 + lane];
-#pragma unroll
-        for (int q = 1; q < kFwdWaves; ++q)
-          sum += sh[(q * 8 + c % 8) * 64 + lane];
-        if (b < nb) dst[(int64_t)(c0 + c) * nb + b] = sum;
-      }
+      for (int x = 1; x < kFwdWaves; ++x) sum += red[(x * 16 + c) * 64 + lane];
+      if (16 * q + c < P.ncol && b0 + lane < nb)
+        dst[(int64_t)(16 * q + c) * nb + b0 + lane] = sum;
     }
   }
 }
 
-// temp[roff_p + r][b] = sum_c chi_p[r][c] X_p[c][b] for the chunk's rows,
-// with X_p[c][b] = (dv sum_j part[poff_p + j ncol_p + c][b]) factor (j in
-// chunk order) formed in registers for this lane's band.  Wave w takes the
-// chunk's rows w * 32 .. w * 32 + 31; chi is staged in LDS as in the
-// forward pass and read as broadcasts.
-__global__ __launch_bounds__(kFwdWaves * 64) void nloc_backward_kernel(
+// X_p[c][b] = (dv sum_j part[poff_p + j ncol_p + c][b]) factor_p[c], the
+// chunk parts added in chunk order; one thread per (column, band), four
+// columns per workgroup.
+__global__ __launch_bounds__(256) void nloc_x_kernel(
     const double* __restrict__ part, const NlocProj* __restrict__ proj,
-    const NlocChunk* __restrict__ chunks, const double* __restrict__ factor,
-    double dv, const double* __restrict__ chi, int nb,
-    double* __restrict__ temp) {
-  __shared__ __align__(16) double sh[kFwdCols * kFwdRows];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int b = blockIdx.y * 64 + lane;
-  const int bc = min(b, nb - 1);
-  const NlocChunk C = chunks[blockIdx.x];
-  const NlocProj P = proj[C.proj];
-  const int nr = min(kFwdRowsPerWave, C.nr - w * kFwdRowsPerWave);
-  double* dst = temp + (P.roff + C.r0 + w * kFwdRowsPerWave) * nb + b;
-  for (int c0 = 0; c0 < P.ncol; c0 += kFwdCols) {
-    const int nc = min(kFwdCols, P.ncol - c0);
-    // The 16 columns' parts of one chunk are loaded together, chunk after
-    // chunk (each column's sum in chunk order).
-    double xv[kFwdCols];
+    const int* __restrict__ colproj, const double* __restrict__ factor,
+    double dv, int nb, int64_t cols, double* __restrict__ X) {
+  const int64_t col = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int b = blockIdx.y * 64 + (threadIdx.x & 63);
+  if (col >= cols || b >= nb) return;
+  const NlocProj P = proj[colproj[col]];
+  const int c = (int)(col - P.coff);
+  const double* src = part + (P.poff + c) * nb + b;
+  double x = 0.0;
+  for (int j = 0; j < P.nchunks; ++j) x += src[(int64_t)j * P.ncol * nb];
+  X[col * nb + b] = (dv * x) * factor[col];
+}
+
+// One segment's operands in registers, loaded ahead of its turn: its chi
+// rows in the A layout of v_mfma_f64_16x16x4 (lane l: rows 16 m + l % 16,
+// columns 4 s + l / 16, s = 0..3, for each block m of 16 rows; the host
+// stores them so, zero past the segment's rows and the projector's
+// columns), its first 16 X columns in the B layout (lane l: column
+// 4 s + l / 16, band 16 j + l % 16 of the wave's 64), and in lane k the
+// block slot of the segment's row k.
+template <int NP>
+struct NlocSegOps {
+  NlocSeg S;
+  double a[NP / 16][4];
+  double x[4][4];
+  int es;
+};
+
+template <int NP>
+__device__ inline void nloc_segment_loads(const NlocSeg& S,
+                                          const double* __restrict__ chia,
+                                          const int* __restrict__ eslot,
+                                          const double* __restrict__ X, int nb,
+                                          int b0, int lane,
+                                          NlocSegOps<NP>& o) {
+  o.S = S;
+  const int n = S.e1 - S.e0;
+  const int kp = (S.ncol + 15) >> 4;  // 16-column passes
+  const int mbs = (n + 15) >> 4;      // 16-row blocks
 #pragma unroll
-    for (int c = 0; c < kFwdCols; ++c) xv[c] = 0.0;
-    const double* src = part + P.poff * nb + bc;
-    for (int j = 0; j < P.nchunks; ++j) {
-      double x[kFwdCols];
+  for (int m = 0; m < NP / 16; ++m) {
+    const double2* src = reinterpret_cast<const double2*>(
+        chia + (S.a0 + (int64_t)min(m, mbs - 1) * kp) * 256 + lane * 4);
+    const double2 u = src[0], v = src[1];
+    o.a[m][0] = u.x;
+    o.a[m][1] = u.y;
+    o.a[m][2] = v.x;
+    o.a[m][3] = v.y;
+  }
+  o.es = eslot[S.e0 + min(lane, n - 1)];
 #pragma unroll
-      for (int c = 0; c < kFwdCols; ++c)
-        x[c] = src[((int64_t)j * P.ncol + min(c0 + c, P.ncol - 1)) * nb];
+  for (int s = 0; s < 4; ++s) {
+    const int c = 4 * s + (lane >> 4);
+    const double* xc = X + (int64_t)(S.coff + min(c, S.ncol - 1)) * nb;
 #pragma unroll
-      for (int c = 0; c < kFwdCols; ++c) xv[c] += x[c];
-    }
-#pragma unroll
-    for (int c = 0; c < kFwdCols; ++c)
-      xv[c] = (dv * xv[c]) * factor[P.coff + min(c0 + c, P.ncol - 1)];
-    if (c0 > 0) __syncthreads();  // the previous pass's chi is read
-    for (int e = tid; e < kFwdCols * kFwdRows; e += kFwdWaves * 64) {
-      const int r = e / kFwdCols, c = e % kFwdCols;
-      sh[e] = (c < nc && r < C.nr)
-                  ? chi[P.choff + (int64_t)(c0 + c) * P.nrow + C.r0 + r]
-                  : 0.0;
-    }
-    __syncthreads();
-    // One row at a time (its 16 chi values are eight 16-byte broadcasts);
-    // a projector of more than 16 columns continues from the sums its
-    // earlier passes stored.
-    const double2* ch =
-        reinterpret_cast<const double2*>(sh + w * kFwdRowsPerWave * kFwdCols);
-    for (int u = 0; u < nr; ++u) {
-      double t = (c0 > 0 && b < nb) ? dst[(int64_t)u * nb] : 0.0;
-#pragma unroll
-      for (int q = 0; q < kFwdCols / 2; ++q) {
-        const double2 x = ch[u * (kFwdCols / 2) + q];
-        t = fma(x.x, xv[2 * q], t);
-        t = fma(x.y, xv[2 * q + 1], t);
-      }
-      if (b < nb) dst[(int64_t)u * nb] = t;
+    for (int j = 0; j < 4; ++j) {
+      const double v = xc[min(b0 + 16 * j + (lane & 15), nb - 1)];
+      o.x[s][j] = c < S.ncol ? v : 0.0;
     }
   }
 }
 
-// result[g][b] += the temp rows of grid point g in projector order.  One
-// lane per band, a wave per (grid point, 64 bands); grid point u (sorted
-// list `point`) receives rows rows_of[ptr[u] .. ptr[u+1]), loaded four at a
-// time from clamped positions and added in order.
-__global__ void nloc_scatter_kernel(const double* __restrict__ temp, int nb,
-                                    const int* __restrict__ point,
-                                    const int* __restrict__ ptr,
-                                    const int* __restrict__ rows_of,
-                                    int64_t npoints, int Nd, int tshift,
-                                    double* __restrict__ result) {
-  const int b = blockIdx.x * 64 + (threadIdx.x & 63);
+// Adds one segment's row products T = chi_seg X_p (rows x 64 bands) to the
+// block's values in LDS.  Each 16-row block is 16 MFMAs (4 column steps x
+// 4 band blocks; columns past 16 in further passes, loaded there); lane l
+// then holds T[16 m + 4 v + l / 16][16 j + l % 16] and adds it to its row's
+// point.  The rows' points are distinct, so every (point, band) value is
+// read and written once per segment.
+template <int NP>
+__device__ inline void nloc_segment_apply(const NlocSegOps<NP>& o,
+                                          double* __restrict__ acc,
+                                          const double* __restrict__ chia,
+                                          const double* __restrict__ X, int nb,
+                                          int b0, int lane) {
+  const NlocSeg& S = o.S;
+  const int n = S.e1 - S.e0;
+  const int kp = (S.ncol + 15) >> 4;
+#pragma unroll
+  for (int m = 0; m < NP / 16; ++m) {
+    if (16 * m >= n) break;
+    v4d d[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) d[j] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        d[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.a[m][s], o.x[s][j], d[j],
+                                                    0, 0, 0);
+    for (int p = 1; p < kp; ++p) {
+      const double* ap = chia + (S.a0 + (int64_t)m * kp + p) * 256 + lane * 4;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int c = 16 * p + 4 * s + (lane >> 4);
+        const double* xc = X + (int64_t)(S.coff + min(c, S.ncol - 1)) * nb;
+        const double av = ap[s];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const double v = xc[min(b0 + 16 * j + (lane & 15), nb - 1)];
+          d[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              av, c < S.ncol ? v : 0.0, d[j], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int e = 16 * m + 4 * v + (lane >> 4);
+      const int sl = __shfl(o.es, min(e, n - 1));
+      double* ap = acc + sl * 64 + (lane & 15);
+      if (e < n) {
+        double t[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) t[j] = ap[16 * j];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ap[16 * j] = t[j] + d[j][v];
+      }
+    }
+  }
+  __syncthreads();  // the next segment's lanes read what these wrote
+}
+
+// result[g][b] += sum over the rows (p, r) touching grid point g, in
+// projector order (then row order), of sum_c chi_p[r][c] X_p[c][b].  The
+// touched points, ascending, are cut into blocks of NP; a block's rows,
+// ordered by (projector, row), are cut into segments of one projector whose
+// rows touch distinct points of the block.  One wave per (block, 64 bands):
+// the block's result values go to LDS ([point][band]) and the segments are
+// applied in order, each one's operands loaded two segments ahead (three
+// register sets in rotation, the loop unrolled by three so that no set is
+// copied: a copy would wait for its loads).  A point's additions all happen
+// on one wave in projector order, so the sums keep the reference's order
+// of projectors with no atomics, and no per-row products are stored.
+// (The row products were first formed with FMAs on chi values read from
+// LDS as broadcasts: 16 values per row, each broadcast returning 1 KB to
+// the wave, bound the kernel by LDS bandwidth at 102 us on 32768 points x
+// 520 bands with 64 projectors.)
+template <int NP>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
+void nloc_apply_kernel(
+    const double* __restrict__ X, int nb, const double* __restrict__ chia,
+    const int* __restrict__ eslot, const NlocSeg* __restrict__ seg,
+    const NlocBlk* __restrict__ blk, const int* __restrict__ point, int Nd,
+    int tshift, double* __restrict__ result) {
+  static_assert(NP % 16 == 0 && NP <= 64, "block points");
+  __shared__ double acc[NP * 64];
+  const int lane = threadIdx.x;
+  const int b0 = blockIdx.y * 64;
+  const int b = b0 + lane;
   const int bc = min(b, nb - 1);
   const int T = 1 << tshift;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  for (int64_t u = (int64_t)blockIdx.y * 4 + wv; u < npoints;
-       u += (int64_t)gridDim.y * 4) {
-    double* dst =
-        result + ((int64_t)(bc >> tshift) * Nd + point[u]) * T + (bc & (T - 1));
-    double acc = *dst;
-    const int j0 = ptr[u], j1 = ptr[u + 1];
-    for (int j = j0; j < j1; j += 4) {
-      double x[4];
+  const int64_t boff = (int64_t)(bc >> tshift) * Nd * T + (bc & (T - 1));
+  const NlocBlk B = blk[blockIdx.x];
+  const int* pts = point + B.u0;  // NP entries (the table is padded)
+  double r[NP];
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
-        x[k] = temp[(int64_t)rows_of[min(j + k, j1 - 1)] * nb + bc];
+  for (int s = 0; s < NP; ++s) r[s] = result[boff + (int64_t)pts[s] * T];
+  NlocSegOps<NP> o0, o1, o2;
+  nloc_segment_loads<NP>(seg[B.s0], chia, eslot, X, nb, b0, lane, o0);
+  nloc_segment_loads<NP>(seg[min(B.s0 + 1, B.s1 - 1)], chia, eslot, X, nb,
+                         b0, lane, o1);
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
-        if (j + k < j1) acc += x[k];
-    }
-    if (b < nb) *dst = acc;
+  for (int s = 0; s < NP; ++s) acc[s * 64 + lane] = r[s];
+  __syncthreads();
+  for (int si = B.s0;; si += 3) {
+    nloc_segment_loads<NP>(seg[min(si + 2, B.s1 - 1)], chia, eslot, X, nb, b0,
+                           lane, o2);
+    nloc_segment_apply<NP>(o0, acc, chia, X, nb, b0, lane);
+    if (si + 1 >= B.s1) break;
+    nloc_segment_loads<NP>(seg[min(si + 3, B.s1 - 1)], chia, eslot, X, nb, b0,
+                           lane, o0);
+    nloc_segment_apply<NP>(o1, acc, chia, X, nb, b0, lane);
+    if (si + 2 >= B.s1) break;
+    nloc_segment_loads<NP>(seg[min(si + 4, B.s1 - 1)], chia, eslot, X, nb, b0,
+                           lane, o1);
+    nloc_segment_apply<NP>(o2, acc, chia, X, nb, b0, lane);
+    if (si + 3 >= B.s1) break;
   }
-}
-
-dim3 rows_grid(int nb, int64_t rows) {
-  const int64_t by = std::min<int64_t>((rows + 3) / 4, 65535);
-  return dim3((nb + 63) / 64, (unsigned)std::max<int64_t>(by, 1));
+#pragma unroll
+  for (int s = 0; s < NP; ++s)
+    if (s < B.np && b < nb) result[boff + (int64_t)pts[s] * T] = acc[s * 64 + lane];
 }
 
 }  // namespace
@@ -249,25 +349,28 @@ struct skgemm_nloc {
   int64_t rows = 0, cols = 0, nchunks = 0, parts = 0;
   // device copies
   int* index = nullptr;       // rows
-  double* chi = nullptr;      // sum nrow * ncol
+  double* chif = nullptr;     // chunks' chi values in the forward A layout
   double* factor = nullptr;   // cols
   NlocProj* proj = nullptr;   // num_proj
   NlocChunk* chunks = nullptr;  // nchunks
-  int* point = nullptr;       // npoints
-  int* ptr = nullptr;         // npoints + 1
-  int* rows_of = nullptr;     // rows touching each point, in order
-  int64_t npoints = 0;
+  int* colproj = nullptr;     // cols: the projector of each X column
+  int* point = nullptr;       // npoints, then kBlkPoints copies of the last
+  NlocBlk* blk = nullptr;     // nblocks
+  NlocSeg* seg = nullptr;     // segments of all blocks
+  int* eslot = nullptr;       // rows in block order: their block slot
+  double* chia = nullptr;     // segments' chi rows in the MFMA A layout
+  int64_t npoints = 0, nblocks = 0;
   // workspace, grown with nb
   double* part = nullptr;     // parts x nb
-  double* temp = nullptr;     // rows x nb
+  double* X = nullptr;        // cols x nb
   int ws_nb = 0;
 };
 
 static void nloc_free(skgemm_nloc* h) {
-  for (void* p : {(void*)h->index, (void*)h->chi, (void*)h->factor,
-                  (void*)h->proj, (void*)h->chunks, (void*)h->point,
-                  (void*)h->ptr, (void*)h->rows_of, (void*)h->part,
-                  (void*)h->temp})
+  for (void* p : {(void*)h->index, (void*)h->chif, (void*)h->factor,
+                  (void*)h->proj, (void*)h->chunks, (void*)h->colproj,
+                  (void*)h->point, (void*)h->blk, (void*)h->seg,
+                  (void*)h->eslot, (void*)h->chia, (void*)h->part, (void*)h->X})
     if (p) hipFree(p);
 }
 
@@ -341,49 +444,119 @@ extern "C" int skgemm_nloc_create(skgemm_ctx* c, int Nd, int num_proj,
   // get none: they contribute nothing).
   std::vector<NlocProj> meta((size_t)num_proj);
   std::vector<NlocChunk> chunks;
+  // Each chunk's chi values in the forward kernel's A layout: per 16-column
+  // pass q a block of 4096 doubles, wave w's lane l holding at
+  // 8 (64 w + l) + s the value of column 16 q + l % 16, chunk row
+  // 32 w + 4 s + l / 16 (zero past the chunk's rows and the columns).
+  std::vector<double> chif;
   int64_t poff = 0;
   for (int p = 0; p < num_proj; ++p) {
     NlocProj m{roff[p], choff[p], coff[p], nrow[p], ncol[p], 0, 0, poff};
     m.chunk0 = (int)chunks.size();
+    const int kp = (ncol[p] + 15) / 16;
     if (ncol[p] > 0)
-      for (int r0 = 0; r0 < nrow[p]; r0 += kFwdRows)
-        chunks.push_back(NlocChunk{p, r0, std::min(kFwdRows, nrow[p] - r0)});
+      for (int r0 = 0; r0 < nrow[p]; r0 += kFwdRows) {
+        const int nr = std::min(kFwdRows, nrow[p] - r0);
+        chunks.push_back(
+            NlocChunk{p, r0, nr, (int64_t)(chif.size() / (kFwdRows * 16))});
+        for (int q = 0; q < kp; ++q)
+          for (int i = 0; i < kFwdRows * 16; ++i) {
+            const int s = i % 8, l = (i / 8) % 64, w = i / 512;
+            const int row = 32 * w + 4 * s + l / 16, col = 16 * q + l % 16;
+            chif.push_back(row < nr && col < ncol[p]
+                               ? chi_all[choff[p] + (int64_t)col * nrow[p] +
+                                         r0 + row]
+                               : 0.0);
+          }
+      }
     m.nchunks = (int)chunks.size() - m.chunk0;
     poff += (int64_t)m.nchunks * ncol[p];
     meta[p] = m;
   }
   h->nchunks = (int64_t)chunks.size();
   h->parts = poff;
-  // Scatter table: grid points in ascending order, each with its rows in
-  // projector order (then row order), skipping projectors without columns.
-  std::vector<std::pair<int, int64_t>> touch;
+  // Point blocks: the touched grid points in ascending order (projectors
+  // without columns skipped), kBlkPoints to a block; a block's rows sorted
+  // by global row, i.e. by (projector, row), and cut into segments at each
+  // new projector and before a row whose point the segment already has.
+  std::vector<std::pair<int, int>> touch;  // (grid point, global row)
   touch.reserve((size_t)rows);
+  std::vector<int> rowproj((size_t)rows);
   for (int p = 0; p < num_proj; ++p) {
+    for (int r = 0; r < nrow[p]; ++r) rowproj[roff[p] + r] = p;
     if (ncol[p] == 0) continue;
     for (int r = 0; r < nrow[p]; ++r)
-      touch.push_back({index_all[roff[p] + r], roff[p] + r});
+      touch.push_back({index_all[roff[p] + r], (int)(roff[p] + r)});
   }
-  std::stable_sort(touch.begin(), touch.end(),
-                   [](const std::pair<int, int64_t>& a,
-                      const std::pair<int, int64_t>& b) { return a.first < b.first; });
-  std::vector<int> point, ptr(1, 0), rows_of;
-  rows_of.reserve(touch.size());
-  for (size_t i = 0; i < touch.size(); ++i) {
-    if (i == 0 || touch[i].first != touch[i - 1].first) {
-      point.push_back(touch[i].first);
-      if (i > 0) ptr.push_back((int)i);
+  std::sort(touch.begin(), touch.end());
+  std::vector<int> point, colproj((size_t)cols);
+  std::vector<NlocBlk> blks;
+  std::vector<NlocSeg> segs;
+  std::vector<int> eslot;   // the block slot of each row, in block order
+  std::vector<int> erow;    // its global row (host only)
+  std::vector<double> chia;  // chi rows of each segment in the MFMA A layout
+  eslot.reserve(touch.size());
+  erow.reserve(touch.size());
+  for (int p = 0; p < num_proj; ++p)
+    for (int j = 0; j < ncol[p]; ++j) colproj[coff[p] + j] = p;
+  std::vector<std::pair<int, int>> brow;  // (global row, slot)
+  for (size_t i = 0; i < touch.size();) {
+    NlocBlk B{(int)point.size(), 0, (int)segs.size(), 0};
+    brow.clear();
+    while (i < touch.size() && B.np < kBlkPoints) {
+      const int g = touch[i].first;
+      for (; i < touch.size() && touch[i].first == g; ++i)
+        brow.push_back({touch[i].second, B.np});
+      point.push_back(g);
+      ++B.np;
     }
-    rows_of.push_back((int)touch[i].second);
+    std::sort(brow.begin(), brow.end());
+    uint64_t seen = 0;
+    for (size_t e = 0; e < brow.size(); ++e) {
+      const int R = brow[e].first, s = brow[e].second, p = rowproj[R];
+      if (e == 0 || p != rowproj[brow[e - 1].first] || ((seen >> s) & 1)) {
+        if (e > 0) segs.back().e1 = (int)eslot.size();
+        segs.push_back(
+            NlocSeg{(int)eslot.size(), 0, (int)coff[p], ncol[p], 0});
+        seen = 0;
+      }
+      seen |= 1ull << s;
+      eslot.push_back(s);
+      erow.push_back(R);
+    }
+    segs.back().e1 = (int)eslot.size();
+    B.s1 = (int)segs.size();
+    blks.push_back(B);
   }
-  if (!touch.empty()) ptr.push_back((int)touch.size());
+  // Each segment's chi rows in the A-operand layout of v_mfma_f64_16x16x4:
+  // per 16-row block m and 16-column pass q, 256 doubles, lane l's four
+  // values (s = 0..3) at 4 l + s: row 16 m + l % 16 of the segment, column
+  // 16 q + 4 s + l / 16 (zero past the rows and columns).
+  for (NlocSeg& S : segs) {
+    const int n = S.e1 - S.e0, kp = (S.ncol + 15) / 16, p = colproj[S.coff];
+    S.a0 = (int64_t)(chia.size() / 256);
+    for (int m = 0; m < (n + 15) / 16; ++m)
+      for (int q = 0; q < kp; ++q)
+        for (int l = 0; l < 64; ++l)
+          for (int s4 = 0; s4 < 4; ++s4) {
+            const int e = 16 * m + l % 16, c = 16 * q + 4 * s4 + l / 16;
+            chia.push_back(e < n && c < S.ncol
+                               ? chi_all[choff[p] + (int64_t)c * nrow[p] +
+                                         (erow[S.e0 + e] - roff[p])]
+                               : 0.0);
+          }
+  }
   h->npoints = (int64_t)point.size();
+  h->nblocks = (int64_t)blks.size();
+  if (!point.empty()) point.resize(point.size() + kBlkPoints, point.back());
   int rc = hipSetDevice(c->device) == hipSuccess
                ? SKGEMM_OK
                : set_error(SKGEMM_ERR_HIP, "hipSetDevice");
   if (rc == SKGEMM_OK)
     rc = upload(&h->index, index_all, (size_t)rows, "index", c->stream);
   if (rc == SKGEMM_OK)
-    rc = upload(&h->chi, chi_all, (size_t)choff[num_proj], "chi", c->stream);
+    rc = upload(&h->chif, chif.data(), chif.size(), "chi (forward)",
+                c->stream);
   if (rc == SKGEMM_OK)
     rc = upload(&h->factor, factor.data(), factor.size(), "gamma", c->stream);
   if (rc == SKGEMM_OK)
@@ -393,12 +566,19 @@ extern "C" int skgemm_nloc_create(skgemm_ctx* c, int Nd, int num_proj,
     rc = upload(&h->chunks, chunks.data(), chunks.size(), "chunk table",
                 c->stream);
   if (rc == SKGEMM_OK)
-    rc = upload(&h->point, point.data(), point.size(), "points", c->stream);
-  if (rc == SKGEMM_OK && h->npoints > 0)
-    rc = upload(&h->ptr, ptr.data(), ptr.size(), "point table", c->stream);
-  if (rc == SKGEMM_OK)
-    rc = upload(&h->rows_of, rows_of.data(), rows_of.size(), "row list",
+    rc = upload(&h->colproj, colproj.data(), colproj.size(), "column table",
                 c->stream);
+  if (rc == SKGEMM_OK)
+    rc = upload(&h->point, point.data(), point.size(), "points", c->stream);
+  if (rc == SKGEMM_OK)
+    rc = upload(&h->blk, blks.data(), blks.size(), "block table", c->stream);
+  if (rc == SKGEMM_OK)
+    rc = upload(&h->seg, segs.data(), segs.size(), "segment table", c->stream);
+  if (rc == SKGEMM_OK)
+    rc = upload(&h->eslot, eslot.data(), eslot.size(), "row slots",
+                c->stream);
+  if (rc == SKGEMM_OK)
+    rc = upload(&h->chia, chia.data(), chia.size(), "chi blocks", c->stream);
   if (rc != SKGEMM_OK) {
     nloc_free(h);
     delete h;
@@ -426,14 +606,14 @@ extern "C" int skgemm_nloc_apply(skgemm_nloc* h, int nb, double dv,
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
   if (nb > h->ws_nb) {
     SKG_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-    for (double** p : {&h->part, &h->temp}) {
+    for (double** p : {&h->part, &h->X}) {
       if (*p) hipFree(*p);
       *p = nullptr;
     }
     h->ws_nb = 0;
     if (hipMalloc(&h->part, (size_t)std::max<int64_t>(h->parts, 1) * nb *
                                 sizeof(double)) != hipSuccess ||
-        hipMalloc(&h->temp, (size_t)h->rows * nb * sizeof(double)) != hipSuccess)
+        hipMalloc(&h->X, (size_t)h->cols * nb * sizeof(double)) != hipSuccess)
       return set_error(SKGEMM_ERR_NOMEM, "nloc workspace for %d bands", nb);
     h->ws_nb = nb;
     ++c->mem_gen;
@@ -442,17 +622,18 @@ extern "C" int skgemm_nloc_apply(skgemm_nloc* h, int nb, double dv,
   if (h->nchunks == 0) return SKGEMM_OK;  // no projector has columns
   const dim3 cgrid((unsigned)h->nchunks, bgroups);
   hipLaunchKernelGGL(nloc_forward_kernel, cgrid, dim3(kFwdWaves * 64), 0,
-                     c->stream, psi16, h->Nd, nb, h->index, h->chi, h->proj,
+                     c->stream, psi16, h->Nd, nb, h->index, h->chif, h->proj,
                      h->chunks, h->part);
   SKG_HIP(hipGetLastError(), "nloc forward launch");
-  hipLaunchKernelGGL(nloc_backward_kernel, cgrid, dim3(kFwdWaves * 64), 0,
-                     c->stream, h->part, h->proj, h->chunks, h->factor, dv,
-                     h->chi, nb, h->temp);
-  SKG_HIP(hipGetLastError(), "nloc backward launch");
-  hipLaunchKernelGGL(nloc_scatter_kernel, rows_grid(nb, h->npoints), dim3(256),
-                     0, c->stream, h->temp, nb, h->point, h->ptr, h->rows_of,
-                     h->npoints, h->Nd, result_tile == 32 ? 5 : 4, result);
-  SKG_HIP(hipGetLastError(), "nloc scatter launch");
+  hipLaunchKernelGGL(nloc_x_kernel, dim3((unsigned)((h->cols + 3) / 4), bgroups),
+                     dim3(256), 0, c->stream, h->part, h->proj, h->colproj,
+                     h->factor, dv, nb, h->cols, h->X);
+  SKG_HIP(hipGetLastError(), "nloc X launch");
+  hipLaunchKernelGGL(nloc_apply_kernel<kBlkPoints>,
+                     dim3((unsigned)h->nblocks, bgroups), dim3(64), 0,
+                     c->stream, h->X, nb, h->chia, h->eslot, h->seg, h->blk,
+                     h->point, h->Nd, result_tile == 32 ? 5 : 4, result);
+  SKG_HIP(hipGetLastError(), "nloc apply launch");
   return SKGEMM_OK;
 }
 

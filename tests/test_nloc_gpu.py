@@ -75,6 +75,36 @@ def test_shared_points_repeated_atoms_empty_projectors(skg, ctx, orc):
     assert rel_fro(got - res0, want - res0) < TOL
 
 
+def test_repeated_rows_and_crowded_points(skg, ctx, orc):
+    """A projector that lists one grid point several times (each row adds in
+    row order), 40 projectors of 16-20 columns sharing one point (more rows
+    than a block holds points), and runs of consecutive grid points longer
+    than a block."""
+    K, nb = 300, 70
+    rng = np.random.default_rng(5)
+    nrows, ncols, atoms, idx = [], [], [], []
+    rep = rng.permutation(K)[:30]
+    rep[[3, 9, 17]] = rep[0]                 # point rep[0] four times
+    nrows.append(30); ncols.append(7); atoms.append(1); idx.append(rep)
+    for p in range(40):                       # all of them touch point 7
+        pts = rng.permutation(K)[:25]
+        pts[pts == 7] = (7 + 1 + p) % K
+        pts[p % 25] = 7
+        nrows.append(25); ncols.append(16 + p % 5); atoms.append(2 + p // 3)
+        idx.append(pts)
+    nrows.append(K); ncols.append(3); atoms.append(99); idx.append(np.arange(K))
+    idx = np.concatenate(idx)
+    chi = rng.uniform(-1, 1, sum(r * c for r, c in zip(nrows, ncols)))
+    gam = rng.uniform(-1, 1, sum(ncols))
+    proj = (nrows, ncols, atoms, idx, chi, gam)
+    for tile in (32, 16):
+        psi16 = band_pack_cm(rng.uniform(-1, 1, K * nb), K, nb, 16)
+        res0 = band_pack_cm(rng.uniform(-1, 1, K * nb), K, nb, tile)
+        got = run_gpu(skg, ctx, K, nb, 0.5, proj, psi16, res0, tile)
+        want = oracle(orc, K, nb, 0.5, proj, psi16, res0, tile)
+        assert rel_fro(got - res0, want - res0) < TOL
+
+
 def c2_projectors(rng, K, count):
     """Compact projector spheres: nrow 512..1024 consecutive-ish grid
     indices around random centres, ncol in {9, 13, 16}."""
