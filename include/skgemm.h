@@ -344,8 +344,8 @@ int skgemm_nloc_create(skgemm_ctx* ctx, int Nd, int num_proj, const int* nrow,
                        const int* index_all, const double* chi_all,
                        const double* gamma_all, skgemm_nloc** out);
 /* Device pointers, asynchronous on the context's stream: psi16
- * [ceil(nb/16)][Nd][16], result [ceil(nb/T)][Nd][T] with T = result_tile
- * (32 or 16).  The handle's workspace grows with nb. */
+ * [ceil(nb/16)][Nd][16] (16-byte aligned), result [ceil(nb/T)][Nd][T] with
+ * T = result_tile (32 or 16).  The handle's workspace grows with nb. */
 int skgemm_nloc_apply(skgemm_nloc* h, int nb, double dv, const double* psi16,
                       double* result, int result_tile);
 int skgemm_nloc_destroy(skgemm_nloc* h);

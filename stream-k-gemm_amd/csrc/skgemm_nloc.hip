@@ -52,6 +52,14 @@ struct NlocChunk {
   int64_t f0;  // its chi values in the forward A layout, in blocks of 4096
 };
 typedef double v4d __attribute__((ext_vector_type(4)));
+
+// Unit u (of `units`) for hardware workgroup g of a one-dimensional grid:
+// workgroup g runs on XCD g % 8, so each XCD gets a contiguous range of
+// units (neighbouring units share data through that XCD's L2).
+__device__ inline int xcd_unit(int g, int units) {
+  const int per = units >> 3, rem = units & 7, xcd = g & 7;
+  return xcd * per + (xcd < rem ? xcd : rem) + (g >> 3);
+}
 // A segment of a point block's rows: rows e0 .. e1 of the block-ordered row
 // list, all of one projector (X columns coff .., ncol of them), touching
 // distinct points of the block; its chi rows are stored from a0 on, in
@@ -89,26 +97,36 @@ __global__ __launch_bounds__(kFwdWaves * 64, 4) void nloc_forward_kernel(
     const int* __restrict__ index, const double* __restrict__ chif,
     const NlocProj* __restrict__ proj, const NlocChunk* __restrict__ chunks,
     double* __restrict__ part) {
-  __shared__ double red[kFwdWaves * 16 * 64];  // each wave's 16 x 64 sums
+  __shared__ __align__(16) double red[kFwdWaves * 16 * 64];  // each wave's 16 x 64 sums
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int b0 = blockIdx.y * 64;
-  const NlocChunk C = chunks[blockIdx.x];
+  // units chunk-major, the band groups of one chunk on one XCD
+  const int u = xcd_unit(blockIdx.x, gridDim.x);
+  const int bgroups = (nb + 63) >> 6;
+  const int ci = u / bgroups;
+  const int b0 = (u - ci * bgroups) * 64;
+  const NlocChunk C = chunks[ci];
   const NlocProj P = proj[C.proj];
   const int* idx = index + P.roff;
   const int rlast = C.r0 + C.nr - 1;
+  // lane l's bands: 4 (l % 16) + j, j = 0..3, four adjacent values of one
+  // 16-band panel (a band group past nb reads the last panel's group: its
+  // values only reach outputs that are not stored)
+  const int bq = min(b0 + 4 * (lane & 15), ((nb + 15) & ~15) - 4);
+  const double* src = psi16 + (int64_t)(bq >> 4) * Nd * 16 + (bq & 15);
   double v[8][4];
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
     const int g = idx[min(C.r0 + 32 * w + 4 * s + (lane >> 4), rlast)];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int bb = min(b0 + 16 * j + (lane & 15), nb - 1);
-      v[s][j] = psi16[((int64_t)(bb >> 4) * Nd + g) * 16 + (bb & 15)];
-    }
+    const double2* pv = reinterpret_cast<const double2*>(src + (int64_t)g * 16);
+    const double2 x0 = pv[0], x1 = pv[1];
+    v[s][0] = x0.x;
+    v[s][1] = x0.y;
+    v[s][2] = x1.x;
+    v[s][3] = x1.y;
   }
-  double* dst = part + (P.poff + (int64_t)(blockIdx.x - P.chunk0) * P.ncol) * nb;
+  double* dst = part + (P.poff + (int64_t)(ci - P.chunk0) * P.ncol) * nb;
   const int kp = (P.ncol + 15) >> 4;
   for (int q = 0; q < kp; ++q) {
     const double2* cf = reinterpret_cast<const double2*>(
@@ -130,13 +148,14 @@ __global__ __launch_bounds__(kFwdWaves * 64, 4) void nloc_forward_kernel(
         d[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], v[s][j], d[j], 0, 0,
                                                     0);
     if (q > 0) __syncthreads();  // the previous pass's sums are read
-    // lane l holds column 4 u + l / 16, band 16 j + l % 16
+    // lane l holds column 4 u + l / 16, band 4 (l % 16) + j
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        red[(w * 16 + 4 * u + (lane >> 4)) * 64 + 16 * j + (lane & 15)] =
-            d[j][u];
+    for (int u = 0; u < 4; ++u) {
+      double2* rp = reinterpret_cast<double2*>(
+          red + (w * 16 + 4 * u + (lane >> 4)) * 64 + 4 * (lane & 15));
+      rp[0] = make_double2(d[0][u], d[1][u]);
+      rp[1] = make_double2(d[2][u], d[3][u]);
+    }
     __syncthreads();
     // wave w adds up columns 2 w and 2 w + 1 over the waves, in wave order
 #pragma unroll
@@ -153,20 +172,24 @@ __global__ __launch_bounds__(kFwdWaves * 64, 4) void nloc_forward_kernel(
 
 // X_p[c][b] = (dv sum_j part[poff_p + j ncol_p + c][b]) factor_p[c], the
 // chunk parts added in chunk order; one thread per (column, band), four
-// columns per workgroup.
+// columns per workgroup.  X rows are xs = 64 ceil(nb / 64) long, zero past
+// nb.
 __global__ __launch_bounds__(256) void nloc_x_kernel(
     const double* __restrict__ part, const NlocProj* __restrict__ proj,
     const int* __restrict__ colproj, const double* __restrict__ factor,
-    double dv, int nb, int64_t cols, double* __restrict__ X) {
+    double dv, int nb, int xs, int64_t cols, double* __restrict__ X) {
   const int64_t col = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int b = blockIdx.y * 64 + (threadIdx.x & 63);
-  if (col >= cols || b >= nb) return;
-  const NlocProj P = proj[colproj[col]];
-  const int c = (int)(col - P.coff);
-  const double* src = part + (P.poff + c) * nb + b;
+  if (col >= cols) return;
   double x = 0.0;
-  for (int j = 0; j < P.nchunks; ++j) x += src[(int64_t)j * P.ncol * nb];
-  X[col * nb + b] = (dv * x) * factor[col];
+  if (b < nb) {
+    const NlocProj P = proj[colproj[col]];
+    const int c = (int)(col - P.coff);
+    const double* src = part + (P.poff + c) * nb + b;
+    for (int j = 0; j < P.nchunks; ++j) x += src[(int64_t)j * P.ncol * nb];
+    x = (dv * x) * factor[col];
+  }
+  X[col * xs + b] = x;
 }
 
 // One segment's operands in registers, loaded ahead of its turn: its chi
@@ -174,7 +197,7 @@ __global__ __launch_bounds__(256) void nloc_x_kernel(
 // columns 4 s + l / 16, s = 0..3, for each block m of 16 rows; the host
 // stores them so, zero past the segment's rows and the projector's
 // columns), its first 16 X columns in the B layout (lane l: column
-// 4 s + l / 16, band 16 j + l % 16 of the wave's 64), and in lane k the
+// 4 s + l / 16, band 4 (l % 16) + j of the wave's 64), and in lane k the
 // block slot of the segment's row k.
 template <int NP>
 struct NlocSegOps {
@@ -188,7 +211,7 @@ template <int NP>
 __device__ inline void nloc_segment_loads(const NlocSeg& S,
                                           const double* __restrict__ chia,
                                           const int* __restrict__ eslot,
-                                          const double* __restrict__ X, int nb,
+                                          const double* __restrict__ X, int xs,
                                           int b0, int lane,
                                           NlocSegOps<NP>& o) {
   o.S = S;
@@ -209,26 +232,29 @@ __device__ inline void nloc_segment_loads(const NlocSeg& S,
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const int c = 4 * s + (lane >> 4);
-    const double* xc = X + (int64_t)(S.coff + min(c, S.ncol - 1)) * nb;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const double v = xc[min(b0 + 16 * j + (lane & 15), nb - 1)];
-      o.x[s][j] = c < S.ncol ? v : 0.0;
-    }
+    const double2* xc = reinterpret_cast<const double2*>(
+        X + (int64_t)(S.coff + min(c, S.ncol - 1)) * xs + b0 +
+        4 * (lane & 15));
+    const double2 x0 = xc[0], x1 = xc[1];
+    const bool in = c < S.ncol;
+    o.x[s][0] = in ? x0.x : 0.0;
+    o.x[s][1] = in ? x0.y : 0.0;
+    o.x[s][2] = in ? x1.x : 0.0;
+    o.x[s][3] = in ? x1.y : 0.0;
   }
 }
 
 // Adds one segment's row products T = chi_seg X_p (rows x 64 bands) to the
 // block's values in LDS.  Each 16-row block is 16 MFMAs (4 column steps x
 // 4 band blocks; columns past 16 in further passes, loaded there); lane l
-// then holds T[16 m + 4 v + l / 16][16 j + l % 16] and adds it to its row's
+// then holds T[16 m + 4 v + l / 16][4 (l % 16) + j] and adds it to its row's
 // point.  The rows' points are distinct, so every (point, band) value is
 // read and written once per segment.
 template <int NP>
 __device__ inline void nloc_segment_apply(const NlocSegOps<NP>& o,
                                           double* __restrict__ acc,
                                           const double* __restrict__ chia,
-                                          const double* __restrict__ X, int nb,
+                                          const double* __restrict__ X, int xs,
                                           int b0, int lane) {
   const NlocSeg& S = o.S;
   const int n = S.e1 - S.e0;
@@ -250,27 +276,24 @@ __device__ inline void nloc_segment_apply(const NlocSegOps<NP>& o,
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         const int c = 16 * p + 4 * s + (lane >> 4);
-        const double* xc = X + (int64_t)(S.coff + min(c, S.ncol - 1)) * nb;
+        const double* xc = X + (int64_t)(S.coff + min(c, S.ncol - 1)) * xs +
+                           b0 + 4 * (lane & 15);
         const double av = ap[s];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const double v = xc[min(b0 + 16 * j + (lane & 15), nb - 1)];
+        for (int j = 0; j < 4; ++j)
           d[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-              av, c < S.ncol ? v : 0.0, d[j], 0, 0, 0);
-        }
+              av, c < S.ncol ? xc[j] : 0.0, d[j], 0, 0, 0);
       }
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int e = 16 * m + 4 * v + (lane >> 4);
       const int sl = __shfl(o.es, min(e, n - 1));
-      double* ap = acc + sl * 64 + (lane & 15);
+      double2* ap = reinterpret_cast<double2*>(acc + sl * 64 + 4 * (lane & 15));
       if (e < n) {
-        double t[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) t[j] = ap[16 * j];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) ap[16 * j] = t[j] + d[j][v];
+        const double2 t0 = ap[0], t1 = ap[1];
+        ap[0] = make_double2(t0.x + d[0][v], t0.y + d[1][v]);
+        ap[1] = make_double2(t1.x + d[2][v], t1.y + d[3][v]);
       }
     }
   }
@@ -300,37 +323,45 @@ void nloc_apply_kernel(
     const NlocBlk* __restrict__ blk, const int* __restrict__ point, int Nd,
     int tshift, double* __restrict__ result) {
   static_assert(NP % 16 == 0 && NP <= 64, "block points");
-  __shared__ double acc[NP * 64];
+  __shared__ __align__(16) double acc[NP * 64];
   const int lane = threadIdx.x;
-  const int b0 = blockIdx.y * 64;
+  // units band-group-major (the result streams through one pair of band
+  // panels at a time), a run of neighbouring blocks, which share
+  // projectors, on each XCD
+  const int u = xcd_unit(blockIdx.x, gridDim.x);
+  const int nblk = gridDim.x / ((nb + 63) >> 6);
+  const int bg = u / nblk;
+  const int bi = u - bg * nblk;
+  const int b0 = bg * 64;
+  const int xs = (nb + 63) & ~63;  // X row length
   const int b = b0 + lane;
   const int bc = min(b, nb - 1);
   const int T = 1 << tshift;
   const int64_t boff = (int64_t)(bc >> tshift) * Nd * T + (bc & (T - 1));
-  const NlocBlk B = blk[blockIdx.x];
+  const NlocBlk B = blk[bi];
   const int* pts = point + B.u0;  // NP entries (the table is padded)
   double r[NP];
 #pragma unroll
   for (int s = 0; s < NP; ++s) r[s] = result[boff + (int64_t)pts[s] * T];
   NlocSegOps<NP> o0, o1, o2;
-  nloc_segment_loads<NP>(seg[B.s0], chia, eslot, X, nb, b0, lane, o0);
-  nloc_segment_loads<NP>(seg[min(B.s0 + 1, B.s1 - 1)], chia, eslot, X, nb,
+  nloc_segment_loads<NP>(seg[B.s0], chia, eslot, X, xs, b0, lane, o0);
+  nloc_segment_loads<NP>(seg[min(B.s0 + 1, B.s1 - 1)], chia, eslot, X, xs,
                          b0, lane, o1);
 #pragma unroll
   for (int s = 0; s < NP; ++s) acc[s * 64 + lane] = r[s];
   __syncthreads();
   for (int si = B.s0;; si += 3) {
-    nloc_segment_loads<NP>(seg[min(si + 2, B.s1 - 1)], chia, eslot, X, nb, b0,
+    nloc_segment_loads<NP>(seg[min(si + 2, B.s1 - 1)], chia, eslot, X, xs, b0,
                            lane, o2);
-    nloc_segment_apply<NP>(o0, acc, chia, X, nb, b0, lane);
+    nloc_segment_apply<NP>(o0, acc, chia, X, xs, b0, lane);
     if (si + 1 >= B.s1) break;
-    nloc_segment_loads<NP>(seg[min(si + 3, B.s1 - 1)], chia, eslot, X, nb, b0,
+    nloc_segment_loads<NP>(seg[min(si + 3, B.s1 - 1)], chia, eslot, X, xs, b0,
                            lane, o0);
-    nloc_segment_apply<NP>(o1, acc, chia, X, nb, b0, lane);
+    nloc_segment_apply<NP>(o1, acc, chia, X, xs, b0, lane);
     if (si + 2 >= B.s1) break;
-    nloc_segment_loads<NP>(seg[min(si + 4, B.s1 - 1)], chia, eslot, X, nb, b0,
+    nloc_segment_loads<NP>(seg[min(si + 4, B.s1 - 1)], chia, eslot, X, xs, b0,
                            lane, o1);
-    nloc_segment_apply<NP>(o2, acc, chia, X, nb, b0, lane);
+    nloc_segment_apply<NP>(o2, acc, chia, X, xs, b0, lane);
     if (si + 3 >= B.s1) break;
   }
 #pragma unroll
@@ -602,6 +633,8 @@ extern "C" int skgemm_nloc_apply(skgemm_nloc* h, int nb, double dv,
   if (psi16 == nullptr || result == nullptr)
     return set_error(SKGEMM_ERR_ARG, "psi16 or result is NULL");
   SKG_TRY(check_dev(psi16, "psi16"));
+  if ((uintptr_t)psi16 % 16 != 0)
+    return set_error(SKGEMM_ERR_ARG, "psi16 must be 16-byte aligned");
   SKG_TRY(check_dev(result, "result"));
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
   if (nb > h->ws_nb) {
@@ -613,24 +646,28 @@ extern "C" int skgemm_nloc_apply(skgemm_nloc* h, int nb, double dv,
     h->ws_nb = 0;
     if (hipMalloc(&h->part, (size_t)std::max<int64_t>(h->parts, 1) * nb *
                                 sizeof(double)) != hipSuccess ||
-        hipMalloc(&h->X, (size_t)h->cols * nb * sizeof(double)) != hipSuccess)
+        hipMalloc(&h->X, (size_t)h->cols * ((nb + 63) & ~63) *
+                             sizeof(double)) != hipSuccess)
       return set_error(SKGEMM_ERR_NOMEM, "nloc workspace for %d bands", nb);
     h->ws_nb = nb;
     ++c->mem_gen;
   }
   const unsigned bgroups = (unsigned)((nb + 63) / 64);
   if (h->nchunks == 0) return SKGEMM_OK;  // no projector has columns
-  const dim3 cgrid((unsigned)h->nchunks, bgroups);
-  hipLaunchKernelGGL(nloc_forward_kernel, cgrid, dim3(kFwdWaves * 64), 0,
+  if (h->nchunks * bgroups >= INT32_MAX || h->nblocks * bgroups >= INT32_MAX)
+    return set_error(SKGEMM_ERR_NOTSUP, "nloc: too many workgroups");
+  hipLaunchKernelGGL(nloc_forward_kernel,
+                     dim3((unsigned)(h->nchunks * bgroups)),
+                     dim3(kFwdWaves * 64), 0,
                      c->stream, psi16, h->Nd, nb, h->index, h->chif, h->proj,
                      h->chunks, h->part);
   SKG_HIP(hipGetLastError(), "nloc forward launch");
   hipLaunchKernelGGL(nloc_x_kernel, dim3((unsigned)((h->cols + 3) / 4), bgroups),
                      dim3(256), 0, c->stream, h->part, h->proj, h->colproj,
-                     h->factor, dv, nb, h->cols, h->X);
+                     h->factor, dv, nb, (int)(bgroups * 64), h->cols, h->X);
   SKG_HIP(hipGetLastError(), "nloc X launch");
   hipLaunchKernelGGL(nloc_apply_kernel<kBlkPoints>,
-                     dim3((unsigned)h->nblocks, bgroups), dim3(64), 0,
+                     dim3((unsigned)(h->nblocks * bgroups)), dim3(64), 0,
                      c->stream, h->X, nb, h->chia, h->eslot, h->seg, h->blk,
                      h->point, h->Nd, result_tile == 32 ? 5 : 4, result);
   SKG_HIP(hipGetLastError(), "nloc apply launch");
