@@ -75,6 +75,7 @@ struct NlocBlk {
   int u0, np, s0, s1;
 };
 constexpr int kBlkPoints = 32;  // points per block of the apply kernel
+constexpr int kApplyNJ = 4;     // its waves cover 16 kApplyNJ bands
 constexpr int kFwdWaves = 8, kFwdRowsPerWave = 32;
 constexpr int kFwdRows = kFwdWaves * kFwdRowsPerWave;  // rows per chunk
 
@@ -192,28 +193,47 @@ __global__ __launch_bounds__(256) void nloc_x_kernel(
   X[col * xs + b] = x;
 }
 
+// A wave of the apply kernel covers W = 16 NJ bands: lane l's bands are
+// NJ (l % 16) + j, j < NJ (NJ adjacent values).
+
+// Loads NJ adjacent doubles (16-byte pieces where NJ >= 2; p is 8 NJ-byte
+// aligned).
+template <int NJ>
+__device__ inline void load_bands(const double* __restrict__ p, double* v) {
+  if constexpr (NJ == 1) {
+    v[0] = p[0];
+  } else {
+#pragma unroll
+    for (int h = 0; h < NJ / 2; ++h) {
+      const double2 x = reinterpret_cast<const double2*>(p)[h];
+      v[2 * h] = x.x;
+      v[2 * h + 1] = x.y;
+    }
+  }
+}
+
 // One segment's operands in registers, loaded ahead of its turn: its chi
 // rows in the A layout of v_mfma_f64_16x16x4 (lane l: rows 16 m + l % 16,
 // columns 4 s + l / 16, s = 0..3, for each block m of 16 rows; the host
 // stores them so, zero past the segment's rows and the projector's
 // columns), its first 16 X columns in the B layout (lane l: column
-// 4 s + l / 16, band 4 (l % 16) + j of the wave's 64), and in lane k the
+// 4 s + l / 16, bands NJ (l % 16) + j of the wave's W), and in lane k the
 // block slot of the segment's row k.
-template <int NP>
+template <int NP, int NJ>
 struct NlocSegOps {
   NlocSeg S;
   double a[NP / 16][4];
-  double x[4][4];
+  double x[4][NJ];
   int es;
 };
 
-template <int NP>
+template <int NP, int NJ>
 __device__ inline void nloc_segment_loads(const NlocSeg& S,
                                           const double* __restrict__ chia,
                                           const int* __restrict__ eslot,
                                           const double* __restrict__ X, int xs,
                                           int b0, int lane,
-                                          NlocSegOps<NP>& o) {
+                                          NlocSegOps<NP, NJ>& o) {
   o.S = S;
   const int n = S.e1 - S.e0;
   const int kp = (S.ncol + 15) >> 4;  // 16-column passes
@@ -232,43 +252,41 @@ __device__ inline void nloc_segment_loads(const NlocSeg& S,
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const int c = 4 * s + (lane >> 4);
-    const double2* xc = reinterpret_cast<const double2*>(
-        X + (int64_t)(S.coff + min(c, S.ncol - 1)) * xs + b0 +
-        4 * (lane & 15));
-    const double2 x0 = xc[0], x1 = xc[1];
-    const bool in = c < S.ncol;
-    o.x[s][0] = in ? x0.x : 0.0;
-    o.x[s][1] = in ? x0.y : 0.0;
-    o.x[s][2] = in ? x1.x : 0.0;
-    o.x[s][3] = in ? x1.y : 0.0;
+    double v[NJ];
+    load_bands<NJ>(X + (int64_t)(S.coff + min(c, S.ncol - 1)) * xs + b0 +
+                       NJ * (lane & 15),
+                   v);
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) o.x[s][j] = c < S.ncol ? v[j] : 0.0;
   }
 }
 
-// Adds one segment's row products T = chi_seg X_p (rows x 64 bands) to the
-// block's values in LDS.  Each 16-row block is 16 MFMAs (4 column steps x
-// 4 band blocks; columns past 16 in further passes, loaded there); lane l
-// then holds T[16 m + 4 v + l / 16][4 (l % 16) + j] and adds it to its row's
-// point.  The rows' points are distinct, so every (point, band) value is
-// read and written once per segment.
-template <int NP>
-__device__ inline void nloc_segment_apply(const NlocSegOps<NP>& o,
+// Adds one segment's row products T = chi_seg X_p (rows x W bands) to the
+// block's values in LDS ([point][W]).  Each 16-row block is 4 NJ MFMAs (4
+// column steps x NJ band blocks; columns past 16 in further passes, loaded
+// there); lane l then holds T[16 m + 4 v + l / 16][NJ (l % 16) + j] and adds
+// it to its row's point.  The rows' points are distinct, so every (point,
+// band) value is read and written once per segment.
+template <int NP, int NJ>
+__device__ inline void nloc_segment_apply(const NlocSegOps<NP, NJ>& o,
                                           double* __restrict__ acc,
                                           const double* __restrict__ chia,
                                           const double* __restrict__ X, int xs,
                                           int b0, int lane) {
+  constexpr int W = 16 * NJ;
   const NlocSeg& S = o.S;
   const int n = S.e1 - S.e0;
   const int kp = (S.ncol + 15) >> 4;
 #pragma unroll
   for (int m = 0; m < NP / 16; ++m) {
     if (16 * m >= n) break;
-    v4d d[4];
+    v4d d[NJ];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) d[j] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < NJ; ++j) d[j] = v4d{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < NJ; ++j)
         d[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.a[m][s], o.x[s][j], d[j],
                                                     0, 0, 0);
     for (int p = 1; p < kp; ++p) {
@@ -277,10 +295,10 @@ __device__ inline void nloc_segment_apply(const NlocSegOps<NP>& o,
       for (int s = 0; s < 4; ++s) {
         const int c = 16 * p + 4 * s + (lane >> 4);
         const double* xc = X + (int64_t)(S.coff + min(c, S.ncol - 1)) * xs +
-                           b0 + 4 * (lane & 15);
+                           b0 + NJ * (lane & 15);
         const double av = ap[s];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < NJ; ++j)
           d[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
               av, c < S.ncol ? xc[j] : 0.0, d[j], 0, 0, 0);
       }
@@ -289,11 +307,18 @@ __device__ inline void nloc_segment_apply(const NlocSegOps<NP>& o,
     for (int v = 0; v < 4; ++v) {
       const int e = 16 * m + 4 * v + (lane >> 4);
       const int sl = __shfl(o.es, min(e, n - 1));
-      double2* ap = reinterpret_cast<double2*>(acc + sl * 64 + 4 * (lane & 15));
+      double* ap = acc + sl * W + NJ * (lane & 15);
       if (e < n) {
-        const double2 t0 = ap[0], t1 = ap[1];
-        ap[0] = make_double2(t0.x + d[0][v], t0.y + d[1][v]);
-        ap[1] = make_double2(t1.x + d[2][v], t1.y + d[3][v]);
+        if constexpr (NJ == 1) {
+          ap[0] += d[0][v];
+        } else {
+#pragma unroll
+          for (int h = 0; h < NJ / 2; ++h) {
+            double2* a2 = reinterpret_cast<double2*>(ap) + h;
+            const double2 t = *a2;
+            *a2 = make_double2(t.x + d[2 * h][v], t.y + d[2 * h + 1][v]);
+          }
+        }
       }
     }
   }
@@ -304,69 +329,78 @@ __device__ inline void nloc_segment_apply(const NlocSegOps<NP>& o,
 // projector order (then row order), of sum_c chi_p[r][c] X_p[c][b].  The
 // touched points, ascending, are cut into blocks of NP; a block's rows,
 // ordered by (projector, row), are cut into segments of one projector whose
-// rows touch distinct points of the block.  One wave per (block, 64 bands):
-// the block's result values go to LDS ([point][band]) and the segments are
-// applied in order, each one's operands loaded two segments ahead (three
-// register sets in rotation, the loop unrolled by three so that no set is
-// copied: a copy would wait for its loads).  A point's additions all happen
-// on one wave in projector order, so the sums keep the reference's order
-// of projectors with no atomics, and no per-row products are stored.
-// (The row products were first formed with FMAs on chi values read from
-// LDS as broadcasts: 16 values per row, each broadcast returning 1 KB to
-// the wave, bound the kernel by LDS bandwidth at 102 us on 32768 points x
-// 520 bands with 64 projectors.)
-template <int NP>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
+// rows touch distinct points of the block.  One wave per (block, W = 16 NJ
+// bands): the block's result values go to LDS ([point][W]) and the
+// segments are applied in order, each one's operands loaded one segment
+// ahead (two register sets in turn, the loop unrolled by two so that no set
+// is copied: a copy would wait for its loads).  A point's additions all
+// happen on one wave in projector order, so the sums keep the reference's
+// order of projectors with no atomics, and no per-row products are stored.
+// Units are band-group-major (the result streams through one band range at
+// a time) with a run of neighbouring blocks, which share projectors, on
+// each XCD.
+// Measured on 32768 points x 520 bands with 64 projectors, W = 64: the row
+// products as FMAs on chi values read from LDS as broadcasts (16 per row,
+// each returning 1 KB to the wave) were bound by LDS bandwidth at 102 us;
+// with MFMA, three register sets two segments ahead took 248 registers (two
+// waves per SIMD) and 75.5 us against 71.2 us for two sets (three waves per
+// SIMD); blocks of 16 points took 89 us.
+template <int NP, int NJ>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3)))
 void nloc_apply_kernel(
     const double* __restrict__ X, int nb, const double* __restrict__ chia,
     const int* __restrict__ eslot, const NlocSeg* __restrict__ seg,
     const NlocBlk* __restrict__ blk, const int* __restrict__ point, int Nd,
     int tshift, double* __restrict__ result) {
   static_assert(NP % 16 == 0 && NP <= 64, "block points");
-  __shared__ __align__(16) double acc[NP * 64];
+  static_assert(NJ == 1 || NJ == 2 || NJ == 4, "band blocks per wave");
+  constexpr int W = 16 * NJ;
+  constexpr int kPer = NP * W / 64;  // result values per lane
+  __shared__ __align__(16) double acc[NP * W];
   const int lane = threadIdx.x;
-  // units band-group-major (the result streams through one pair of band
-  // panels at a time), a run of neighbouring blocks, which share
-  // projectors, on each XCD
   const int u = xcd_unit(blockIdx.x, gridDim.x);
-  const int nblk = gridDim.x / ((nb + 63) >> 6);
+  const int nblk = gridDim.x / ((nb + W - 1) / W);
   const int bg = u / nblk;
   const int bi = u - bg * nblk;
-  const int b0 = bg * 64;
+  const int b0 = bg * W;
   const int xs = (nb + 63) & ~63;  // X row length
-  const int b = b0 + lane;
-  const int bc = min(b, nb - 1);
   const int T = 1 << tshift;
-  const int64_t boff = (int64_t)(bc >> tshift) * Nd * T + (bc & (T - 1));
   const NlocBlk B = blk[bi];
   const int* pts = point + B.u0;  // NP entries (the table is padded)
-  double r[NP];
+  // value i of lane l: point (64 i + l) / W, band b0 + l % W
+  const int bw = lane % W;
+  const int bc = min(b0 + bw, nb - 1);
+  const int64_t boff = (int64_t)(bc >> tshift) * Nd * T + (bc & (T - 1));
+  // the grid point of value i (uniform when W = 64: scalar loads)
+  int gp[kPer];
 #pragma unroll
-  for (int s = 0; s < NP; ++s) r[s] = result[boff + (int64_t)pts[s] * T];
-  NlocSegOps<NP> o0, o1, o2;
-  nloc_segment_loads<NP>(seg[B.s0], chia, eslot, X, xs, b0, lane, o0);
-  nloc_segment_loads<NP>(seg[min(B.s0 + 1, B.s1 - 1)], chia, eslot, X, xs,
-                         b0, lane, o1);
+  for (int i = 0; i < kPer; ++i)
+    gp[i] = W == 64 ? pts[i] : pts[(64 * i + lane) / W];
+  double r[kPer];
 #pragma unroll
-  for (int s = 0; s < NP; ++s) acc[s * 64 + lane] = r[s];
+  for (int i = 0; i < kPer; ++i) r[i] = result[boff + (int64_t)gp[i] * T];
+  NlocSegOps<NP, NJ> o0, o1;
+  nloc_segment_loads<NP, NJ>(seg[B.s0], chia, eslot, X, xs, b0, lane, o0);
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) acc[64 * i + lane] = r[i];
   __syncthreads();
-  for (int si = B.s0;; si += 3) {
-    nloc_segment_loads<NP>(seg[min(si + 2, B.s1 - 1)], chia, eslot, X, xs, b0,
-                           lane, o2);
-    nloc_segment_apply<NP>(o0, acc, chia, X, xs, b0, lane);
+  for (int si = B.s0;; si += 2) {
+    nloc_segment_loads<NP, NJ>(seg[min(si + 1, B.s1 - 1)], chia, eslot, X, xs,
+                               b0, lane, o1);
+    nloc_segment_apply<NP, NJ>(o0, acc, chia, X, xs, b0, lane);
     if (si + 1 >= B.s1) break;
-    nloc_segment_loads<NP>(seg[min(si + 3, B.s1 - 1)], chia, eslot, X, xs, b0,
-                           lane, o0);
-    nloc_segment_apply<NP>(o1, acc, chia, X, xs, b0, lane);
+    nloc_segment_loads<NP, NJ>(seg[min(si + 2, B.s1 - 1)], chia, eslot, X, xs,
+                               b0, lane, o0);
+    nloc_segment_apply<NP, NJ>(o1, acc, chia, X, xs, b0, lane);
     if (si + 2 >= B.s1) break;
-    nloc_segment_loads<NP>(seg[min(si + 4, B.s1 - 1)], chia, eslot, X, xs, b0,
-                           lane, o1);
-    nloc_segment_apply<NP>(o2, acc, chia, X, xs, b0, lane);
-    if (si + 3 >= B.s1) break;
   }
 #pragma unroll
-  for (int s = 0; s < NP; ++s)
-    if (s < B.np && b < nb) result[boff + (int64_t)pts[s] * T] = acc[s * 64 + lane];
+  for (int i = 0; i < kPer; ++i) {
+    const int sp = W == 64 ? i : (64 * i + lane) / W;
+    const int g = W == 64 ? pts[i] : gp[i];  // W = 64: a scalar reload
+    if (sp < B.np && b0 + bw < nb)
+      result[boff + (int64_t)g * T] = acc[64 * i + lane];
+  }
 }
 
 }  // namespace
@@ -654,7 +688,8 @@ extern "C" int skgemm_nloc_apply(skgemm_nloc* h, int nb, double dv,
   }
   const unsigned bgroups = (unsigned)((nb + 63) / 64);
   if (h->nchunks == 0) return SKGEMM_OK;  // no projector has columns
-  if (h->nchunks * bgroups >= INT32_MAX || h->nblocks * bgroups >= INT32_MAX)
+  const int64_t agroups = (nb + 16 * kApplyNJ - 1) / (16 * kApplyNJ);
+  if (h->nchunks * bgroups >= INT32_MAX || h->nblocks * agroups >= INT32_MAX)
     return set_error(SKGEMM_ERR_NOTSUP, "nloc: too many workgroups");
   hipLaunchKernelGGL(nloc_forward_kernel,
                      dim3((unsigned)(h->nchunks * bgroups)),
@@ -666,8 +701,8 @@ extern "C" int skgemm_nloc_apply(skgemm_nloc* h, int nb, double dv,
                      dim3(256), 0, c->stream, h->part, h->proj, h->colproj,
                      h->factor, dv, nb, (int)(bgroups * 64), h->cols, h->X);
   SKG_HIP(hipGetLastError(), "nloc X launch");
-  hipLaunchKernelGGL(nloc_apply_kernel<kBlkPoints>,
-                     dim3((unsigned)(h->nblocks * bgroups)), dim3(64), 0,
+  hipLaunchKernelGGL((nloc_apply_kernel<kBlkPoints, kApplyNJ>),
+                     dim3((unsigned)(h->nblocks * agroups)), dim3(64), 0,
                      c->stream, h->X, nb, h->chia, h->eslot, h->seg, h->blk,
                      h->point, h->Nd, result_tile == 32 ? 5 : 4, result);
   SKG_HIP(hipGetLastError(), "nloc apply launch");
