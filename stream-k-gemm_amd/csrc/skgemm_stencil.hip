@@ -32,7 +32,7 @@ struct StencilCoef {
 
 // One workgroup: 16 band lanes x an 8 x 8 block of (i, j) points of one
 // 16-band tile (1024 threads), marching through kStencilPlanes planes of k;
-// grid (ceil(ni/8) ceil(nj/8), ceil(nk/kStencilPlanes), tiles16).  A thread
+// ceil(ni/8) ceil(nj/8) ceil(nk/kStencilPlanes) tiles16 workgroups.  A thread
 // keeps its column's 2F + 1 values along k in registers (one new z load per
 // plane).  Per plane the block's centres and the cross-shaped x / y halo
 // (F points either side of each block row and column, 32 F points) go into
@@ -59,13 +59,23 @@ __attribute__((amdgpu_waves_per_eu(F <= 6 ? 8 : 4, 8))) void stencil_kernel(
   const int tid = threadIdx.x;
   const int bl = tid & 15;
   const int p = tid >> 4;                    // 0..63
+  // Work unit u = (tile, k chunk, i-j block), block fastest; hardware
+  // workgroup g runs on XCD g % 8 and each XCD takes a contiguous range of
+  // units, so the blocks and k chunks of one tile (whose x / y halos and z
+  // prologues overlap) meet in one L2.
   const int nbi = (ni + kSB - 1) / kSB;
-  const int i0 = (blockIdx.x % nbi) * kSB, j0 = (blockIdx.x / nbi) * kSB;
+  const int nblk = nbi * ((nj + kSB - 1) / kSB);
+  const int nkc = (nk + kStencilPlanes - 1) / kStencilPlanes;
+  const int units = gridDim.x;
+  const int xcd = blockIdx.x & 7, per = units >> 3, xr = units & 7;
+  const int u = xcd * per + (xcd < xr ? xcd : xr) + (blockIdx.x >> 3);
+  const int blkid = u % nblk;
+  const int i0 = (blkid % nbi) * kSB, j0 = (blkid / nbi) * kSB;
   const int pi = p % kSB, pj = p / kSB;
   const int i = i0 + pi, j = j0 + pj;
-  const int k0 = blockIdx.y * kStencilPlanes;
+  const int k0 = (u / nblk) % nkc * kStencilPlanes;
   const int k1 = k0 + kStencilPlanes < nk ? k0 + kStencilPlanes : nk;
-  const int t16 = blockIdx.z;
+  const int t16 = u / (nblk * nkc);
   const int b = t16 * 16 + bl;
   const bool band_ok = b < nb;
   const bool own = band_ok && i < ni && j < nj;
@@ -85,7 +95,7 @@ __attribute__((amdgpu_waves_per_eu(F <= 6 ? 8 : 4, 8))) void stencil_kernel(
   // block row h / (2F), offset h % (2F) -> i = -F .. -1, kSB .. kSB + F - 1;
   // else the y halo of block column (h') / (2F) likewise along j.
   int hoff[kHs];     // LDS offset (points) of halo point m, -1: none
-  int64_t hcol[kHs]; // extended column index (i + F) + (j + F) sj
+  int hcol[kHs];     // extended column index (i + F) + (j + F) sj
 #pragma unroll
   for (int m = 0; m < kHs; ++m) {
     const int h = p + 64 * m;
@@ -110,12 +120,13 @@ __attribute__((amdgpu_waves_per_eu(F <= 6 ? 8 : 4, 8))) void stencil_kernel(
                                            : (i0 + line < ni && gj < nj + F);
       if (need) {
         hoff[m] = (hj + F) * E + (hi + F);
-        hcol[m] = (int64_t)(gi + F) + (int64_t)(gj + F) * sj;
+        hcol[m] = (gi + F) + (gj + F) * nie;
       }
     }
   }
   // Column of this thread's own point.
-  const int64_t col = (int64_t)(i + F) + (int64_t)(j + F) * sj;
+  // (32-bit: the host keeps an extended plane below 2^31 points)
+  const int col = (i + F) + (j + F) * nie;
   double q[2 * F + 1];  // q[F + d] = psi at plane k + d
   double hv[kHs];
 #pragma unroll
@@ -127,9 +138,17 @@ __attribute__((amdgpu_waves_per_eu(F <= 6 ? 8 : 4, 8))) void stencil_kernel(
       hv[m] = hoff[m] >= 0 ? s[(hcol[m] + (int64_t)(k + F) * sk) * 16] : 0.0;
   };
   load_halo(k0);
+  // The column's value at extended plane k + 2F is loaded one plane ahead
+  // (qn), and the diagonal before the plane's LDS stores and barrier, so
+  // neither load is waited for right after it is issued.
+  const int64_t gcol = i + (int64_t)j * ni;
+  double qn = inext ? s[(col + (int64_t)(k0 + 2 * F) * sk) * 16] : 0.0;
   const int me = ((pj + F) * E + (pi + F)) * 16 + bl;
   for (int k = k0; k < k1; ++k) {
-    q[2 * F] = inext ? s[(col + (int64_t)(k + 2 * F) * sk) * 16] : 0.0;
+    q[2 * F] = qn;
+    const double ak = own ? A_diag[gcol + (int64_t)k * ni * nj] : 0.0;
+    if (k + 1 < k1)
+      qn = inext ? s[(col + (int64_t)(k + 1 + 2 * F) * sk) * 16] : 0.0;
     sl[me] = q[F];
 #pragma unroll
     for (int m = 0; m < kHs; ++m)
@@ -140,11 +159,11 @@ __attribute__((amdgpu_waves_per_eu(F <= 6 ? 8 : 4, 8))) void stencil_kernel(
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
     if (own) {
-      const int64_t g = i + (int64_t)j * ni + (int64_t)k * ni * nj;
+      const int64_t g = gcol + (int64_t)k * ni * nj;
       // Same order as the reference's scalar version: centre, x terms, y
       // terms, z terms, r ascending in each (each term one fused
       // multiply-add).
-      double res = q[F] * (A_diag[g] + c.c0);
+      double res = q[F] * (ak + c.c0);
 #pragma unroll
       for (int r = 1; r <= F; ++r)
         res = fma(c.x[r], sl[me + r * 16] + sl[me - r * 16], res);
@@ -219,7 +238,9 @@ int check_grid(int ni, int nj, int nk, int nb, int F) {
   // stencil (blocks of the i-j plane, k chunks, tiles).
   const int64_t halo_y = (nk + 2 * F + kHaloPlanes - 1) / kHaloPlanes * tiles;
   const int64_t st_x = (int64_t)((ni + kSB - 1) / kSB) * ((nj + kSB - 1) / kSB);
-  if (ex * 16 >= ((int64_t)1 << 40) || halo_y > 65535 || tiles > 65535 ||
+  const int64_t plane = (int64_t)(ni + 2 * F) * (nj + 2 * F);
+  if (ex * 16 >= ((int64_t)1 << 40) || plane >= 0x7fffffff ||
+      halo_y > 65535 || tiles > 65535 ||
       st_x > 0x7fffffff ||
       (nk + kStencilPlanes - 1) / kStencilPlanes > 65535)
     return set_error(SKGEMM_ERR_NOTSUP, "grid too large for one launch");
@@ -283,8 +304,11 @@ extern "C" int skgemm_stencil_apply(skgemm_ctx* c, int ni, int nj, int nk,
   cf.c0 = coef_0;
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
   const int kchunks = (nk + kStencilPlanes - 1) / kStencilPlanes;
-  const dim3 grid(((ni + kSB - 1) / kSB) * ((nj + kSB - 1) / kSB), kchunks,
-                  (nb + 15) / 16);
+  const int64_t units = (int64_t)((ni + kSB - 1) / kSB) *
+                        ((nj + kSB - 1) / kSB) * kchunks * ((nb + 15) / 16);
+  if (units >= 0x7fffffff)
+    return set_error(SKGEMM_ERR_NOTSUP, "stencil: too many workgroups");
+  const dim3 grid((unsigned)units);
   const int ts = out_tile == 32 ? 5 : 4;
   c->record_begin();
 #define SKG_STENCIL(F)                                                     \
