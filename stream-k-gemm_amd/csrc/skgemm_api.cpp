@@ -1261,12 +1261,14 @@ extern "C" int skgemm_copy_probe(skgemm_ctx* c, size_t bytes, double* gbps) {
   }
   hipMemsetAsync(a, 0, bytes, c->stream);
   const int64_t n = bytes / 8;
-  launch_copy_probe((double*)a, (double*)b, n, c->stream);
+  e = launch_copy_probe((double*)a, (double*)b, n, c->stream);
   hipEventRecord(c->ev0, c->stream);
   const int reps = 10;
-  for (int i = 0; i < reps; ++i) launch_copy_probe((double*)a, (double*)b, n, c->stream);
+  for (int i = 0; i < reps && e == hipSuccess; ++i)
+    e = launch_copy_probe((double*)a, (double*)b, n, c->stream);
   hipEventRecord(c->ev1, c->stream);
-  e = hipEventSynchronize(c->ev1);
+  if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+  else hipEventSynchronize(c->ev1);
   float ms = 0.f;
   hipEventElapsedTime(&ms, c->ev0, c->ev1);
   hipFree(a);

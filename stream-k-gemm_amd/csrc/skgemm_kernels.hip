@@ -1455,16 +1455,31 @@ hipError_t launch_peak_probe(int which, int blocks, int iters, double* sink,
   return hipGetLastError();
 }
 
-__global__ void copy_kernel(const double2* __restrict__ src,
-                            double2* __restrict__ dst, int64_t n2) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n2;
-       i += (int64_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
+// Each workgroup copies one contiguous 64 KB chunk (256 threads, four
+// 16-byte loads each in flight): 5.36-5.5 TB/s on 1 GiB, against 4.8 TB/s
+// for a grid-stride loop whose workgroups interleave at 4 KB over the whole
+// buffer (reads alone 5.2-5.5 TB/s, writes alone 3.9-4.4 TB/s).
+constexpr int64_t kCopyChunk = 4096;  // 16-byte elements per workgroup
+__global__ __launch_bounds__(256) void copy_kernel(
+    const double2* __restrict__ src, double2* __restrict__ dst, int64_t n2) {
+  const int64_t b0 = blockIdx.x * kCopyChunk;
+  const int64_t b1 = b0 + kCopyChunk < n2 ? b0 + kCopyChunk : n2;
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += 4 * 256) {
+    double2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (i + 256 * u < b1) v[u] = src[i + 256 * u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (i + 256 * u < b1) dst[i + 256 * u] = v[u];
+  }
 }
 hipError_t launch_copy_probe(const double* src, double* dst, int64_t n,
                              hipStream_t s) {
   const int64_t n2 = n / 2;
-  hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(256), 0, s,
+  const int64_t blocks = (n2 + kCopyChunk - 1) / kCopyChunk;
+  if (blocks <= 0 || blocks >= 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, s,
                      reinterpret_cast<const double2*>(src),
                      reinterpret_cast<double2*>(dst), n2);
   return hipGetLastError();
