@@ -2,7 +2,9 @@
 the oracle's restatement of include/nloc_packed_opt2.hpp and the reference
 test's unpacked formulation: the reference test's five configurations for
 both output packings, shared grid points, repeated atoms, empty projectors,
-a c2-sized case (Nd 32768, 520 bands), determinism and argument errors.
+points listed twice by one projector, crowded points, band counts that
+leave partial band groups (bands past nb untouched), a c2-sized case (Nd
+32768, 520 bands), determinism and argument errors.
 Bar: the added term (result after - before) within 1e-13 relative Frobenius
 of the oracle's."""
 import numpy as np
@@ -103,6 +105,39 @@ def test_repeated_rows_and_crowded_points(skg, ctx, orc):
         got = run_gpu(skg, ctx, K, nb, 0.5, proj, psi16, res0, tile)
         want = oracle(orc, K, nb, 0.5, proj, psi16, res0, tile)
         assert rel_fro(got - res0, want - res0) < TOL
+
+
+@pytest.mark.parametrize("nb", [1, 3, 17, 65, 100, 130])
+def test_odd_band_counts_random_projectors(skg, ctx, orc, nb):
+    """Band counts that leave partial 16- and 64-band groups, random
+    projector sets with 1..24 columns (past 16: the extra column pass),
+    repeated atoms and grid points listed twice by one projector."""
+    K = 600
+    rng = np.random.default_rng(1000 + nb)
+    count = 9
+    nrows = [int(rng.integers(1, 200)) for _ in range(count)]
+    ncols = [int(rng.integers(1, 25)) for _ in range(count)]
+    atoms = sorted(int(a) for a in rng.integers(1, 5, count))
+    idx = []
+    for n in nrows:
+        pts = rng.integers(0, K, n)          # repeats within a projector too
+        idx.append(pts)
+    idx = np.concatenate(idx)
+    chi = rng.uniform(-1, 1, sum(r * c for r, c in zip(nrows, ncols)))
+    gam = rng.uniform(-1, 1, sum(ncols))
+    proj = (nrows, ncols, atoms, idx, chi, gam)
+    for tile in (32, 16):
+        print("nb", nb, "tile", tile, flush=True)
+        psi16 = band_pack_cm(rng.uniform(-1, 1, K * nb), K, nb, 16)
+        res0 = band_pack_cm(rng.uniform(-1, 1, K * nb), K, nb, tile)
+        got = run_gpu(skg, ctx, K, nb, 0.3, proj, psi16, res0, tile)
+        want = oracle(orc, K, nb, 0.3, proj, psi16, res0, tile)
+        assert rel_fro(got - res0, want - res0) < TOL
+        # bands past nb in the last packed tile are not written
+        pad = np.ones(((nb + tile - 1) // tile, K, tile), bool)
+        for b in range(nb):
+            pad[b // tile, :, b % tile] = False
+        assert np.array_equal(got[pad.ravel()], res0[pad.ravel()])
 
 
 def c2_projectors(rng, K, count):
