@@ -35,6 +35,10 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
+// Stage count flag of a DMA configuration: two K steps per barrier (four
+// stages as two pairs).
+constexpr int kPairSteps = 0x100;
+
 static const TileCfg kCfgs[] = {
     {128, 128, 16, 2, 2},  // 0: 4 waves, 64x64 per wave
     {64, 64, 16, 2, 2},    // 1: 4 waves, 32x32 per wave
@@ -57,6 +61,9 @@ static const TileCfg kCfgs[] = {
     {64, 64, 16, 2, 2, 2, 1},    // 14: as 12 with 2 stages
     {128, 128, 16, 4, 2, 2},     // 15: strip twin of 13 (batched tables)
     {64, 64, 16, 2, 2, 2},       // 16: strip twin of 14
+    // Paired steps: 4 stages as two pairs, one barrier per two K steps.
+    {64, 64, 16, 2, 2, 4 | kPairSteps, 1},  // 17: as 12, paired steps
+    {64, 64, 16, 2, 2, 4 | kPairSteps},     // 18: strip twin of 17
 };
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
@@ -541,6 +548,8 @@ __device__ __forceinline__ void mac_loop_dma(
   using TB = DmaTile<BN, BK, BKC>;
   using TX = DmaTile<16, BK, (XT & kStripRow) != 0 ? AKC : BKC>;
   using F = DmaFrag<BM, BN, BK, WM, WN>;
+  constexpr int NSt = NS & 0xff;  // stages
+  constexpr bool PAIR = (NS & kPairSteps) != 0;
   constexpr int NW = WM * WN;
   constexpr int NDA = TA::kNumDma, NDB = TB::kNumDma;
   static_assert((NDA + NDB) % NW == 0, "DMA instructions vs waves");
@@ -549,8 +558,8 @@ __device__ __forceinline__ void mac_loop_dma(
   constexpr int kRegA = dma_region<BM, BK>();
   constexpr int kRegB = dma_region<BN, BK>();
   constexpr int kStage = kRegA + kRegB + (XT != 0 ? dma_region<16, BK>() : 0);
-  constexpr int kAhead = (NS - 2) * DPW;
-  constexpr int kAheadX = (NS - 2) * (DPW + 1);
+  constexpr int kAhead = (NSt - 2) * DPW;
+  constexpr int kAheadX = (NSt - 2) * (DPW + 1);
   static_assert(kAheadX < 64, "vmcnt range");
   // vmcnt: this step's copies have landed; lgkmcnt 0: this wave's fragment
   // reads are complete, so after the barrier no wave still reads a stage
@@ -660,8 +669,74 @@ __device__ __forceinline__ void mac_loop_dma(
     asm volatile("" ::: "memory");
   };
 
+  if constexpr (PAIR) {
+    // Two K steps per barrier: stages {0, 1} and {2, 3} hold one pair each.
+    // At a barrier every copy issued so far has landed (the next pair's,
+    // issued one pair earlier) and every wave has read the pair it just
+    // multiplied out; the freed pair of stages then takes the pair after
+    // next.  The strip fragment (one register set) is read for a step after
+    // the previous step's last use of it.
+    static_assert(NSt == 4, "paired steps: two pairs of stages");
+    constexpr int kP0 = 2 * DPW, kP0X = 2 * (DPW + 1);
+    constexpr int kW0 = (kP0 & 15) | ((kP0 >> 4) << 14) | (7 << 4);
+    constexpr int kW0X = (kP0X & 15) | ((kP0X >> 4) << 14) | (7 << 4);
+    constexpr int kWAll = 7 << 4;  // vmcnt(0) lgkmcnt(0)
+    auto barrier = [&](bool first) {
+      __builtin_amdgcn_sched_barrier(0);
+      if (!first)
+        __builtin_amdgcn_s_waitcnt(kWAll);
+      else if (xwave)
+        __builtin_amdgcn_s_waitcnt(kW0X);
+      else
+        __builtin_amdgcn_s_waitcnt(kW0);
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_s_barrier();
+      asm volatile("" ::: "memory");
+    };
 #pragma unroll
-  for (int s0 = 0; s0 < NS - 1; ++s0) issue(s0, kk + s0);
+    for (int s0 = 0; s0 < 4; ++s0) issue(s0, kk + s0);
+    barrier(true);  // pair 0 landed; pair 1 may be in flight
+    F f0, f1;
+    read(f0, 0);
+    readx(0);
+    int a = 0;  // first stage of the current pair
+#pragma unroll 1
+    for (; kk + 2 < kend; kk += 2) {
+      // step kk + 1's fragments are read once half of step kk's are dead
+      dma_mma<BM, BN, BK, WM, WN, MASKED, XT, 0, 2>(acc, xacc, cacc, f0, xf,
+                                                     mask, wm, wn);
+      read(f1, a + 1);
+      __builtin_amdgcn_sched_barrier(0);  // reads issue before the MFMAs
+      dma_mma<BM, BN, BK, WM, WN, MASKED, XT, 2, 4>(acc, xacc, cacc, f0, xf,
+                                                     mask, wm, wn);
+      readx(a + 1);
+      dma_mma<BM, BN, BK, WM, WN, MASKED, XT, 0, 3>(acc, xacc, cacc, f1, xf,
+                                                     mask, wm, wn);
+      barrier(false);
+      issue(a, kk + 4);
+      issue(a + 1, kk + 5);
+      a = 2 - a;
+      read(f0, a);
+      __builtin_amdgcn_sched_barrier(0);
+      dma_mma<BM, BN, BK, WM, WN, MASKED, XT, 3, 4>(acc, xacc, cacc, f1, xf,
+                                                     mask, wm, wn);
+      readx(a);
+    }
+    const bool two = kk + 1 < kend;
+    if (two) read(f1, a + 1);
+    dma_mma<BM, BN, BK, WM, WN, MASKED, XT, 0, 4>(acc, xacc, cacc, f0, xf,
+                                                   mask, wm, wn);
+    if (two) {
+      readx(a + 1);
+      dma_mma<BM, BN, BK, WM, WN, MASKED, XT, 0, 4>(acc, xacc, cacc, f1, xf,
+                                                     mask, wm, wn);
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // clamped tail copies
+    __syncthreads();
+    return;
+  }
+#pragma unroll
+  for (int s0 = 0; s0 < NSt - 1; ++s0) issue(s0, kk + s0);
   publish();
   F f0, f1;
   read(f0, 0);
@@ -669,11 +744,11 @@ __device__ __forceinline__ void mac_loop_dma(
   int st = 0;  // stage of step kk
 #pragma unroll 1
   for (; kk + 1 < kend; ++kk) {
-    issue(st == 0 ? NS - 1 : st - 1, kk + NS - 1);
+    issue(st == 0 ? NSt - 1 : st - 1, kk + NSt - 1);
     dma_mma<BM, BN, BK, WM, WN, MASKED, XT, 0, 3>(acc, xacc, cacc, f0, xf,
                                                    mask, wm, wn);
     publish();
-    st = st + 1 == NS ? 0 : st + 1;
+    st = st + 1 == NSt ? 0 : st + 1;
     read(f1, st);
     __builtin_amdgcn_sched_barrier(0);  // reads issue before the MFMAs
     dma_mma<BM, BN, BK, WM, WN, MASKED, XT, 3, 4>(acc, xacc, cacc, f0, xf,
@@ -704,9 +779,13 @@ __device__ __forceinline__ T load_uniform(const T* p) {
 // XS = kStripCol (kStripRow): tiles of the last tile column (row) may carry
 // the column (row) strip (Problem::xcol / xrow > 0; single-problem DMA
 // launches only).  One kernel per strip kind keeps each within registers.
+// (The paired-step row-strip kernel is held to 256 registers, two waves per
+// SIMD; unconstrained it took 262.)
 template <int BM, int BN, int BK, int WM, int WN, int NS, bool BATCHED,
           int XS = 0>
-__global__ __launch_bounds__(WM* WN * 64) void sk_dgemm_kernel(Launch L) {
+__global__ __launch_bounds__(WM* WN * 64) __attribute__((amdgpu_waves_per_eu(
+    ((NS & kPairSteps) != 0 && XS == kStripRow) ? 2 : 1))) void
+sk_dgemm_kernel(Launch L) {
   using S = TileShape<BM, BN, BK, WM, WN>;
   constexpr int NT = S::NT, TM = S::TM, TN = S::TN, MB = S::MB, NB = S::NB;
   static_assert(!XS || (NS >= 2 && !BATCHED), "strip: single DMA launches");
@@ -1111,7 +1190,7 @@ static size_t lds_bytes_t() {
   size_t d = 2ull * TileShape<BM, BN, BK, WM, WN>::kBufDoubles;
   if constexpr (NS >= 2) {
     const size_t dma =
-        (size_t)NS * (dma_region<BM, BK>() + dma_region<BN, BK>() +
+        (size_t)(NS & 0xff) * (dma_region<BM, BK>() + dma_region<BN, BK>() +
                       (XS ? dma_region<16, BK>() : 0));
     if (dma > d) d = dma;
   }
@@ -1200,6 +1279,8 @@ static int occupancy_t() {
     case 14: return FN<64, 64, 16, 2, 2, 2, true>(__VA_ARGS__);   \
     case 15: return FN<128, 128, 16, 4, 2, 2>(__VA_ARGS__);  \
     case 16: return FN<64, 64, 16, 2, 2, 2>(__VA_ARGS__);    \
+    case 17: return FN<64, 64, 16, 2, 2, 4 | kPairSteps, true>(__VA_ARGS__); \
+    case 18: return FN<64, 64, 16, 2, 2, 4 | kPairSteps>(__VA_ARGS__);       \
     default: break;                                        \
   }
 
