@@ -268,7 +268,9 @@ def test_every_square_tile_column_major(skg, ctx, orc, rng):
                                       (12, 130, 520), (12, 200, 64),
                                       (11, 136, 70), (11, 144, 37),
                                       (14, 80, 33), (14, 200, 64),
-                                      (13, 136, 70)])
+                                      (13, 136, 70), (17, 72, 16),
+                                      (17, 80, 33), (17, 130, 520),
+                                      (17, 200, 64), (17, 200, 80)])
 def test_strip_tiles(skg, ctx, rng, tile, N, K):
     """Strip tiles on a SYRK: N = T x BM + r (r <= 16) tiles a T x T
     triangle whose last tile row carries the r remainder rows; wave 0 of the
@@ -276,7 +278,8 @@ def test_strip_tiles(skg, ctx, rng, tile, N, K):
     layout, alpha and beta, default grid, split tiles and K chunks; the
     upper triangle is never written.  (N = 200 has remainder 8 on 64-wide
     tiles; tile 12 is 64 x 64, tile 11 128 x 128, tiles 14 and 13 the same
-    with two DMA stages.)"""
+    with two DMA stages, tile 17 as 12 with paired steps: K = 16, 33, 64 and
+    80 give units of 1, 3, 4 and 5 steps.)"""
     A = rng.uniform(-1, 1, (K, N))
     C0 = rng.uniform(-1, 1, (N, N))
     ref = 0.5 * (A.T @ A) - 1.25 * C0
