@@ -84,8 +84,9 @@ constexpr int kFwdRows = kFwdWaves * kFwdRowsPerWave;  // rows per chunk
 // Wave w takes rows 32 w .. 32 w + 31 of the chunk as eight steps of
 // v_mfma_f64_16x16x4 per 16-band block (A = 16 chi columns x 4 rows, B = 4
 // psi rows x 16 bands): lane l gathers psi at row 4 s + l / 16, bands
-// 16 j + l % 16 of the workgroup's 64 (32 loads issued at once, on clamped
-// rows and bands) and reads its chi values from a copy the host stores in
+// 4 (l % 16) + j of the workgroup's 64 (four adjacent values of one 16-band
+// panel; 32 loads issued at once, on clamped rows) and reads its chi values
+// from a copy the host stores in
 // the A layout (zero past the chunk's rows and the projector's columns, so
 // the clamped rows add nothing).  The 8 waves' sums go through LDS and are
 // added in wave order.  (A non-finite psi value at a chunk's last row can
