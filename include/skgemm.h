@@ -112,6 +112,15 @@ int skgemm_synchronize(skgemm_ctx* ctx);
  * error instead of a hang. */
 int skgemm_set_spin_limit(skgemm_ctx* ctx, int64_t limit);
 
+/* Failure-path test hook for the plain FP64 kernel's waiting fix-up.  1:
+ * the workgroups that finish a split tile store their partials but never
+ * raise their ready flags, so every owner's wait runs out (the spin limit is
+ * clamped to 64 rounds meanwhile) and the call reports SKGEMM_ERR_TIMEOUT;
+ * results are then undefined.  0 (default): normal.  No effect on the
+ * wait-free fix-up.  No counterpart in the reference (it has no bounded
+ * waits); for tests that force a timeout deterministically. */
+int skgemm_set_fixup_hold(skgemm_ctx* ctx, int on);
+
 /* Stream-K fix-up of the plain FP64 kernel on this context (single,
  * batched and host-pointer forms).  0 (default): the workgroup that starts
  * a split tile waits for the partials of the workgroups that finish it,

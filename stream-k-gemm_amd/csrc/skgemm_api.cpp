@@ -294,6 +294,13 @@ extern "C" int skgemm_set_spin_limit(skgemm_ctx* c, int64_t limit) {
   return SKGEMM_OK;
 }
 
+extern "C" int skgemm_set_fixup_hold(skgemm_ctx* c, int on) {
+  SKG_TRY(check_ctx(c));
+  if (on != 0 && on != 1) return set_error(SKGEMM_ERR_ARG, "on must be 0 or 1");
+  c->fixup_hold = on != 0;
+  return SKGEMM_OK;
+}
+
 extern "C" int skgemm_set_syrk_store(skgemm_ctx* c, int mode) {
   SKG_TRY(check_ctx(c));
   if (mode != SKGEMM_SYRK_LOWER && mode != SKGEMM_SYRK_DIAG_BLOCKS)
@@ -768,8 +775,9 @@ int run_single(skgemm_ctx* c, Problem& p, const Plan& plan_in,
     L.flags = c->flags;
   }
   L.status = c->status;
-  L.spin_limit = c->spin_limit;
+  L.spin_limit = c->fixup_spin_limit();
   L.waitfree = c->waitfree ? 1 : 0;
+  L.hold = c->fixup_hold ? 1 : 0;
   if (timed) c->record_begin();
   SKG_HIP(launch_dgemm_sk(pl.cfg, L, c->stream), "Stream-K launch");
   if (p.kchunks > 1)

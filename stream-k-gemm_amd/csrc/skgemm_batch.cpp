@@ -199,8 +199,9 @@ int launch_plain_table(skgemm_ctx* c, const PlainTable& tb,
   L.ws = c->ws;
   L.flags = c->flags;
   L.status = c->status;
-  L.spin_limit = c->spin_limit;
+  L.spin_limit = c->fixup_spin_limit();
   L.waitfree = c->waitfree ? 1 : 0;
+  L.hold = c->fixup_hold ? 1 : 0;
   SKG_HIP(launch_dgemm_sk(tb.cfg, L, c->stream), "batched launch");
   return SKGEMM_OK;
 }
@@ -244,8 +245,9 @@ int run_plain(skgemm_ctx* c, const skgemm_problem* q, int n, int use_graph) {
   L.ws = c->ws;
   L.flags = c->flags;
   L.status = c->status;
-  L.spin_limit = c->spin_limit;
+  L.spin_limit = c->fixup_spin_limit();
   L.waitfree = c->waitfree ? 1 : 0;
+  L.hold = c->fixup_hold ? 1 : 0;
   c->record_begin();
   if (!use_graph) {
     SKG_HIP(launch_dgemm_sk(cfg, L, c->stream), "batched launch");

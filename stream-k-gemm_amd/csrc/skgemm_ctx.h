@@ -47,7 +47,15 @@ struct skgemm_ctx {
   int64_t spin_limit = 1ll << 26;
   // Last-arriver fix-up (no waits between workgroups): twice the slots.
   bool waitfree = false;
+  // Failure-path test hook: peers never raise their ready flags, so every
+  // waiting owner runs out its (clamped) spin limit.
+  bool fixup_hold = false;
   size_t ws_doubles(int cfg, int grid) const;
+  // The spin limit a launch carries: at most 64 rounds while the hold
+  // hook is on, so a held wait ends within microseconds.
+  int64_t fixup_spin_limit() const {
+    return fixup_hold && spin_limit > 64 ? 64 : spin_limit;
+  }
   // SYRK outputs: SKGEMM_SYRK_LOWER or SKGEMM_SYRK_DIAG_BLOCKS.
   int syrk_store = SKGEMM_SYRK_LOWER;
   // Per-chunk sums of K-chunked launches.

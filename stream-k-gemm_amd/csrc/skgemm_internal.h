@@ -124,6 +124,8 @@ struct Launch {
   int* status;          // bit 0: a fix-up wait hit its spin limit
   int64_t spin_limit;   // s_sleep rounds before a wait gives up
   int waitfree;         // 1: last-arriver fix-up, no inter-workgroup waits
+  int hold;             // 1: peers store partials but never raise their
+                        // flags (failure-path tests; waiting fix-up only)
 };
 
 // Tile configurations (index = kernel table entry).

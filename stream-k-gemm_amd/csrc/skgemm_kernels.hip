@@ -1059,7 +1059,7 @@ sk_dgemm_kernel(Launch L) {
       spill(wsd + (int64_t)w * kSlot);
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
-      if (tid == 0)
+      if (tid == 0 && !L.hold)
         __hip_atomic_store(&L.flags[w], 1, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
       write_out = false;

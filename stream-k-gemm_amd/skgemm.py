@@ -26,7 +26,7 @@ ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_TIMEOUT, ERR_NOTSUP, ERR_STATE = -1, -2, -3, -4
 EXPORTS = [
     "skgemm_last_error", "skgemm_version", "skgemm_create", "skgemm_destroy",
     "skgemm_set_stream", "skgemm_synchronize", "skgemm_set_spin_limit",
-    "skgemm_set_waitfree",
+    "skgemm_set_waitfree", "skgemm_set_fixup_hold",
     "skgemm_set_syrk_store",
     "skgemm_device_info", "skgemm_dev_alloc", "skgemm_dev_free",
     "skgemm_memcpy_h2d", "skgemm_memcpy_d2h", "skgemm_dgemm",
@@ -90,6 +90,7 @@ def lib():
         "skgemm_synchronize": (c_int, [P]),
         "skgemm_set_spin_limit": (c_int, [P, c_int64]),
         "skgemm_set_waitfree": (c_int, [P, c_int]),
+        "skgemm_set_fixup_hold": (c_int, [P, c_int]),
         "skgemm_set_syrk_store": (c_int, [P, c_int]),
         "skgemm_device_info": (c_int, [P, P, P, P, P]),
         "skgemm_dev_alloc": (c_int, [P, c_size_t, c_int, ctypes.POINTER(P)]),
@@ -234,6 +235,11 @@ class Context:
     def set_waitfree(self, on):
         """Last-arriver Stream-K fix-up (no waits between workgroups)."""
         check(lib().skgemm_set_waitfree(self.h, 1 if on else 0))
+
+    def set_fixup_hold(self, on):
+        """Test hook: peers never raise their fix-up flags, so waiting owners
+        time out (spin limit clamped to 64 rounds)."""
+        check(lib().skgemm_set_fixup_hold(self.h, 1 if on else 0))
 
     def set_syrk_store(self, mode):
         """SYRK_LOWER or SYRK_DIAG_BLOCKS (whole 16 x 16 diagonal blocks)."""
