@@ -226,6 +226,61 @@ __global__ __launch_bounds__(256) void halo_extend_kernel(
   }
 }
 
+// The same fill as 16-byte moves, for 16-byte aligned grids whose extended
+// plane holds fewer than 2^31 double pairs: a workgroup takes kHaloChunk
+// consecutive pairs of one extended plane of one tile (nje rows of nie
+// points of 8 pairs, as stored), and each thread issues the loads of its
+// kHaloBatch pairs before it stores any of them.  Grid: tiles16 x nke x
+// ceil(plane pairs / kHaloChunk), plane-major.  On the 32^3 grid with 520
+// bands and FDn 6, 16 KB per workgroup and 4 loads per thread ran at the
+// copy kernel's rate (96-97 us); 32 and 64 KB workgroups took 103-111 us,
+// 4 KB ones (one pair per thread) 127 us.
+typedef double hpair __attribute__((ext_vector_type(2)));
+constexpr int kHaloChunk = 1024;  // pairs (16 KB) per workgroup
+constexpr int kHaloBatch = 4;     // pairs per thread in flight
+
+__global__ __launch_bounds__(256) void halo_extend_vec_kernel(
+    const hpair* __restrict__ local, int ni, int nj, int nk, int F, int mode,
+    int chunks, hpair* __restrict__ ext) {
+  const int nie = ni + 2 * F, nje = nj + 2 * F, nke = nk + 2 * F;
+  const int nie8 = nie * 8;
+  const int pp = nje * nie8;  // pairs per extended plane
+  const int pl = blockIdx.x / chunks;  // t16 * nke + ke
+  const int c = blockIdx.x - pl * chunks;
+  const int t16 = pl / nke;
+  const int ke = pl - t16 * nke;
+  const int k = ke - F;
+  const bool kin = k >= 0 && k < nk;
+  if (!kin && mode == 2) return;  // a plane of ghost points only
+  const int kk = kin ? k : wrap(k, nk);
+  const hpair* src = local + ((int64_t)t16 * nk + kk) * nj * ni * 8;
+  hpair* dst = ext + (int64_t)pl * pp;
+  const int e1 = min(c * kHaloChunk + kHaloChunk, pp);
+  for (int b = c * kHaloChunk + (int)threadIdx.x; b < e1;
+       b += 256 * kHaloBatch) {
+    hpair v[kHaloBatch];
+    bool st[kHaloBatch];
+#pragma unroll
+    for (int q = 0; q < kHaloBatch; ++q) {
+      const int e = b + 256 * q;
+      const int je = e / nie8;
+      const int r = e - je * nie8;
+      const int j = je - F, i = (r >> 3) - F;
+      const bool in = kin && j >= 0 && j < nj && i >= 0 && i < ni;
+      st[q] = e < e1 && (in || mode != 2);
+      v[q] = hpair{0.0, 0.0};
+      if (e < e1 && (in || mode == 1)) {
+        const int jj = in ? j : wrap(j, nj);
+        const int ii = in ? i : wrap(i, ni);
+        v[q] = src[((int64_t)jj * ni + ii) * 8 + (r & 7)];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kHaloBatch; ++q)
+      if (st[q]) dst[b + 256 * q] = v[q];
+  }
+}
+
 int check_grid(int ni, int nj, int nk, int nb, int F) {
   if (ni <= 0 || nj <= 0 || nk <= 0 || nb < 0)
     return set_error(SKGEMM_ERR_ARG, "grid %d x %d x %d, %d bands", ni, nj, nk,
@@ -266,11 +321,25 @@ extern "C" int skgemm_halo_extend(skgemm_ctx* c, int ni, int nj, int nk,
   SKG_TRY(check_dev(local16, "local16"));
   SKG_TRY(check_dev(ext16, "ext16"));
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
-  const int nje = nj + 2 * FDn, nke = nk + 2 * FDn;
-  const dim3 grid(nje, ((nke + kHaloPlanes - 1) / kHaloPlanes) * ((nb + 15) / 16));
+  const int nie = ni + 2 * FDn, nje = nj + 2 * FDn, nke = nk + 2 * FDn;
+  const int64_t tiles = (nb + 15) / 16;
+  const int64_t pairs = (int64_t)nje * nie * 8;  // per extended plane
+  const int64_t chunks = (pairs + kHaloChunk - 1) / kHaloChunk;
+  const bool vec = ((uintptr_t)local16 & 15) == 0 &&
+                   ((uintptr_t)ext16 & 15) == 0 &&
+                   pairs <= 0x7fffffff - 2 * kHaloChunk &&
+                   tiles * nke * chunks < 0x7fffffff;
   c->record_begin();
-  hipLaunchKernelGGL(halo_extend_kernel, grid, dim3(256), 0, c->stream,
-                     local16, ni, nj, nk, FDn, boundary, ext16);
+  if (vec) {
+    hipLaunchKernelGGL(halo_extend_vec_kernel,
+                       dim3((unsigned)(tiles * nke * chunks)), dim3(256), 0,
+                       c->stream, (const hpair*)local16, ni, nj, nk, FDn,
+                       boundary, (int)chunks, (hpair*)ext16);
+  } else {
+    const dim3 grid(nje, ((nke + kHaloPlanes - 1) / kHaloPlanes) * tiles);
+    hipLaunchKernelGGL(halo_extend_kernel, grid, dim3(256), 0, c->stream,
+                       local16, ni, nj, nk, FDn, boundary, ext16);
+  }
   SKG_HIP(hipGetLastError(), "halo launch");
   c->record_end();
   return SKGEMM_OK;

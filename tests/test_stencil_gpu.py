@@ -98,3 +98,36 @@ def test_interior_copy_leaves_ghosts(skg, ctx, orc, rng):
     inside = np.repeat(np.tile(e.ravel(), (nb + 15) // 16), 16)
     assert np.array_equal(got[inside], zero[inside])
     assert np.all(got[~inside] == 7.0)
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1, 3, 1), (3, 2, 5, 17, 3),
+                                   (50, 3, 2, 16, 4), (7, 5, 9, 40, 8)])
+@pytest.mark.parametrize("offset", [0, 1])
+def test_halo_shapes_modes_alignment(skg, ctx, orc, shape, offset):
+    """Every boundary mode on planes smaller than one workgroup's share,
+    ghost widths beyond the grid (periodic wraps more than once), and grids
+    one double off 16-byte alignment (offset 1: the one-double-per-lane
+    kernel; offset 0: the 16-byte kernel)."""
+    ni, nj, nk, nb, F = shape
+    print("halo", shape, "offset", offset, flush=True)
+    rng = np.random.default_rng(sum(shape) + offset)
+    t = (nb + 15) // 16
+    local = rng.uniform(-1, 1, t * ni * nj * nk * 16)
+    n_ext = t * (ni + 2 * F) * (nj + 2 * F) * (nk + 2 * F) * 16
+    e = np.zeros((nk + 2 * F, nj + 2 * F, ni + 2 * F), bool)
+    e[F:F + nk, F:F + nj, F:F + ni] = True
+    inside = np.repeat(np.tile(e.ravel(), t), 16)
+    d_loc = skg.DeviceArray.from_host(ctx, np.concatenate([[0.0] * offset,
+                                                           local]))
+    for mode in (0, 1, 2):
+        d_ext = skg.DeviceArray.from_host(ctx, np.full(n_ext + offset, 7.0))
+        ctx.halo_extend(ni, nj, nk, nb, F, d_loc.ptr + 8 * offset,
+                        d_ext.ptr + 8 * offset, boundary=mode)
+        ctx.synchronize()
+        got = d_ext.download()[offset:]
+        want = orc.halo_extend(ni, nj, nk, nb, F, local, mode == 1)
+        if mode == 2:
+            assert np.array_equal(got[inside], want[inside])
+            assert np.all(got[~inside] == 7.0)
+        else:
+            assert np.array_equal(got, want), mode
