@@ -131,3 +131,53 @@ def test_halo_shapes_modes_alignment(skg, ctx, orc, shape, offset):
             assert np.all(got[~inside] == 7.0)
         else:
             assert np.array_equal(got, want), mode
+
+
+@pytest.mark.parametrize("shape", [(7, 5, 9, 33, 6), (9, 17, 3, 17, 4),
+                                   (16, 8, 17, 31, 2)])
+@pytest.mark.parametrize("tile", [32, 16])
+def test_odd_bands_short_chunks(skg, ctx, orc, shape, tile):
+    """Odd band counts (a last band pair with one band), grids that leave
+    partial 8 x 8 blocks and a last k chunk shorter than 8 planes."""
+    ni, nj, nk, nb, F = shape
+    print("stencil", shape, "tile", tile, flush=True)
+    rng = np.random.default_rng(sum(shape) + tile)
+    cx, cy, cz = (rng.uniform(-1, 1, F + 1) for _ in range(3))
+    run(skg, ctx, orc, ni, nj, nk, nb, F, True, tile, cx, cy, cz, 0.3,
+        sum(shape))
+
+
+@pytest.mark.parametrize("F", [2, 6])
+def test_aligned_and_unaligned_kernels_agree(skg, ctx, orc, F):
+    """16-byte aligned grids and outputs take the band-pair kernel, grids
+    one double off alignment the one-double kernel: the same terms in the
+    same order, so the two give identical bits (and the oracle's values)."""
+    ni, nj, nk, nb, tile = 10, 9, 11, 37, 32
+    rng = np.random.default_rng(70 + F)
+    cx, cy, cz = (rng.uniform(-1, 1, F + 1) for _ in range(3))
+    local = grid_pack16(rng.uniform(-1, 1, (nb, nk, nj, ni)), nb)
+    K = ni * nj * nk
+    A = rng.uniform(-1, 1, K)
+    t16 = (nb + 15) // 16
+    n_ext = t16 * (ni + 2 * F) * (nj + 2 * F) * (nk + 2 * F) * 16
+    n_out = ((nb + tile - 1) // tile) * K * tile
+    d_A = skg.DeviceArray.from_host(ctx, A)
+    outs = []
+    for off in (0, 1):
+        print("FDn", F, "offset", off, flush=True)
+        d_loc = skg.DeviceArray.from_host(ctx, np.concatenate([[0.0] * off,
+                                                               local]))
+        d_ext = skg.DeviceArray(ctx, n_ext + off, zero=True)
+        d_out = skg.DeviceArray.from_host(ctx, np.full(n_out + off, -3.5))
+        ctx.halo_extend(ni, nj, nk, nb, F, d_loc.ptr + 8 * off,
+                        d_ext.ptr + 8 * off, True)
+        ctx.stencil_apply(ni, nj, nk, nb, F, d_ext.ptr + 8 * off, d_A.ptr,
+                          cx, cy, cz, 0.4, d_out.ptr + 8 * off, tile)
+        ctx.synchronize()
+        outs.append(d_out.download()[off:])
+    assert np.array_equal(outs[0], outs[1])
+    ext_want = orc.halo_extend(ni, nj, nk, nb, F, local, True)
+    want = orc.stencil_apply(ni, nj, nk, nb, F, ext_want, A, cx, cy, cz, 0.4,
+                             tile, out=np.full(n_out, -3.5))
+    assert np.array_equal(outs[0] == -3.5, want == -3.5)
+    assert rel_fro(outs[0], want) < TOL
