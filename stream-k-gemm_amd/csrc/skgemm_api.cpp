@@ -1082,9 +1082,20 @@ static int convert(skgemm_ctx* c, bool swap, int src_layout, const double* src,
   pack_side(n_unit, m_unit, d.pshift, &n_fast, &w);
   map.dst_fast_o = (n_fast != swap) ? 1 : 0;
   map.dst_wshift = w;
+  // 16-byte pairs along a side's lane order: a 16-byte aligned base, unit
+  // stride along that index, the other index's stride and the panel stride
+  // even, panels at least two wide (a pair starts at an even index).
+  const bool s_al = ((uintptr_t)src & 15) == 0 && s.pstride % 2 == 0 &&
+                    s.pshift >= 1;
+  const bool vs = s_al && (map.src_fast_o ? (s.os == 1 && s.ks % 2 == 0)
+                                          : (s.ks == 1 && s.os % 2 == 0));
+  const bool d_al = ((uintptr_t)dst & 15) == 0 && d.pstride % 2 == 0 &&
+                    d.pshift >= 1;
+  const bool vd = d_al && (n_fast ? (d.ns == 1 && d.ms % 2 == 0)
+                                  : (d.ms == 1 && d.ns % 2 == 0));
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
   c->record_begin();
-  SKG_HIP(launch_pack(s, r_extent, d, map, c->stream), name);
+  SKG_HIP(launch_pack(s, r_extent, d, map, vs, vd, c->stream), name);
   c->record_end();
   return SKGEMM_OK;
 }

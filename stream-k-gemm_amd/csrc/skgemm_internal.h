@@ -175,8 +175,10 @@ struct PackMap {
   int src_fast_o, src_wshift;
   int dst_fast_o, dst_wshift;
 };
+// vs / vd: 16-byte pairs along the source / destination lane order are
+// contiguous and aligned (pack2_kernel), else one element per lane.
 hipError_t launch_pack(const Operand& src, int r_extent, const Output& dst,
-                       const PackMap& map, hipStream_t s);
+                       const PackMap& map, bool vs, bool vd, hipStream_t s);
 hipError_t launch_zero_ints(int* p, int n, hipStream_t s);
 hipError_t launch_peak_probe(int which, int blocks, int iters, double* sink,
                              hipStream_t s);

@@ -1461,14 +1461,99 @@ __global__ __launch_bounds__(256) void pack_kernel(Operand src, int r_extent,
   }
 }
 
+// pack_kernel with 16-byte global accesses on the sides where a pair of
+// elements along the unit-stride index is contiguous and 16-byte aligned
+// (VS: source, VD: destination; the host checks base alignment and even
+// strides): a thread moves 8 pairs of the 64 x 64 tile, its pair i being
+// elements 2 (t + 256 i) and 2 (t + 256 i) + 1 of that side's lane order;
+// the LDS tile is accessed per element as in pack_kernel.  A pair whose
+// second element lies past the extent moves its first element alone.
+typedef double pk2 __attribute__((ext_vector_type(2)));
+
+template <bool VS, bool VD>
+__global__ __launch_bounds__(256) void pack2_kernel(Operand src, int r_extent,
+                                                    Output dst, PackMap map) {
+  __shared__ double tile[kPackTile * kPackPitch];
+  const int t = threadIdx.x;
+  const int r0 = blockIdx.x * kPackTile;
+  const int o_blocks = (src.extent + kPackTile - 1) / kPackTile;
+  constexpr int NS = VS ? 8 : 16, ND = VD ? 8 : 16;
+  for (int ob = blockIdx.y; ob < o_blocks; ob += gridDim.y) {
+    const int o0 = ob * kPackTile;
+    double v0[NS], v1[NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      int o, r;
+      pack_coords((VS ? 2 : 1) * (t + 256 * i), map.src_fast_o,
+                  map.src_wshift, &o, &r);
+      const bool in0 = o0 + o < src.extent && r0 + r < r_extent;
+      const bool in1 = VS && (map.src_fast_o ? o0 + o + 1 < src.extent
+                                             : r0 + r + 1 < r_extent) && in0;
+      const double* a = src.ptr + op_addr(src, o0 + o, r0 + r);
+      v0[i] = 0.0;
+      v1[i] = 0.0;
+      if (in1) {
+        const pk2 x = *(const pk2*)a;
+        v0[i] = x.x;
+        v1[i] = x.y;
+      } else if (in0) {
+        v0[i] = a[0];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      int o, r;
+      pack_coords((VS ? 2 : 1) * (t + 256 * i), map.src_fast_o,
+                  map.src_wshift, &o, &r);
+      tile[o * kPackPitch + r] = v0[i];
+      if (VS) {
+        if (map.src_fast_o)
+          tile[(o + 1) * kPackPitch + r] = v1[i];
+        else
+          tile[o * kPackPitch + r + 1] = v1[i];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      int o, r;
+      pack_coords((VD ? 2 : 1) * (t + 256 * i), map.dst_fast_o,
+                  map.dst_wshift, &o, &r);
+      const bool in0 = o0 + o < src.extent && r0 + r < r_extent;
+      if (!in0) continue;
+      const int o1 = VD && map.dst_fast_o ? o + 1 : o;
+      const int r1 = VD && !map.dst_fast_o ? r + 1 : r;
+      const bool in1 = VD && o0 + o1 < src.extent && r0 + r1 < r_extent;
+      const int64_t a = map.swap ? out_addr(dst, o0 + o, r0 + r)
+                                 : out_addr(dst, r0 + r, o0 + o);
+      const double x0 = tile[o * kPackPitch + r];
+      if (in1)
+        *(pk2*)(dst.ptr + a) = pk2{x0, tile[o1 * kPackPitch + r1]};
+      else
+        dst.ptr[a] = x0;
+    }
+    __syncthreads();
+  }
+}
+
 hipError_t launch_pack(const Operand& src, int r_extent, const Output& dst,
-                       const PackMap& map, hipStream_t s) {
+                       const PackMap& map, bool vs, bool vd, hipStream_t s) {
   if (src.extent <= 0 || r_extent <= 0) return hipSuccess;
   const int bx = (r_extent + kPackTile - 1) / kPackTile;
   const int ob = (src.extent + kPackTile - 1) / kPackTile;
   const int by = ob < 65535 ? ob : 65535;
-  hipLaunchKernelGGL(pack_kernel, dim3(bx, by), dim3(256), 0, s, src,
-                     r_extent, dst, map);
+  if (vs && vd)
+    hipLaunchKernelGGL((pack2_kernel<true, true>), dim3(bx, by), dim3(256), 0,
+                       s, src, r_extent, dst, map);
+  else if (vs)
+    hipLaunchKernelGGL((pack2_kernel<true, false>), dim3(bx, by), dim3(256),
+                       0, s, src, r_extent, dst, map);
+  else if (vd)
+    hipLaunchKernelGGL((pack2_kernel<false, true>), dim3(bx, by), dim3(256),
+                       0, s, src, r_extent, dst, map);
+  else
+    hipLaunchKernelGGL(pack_kernel, dim3(bx, by), dim3(256), 0, s, src,
+                       r_extent, dst, map);
   return hipGetLastError();
 }
 

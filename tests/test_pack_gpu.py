@@ -24,27 +24,47 @@ def expected_buffer(X, layout, pad):
     return out, ld
 
 
-def run_convert(skg, ctx, swap, X, ls, ldst, pad_src=3, pad_dst=2):
-    """X is R x O (element (r, o)); src is encoded in the B convention."""
+def run_convert(skg, ctx, swap, X, ls, ldst, pad_src=3, pad_dst=2, off=0):
+    """X is R x O (element (r, o)); src is encoded in the B convention.
+    off: both buffers start `off` doubles into their allocations."""
     R, O = X.shape
     src, ld_s = encode_b(X, ls, pad_src)
     want, ld_d = expected_buffer(X.T if swap else X, ldst, pad_dst)
-    d_src = skg.DeviceArray.from_host(ctx, src)
-    d_dst = skg.DeviceArray.from_host(ctx, np.full(want.shape, SENTINEL))
+    d_src = skg.DeviceArray.from_host(ctx, np.concatenate([[0.0] * off, src]))
+    d_dst = skg.DeviceArray.from_host(ctx, np.full(want.size + off,
+                                                   SENTINEL))
     f = ctx.transpose if swap else ctx.pack
-    f(ls, d_src.ptr, ld_s, ldst, d_dst.ptr, ld_d, O, R)
+    f(ls, d_src.ptr + 8 * off, ld_s, ldst, d_dst.ptr + 8 * off, ld_d, O, R)
     ctx.synchronize()
-    return d_dst.download(), want
+    got = d_dst.download()
+    assert np.all(got[:off] == SENTINEL)
+    return got[off:], want
 
 
 @pytest.mark.parametrize("swap", [False, True])
 @pytest.mark.parametrize("shape", [(45, 70), (130, 300), (64, 64), (1, 17)])
-def test_every_layout_pair(skg, ctx, rng, swap, shape):
+@pytest.mark.parametrize("pads", [(3, 2), (2, 4)])
+def test_every_layout_pair(skg, ctx, rng, swap, shape, pads):
+    """Odd and even leading dimensions (pads): 16-byte pairs on neither,
+    one or both sides of the conversion; odd extents leave single
+    elements at the edges."""
     X = rng.uniform(-1, 1, shape)
     for ls in range(4):
         for ldst in range(4):
-            print("swap", swap, "shape", shape, "layouts", ls, ldst, flush=True)
-            got, want = run_convert(skg, ctx, swap, X, ls, ldst)
+            print("swap", swap, "shape", shape, "pads", pads, "layouts", ls,
+                  ldst, flush=True)
+            got, want = run_convert(skg, ctx, swap, X, ls, ldst, *pads)
+            assert np.array_equal(got, want), (ls, ldst)
+
+
+@pytest.mark.parametrize("swap", [False, True])
+def test_unaligned_buffers(skg, ctx, rng, swap):
+    """Buffers one double off 16-byte alignment take the one-element path."""
+    X = rng.uniform(-1, 1, (70, 45))
+    for ls in range(4):
+        for ldst in range(4):
+            print("unaligned swap", swap, "layouts", ls, ldst, flush=True)
+            got, want = run_convert(skg, ctx, swap, X, ls, ldst, 2, 4, off=1)
             assert np.array_equal(got, want), (ls, ldst)
 
 
