@@ -290,6 +290,9 @@ __device__ inline void nloc_segment_apply(const NlocSegOps<NP, NJ>& o,
       for (int j = 0; j < NJ; ++j)
         d[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.a[m][s], o.x[s][j], d[j],
                                                     0, 0, 0);
+    // Further 16-column passes.  Loads are unconditional (the column index
+    // is clamped, so every address is valid; columns past ncol are zeroed
+    // by a select), so none waits alone in a branch next to its use.
     for (int p = 1; p < kp; ++p) {
       const double* ap = chia + (S.a0 + (int64_t)m * kp + p) * 256 + lane * 4;
 #pragma unroll
@@ -298,10 +301,16 @@ __device__ inline void nloc_segment_apply(const NlocSegOps<NP, NJ>& o,
         const double* xc = X + (int64_t)(S.coff + min(c, S.ncol - 1)) * xs +
                            b0 + NJ * (lane & 15);
         const double av = ap[s];
+        double xv[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          const double x = xc[j];
+          xv[j] = c < S.ncol ? x : 0.0;
+        }
 #pragma unroll
         for (int j = 0; j < NJ; ++j)
-          d[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-              av, c < S.ncol ? xc[j] : 0.0, d[j], 0, 0, 0);
+          d[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, xv[j], d[j], 0, 0,
+                                                      0);
       }
     }
 #pragma unroll
