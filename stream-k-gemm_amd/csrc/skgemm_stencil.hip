@@ -292,11 +292,15 @@ __global__ __launch_bounds__(256) void halo_extend_vec_kernel(
 // accesses are buffer operations from resources based at the workgroup's
 // first plane, with fixed per-thread offsets and a uniform per-plane one.
 // Each plane's column value, diagonal and halo are loaded one plane ahead.
-// For 16-byte aligned grids and outputs whose kStencilPlanes + 2 FDn
-// extended planes and kStencilPlanes output planes span fewer than
+// For 16-byte aligned grids and outputs whose kPairPlanes + 2 FDn
+// extended planes and kPairPlanes output planes span fewer than
 // kStencilRecords bytes.  On the 32^3 grid with 520 bands and FDn 6:
-// 124 us against 143 us for stencil_kernel (one band per lane in this form
+// 120 us against 143 us for stencil_kernel (one band per lane in this form
 // took 132 us; loads two planes ahead 127 us).
+// Planes per workgroup: 16 here (the z prologue of 2 FDn planes is then
+// shared by twice the outputs; 120 us against 127 us with 8 on one box),
+// 8 for stencil_kernel (16 measured slower there).
+constexpr int kPairPlanes = 16;
 constexpr int kStencilRsrc3 = 0x00020000;        // raw buffer (gfx9 family)
 constexpr int kStencilRecords = 0x7FFFFF00;      // bytes addressable
 
@@ -325,7 +329,7 @@ __attribute__((amdgpu_waves_per_eu(2, 8))) void stencil_pair_kernel(
   const int p = tid >> 3;                    // 0..63
   const int nbi = (ni + kSB - 1) / kSB;
   const int nblk = nbi * ((nj + kSB - 1) / kSB);
-  const int nkc = (nk + kStencilPlanes - 1) / kStencilPlanes;
+  const int nkc = (nk + kPairPlanes - 1) / kPairPlanes;
   const int units = gridDim.x;
   const int xcd = blockIdx.x & 7, per = units >> 3, xr = units & 7;
   const int u = xcd * per + (xcd < xr ? xcd : xr) + (blockIdx.x >> 3);
@@ -333,8 +337,8 @@ __attribute__((amdgpu_waves_per_eu(2, 8))) void stencil_pair_kernel(
   const int i0 = (blkid % nbi) * kSB, j0 = (blkid / nbi) * kSB;
   const int pi = p % kSB, pj = p / kSB;
   const int i = i0 + pi, j = j0 + pj;
-  const int k0 = (u / nblk) % nkc * kStencilPlanes;
-  const int k1 = k0 + kStencilPlanes < nk ? k0 + kStencilPlanes : nk;
+  const int k0 = (u / nblk) % nkc * kPairPlanes;
+  const int k1 = k0 + kPairPlanes < nk ? k0 + kPairPlanes : nk;
   const int t16 = u / (nblk * nkc);
   const int b = t16 * 16 + 2 * bp;
   const bool band_ok = b < nb;
@@ -420,13 +424,13 @@ __attribute__((amdgpu_waves_per_eu(2, 8))) void stencil_pair_kernel(
   load_next(0);
   const int me = ((pj + F) * E + (pi + F)) * 8 + bp;
 #pragma unroll
-  for (int t = 0; t < kStencilPlanes; ++t) {
+  for (int t = 0; t < kPairPlanes; ++t) {
     q[2 * F] = qn;
     const double dg = ak + c.c0;
     sl[me] = q[F];
 #pragma unroll
     for (int m = 0; m < kHs; ++m) sl[hoff[m] * 8 + bp] = hv[m];
-    if (t + 1 < kStencilPlanes) load_next(t + 1);
+    if (t + 1 < kPairPlanes) load_next(t + 1);
     __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): LDS stores done
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
@@ -560,7 +564,17 @@ extern "C" int skgemm_stencil_apply(skgemm_ctx* c, int ni, int nj, int nk,
   }
   cf.c0 = coef_0;
   SKG_HIP(hipSetDevice(c->device), "hipSetDevice");
-  const int kchunks = (nk + kStencilPlanes - 1) / kStencilPlanes;
+  // The band-pair kernel's resources span kPairPlanes + 2 FDn extended
+  // planes and kPairPlanes diagonal and output planes from a chunk's first
+  // plane.
+  const int64_t ep = (int64_t)(ni + 2 * FDn) * (nj + 2 * FDn);
+  const int64_t gp = (int64_t)ni * nj;
+  const bool pair = ((uintptr_t)ext16 & 15) == 0 &&
+                    ((uintptr_t)out & 15) == 0 &&
+                    (kPairPlanes + 2 * FDn) * ep * 128 < kStencilRecords &&
+                    kPairPlanes * gp * 8 * out_tile < kStencilRecords;
+  const int planes = pair ? kPairPlanes : kStencilPlanes;
+  const int kchunks = (nk + planes - 1) / planes;
   const int64_t units = (int64_t)((ni + kSB - 1) / kSB) *
                         ((nj + kSB - 1) / kSB) * kchunks * ((nb + 15) / 16);
   if (units >= 0x7fffffff)
@@ -568,15 +582,6 @@ extern "C" int skgemm_stencil_apply(skgemm_ctx* c, int ni, int nj, int nk,
   const dim3 grid((unsigned)units);
   const int ts = out_tile == 32 ? 5 : 4;
   c->record_begin();
-  // The band-pair kernel's resources span kStencilPlanes + 2 FDn extended
-  // planes and kStencilPlanes diagonal and output planes from a chunk's
-  // first plane.
-  const int64_t ep = (int64_t)(ni + 2 * FDn) * (nj + 2 * FDn);
-  const int64_t gp = (int64_t)ni * nj;
-  const bool pair = ((uintptr_t)ext16 & 15) == 0 &&
-                    ((uintptr_t)out & 15) == 0 &&
-                    (kStencilPlanes + 2 * FDn) * ep * 128 < kStencilRecords &&
-                    kStencilPlanes * gp * 8 * out_tile < kStencilRecords;
 #define SKG_STENCIL(F)                                                     \
   case F:                                                                  \
     if (pair)                                                              \
